@@ -1,0 +1,189 @@
+"""Benchmark: Richardson–Lucy voxel-iterations/s at 64x512x512 float32 on one MI355X.
+
+A step is one RL iteration (forward convolution with the fused ratio
+epilogue + adjoint correlation with the fused multiplicative update) on a
+64x512x512 volume with a full-size widefield PSF (FFT canvas 128x1024x1024).
+Inputs are resident in HBM before the timed region starts.
+
+Prints one JSON line (see the keys below).  ``--gpus N`` with N > 1 exits:
+every run here uses one GPU.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+METRIC = "voxel-iters/s (RL, PDHG-Hessian) at 64x512x512 fp32; serving p50/p99 latency"
+SHAPE = (64, 512, 512)
+LATERAL, AXIAL = 0.085, 0.1
+N_BLOBS = 800          # config-0 blob density (200 per 64x256x256) on a 64x512x512 volume
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def conv_pass_bytes(S, P):
+    """Algorithmic HBM bytes of the five passes of one pruned convolution (complex64 = 8 B)."""
+    s0, s1, s2 = S
+    p0, p1, p2 = P
+    H = p2 // 2 + 1
+    return {
+        "x_fwd": 4 * s0 * s1 * s2 + 8 * s0 * s1 * H,
+        "y_fwd": 8 * s0 * s1 * H + 8 * s0 * p1 * H,
+        "z_conv": 8 * s0 * p1 * H + 8 * p0 * p1 * H + 8 * s0 * p1 * H,
+        "y_inv": 8 * s0 * p1 * H + 8 * s0 * s1 * H,
+        "x_inv": 8 * s0 * s1 * H + 4 * s0 * s1 * s2,
+    }
+
+
+def make_inputs(torch):
+    from deconlib_amd import LinearFFTConvolver
+    from deconlib_amd.psf import widefield_psf
+    from deconlib_amd.synth import CONFIG0_OPTICS, blob_volume
+
+    t0 = time.time()
+    psf = widefield_psf(SHAPE, LATERAL, AXIAL, CONFIG0_OPTICS)
+    obj = blob_volume(SHAPE, N_BLOBS, np.random.default_rng(1000)).astype(np.float32)
+    blur = LinearFFTConvolver(psf, SHAPE)
+    clean = np.maximum(blur.forward(torch.from_numpy(obj).cuda()).cpu().numpy(), 0.0)
+    prng = np.random.default_rng(np.random.SeedSequence(1000).spawn(1)[0])
+    obs = prng.poisson(clean).astype(np.float32)
+    log(f"inputs ready in {time.time() - t0:.1f}s")
+    return psf, obs, blur
+
+
+def cpu_baseline(psf, obs, iters):
+    """The oracle's RL iteration (fp32 NumPy/SciPy restatement) timed on the host cores."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import deconv as O
+
+    cores = len(os.sched_getaffinity(0))
+    A = O.LinearFFTConvolver(psf, obs.shape)
+    t = np.float32
+    x = np.maximum(A.adjoint(np.maximum(obs, t(1e-6))), t(1e-6))
+    s = A.adjoint(np.ones_like(obs))
+    act = s > t(max(float(s.max()) * 1e-6, 1e-10))
+    s = np.where(act, s, t(1.0))
+    eps = t(1e-10)
+
+    def step(x):
+        r = obs / np.maximum(A.forward(x), eps)
+        return np.maximum(x * np.where(act, A.adjoint(r) / s, t(0.0)), eps)
+
+    x = step(x)  # warm-up iteration
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        x = step(x)
+    dt = time.perf_counter() - t0
+    return {
+        "value": float(np.prod(obs.shape) * iters / dt),
+        "unit": "voxel-iters/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{iters} RL iterations of the fp32 NumPy/SciPy restatement at 64x512x512 "
+                  f"(scipy.fft workers={cores}), after 1 warm-up iteration; {dt:.1f}s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-iters", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
+              "(multi-GPU sharding is a design note, not a measured path)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from deconlib_amd import _native as N
+    from deconlib_amd.rl import _RLState
+
+    psf, obs, blur = make_inputs(torch)
+    st = _RLState(obs, blur, background=0.0, num_iter=args.warmup + args.steps + 1, eval_interval=10)
+    for k in range(args.warmup):
+        st.step(k)
+    torch.cuda.synchronize()
+    N.call("dh_timing_read", None, None, 1)
+    N.call("dh_timing_enable", 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.warmup, args.warmup + args.steps):
+        st.step(k)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    N.call("dh_timing_enable", 0)
+    import ctypes as C
+
+    ms = (C.c_double * 5)()
+    cnt = (C.c_int64 * 5)()
+    N.call("dh_timing_read", ms, cnt, 1)
+    vox = int(np.prod(SHAPE))
+    value = vox * args.steps / dt
+    P = blur.canvas_shape
+    pb = conv_pass_bytes(SHAPE, P)
+    per_pass = {}
+    for i, name in enumerate(N.PASS_NAMES):
+        if cnt[i]:
+            avg = ms[i] / cnt[i]
+            per_pass[name] = {"avg_ms": avg, "gbs": pb[name] / (avg * 1e-3) / 1e9, "launches": int(cnt[i])}
+    z = per_pass["z_conv"]
+    log(f"per-pass: {json.dumps(per_pass)}")
+    ok = bool(torch.isfinite(st.x).all().item())
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(psf, obs, args.cpu_iters)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "voxel-iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "Richardson-Lucy, 64x512x512 float32, widefield PSF 64x512x512 "
+                        "(NA 1.4, 0.525 um, 0.085/0.1 um), FFT canvas "
+                        f"{P[0]}x{P[1]}x{P[2]}, {N_BLOBS} Gaussian blobs + 10, Poisson",
+            "volume": list(SHAPE),
+            "canvas": list(P),
+            "solver": "richardson_lucy_with_operator (fused)",
+            "finite": ok,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "z_conv (fused Z FFT x OTF x inverse Z FFT)",
+            "achieved": z["gbs"],
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": z["gbs"] / PEAK_HBM_GBS,
+            "traffic": None,
+            "algorithmic_bytes": pb["z_conv"],
+            "avg_ms": z["avg_ms"],
+        },
+        "passes": per_pass,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

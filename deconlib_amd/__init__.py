@@ -1,0 +1,32 @@
+"""deconlib_amd — MI355X-native FFT-operator deconvolution.
+
+Drop-in for the hot path of ``deconlib.deconvolution``: the operator protocol
+and composition, ``LinearFFTConvolver`` / ``FFTConvolver``, ``Pad`` / ``Crop`` /
+``FractionalAreaDownsample``, ``Hessian3D``, Richardson–Lucy and PDHG with
+Hessian regularization.  All arithmetic runs in hand-written HIP kernels for
+gfx950 (``deconlib_amd/_lib/libdeconhip.so``, C ABI in ``include/deconhip.h``);
+PyTorch only owns device buffers and streams.
+"""
+
+from .base import DeconvolutionResult, MLXDeconvolutionResult
+from .composition import Compose, LinearOperator, compose
+from .core_operators import Crop, FractionalAreaDownsample, Pad
+from .linops import FFTConvolver, LinearFFTConvolver, fast_padded_shape
+from .rl import RLResult, poisson_i_divergence, richardson_lucy_with_operator
+
+__all__ = [
+    "MLXDeconvolutionResult",
+    "DeconvolutionResult",
+    "RLResult",
+    "richardson_lucy_with_operator",
+    "poisson_i_divergence",
+    "FFTConvolver",
+    "LinearFFTConvolver",
+    "fast_padded_shape",
+    "Compose",
+    "LinearOperator",
+    "compose",
+    "Pad",
+    "Crop",
+    "FractionalAreaDownsample",
+]

@@ -1,0 +1,134 @@
+"""ctypes binding of libdeconhip.so (the C ABI declared in include/deconhip.h).
+
+The HIP library is the only compute path: if it is missing this module raises
+at import time, so nothing can silently fall back to a CPU or PyTorch route.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import torch  # loads the HIP runtime first, so the library shares it
+
+_LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libdeconhip.so"
+
+if not _LIB_PATH.exists():
+    raise ImportError(
+        f"libdeconhip.so not found at {_LIB_PATH}; build it with "
+        "`python -m deconlib_amd._build` (hipcc, gfx950)"
+    )
+
+lib = C.CDLL(str(_LIB_PATH))
+
+ABI_VERSION = 1
+
+DH_OK, DH_EINVAL, DH_EHIP, DH_EWORKSPACE, DH_EUNSUPPORTED = 0, -1, -2, -3, -4
+
+PRO_NONE, PRO_ONES, PRO_SUB_FLOOR = 0, 1, 2
+EPI_NONE, EPI_FLOOR, EPI_RL_RATIO, EPI_RL_UPDATE, EPI_ADD = 0, 1, 2, 3, 4
+RED_SUM, RED_MAX, RED_SUMSQ = 0, 1, 2
+
+
+class PlanInfo(C.Structure):
+    _fields_ = [
+        ("ndim", C.c_int32),
+        ("signal_shape", C.c_int32 * 3),
+        ("kernel_shape", C.c_int32 * 3),
+        ("fft_shape", C.c_int32 * 3),
+        ("canvas_shape", C.c_int32 * 3),
+        ("otf_bytes", C.c_int64),
+        ("work_bytes", C.c_int64),
+        ("twiddle_bytes", C.c_int64),
+    ]
+
+
+class FuseArgs(C.Structure):
+    _fields_ = [
+        ("prologue", C.c_int32),
+        ("pro_a", C.c_float),
+        ("pro_b", C.c_float),
+        ("epilogue", C.c_int32),
+        ("epi_a", C.c_float),
+        ("epi_b", C.c_float),
+        ("aux0", C.c_void_p),
+        ("aux1", C.c_void_p),
+        ("reduce_out", C.c_void_p),
+    ]
+
+
+_vp, _i64, _int, _f, _d = C.c_void_p, C.c_int64, C.c_int, C.c_float, C.c_double
+
+
+def _sig(name, res, *args):
+    fn = getattr(lib, name)
+    fn.restype = res
+    fn.argtypes = list(args)
+    return fn
+
+
+# Every symbol the header declares; a missing one fails the import.
+SYMBOLS = {
+    "dh_abi_version": (_int,),
+    "dh_last_error": (C.c_char_p,),
+    "dh_conv_plan_create": (_int, _int, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64), _int, C.POINTER(_vp)),
+    "dh_conv_plan_info": (_int, _vp, C.POINTER(PlanInfo)),
+    "dh_conv_plan_bind": (_int, _vp, _vp, _vp),
+    "dh_conv_plan_destroy": (_int, _vp),
+    "dh_otf_build": (_int, _vp, _vp, _int, _vp, _vp, _i64, C.POINTER(_d), _vp),
+    "dh_conv_apply": (_int, _vp, _int, _vp, _vp, _vp, _vp, _i64, C.POINTER(FuseArgs), _vp),
+    "dh_reduce_partials": (_i64, _i64),
+    "dh_reduce": (_int, _int, _vp, _i64, _vp, _vp, _vp),
+    "dh_rl_support": (_int, _vp, _vp, _i64, _vp, _vp),
+    "dh_clamp_min": (_int, _vp, _vp, _i64, _f, _vp),
+    "dh_sub_floor": (_int, _vp, _vp, _i64, _f, _f, _vp),
+    "dh_rl_ratio": (_int, _vp, _vp, _vp, _i64, _f, _f, _vp, _vp, _vp),
+    "dh_rl_update": (_int, _vp, _vp, _vp, _i64, _f, _vp),
+    "dh_add_scalar": (_int, _vp, _vp, _vp, _i64, _f, _vp, _vp, _vp),
+    "dh_box_copy": (_int, _vp, C.POINTER(_i64), _vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64),
+                    C.POINTER(_i64), _vp),
+    "dh_banded_apply": (_int, _vp, _vp, _i64, _i64, _i64, _i64, _vp, _vp, _int, _vp),
+    "dh_timing_enable": (_int, _int),
+    "dh_timing_read": (_int, C.POINTER(_d), C.POINTER(_i64), _int),
+}
+
+PASS_NAMES = ("x_fwd", "y_fwd", "z_conv", "y_inv", "x_inv")
+
+fns = {name: _sig(name, sig[0], *sig[1:]) for name, sig in SYMBOLS.items()}
+
+if fns["dh_abi_version"]() != ABI_VERSION:
+    raise ImportError(
+        f"libdeconhip.so ABI {fns['dh_abi_version']()} != expected {ABI_VERSION}; rebuild it"
+    )
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+def check(rc: int, what: str = "") -> None:
+    """Map a status code to the reference's exception types."""
+    if rc == DH_OK:
+        return
+    msg = fns["dh_last_error"]().decode(errors="replace")
+    text = f"{what}: {msg}" if what else msg
+    if rc == DH_EINVAL:
+        raise ValueError(text)
+    if rc == DH_EUNSUPPORTED:
+        raise NotImplementedError(text)
+    raise NativeError(f"{text} (status {rc})")
+
+
+def call(name: str, *args) -> int:
+    rc = fns[name](*args)
+    if isinstance(rc, int) and name not in ("dh_reduce_partials", "dh_abi_version"):
+        check(rc, name)
+    return rc
+
+
+def stream_ptr() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def ptr(t) -> int | None:
+    return None if t is None else t.data_ptr()

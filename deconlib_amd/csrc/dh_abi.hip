@@ -1,0 +1,487 @@
+// C ABI of libdeconhip.so: plans, OTF construction, one convolution
+// application, and the RL elementwise / reduction entry points.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/deconhip.h"
+#include "dh_fft.hpp"
+#include "dh_ops.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+int hip_check(hipError_t e, const char* where) {
+  if (e != hipSuccess) return fail(DH_EHIP, std::string(where) + ": " + hipGetErrorString(e));
+  return DH_OK;
+}
+
+#define DH_TRY(expr)            \
+  do {                          \
+    int _rc = (expr);           \
+    if (_rc != DH_OK) return _rc; \
+  } while (0)
+
+bool is_smooth(int n) {
+  if (n < 1) return false;
+  for (int p : {2, 3, 5})
+    while (n % p == 0) n /= p;
+  return n == 1;
+}
+
+int next_smooth(int n) {
+  while (!is_smooth(n)) ++n;
+  return n;
+}
+
+// Radix schedule for a 5-smooth n: 16s first, then 8/4/2, then 5s and 3s.
+bool make_sched(int n, dh::AxisSched& f, dh::AxisSched& iv) {
+  if (!is_smooth(n)) return false;
+  int m = n, c2 = 0, c3 = 0, c5 = 0;
+  while (m % 2 == 0) m /= 2, ++c2;
+  while (m % 3 == 0) m /= 3, ++c3;
+  while (m % 5 == 0) m /= 5, ++c5;
+  std::vector<int> r;
+  int n16 = c2 / 4, rem = c2 % 4;
+  if (rem == 1 && n16 > 0) {
+    --n16;
+    for (int i = 0; i < n16; ++i) r.push_back(16);
+    r.push_back(8);
+    r.push_back(4);
+  } else {
+    for (int i = 0; i < n16; ++i) r.push_back(16);
+    if (rem == 3) r.push_back(8);
+    if (rem == 2) r.push_back(4);
+    if (rem == 1) r.push_back(2);
+  }
+  for (int i = 0; i < c5; ++i) r.push_back(5);
+  for (int i = 0; i < c3; ++i) r.push_back(3);
+  if ((int)r.size() > dh::kMaxPasses) return false;
+  f.n = iv.n = n;
+  f.np = iv.np = (int)r.size();
+  int sp = 1;
+  for (size_t p = 0; p < r.size(); ++p) {
+    f.radix[p] = r[p];
+    f.span[p] = sp;
+    sp *= r[p];
+  }
+  sp = 1;
+  for (size_t p = 0; p < r.size(); ++p) {
+    iv.radix[p] = r[r.size() - 1 - p];
+    iv.span[p] = sp;
+    sp *= iv.radix[p];
+  }
+  return true;
+}
+
+void push_table(std::vector<float2>& tw, int n, int count) {
+  for (int m = 0; m < count; ++m) {
+    const double a = -2.0 * M_PI * (double)m / (double)n;
+    tw.push_back(make_float2((float)std::cos(a), (float)std::sin(a)));
+  }
+}
+
+// Per-pass event timing (measurement only; off unless dh_timing_enable(1)).
+struct TimedPass {
+  int pass;
+  hipEvent_t a, b;
+};
+bool g_timing = false;
+std::vector<TimedPass> g_pending;
+double g_ms[DH_PASS_COUNT] = {0};
+int64_t g_cnt[DH_PASS_COUNT] = {0};
+
+int timing_begin(int pass, hipStream_t st) {
+  if (!g_timing) return DH_OK;
+  TimedPass t{pass, nullptr, nullptr};
+  DH_TRY(hip_check(hipEventCreate(&t.a), "event create"));
+  DH_TRY(hip_check(hipEventCreate(&t.b), "event create"));
+  DH_TRY(hip_check(hipEventRecord(t.a, st), "event record"));
+  g_pending.push_back(t);
+  return DH_OK;
+}
+
+int timing_end(hipStream_t st) {
+  if (!g_timing) return DH_OK;
+  return hip_check(hipEventRecord(g_pending.back().b, st), "event record");
+}
+
+}  // namespace
+
+struct dh_conv_plan {
+  int ndim = 3;
+  int circular = 0;
+  int s[3], k[3], fft[3], P[3];
+  int H = 0, M = 0;
+  int klo[3], khi[3];
+  dh::AxisSched zf, zi, yf, yi, xf, xi;
+  int64_t tw_z = 0, tw_y = 0, tw_xm = 0, tw_xp = 0;
+  std::vector<float2> tw_host;
+  const float2* tw_dev = nullptr;
+  int64_t b1_off = 0, b2_off = 0, part_off = 0, scal_off = 0, ws_bytes = 0;
+  int64_t n_partials = 0;
+};
+
+extern "C" {
+
+int dh_abi_version(void) { return DH_ABI_VERSION; }
+
+const char* dh_last_error(void) { return g_err.c_str(); }
+
+int dh_conv_plan_create(int ndim, const int64_t* signal_shape, const int64_t* kernel_shape,
+                        const int64_t* fft_shape, int circular, dh_conv_plan** out) {
+  if (!out || !signal_shape || !kernel_shape) return fail(DH_EINVAL, "null argument");
+  if (ndim < 1 || ndim > 3) return fail(DH_EINVAL, "ndim must be 1, 2 or 3");
+  auto* p = new dh_conv_plan();
+  p->ndim = ndim;
+  p->circular = circular ? 1 : 0;
+  const int off = 3 - ndim;
+  for (int a = 0; a < 3; ++a) {
+    p->s[a] = p->k[a] = p->fft[a] = 1;
+  }
+  for (int a = 0; a < ndim; ++a) {
+    if (signal_shape[a] < 1 || kernel_shape[a] < 1 || signal_shape[a] > (1 << 20) || kernel_shape[a] > (1 << 20)) {
+      delete p;
+      return fail(DH_EINVAL, "shape entries must be in [1, 2^20]");
+    }
+    p->s[off + a] = (int)signal_shape[a];
+    p->k[off + a] = (int)kernel_shape[a];
+  }
+  for (int a = 0; a < 3; ++a) {
+    int minimum = p->s[a] + p->k[a] - 1;
+    int f;
+    if (fft_shape && a >= off) {
+      f = (int)fft_shape[a - off];
+    } else if (circular) {
+      f = p->s[a];
+    } else {
+      f = next_smooth(std::max(p->s[a] + p->k[a] - 1, p->k[a]));
+    }
+    if (circular) {
+      if (p->s[a] != p->k[a] || f != p->s[a]) {
+        delete p;
+        return fail(DH_EINVAL, "circular plan needs equal signal, kernel and fft shapes");
+      }
+    } else if (f < minimum) {
+      delete p;
+      return fail(DH_EINVAL, "fft_shape is too small for linear convolution; minimum is n + m - 1 per axis");
+    }
+    p->fft[a] = f;
+  }
+  // Canvas this plan computes on: any canvas >= n + m - 1 gives the same
+  // linear convolution, so non-5-smooth or odd x extents are rounded up to
+  // the next even 5-smooth length (the real-to-complex pass needs even x).
+  for (int a = 0; a < 3; ++a) {
+    int c = p->fft[a];
+    if (!circular) {
+      c = next_smooth(c);
+      if (a == 2) {
+        while (c % 2 != 0 || !is_smooth(c)) ++c;
+      }
+    } else if (!is_smooth(c) || (a == 2 && c % 2 != 0 && c != 1)) {
+      delete p;
+      return fail(DH_EUNSUPPORTED, "circular plan needs 5-smooth extents and an even last axis");
+    }
+    p->P[a] = c;
+  }
+  if (p->P[2] == 1) {
+    // 1-point x axis: treat as 2-point canvas is wrong for circular; reject
+    delete p;
+    return fail(DH_EUNSUPPORTED, "last axis of length 1 is not supported");
+  }
+  for (int a = 0; a < 3; ++a) {
+    if (circular) {
+      p->klo[a] = p->k[a];
+      p->khi[a] = 0;
+    } else {
+      p->klo[a] = (p->k[a] + 1) / 2;
+      p->khi[a] = p->k[a] - p->klo[a];
+    }
+  }
+  p->M = p->P[2] / 2;
+  p->H = p->M + 1;
+  if (!make_sched(p->P[0], p->zf, p->zi) || !make_sched(p->P[1], p->yf, p->yi) ||
+      !make_sched(p->M, p->xf, p->xi)) {
+    delete p;
+    return fail(DH_EUNSUPPORTED, "canvas extents must be 5-smooth");
+  }
+  // twiddle tables
+  p->tw_z = 0;
+  push_table(p->tw_host, p->P[0], p->P[0]);
+  p->tw_y = (int64_t)p->tw_host.size();
+  push_table(p->tw_host, p->P[1], p->P[1]);
+  p->tw_xm = (int64_t)p->tw_host.size();
+  push_table(p->tw_host, p->M, p->M);
+  p->tw_xp = (int64_t)p->tw_host.size();
+  push_table(p->tw_host, p->P[2], p->M + 1);
+  // workspace layout
+  const int64_t H = p->H;
+  const int64_t rows1 = std::max<int64_t>((int64_t)p->s[0] * p->s[1], (int64_t)p->k[0] * p->k[1]);
+  const int64_t b1 = rows1 * H * 8;
+  const int64_t b2 = (int64_t)std::max(p->s[0], p->k[0]) * p->P[1] * H * 8;
+  p->n_partials = (int64_t)p->s[0] * p->s[1] + 2048;
+  auto align = [](int64_t v) { return (v + 255) / 256 * 256; };
+  p->b1_off = 0;
+  p->b2_off = align(b1);
+  p->part_off = p->b2_off + align(b2);
+  p->scal_off = p->part_off + align(p->n_partials * 8);
+  p->ws_bytes = p->scal_off + 256;
+  *out = p;
+  return DH_OK;
+}
+
+int dh_conv_plan_info(const dh_conv_plan* p, dh_plan_info* info) {
+  if (!p || !info) return fail(DH_EINVAL, "null argument");
+  info->ndim = p->ndim;
+  for (int a = 0; a < 3; ++a) {
+    info->signal_shape[a] = p->s[a];
+    info->kernel_shape[a] = p->k[a];
+    info->fft_shape[a] = p->fft[a];
+    info->canvas_shape[a] = p->P[a];
+  }
+  info->otf_bytes = (int64_t)p->P[0] * p->P[1] * p->H * 8;
+  info->work_bytes = p->ws_bytes;
+  info->twiddle_bytes = (int64_t)p->tw_host.size() * 8;
+  return DH_OK;
+}
+
+int dh_conv_plan_bind(dh_conv_plan* p, void* twiddles_dev, void* stream) {
+  if (!p || !twiddles_dev) return fail(DH_EINVAL, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  DH_TRY(hip_check(hipMemcpyAsync(twiddles_dev, p->tw_host.data(), p->tw_host.size() * 8,
+                                  hipMemcpyHostToDevice, st),
+                   "twiddle upload"));
+  DH_TRY(hip_check(hipStreamSynchronize(st), "twiddle upload sync"));
+  p->tw_dev = (const float2*)twiddles_dev;
+  return DH_OK;
+}
+
+int dh_conv_plan_destroy(dh_conv_plan* p) {
+  delete p;
+  return DH_OK;
+}
+
+int dh_otf_build(const dh_conv_plan* p, const float* kernel_dev, int normalize, void* otf_dev, void* work_dev,
+                 int64_t work_bytes, double* norm_sq_host, void* stream) {
+  if (!p || !kernel_dev || !otf_dev || !work_dev) return fail(DH_EINVAL, "null argument");
+  if (!p->tw_dev) return fail(DH_EINVAL, "plan is not bound (dh_conv_plan_bind)");
+  if (work_bytes < p->ws_bytes) return fail(DH_EWORKSPACE, "workspace too small");
+  hipStream_t st = (hipStream_t)stream;
+  char* ws = (char*)work_dev;
+  float2* B1 = (float2*)(ws + p->b1_off);
+  float2* B2 = (float2*)(ws + p->b2_off);
+  double* part = (double*)(ws + p->part_off);
+  double* scal = (double*)(ws + p->scal_off);
+  const int64_t nk = (int64_t)p->k[0] * p->k[1] * p->k[2];
+  dh::Pro pro{dh::PRO_NONE, 0.f, 0.f, nullptr};
+  if (normalize) {
+    DH_TRY(hip_check(dh::launch_reduce(dh::RED_SUM, kernel_dev, nk, part, scal, st), "kernel sum"));
+    pro.kind = dh::PRO_DIV_DEV;
+    pro.dptr = scal;
+  }
+  const int64_t H = p->H;
+  DH_TRY(hip_check(dh::launch_x_r2c(p->xf, p->tw_dev + p->tw_xm, p->tw_dev + p->tw_xp, kernel_dev, B1,
+                                    (int64_t)p->k[0] * p->k[1], p->k[2], p->klo[2], p->khi[2], p->H, pro, st),
+                   "otf x pass"));
+  float2* zsrc = B1;
+  if (p->P[1] > 1) {
+    DH_TRY(hip_check(dh::launch_col_fft(-1, p->yf, p->tw_dev + p->tw_y, B1, B2, H, p->k[0], (int64_t)p->k[1] * H,
+                                        (int64_t)p->P[1] * H, H, H, p->klo[1], p->khi[1], p->P[1], 1.0f, st),
+                     "otf y pass"));
+    zsrc = B2;
+  }
+  const double total = (double)p->P[0] * p->P[1] * p->P[2];
+  const int64_t ncols = (int64_t)p->P[1] * H;
+  DH_TRY(hip_check(dh::launch_col_fft(-1, p->zf, p->tw_dev + p->tw_z, zsrc, (float2*)otf_dev, ncols, 1, 0, 0,
+                                      ncols, ncols, p->klo[0], p->khi[0], p->P[0], (float)(1.0 / total), st),
+                   "otf z pass"));
+  if (norm_sq_host) {
+    DH_TRY(hip_check(dh::launch_cmax_abs2((const float2*)otf_dev, (int64_t)p->P[0] * ncols, part, scal + 1, st),
+                     "otf norm"));
+    double v = 0.0;
+    DH_TRY(hip_check(hipMemcpyAsync(&v, scal + 1, sizeof(double), hipMemcpyDeviceToHost, st), "otf norm copy"));
+    DH_TRY(hip_check(hipStreamSynchronize(st), "otf norm sync"));
+    *norm_sq_host = v * total * total;
+  }
+  return DH_OK;
+}
+
+int dh_conv_apply(const dh_conv_plan* p, int adjoint, const float* in, float* out, const void* otf_dev,
+                  void* work_dev, int64_t work_bytes, const dh_fuse_args* fuse, void* stream) {
+  if (!p || !out || !otf_dev || !work_dev) return fail(DH_EINVAL, "null argument");
+  if (!p->tw_dev) return fail(DH_EINVAL, "plan is not bound (dh_conv_plan_bind)");
+  if (work_bytes < p->ws_bytes) return fail(DH_EWORKSPACE, "workspace too small");
+  dh::Pro pro{dh::PRO_NONE, 0.f, 0.f, nullptr};
+  dh::Epi epi{dh::EPI_NONE, 0.f, 0.f, nullptr, nullptr, nullptr};
+  double* reduce_out = nullptr;
+  char* ws = (char*)work_dev;
+  if (fuse) {
+    pro.kind = fuse->prologue;
+    pro.a = fuse->pro_a;
+    pro.b = fuse->pro_b;
+    epi.kind = fuse->epilogue;
+    epi.a = fuse->epi_a;
+    epi.b = fuse->epi_b;
+    epi.aux0 = fuse->aux0;
+    epi.aux1 = fuse->aux1;
+    if (pro.kind < DH_PRO_NONE || pro.kind > DH_PRO_SUB_FLOOR) return fail(DH_EINVAL, "unknown prologue");
+    if (epi.kind < DH_EPI_NONE || epi.kind > DH_EPI_ADD) return fail(DH_EINVAL, "unknown epilogue");
+    if ((epi.kind == DH_EPI_RL_RATIO || epi.kind == DH_EPI_RL_UPDATE) && !epi.aux0)
+      return fail(DH_EINVAL, "epilogue needs aux0");
+    if (epi.kind == DH_EPI_RL_UPDATE && !epi.aux1) return fail(DH_EINVAL, "epilogue needs aux1");
+    if (fuse->reduce_out) {
+      if (!(epi.kind == DH_EPI_RL_RATIO || (epi.kind == DH_EPI_ADD && epi.aux0)))
+        return fail(DH_EINVAL, "this epilogue has no reduction");
+      reduce_out = fuse->reduce_out;
+      epi.partials = (double*)(ws + p->part_off);
+    }
+  }
+  if (!in && pro.kind != DH_PRO_ONES) return fail(DH_EINVAL, "null input");
+  hipStream_t st = (hipStream_t)stream;
+  float2* B1 = (float2*)(ws + p->b1_off);
+  float2* B2 = (float2*)(ws + p->b2_off);
+  const int s0 = p->s[0], s1 = p->s[1], s2 = p->s[2];
+  const int64_t H = p->H;
+  const int64_t nrows = (int64_t)s0 * s1;
+  DH_TRY(timing_begin(DH_PASS_X_FWD, st));
+  DH_TRY(hip_check(dh::launch_x_r2c(p->xf, p->tw_dev + p->tw_xm, p->tw_dev + p->tw_xp, in, B1, nrows, s2, s2, 0,
+                                    p->H, pro, st),
+                   "x forward pass"));
+  DH_TRY(timing_end(st));
+  float2* zb = B1;
+  if (p->P[1] > 1) {
+    DH_TRY(timing_begin(DH_PASS_Y_FWD, st));
+    DH_TRY(hip_check(dh::launch_col_fft(-1, p->yf, p->tw_dev + p->tw_y, B1, B2, H, s0, (int64_t)s1 * H,
+                                        (int64_t)p->P[1] * H, H, H, s1, 0, p->P[1], 1.0f, st),
+                     "y forward pass"));
+    DH_TRY(timing_end(st));
+    zb = B2;
+  }
+  DH_TRY(timing_begin(DH_PASS_Z, st));
+  DH_TRY(hip_check(dh::launch_z_conv(p->zf, p->zi, p->tw_dev + p->tw_z, zb, (const float2*)otf_dev,
+                                     (int64_t)p->P[1] * H, s0, s0, adjoint ? 1 : 0, st),
+                   "z pass"));
+  DH_TRY(timing_end(st));
+  if (p->P[1] > 1) {
+    DH_TRY(timing_begin(DH_PASS_Y_INV, st));
+    DH_TRY(hip_check(dh::launch_col_fft(1, p->yi, p->tw_dev + p->tw_y, B2, B1, H, s0, (int64_t)p->P[1] * H,
+                                        (int64_t)s1 * H, H, H, p->P[1], 0, s1, 1.0f, st),
+                     "y inverse pass"));
+    DH_TRY(timing_end(st));
+  }
+  int64_t nblocks = 0;
+  DH_TRY(timing_begin(DH_PASS_X_INV, st));
+  DH_TRY(hip_check(dh::launch_x_c2r(p->xi, p->tw_dev + p->tw_xm, p->tw_dev + p->tw_xp, B1, out, nrows, s2, p->H,
+                                    epi, &nblocks, st),
+                   "x inverse pass"));
+  DH_TRY(timing_end(st));
+  if (reduce_out) {
+    if (nblocks > p->n_partials) return fail(DH_EWORKSPACE, "partials area too small");
+    DH_TRY(hip_check(dh::launch_sum_partials(epi.partials, nblocks, reduce_out, st), "epilogue reduction"));
+  }
+  return DH_OK;
+}
+
+int dh_timing_enable(int on) {
+  g_timing = on != 0;
+  return DH_OK;
+}
+
+int dh_timing_read(double* ms, int64_t* counts, int reset) {
+  for (auto& t : g_pending) {
+    DH_TRY(hip_check(hipEventSynchronize(t.b), "event sync"));
+    float e = 0.f;
+    DH_TRY(hip_check(hipEventElapsedTime(&e, t.a, t.b), "event elapsed"));
+    g_ms[t.pass] += e;
+    g_cnt[t.pass] += 1;
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+  }
+  g_pending.clear();
+  for (int i = 0; i < DH_PASS_COUNT; ++i) {
+    if (ms) ms[i] = g_ms[i];
+    if (counts) counts[i] = g_cnt[i];
+    if (reset) {
+      g_ms[i] = 0.0;
+      g_cnt[i] = 0;
+    }
+  }
+  return DH_OK;
+}
+
+int64_t dh_reduce_partials(int64_t n) { return dh::reduce_nblocks(n); }
+
+int dh_reduce(int op, const float* x, int64_t n, double* partials_dev, double* result_dev, void* stream) {
+  if (!x || !partials_dev || !result_dev || n < 1) return fail(DH_EINVAL, "bad reduce arguments");
+  if (op < DH_RED_SUM || op > DH_RED_SUMSQ) return fail(DH_EINVAL, "unknown reduction");
+  return hip_check(dh::launch_reduce(op, x, n, partials_dev, result_dev, (hipStream_t)stream), "reduce");
+}
+
+int dh_rl_support(float* sens, float* x, int64_t n, const double* maxval_dev, void* stream) {
+  if (!sens || !maxval_dev) return fail(DH_EINVAL, "null argument");
+  return hip_check(dh::launch_rl_support(sens, x, n, maxval_dev, (hipStream_t)stream), "rl_support");
+}
+
+int dh_clamp_min(const float* in, float* out, int64_t n, float lo, void* stream) {
+  if (!in || !out) return fail(DH_EINVAL, "null argument");
+  return hip_check(dh::launch_clamp_min(in, out, n, lo, (hipStream_t)stream), "clamp_min");
+}
+
+int dh_sub_floor(const float* in, float* out, int64_t n, float a, float b, void* stream) {
+  if (!in || !out) return fail(DH_EINVAL, "null argument");
+  return hip_check(dh::launch_sub_floor(in, out, n, a, b, (hipStream_t)stream), "sub_floor");
+}
+
+int dh_rl_ratio(const float* model, const float* obs, float* out, int64_t n, float bg, float eps,
+                double* partials_dev, double* reduce_out, void* stream) {
+  if (!model || !obs || !out) return fail(DH_EINVAL, "null argument");
+  if (reduce_out && !partials_dev) return fail(DH_EINVAL, "reduction needs partials");
+  return hip_check(dh::launch_rl_ratio(model, obs, out, n, bg, eps, partials_dev, reduce_out, (hipStream_t)stream),
+                   "rl_ratio");
+}
+
+int dh_rl_update(float* x, const float* corr, const float* sens, int64_t n, float eps, void* stream) {
+  if (!x || !corr || !sens) return fail(DH_EINVAL, "null argument");
+  return hip_check(dh::launch_rl_update(x, corr, sens, n, eps, (hipStream_t)stream), "rl_update");
+}
+
+int dh_add_scalar(const float* in, const float* obs, float* out, int64_t n, float a, double* partials_dev,
+                  double* reduce_out, void* stream) {
+  if (!in || !out) return fail(DH_EINVAL, "null argument");
+  if (reduce_out && (!partials_dev || !obs)) return fail(DH_EINVAL, "reduction needs partials and obs");
+  return hip_check(dh::launch_add_scalar(in, obs, out, n, a, partials_dev, reduce_out, (hipStream_t)stream),
+                   "add_scalar");
+}
+
+int dh_box_copy(const float* src, const int64_t* src_shape, float* dst, const int64_t* dst_shape,
+                const int64_t* src_start, const int64_t* dst_start, const int64_t* extent, void* stream) {
+  if (!src || !dst || !src_shape || !dst_shape || !src_start || !dst_start || !extent)
+    return fail(DH_EINVAL, "null argument");
+  for (int a = 0; a < 3; ++a) {
+    if (extent[a] < 0 || src_start[a] < 0 || dst_start[a] < 0 || src_start[a] + extent[a] > src_shape[a] ||
+        dst_start[a] + extent[a] > dst_shape[a])
+      return fail(DH_EINVAL, "box outside the volumes");
+  }
+  return hip_check(dh::launch_box_copy(src, src_shape, dst, dst_shape, src_start, dst_start, extent,
+                                       (hipStream_t)stream),
+                   "box_copy");
+}
+
+int dh_banded_apply(const float* src, float* dst, int64_t outer, int64_t n_in, int64_t inner, int64_t n_out,
+                    const int32_t* idx, const float* wt, int width, void* stream) {
+  if (!src || !dst || !idx || !wt) return fail(DH_EINVAL, "null argument");
+  if (outer < 1 || n_in < 1 || inner < 1 || n_out < 1 || width < 1) return fail(DH_EINVAL, "bad extents");
+  return hip_check(dh::launch_banded(src, dst, outer, n_in, inner, n_out, idx, wt, width, (hipStream_t)stream),
+                   "banded_apply");
+}
+
+}  // extern "C"

@@ -1,0 +1,84 @@
+// Shared declarations for the pruned 3-D FFT convolution passes.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dh {
+
+constexpr int kMaxPasses = 16;
+// Elements of one line a thread keeps in registers per pass (upper bound
+// that fixes the per-radix butterfly count, see kMB below).
+constexpr int kEPT = 16;
+
+// A Stockham schedule for one axis: radices in pass order and the span
+// (product of the previous radices) of each pass.
+struct AxisSched {
+  int n;
+  int np;
+  int radix[kMaxPasses];
+  int span[kMaxPasses];
+};
+
+// Fused prologue (input sample of the forward X pass) and epilogue (cropped
+// output sample of the inverse X pass).
+enum ProKind { PRO_NONE = 0, PRO_ONES = 1, PRO_SUB_FLOOR = 2, PRO_DIV_DEV = 100 };
+enum EpiKind { EPI_NONE = 0, EPI_FLOOR = 1, EPI_RL_RATIO = 2, EPI_RL_UPDATE = 3, EPI_ADD = 4 };
+
+struct Pro {
+  int kind;
+  float a, b;
+  const double* dptr;  // PRO_DIV_DEV: divide by (float)*dptr
+};
+
+struct Epi {
+  int kind;
+  float a, b;
+  const float* aux0;
+  const float* aux1;
+  double* partials;  // per-block partial sums (nullptr: no reduction)
+};
+
+// Launch helpers (host).
+struct ColLaunch {
+  int L;        // columns per workgroup
+  int T;        // threads per column
+};
+struct RowLaunch {
+  int R;        // rows per workgroup
+  int T;        // threads per row
+};
+
+ColLaunch col_launch_for(int n, int64_t ncols);
+RowLaunch row_launch_for(int m);
+
+// Column FFT along a strided axis: for every batch b and column c,
+//   line e (canvas index) = src[b*src_b + srow(e)*rstride + c], where rows
+//   [0,lo) map to themselves, rows [n-hi,n) map to lo..lo+hi-1, others are 0;
+//   stores rows [0, n_out) of the transform (times scale) to dst.
+hipError_t launch_col_fft(int dir, const AxisSched& s, const float2* tw, const float2* src,
+                          float2* dst, int64_t ncols, int nbatch, int64_t src_b, int64_t dst_b,
+                          int64_t src_rstride, int64_t dst_rstride, int lo, int hi, int n_out,
+                          float scale, hipStream_t st);
+
+// Fused Z pass: forward FFT (pruned input rows lo/hi), multiply by otf (or its
+// conjugate), inverse FFT, store rows [0, n_out).  In place on buf.
+hipError_t launch_z_conv(const AxisSched& f, const AxisSched& iv, const float2* tw, float2* buf,
+                         const float2* otf, int64_t ncols, int lo, int n_out, int conj,
+                         hipStream_t st);
+
+// Forward real-to-complex X pass on rows of a real volume (row length
+// rowlen = lo + hi source samples mapped into a canvas of 2*m).
+hipError_t launch_x_r2c(const AxisSched& s, const float2* twm, const float2* twp, const float* src,
+                        float2* dst, int64_t nrows, int rowlen, int lo, int hi, int H,
+                        const Pro& pro, hipStream_t st);
+
+// Inverse complex-to-real X pass; stores the first s2 samples with the epilogue.
+// Returns the number of workgroups (= partials written) through nblocks.
+hipError_t launch_x_c2r(const AxisSched& s, const float2* twm, const float2* twp, const float2* src,
+                        float* dst, int64_t nrows, int s2, int H, const Epi& epi, int64_t* nblocks,
+                        hipStream_t st);
+
+// Deterministic final reduction of n partials into out[0] (sum).
+hipError_t launch_sum_partials(const double* partials, int64_t n, double* out, hipStream_t st);
+
+}  // namespace dh

@@ -1,0 +1,268 @@
+// Elementwise steps and deterministic reductions for the RL / PDHG loops.
+//
+// Reductions are two-stage: a fixed grid writes float64 partials, one
+// workgroup sums them in a fixed order, so a result never depends on timing.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "dh_ops.hpp"
+
+namespace dh {
+
+constexpr int kRedBlocks = 1024;
+constexpr int kRedThreads = 256;
+
+__device__ static double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+__device__ static double wave_max(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  return v;
+}
+
+__device__ static double block_reduce(double v, int op, double* sh) {
+  v = (op == RED_MAX) ? wave_max(v) : wave_sum(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double r = (op == RED_MAX) ? -INFINITY : 0.0;
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int i = 0; i < nw; ++i) r = (op == RED_MAX) ? fmax(r, sh[i]) : r + sh[i];
+  }
+  return r;
+}
+
+__global__ void k_reduce_stage1(int op, const float* __restrict__ x, int64_t n, double* partials) {
+  __shared__ double sh[16];
+  double acc = (op == RED_MAX) ? -INFINITY : 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double v = x[i];
+    if (op == RED_MAX) acc = fmax(acc, v);
+    else if (op == RED_SUMSQ) acc += v * v;
+    else acc += v;
+  }
+  double r = block_reduce(acc, op, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+__global__ void k_reduce_stage2(int op, const double* __restrict__ partials, int64_t n, double* out) {
+  __shared__ double sh[16];
+  double acc = (op == RED_MAX) ? -INFINITY : 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
+    acc = (op == RED_MAX) ? fmax(acc, partials[i]) : acc + partials[i];
+  double r = block_reduce(acc, op == RED_SUMSQ ? RED_SUM : op, sh);
+  if (threadIdx.x == 0) out[0] = r;
+}
+
+int64_t reduce_nblocks(int64_t n) {
+  int64_t b = (n + kRedThreads - 1) / kRedThreads;
+  if (b > kRedBlocks) b = kRedBlocks;
+  if (b < 1) b = 1;
+  return b;
+}
+
+hipError_t launch_reduce(int op, const float* x, int64_t n, double* partials, double* out, hipStream_t st) {
+  const int64_t nb = reduce_nblocks(n);
+  hipLaunchKernelGGL(k_reduce_stage1, dim3((unsigned)nb), dim3(kRedThreads), 0, st, op, x, n, partials);
+  hipLaunchKernelGGL(k_reduce_stage2, dim3(1), dim3(1024), 0, st, op, (const double*)partials, nb, out);
+  return hipGetLastError();
+}
+
+// complex max |z|^2
+__global__ void k_cmax_stage1(const float2* __restrict__ z, int64_t n, double* partials) {
+  __shared__ double sh[16];
+  double acc = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const float2 v = z[i];
+    acc = fmax(acc, (double)v.x * v.x + (double)v.y * v.y);
+  }
+  double r = block_reduce(acc, RED_MAX, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
+hipError_t launch_cmax_abs2(const float2* z, int64_t n, double* partials, double* out, hipStream_t st) {
+  const int64_t nb = reduce_nblocks(n);
+  hipLaunchKernelGGL(k_cmax_stage1, dim3((unsigned)nb), dim3(kRedThreads), 0, st, z, n, partials);
+  hipLaunchKernelGGL(k_reduce_stage2, dim3(1), dim3(1024), 0, st, (int)RED_MAX, (const double*)partials, nb, out);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// RL elementwise steps
+// ---------------------------------------------------------------------------
+
+static dim3 ew_grid(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  if (b > 65535LL * 64) b = 65535LL * 64;
+  return dim3((unsigned)b);
+}
+
+#define DH_EW_LOOP(i, n) \
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
+
+__global__ void k_rl_support(float* sens, float* x, int64_t n, const double* maxval) {
+  const float mx = (float)maxval[0];
+  const float floor_ = fmaxf(mx * 1e-6f, 1e-10f);
+  DH_EW_LOOP(i, n) {
+    const float s = sens[i];
+    const bool act = s > floor_;
+    sens[i] = act ? s : INFINITY;
+    if (x) x[i] = act ? x[i] : 0.0f;
+  }
+}
+
+__global__ void k_clamp_min(const float* in, float* out, int64_t n, float lo) {
+  DH_EW_LOOP(i, n) out[i] = fmaxf(in[i], lo);
+}
+
+__global__ void k_sub_floor(const float* in, float* out, int64_t n, float a, float b) {
+  DH_EW_LOOP(i, n) out[i] = fmaxf(in[i] - a, b);
+}
+
+__device__ __forceinline__ float idiv_term_ew(float o, float mdl) {
+  const float os = fmaxf(o, 1e-6f), ms = fmaxf(mdl, 1e-6f);
+  return o * logf(os / ms) - (o - ms);
+}
+
+__global__ void k_rl_ratio(const float* model, const float* obs, float* out, int64_t n, float bg, float eps,
+                           double* partials) {
+  __shared__ double sh[16];
+  double acc = 0.0;
+  DH_EW_LOOP(i, n) {
+    const float m = model[i] + bg;
+    const float o = obs[i];
+    if (partials) acc += (double)idiv_term_ew(o, m);
+    out[i] = o / fmaxf(m, eps);
+  }
+  if (partials) {
+    double r = block_reduce(acc, RED_SUM, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+  }
+}
+
+__global__ void k_add_scalar(const float* in, const float* obs, float* out, int64_t n, float a,
+                             double* partials) {
+  __shared__ double sh[16];
+  double acc = 0.0;
+  DH_EW_LOOP(i, n) {
+    const float m = in[i] + a;
+    if (partials && obs) acc += (double)idiv_term_ew(obs[i], m);
+    out[i] = m;
+  }
+  if (partials) {
+    double r = block_reduce(acc, RED_SUM, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = r;
+  }
+}
+
+__global__ void k_rl_update(float* x, const float* corr, const float* sens, int64_t n, float eps) {
+  DH_EW_LOOP(i, n) x[i] = fmaxf(x[i] * (corr[i] / sens[i]), eps);
+}
+
+hipError_t launch_rl_support(float* sens, float* x, int64_t n, const double* maxval, hipStream_t st) {
+  hipLaunchKernelGGL(k_rl_support, ew_grid(n), dim3(256), 0, st, sens, x, n, maxval);
+  return hipGetLastError();
+}
+hipError_t launch_clamp_min(const float* in, float* out, int64_t n, float lo, hipStream_t st) {
+  hipLaunchKernelGGL(k_clamp_min, ew_grid(n), dim3(256), 0, st, in, out, n, lo);
+  return hipGetLastError();
+}
+hipError_t launch_sub_floor(const float* in, float* out, int64_t n, float a, float b, hipStream_t st) {
+  hipLaunchKernelGGL(k_sub_floor, ew_grid(n), dim3(256), 0, st, in, out, n, a, b);
+  return hipGetLastError();
+}
+hipError_t launch_rl_ratio(const float* model, const float* obs, float* out, int64_t n, float bg, float eps,
+                           double* partials, double* reduce_out, hipStream_t st) {
+  const int64_t nb = reduce_nblocks(n);
+  hipLaunchKernelGGL(k_rl_ratio, dim3((unsigned)nb), dim3(kRedThreads), 0, st, model, obs, out, n, bg, eps,
+                     reduce_out ? partials : nullptr);
+  if (reduce_out)
+    hipLaunchKernelGGL(k_reduce_stage2, dim3(1), dim3(1024), 0, st, (int)RED_SUM, (const double*)partials, nb,
+                       reduce_out);
+  return hipGetLastError();
+}
+hipError_t launch_add_scalar(const float* in, const float* obs, float* out, int64_t n, float a,
+                             double* partials, double* reduce_out, hipStream_t st) {
+  const int64_t nb = reduce_nblocks(n);
+  hipLaunchKernelGGL(k_add_scalar, dim3((unsigned)nb), dim3(kRedThreads), 0, st, in, obs, out, n, a,
+                     reduce_out ? partials : nullptr);
+  if (reduce_out)
+    hipLaunchKernelGGL(k_reduce_stage2, dim3(1), dim3(1024), 0, st, (int)RED_SUM, (const double*)partials, nb,
+                       reduce_out);
+  return hipGetLastError();
+}
+hipError_t launch_rl_update(float* x, const float* corr, const float* sens, int64_t n, float eps,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(k_rl_update, ew_grid(n), dim3(256), 0, st, x, corr, sens, n, eps);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Layout operators
+// ---------------------------------------------------------------------------
+
+struct Box {
+  int64_t ss[3], ds[3], s0[3], d0[3], ext[3];
+};
+
+__global__ void k_box_copy(const float* __restrict__ src, float* __restrict__ dst, Box b) {
+  const int64_t n = b.ds[0] * b.ds[1] * b.ds[2];
+  DH_EW_LOOP(i, n) {
+    const int64_t x = i % b.ds[2];
+    const int64_t y = (i / b.ds[2]) % b.ds[1];
+    const int64_t z = i / (b.ds[2] * b.ds[1]);
+    const int64_t rz = z - b.d0[0], ry = y - b.d0[1], rx = x - b.d0[2];
+    float v = 0.0f;
+    if (rz >= 0 && rz < b.ext[0] && ry >= 0 && ry < b.ext[1] && rx >= 0 && rx < b.ext[2])
+      v = src[((rz + b.s0[0]) * b.ss[1] + (ry + b.s0[1])) * b.ss[2] + (rx + b.s0[2])];
+    dst[i] = v;
+  }
+}
+
+hipError_t launch_box_copy(const float* src, const int64_t* ss, float* dst, const int64_t* ds,
+                           const int64_t* s0, const int64_t* d0, const int64_t* ext, hipStream_t st) {
+  Box b;
+  for (int a = 0; a < 3; ++a) {
+    b.ss[a] = ss[a];
+    b.ds[a] = ds[a];
+    b.s0[a] = s0[a];
+    b.d0[a] = d0[a];
+    b.ext[a] = ext[a];
+  }
+  const int64_t n = ds[0] * ds[1] * ds[2];
+  hipLaunchKernelGGL(k_box_copy, ew_grid(n), dim3(256), 0, st, src, dst, b);
+  return hipGetLastError();
+}
+
+__global__ void k_banded(const float* __restrict__ src, float* __restrict__ dst, int64_t outer, int64_t n_in,
+                         int64_t inner, int64_t n_out, const int32_t* __restrict__ idx,
+                         const float* __restrict__ wt, int width) {
+  const int64_t n = outer * n_out * inner;
+  DH_EW_LOOP(i, n) {
+    const int64_t q = i % inner;
+    const int64_t o = (i / inner) % n_out;
+    const int64_t b = i / (inner * n_out);
+    const float* sb = src + b * n_in * inner + q;
+    float acc = 0.0f;
+    for (int k = 0; k < width; ++k) {
+      const int32_t j = idx[o * width + k];
+      acc += sb[(int64_t)j * inner] * wt[o * width + k];
+    }
+    dst[i] = acc;
+  }
+}
+
+hipError_t launch_banded(const float* src, float* dst, int64_t outer, int64_t n_in, int64_t inner, int64_t n_out,
+                         const int32_t* idx, const float* wt, int width, hipStream_t st) {
+  const int64_t n = outer * n_out * inner;
+  hipLaunchKernelGGL(k_banded, ew_grid(n), dim3(256), 0, st, src, dst, outer, n_in, inner, n_out, idx, wt, width);
+  return hipGetLastError();
+}
+
+}  // namespace dh

@@ -1,0 +1,188 @@
+"""FFT convolution operators on MI355X (drop-in for deconlib's linops_mlx).
+
+``FFTConvolver`` (linops_mlx.py:384-432) and ``LinearFFTConvolver``
+(linops_mlx.py:435-533) keep the reference's constructor signatures,
+attributes (``signal_shape``, ``in_shape``, ``out_shape``, ``kernel_shape``,
+``fft_shape``, ``operator_norm_sq``) and ``ValueError`` behaviour.  Every
+application runs the pruned five-pass HIP pipeline of libdeconhip; inputs are
+device ``torch.Tensor`` (numpy is accepted and uploaded), outputs are new
+device tensors, matching the reference's "every operator returns a new array".
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Tuple
+
+import numpy as np
+import torch
+
+from . import _native as N
+from ._tensor import as_f32, device, empty_f32
+
+
+def _next_smooth_number(n: int) -> int:
+    """Smallest integer >= n whose prime factors are 2, 3, 5 (linops_mlx.py:956-972)."""
+    c = int(n)
+    while True:
+        m = c
+        for p in (2, 3, 5):
+            while m % p == 0:
+                m //= p
+        if m == 1:
+            return c
+        c += 1
+
+
+def fast_padded_shape(signal_shape, kernel_shape, min_pad=None) -> Tuple[int, ...]:
+    """Smallest 5-smooth wrap-free canvas (linops_mlx.py:975-1032)."""
+    nd = len(signal_shape)
+    if min_pad is None:
+        pads = (None,) * nd
+    elif isinstance(min_pad, int):
+        pads = (min_pad,) * nd
+    else:
+        pads = tuple(min_pad)
+    out = []
+    for n, m, p in zip(signal_shape, kernel_shape, pads):
+        need = (m - 1) if p is None else int(p)
+        out.append(_next_smooth_number(max(n + need, m)))
+    return tuple(out)
+
+
+class _ConvPlan:
+    """Owns one native plan and its device buffers (twiddles, OTF, workspace)."""
+
+    def __init__(self, kernel: torch.Tensor, signal_shape, fft_shape, circular: bool, normalize: bool):
+        nd = len(signal_shape)
+        sig = (C.c_int64 * nd)(*signal_shape)
+        ker = (C.c_int64 * nd)(*kernel.shape)
+        fft = (C.c_int64 * nd)(*fft_shape) if fft_shape is not None else None
+        h = C.c_void_p()
+        N.call("dh_conv_plan_create", nd, sig, ker, fft, int(circular), C.byref(h))
+        self._h = h
+        info = N.PlanInfo()
+        N.call("dh_conv_plan_info", h, C.byref(info))
+        self.info = info
+        dev = device()
+        self.twiddles = torch.empty(int(info.twiddle_bytes), dtype=torch.uint8, device=dev)
+        self.otf_buf = torch.empty(int(info.otf_bytes), dtype=torch.uint8, device=dev)
+        self.work = torch.empty(int(info.work_bytes), dtype=torch.uint8, device=dev)
+        N.call("dh_conv_plan_bind", h, self.twiddles.data_ptr(), N.stream_ptr())
+        nsq = C.c_double()
+        N.call("dh_otf_build", h, kernel.data_ptr(), int(normalize), self.otf_buf.data_ptr(),
+               self.work.data_ptr(), int(info.work_bytes), C.byref(nsq), N.stream_ptr())
+        self.norm_sq = float(nsq.value)
+        self.canvas_shape = tuple(int(v) for v in info.canvas_shape)[3 - nd:]
+
+    def apply(self, x: Optional[torch.Tensor], adjoint: bool, out: torch.Tensor,
+              fuse: Optional[N.FuseArgs] = None) -> torch.Tensor:
+        N.call("dh_conv_apply", self._h, int(adjoint), N.ptr(x), out.data_ptr(),
+               self.otf_buf.data_ptr(), self.work.data_ptr(), int(self.info.work_bytes),
+               C.byref(fuse) if fuse is not None else None, N.stream_ptr())
+        return out
+
+    def otf(self) -> torch.Tensor:
+        """Unscaled complex64 OTF (P0, P1, P2//2+1), as the reference stores it."""
+        p = self.info.canvas_shape
+        h = p[2] // 2 + 1
+        z = self.otf_buf.view(torch.complex64).reshape(p[0], p[1], h)
+        return z * float(p[0] * p[1] * p[2])
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            N.fns["dh_conv_plan_destroy"](h)
+            self._h = None
+
+
+class FFTConvolver:
+    """Circular convolution with a stored OTF (linops_mlx.py:384-432)."""
+
+    def __init__(self, kernel, normalize: bool = True):
+        k = as_f32(kernel)
+        self.shape = tuple(int(s) for s in k.shape)
+        self.in_shape = self.shape
+        self.out_shape = self.in_shape
+        self.axes = tuple(range(-len(self.shape), 0))
+        self._plan = _ConvPlan(k, self.shape, self.shape, True, normalize)
+        self.operator_norm_sq = self._plan.norm_sq
+
+    @property
+    def otf(self) -> torch.Tensor:
+        return self._plan.otf()
+
+    @property
+    def otf_conj(self) -> torch.Tensor:
+        return torch.conj(self.otf)
+
+    def forward(self, x) -> torch.Tensor:
+        x = as_f32(x)
+        return self._plan.apply(x, False, empty_f32(self.shape))
+
+    def adjoint(self, y) -> torch.Tensor:
+        y = as_f32(y)
+        return self._plan.apply(y, True, empty_f32(self.shape))
+
+    def __call__(self, x) -> torch.Tensor:
+        return self.forward(x)
+
+
+class LinearFFTConvolver:
+    """Wrap-free linear convolution through a pruned padded FFT (linops_mlx.py:435-533).
+
+    Domain and range are both ``signal_shape``; the ``N + M - 1`` padding
+    happens inside (never materialised: the HIP passes skip the zero rows and
+    crop on store).
+    """
+
+    def __init__(self, kernel, signal_shape: Tuple[int, ...], normalize: bool = True,
+                 fft_shape: Optional[Tuple[int, ...]] = None):
+        if isinstance(kernel, torch.Tensor):
+            kshape = tuple(int(s) for s in kernel.shape)
+        else:
+            kshape = tuple(int(s) for s in np.shape(kernel))
+        self.signal_shape = tuple(int(s) for s in signal_shape)
+        self.in_shape = self.signal_shape
+        self.out_shape = self.signal_shape
+        self.kernel_shape = kshape
+        if len(self.signal_shape) != len(self.kernel_shape):
+            raise ValueError("signal_shape and kernel must have the same number of dimensions")
+        minimum = tuple(n + m - 1 for n, m in zip(self.signal_shape, self.kernel_shape))
+        if fft_shape is None:
+            fft_shape = fast_padded_shape(self.signal_shape, self.kernel_shape)
+        self.fft_shape = tuple(int(s) for s in fft_shape)
+        if len(self.fft_shape) != len(self.signal_shape):
+            raise ValueError("fft_shape must match signal_shape ndim")
+        if any(p < q for p, q in zip(self.fft_shape, minimum)):
+            raise ValueError(
+                f"fft_shape {self.fft_shape} is too small for linear "
+                f"convolution; minimum is {minimum}"
+            )
+        k = as_f32(kernel)
+        self._plan = _ConvPlan(k, self.signal_shape, self.fft_shape, False, normalize)
+        self.canvas_shape = self._plan.canvas_shape
+        self.operator_norm_sq = self._plan.norm_sq
+
+    @property
+    def otf(self) -> torch.Tensor:
+        return self._plan.otf()
+
+    def forward(self, x) -> torch.Tensor:
+        """Zero-boundary linear convolution."""
+        x = as_f32(x)
+        self._check(x)
+        return self._plan.apply(x, False, empty_f32(self.signal_shape))
+
+    def adjoint(self, y) -> torch.Tensor:
+        """Adjoint (zero-boundary correlation)."""
+        y = as_f32(y)
+        self._check(y)
+        return self._plan.apply(y, True, empty_f32(self.signal_shape))
+
+    def _check(self, x: torch.Tensor) -> None:
+        if tuple(x.shape) != self.signal_shape:
+            raise ValueError(f"expected input of shape {self.signal_shape}, got {tuple(x.shape)}")
+
+    def __call__(self, x) -> torch.Tensor:
+        return self.forward(x)

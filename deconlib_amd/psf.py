@@ -1,0 +1,195 @@
+"""Widefield PSF synthesis on the host (the input producer of the solvers).
+
+Restates the reference's scalar PSF model so that workloads can be generated
+without the reference installed:
+
+* ``Optics``                      psf/optics.py:10-53
+* ``make_geometry``               psf/optics.py:96-225 (anti-aliased support weight)
+* ``make_pupil``                  psf/pupil.py:17-45 (uniform pupil, no apodization)
+* ``fft_coords``                  utils/fourier.py:11-36
+* ``pupil_to_psf``                psf/widefield.py:18-73 (DC at corner, normalised)
+* ``IndexMismatch``               psf/aberrations/geometric.py:11-61 (Gibson–Lanni OPD)
+* ``ZernikeAberration``           psf/aberrations/zernike.py:55-127, utils/zernike.py:21-120
+* ``apply_aberrations``           psf/aberrations/base.py:40-71
+
+``pupil_to_psf`` works z-plane by z-plane (the reference forms the whole
+(nz, ny, nx) complex128 stack at once), so large PSFs fit in host memory.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+from math import factorial
+from typing import Dict, Tuple, Union
+
+import numpy as np
+
+
+@dataclass(frozen=True)
+class Optics:
+    wavelength: float
+    na: float
+    ni: float
+    ns: float = None
+
+    def __post_init__(self):
+        if self.ns is None:
+            object.__setattr__(self, "ns", self.ni)
+        if self.na > self.ni:
+            raise ValueError(f"NA ({self.na}) cannot exceed immersion refractive index ({self.ni})")
+        if self.wavelength <= 0:
+            raise ValueError(f"Wavelength must be positive, got {self.wavelength}")
+
+    @property
+    def k_cutoff(self) -> float:
+        return self.na / self.wavelength
+
+    @property
+    def k_immersion(self) -> float:
+        return self.ni / self.wavelength
+
+
+@dataclass(frozen=True)
+class Geometry:
+    kx: np.ndarray
+    ky: np.ndarray
+    kz: np.ndarray
+    rho: np.ndarray
+    phi: np.ndarray
+    mask: np.ndarray
+    support_weight: np.ndarray
+    cos_theta: np.ndarray
+    sin_theta: np.ndarray
+
+    @property
+    def shape(self) -> Tuple[int, int]:
+        return self.mask.shape
+
+
+def make_geometry(shape, spacing, optics: Optics, oversample: int = 8) -> Geometry:
+    ny, nx = shape
+    if isinstance(spacing, (int, float)):
+        dy = dx = float(spacing)
+    else:
+        dy, dx = spacing
+    if dy <= 0 or dx <= 0:
+        raise ValueError(f"Spacing must be positive, got ({dy}, {dx})")
+    if oversample < 1:
+        raise ValueError(f"oversample must be >= 1, got {oversample}")
+    kx1 = np.fft.fftfreq(nx, dx)
+    ky1 = np.fft.fftfreq(ny, dy)
+    kx, ky = np.meshgrid(kx1, ky1, indexing="xy")
+    kr = np.sqrt(kx ** 2 + ky ** 2)
+    mask = kr <= optics.k_cutoff
+    dkx, dky = 1.0 / (nx * dx), 1.0 / (ny * dy)
+    kc2 = optics.k_cutoff ** 2
+    if oversample == 1:
+        w = mask.astype(np.float64)
+    else:
+        sub = (np.arange(oversample) + 0.5) / oversample - 0.5
+        w = np.zeros_like(kr)
+        for jx in range(oversample):
+            kxs2 = (kx + sub[jx] * dkx) ** 2
+            for jy in range(oversample):
+                kys = ky + sub[jy] * dky
+                w += kxs2 + kys * kys <= kc2
+        w /= oversample * oversample
+    rho = np.zeros_like(kr)
+    rho[mask] = kr[mask] / optics.k_cutoff
+    phi = np.arctan2(ky, kx)
+    sin_t = np.clip(optics.wavelength * kr / optics.ni, 0.0, 1.0)
+    cos_t = np.sqrt(1.0 - sin_t ** 2)
+    kz = np.sqrt(np.maximum(0.0, optics.k_immersion ** 2 - kr ** 2))
+    return Geometry(kx=kx, ky=ky, kz=kz, rho=rho, phi=phi, mask=mask, support_weight=w,
+                    cos_theta=cos_t, sin_theta=sin_t)
+
+
+def make_pupil(geom: Geometry) -> np.ndarray:
+    return geom.support_weight.astype(np.complex128)
+
+
+def fft_coords(n: int, spacing: float = 1.0) -> np.ndarray:
+    return np.fft.fftfreq(n) * n * spacing
+
+
+def pupil_to_psf(pupil: np.ndarray, geom: Geometry, z: np.ndarray, normalize: bool = True,
+                 dtype=np.float64) -> np.ndarray:
+    """|IFFT2(P exp(2 pi i kz z))|^2 per z-plane, DC at the corner."""
+    z = np.atleast_1d(z)
+    out = np.empty((len(z),) + pupil.shape, dtype=dtype)
+    total = 0.0
+    for i, zi in enumerate(z):
+        a = np.fft.ifft2(pupil * np.exp(2j * np.pi * geom.kz * zi))
+        plane = a.real ** 2 + a.imag ** 2
+        total += float(plane.sum())
+        out[i] = plane
+    if normalize and total > 0:
+        out /= total
+    return out
+
+
+class IndexMismatch:
+    """Gibson–Lanni depth aberration: exp(2 pi i d (n2 cos t2 - n1 cos t1) / lambda)."""
+
+    def __init__(self, depth: float):
+        self.depth = depth
+
+    def __call__(self, geom: Geometry, optics: Optics) -> np.ndarray:
+        n1, n2 = optics.ni, optics.ns
+        s2 = np.clip((n1 / n2) * geom.sin_theta, 0.0, 1.0)
+        c2 = np.sqrt(1.0 - s2 ** 2)
+        opd = self.depth * (n2 * c2 - n1 * geom.cos_theta)
+        return np.exp(1j * 2.0 * np.pi * opd / optics.wavelength)
+
+
+def _ansi_to_nm(j: int):
+    n = int(np.ceil((-3 + np.sqrt(9 + 8 * j)) / 2))
+    return n, 2 * j - n * (n + 2)
+
+
+def zernike_polynomial(j: int, rho: np.ndarray, phi: np.ndarray) -> np.ndarray:
+    """Orthonormal OSA/ANSI Zernike Z_j."""
+    n, m = _ansi_to_nm(j)
+    am = abs(m)
+    R = np.zeros_like(rho)
+    for k in range((n - am) // 2 + 1):
+        c = (-1) ** k * factorial(n - k) / (
+            factorial(k) * factorial((n + am) // 2 - k) * factorial((n - am) // 2 - k))
+        R += c * rho ** (n - 2 * k)
+    norm = np.sqrt(2.0 * (n + 1) / (1.0 + float(m == 0)))
+    return norm * R * np.cos(m * phi) if m >= 0 else -norm * R * np.sin(am * phi)
+
+
+class ZernikeAberration:
+    SPHERICAL = 12
+
+    def __init__(self, coefficients: Union[Dict[int, float], np.ndarray]):
+        self.coefficients = dict(coefficients) if isinstance(coefficients, dict) else {
+            j: c for j, c in enumerate(np.asarray(coefficients))}
+
+    def __call__(self, geom: Geometry, optics: Optics) -> np.ndarray:
+        phase = np.zeros_like(geom.rho)
+        for j, c in self.coefficients.items():
+            if c != 0:
+                phase += c * zernike_polynomial(int(j), geom.rho, geom.phi)
+        return np.exp(1j * phase * geom.mask)
+
+
+def apply_aberrations(pupil, geom, optics, aberrations) -> np.ndarray:
+    out = pupil.copy()
+    for a in aberrations:
+        out = out * a(geom, optics)
+    return out
+
+
+def widefield_psf(shape, lateral: float, axial: float, optics: Optics = None, aberrations=(),
+                  dtype=np.float32) -> np.ndarray:
+    """(nz, ny, nx) corner-origin PSF normalised to sum 1 (the configs' recipe)."""
+    optics = optics or Optics(wavelength=0.525, na=1.4, ni=1.515, ns=1.334)
+    nz, ny, nx = shape
+    geom = make_geometry((ny, nx), lateral, optics)
+    pupil = make_pupil(geom)
+    if aberrations:
+        pupil = apply_aberrations(pupil, geom, optics, list(aberrations))
+    psf = pupil_to_psf(pupil, geom, fft_coords(nz, axial), normalize=True)
+    return psf.astype(dtype)

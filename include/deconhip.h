@@ -1,0 +1,164 @@
+/*
+ * deconhip.h -- C ABI of libdeconhip.so, the MI355X (gfx950) implementation of
+ * deconlib's FFT-operator deconvolution hot path.
+ *
+ * Conventions
+ *   - Every device buffer is allocated by the caller (PyTorch in the Python
+ *     package) and passed in as a plain pointer.  The library never allocates
+ *     or frees device memory; plans are host-side structs.
+ *   - Every call enqueues work on the caller's HIP stream (passed as void*;
+ *     NULL = the null stream) and never synchronises, except the calls that
+ *     return a host value (documented below).
+ *   - Status: 0 = OK, -1 = invalid argument (the Python layer raises
+ *     ValueError, as the reference does for shape errors), -2 = HIP error,
+ *     -3 = workspace too small, -4 = unsupported.  dh_last_error() returns a
+ *     thread-local message for the last failure.
+ *   - Volumes are C-contiguous float32 (z, y, x); 1-D and 2-D inputs are
+ *     promoted to 3-D with leading unit axes.  Spectra are interleaved
+ *     complex64 laid out (P0, P1, P2/2+1).
+ */
+#ifndef DECONHIP_H
+#define DECONHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DH_ABI_VERSION 1
+
+#define DH_OK 0
+#define DH_EINVAL (-1)
+#define DH_EHIP (-2)
+#define DH_EWORKSPACE (-3)
+#define DH_EUNSUPPORTED (-4)
+
+typedef struct dh_conv_plan dh_conv_plan;
+
+typedef struct {
+  int32_t ndim;
+  int32_t signal_shape[3];     /* promoted to 3-D                          */
+  int32_t kernel_shape[3];
+  int32_t fft_shape[3];        /* canvas the reference would use            */
+  int32_t canvas_shape[3];     /* canvas this plan computes on (x made even) */
+  int64_t otf_bytes;           /* complex64 (P0, P1, P2/2+1)                */
+  int64_t work_bytes;          /* scratch for the pruned passes + partials  */
+  int64_t twiddle_bytes;       /* device twiddle tables                     */
+} dh_plan_info;
+
+/* Fused prologue applied to each input sample of the forward FFT pass
+ * (the padded canvas never exists in memory). */
+#define DH_PRO_NONE 0        /* v = in[i]                                 */
+#define DH_PRO_ONES 1        /* v = 1            (RL sensitivity A^T 1)   */
+#define DH_PRO_SUB_FLOOR 2   /* v = max(in[i] - a, b)  (RL max(obs-bg,1e-6)) */
+
+/* Fused epilogue applied to each cropped output sample v of the inverse pass. */
+#define DH_EPI_NONE 0        /* out = v                                   */
+#define DH_EPI_FLOOR 1       /* out = max(v, a)                           */
+#define DH_EPI_RL_RATIO 2    /* m = v + a; out = aux0 / max(m, b)         */
+                             /*   reduce: sum aux0*log(max(aux0,1e-6)/max(m,1e-6)) - (aux0 - max(m,1e-6)) */
+#define DH_EPI_RL_UPDATE 3   /* out = max(aux0 * (v / aux1), b)  (aux1 = +inf off-support) */
+#define DH_EPI_ADD 4         /* out = v + a      reduce: same I-divergence as RL_RATIO when aux0 != NULL */
+
+typedef struct {
+  int32_t prologue;
+  float pro_a, pro_b;
+  int32_t epilogue;
+  float epi_a, epi_b;
+  const float* aux0;         /* epilogue operand (observed / x), S-shaped */
+  const float* aux1;         /* epilogue operand (sensitivity), S-shaped  */
+  double* reduce_out;        /* device; if non-NULL the epilogue's sum is written here */
+} dh_fuse_args;
+
+int dh_abi_version(void);
+const char* dh_last_error(void);
+
+/* Plan for LinearFFTConvolver (linops_mlx.py:435-533) when circular == 0,
+ * FFTConvolver (linops_mlx.py:384-432) when circular == 1 (then signal,
+ * kernel and fft shapes must all be equal).  fft_shape may be NULL: the
+ * reference's fast_padded_shape (linops_mlx.py:975-1032) is used. */
+int dh_conv_plan_create(int ndim, const int64_t* signal_shape, const int64_t* kernel_shape,
+                        const int64_t* fft_shape, int circular, dh_conv_plan** out);
+int dh_conv_plan_info(const dh_conv_plan* plan, dh_plan_info* info);
+/* Upload the twiddle tables into twiddles_dev (twiddle_bytes). */
+int dh_conv_plan_bind(dh_conv_plan* plan, void* twiddles_dev, void* stream);
+int dh_conv_plan_destroy(dh_conv_plan* plan);
+
+/* OTF = rfftn(pad_corner_origin_kernel(kernel / sum(kernel)))   (linops_mlx.py:408-411,
+ * utils/padding.py:142-190), stored pre-scaled by 1/prod(canvas).  Writes
+ * max|OTF|^2 (operator_norm_sq, linops_mlx.py:415) to *norm_sq_host:
+ * this call synchronises the stream. */
+int dh_otf_build(const dh_conv_plan* plan, const float* kernel_dev, int normalize,
+                 void* otf_dev, void* work_dev, int64_t work_bytes,
+                 double* norm_sq_host, void* stream);
+
+/* One forward (adjoint == 0: convolution, linops_mlx.py:520-524) or adjoint
+ * (adjoint == 1: correlation, linops_mlx.py:526-530) application, S -> S.
+ * fuse may be NULL.  in and out may alias only for elementwise-safe epilogues. */
+int dh_conv_apply(const dh_conv_plan* plan, int adjoint, const float* in, float* out,
+                  const void* otf_dev, void* work_dev, int64_t work_bytes,
+                  const dh_fuse_args* fuse, void* stream);
+
+/* ---- reductions and elementwise steps (rl_mlx.py:123-150) ---- */
+#define DH_RED_SUM 0
+#define DH_RED_MAX 1
+#define DH_RED_SUMSQ 2
+/* result_dev[0] = reduction of x[0:n] in float64, fixed order (deterministic).
+ * partials_dev must hold dh_reduce_partials(n) doubles. */
+int64_t dh_reduce_partials(int64_t n);
+int dh_reduce(int op, const float* x, int64_t n, double* partials_dev, double* result_dev,
+              void* stream);
+
+/* RL support set-up (rl_mlx.py:129-133): floor = max(float(maxval)*1e-6f, 1e-10f);
+ * active = sens > floor; sens = active ? sens : +inf; x = active ? x : 0. */
+int dh_rl_support(float* sens, float* x, int64_t n, const double* maxval_dev, void* stream);
+/* out = max(in, lo) */
+int dh_clamp_min(const float* in, float* out, int64_t n, float lo, void* stream);
+/* out = max(in - a, b) */
+int dh_sub_floor(const float* in, float* out, int64_t n, float a, float b, void* stream);
+/* RL ratio on an arbitrary operator's output (rl_mlx.py:147-148):
+ * out = obs / max(model + bg, eps); reduce_out (may be NULL) gets the
+ * I-divergence sum of obs against model + bg. */
+int dh_rl_ratio(const float* model, const float* obs, float* out, int64_t n, float bg,
+                float eps, double* partials_dev, double* reduce_out, void* stream);
+/* RL update (rl_mlx.py:149-150): x = max(x * (corr / sens), eps). */
+int dh_rl_update(float* x, const float* corr, const float* sens, int64_t n, float eps,
+                 void* stream);
+/* out = in + a (forward model plus background); reduce as dh_rl_ratio. */
+int dh_add_scalar(const float* in, const float* obs, float* out, int64_t n, float a,
+                  double* partials_dev, double* reduce_out, void* stream);
+
+/* ---- measurement ---- */
+/* Pass ids for dh_timing_read. */
+#define DH_PASS_X_FWD 0
+#define DH_PASS_Y_FWD 1
+#define DH_PASS_Z 2
+#define DH_PASS_Y_INV 3
+#define DH_PASS_X_INV 4
+#define DH_PASS_COUNT 5
+/* When enabled, dh_conv_apply records a HIP event pair around each pass on
+ * the caller's stream.  dh_timing_read synchronises on those events and
+ * returns per-pass accumulated milliseconds and launch counts
+ * (DH_PASS_COUNT entries each); reset != 0 clears the accumulators. */
+int dh_timing_enable(int on);
+int dh_timing_read(double* ms, int64_t* counts, int reset);
+
+/* ---- layout operators (core_operators.py:29-352) ---- */
+/* Box copy between C-contiguous 3-D volumes: dst[d] = src[d - dst_start + src_start]
+ * inside the box of the given extent, 0 elsewhere in dst.  Pad.forward / Crop.adjoint
+ * (zero-pad) and Pad.adjoint / Crop.forward (slice) are both this call. */
+int dh_box_copy(const float* src, const int64_t* src_shape, float* dst, const int64_t* dst_shape,
+                const int64_t* src_start, const int64_t* dst_start, const int64_t* extent, void* stream);
+/* Banded resampling along one axis (core_operators.py:205-219): the volume is
+ * viewed as (outer, n_in, inner) -> (outer, n_out, inner) with
+ * out[o, i, q] = sum_k wt[i, k] * in[o, idx[i, k], q], k < width.  idx (int32)
+ * and wt (float32) are device arrays of n_out * width entries. */
+int dh_banded_apply(const float* src, float* dst, int64_t outer, int64_t n_in, int64_t inner, int64_t n_out,
+                    const int32_t* idx, const float* wt, int width, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DECONHIP_H */

@@ -1,0 +1,69 @@
+"""The C-ABI library loads, exports every declared symbol, and its host-side
+plan logic (no GPU call) matches the reference's sizing and error behaviour."""
+
+import ctypes as C
+import re
+from pathlib import Path
+
+import pytest
+
+from deconlib_amd import _native as N
+from oracle import deconv as O
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def _declared_symbols():
+    txt = (ROOT / "include" / "deconhip.h").read_text()
+    return sorted(set(re.findall(r"\b(dh_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_every_declared_symbol_is_exported():
+    names = _declared_symbols()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(N.lib, n), n
+        assert n in N.SYMBOLS, f"{n} has no ctypes signature"
+
+
+def test_abi_version():
+    assert N.fns["dh_abi_version"]() == N.ABI_VERSION
+
+
+def _plan(sig, ker, fft=None, circular=0):
+    nd = len(sig)
+    h = C.c_void_p()
+    rc = N.fns["dh_conv_plan_create"](nd, (C.c_int64 * nd)(*sig), (C.c_int64 * nd)(*ker),
+                                      (C.c_int64 * nd)(*fft) if fft else None, circular, C.byref(h))
+    return rc, h
+
+
+@pytest.mark.parametrize("sig,ker", [((64, 256, 256), (64, 256, 256)), ((128, 512, 512), (128, 512, 512)),
+                                     ((101, 589, 549), (101, 256, 256)), ((17, 33), (5, 9)), ((100,), (7,))])
+def test_plan_canvas_matches_fast_padded_shape(sig, ker):
+    rc, h = _plan(sig, ker)
+    assert rc == 0
+    info = N.PlanInfo()
+    assert N.fns["dh_conv_plan_info"](h, C.byref(info)) == 0
+    nd = len(sig)
+    ref = O.fast_padded_shape(sig, ker)
+    assert tuple(info.fft_shape)[3 - nd:] == ref
+    canvas = tuple(info.canvas_shape)[3 - nd:]
+    assert all(c >= r for c, r in zip(canvas, ref))
+    assert canvas[-1] % 2 == 0
+    P = tuple(info.canvas_shape)
+    assert info.otf_bytes == P[0] * P[1] * (P[2] // 2 + 1) * 8
+    N.fns["dh_conv_plan_destroy"](h)
+
+
+def test_plan_errors_map_to_reference_exceptions():
+    rc, _ = _plan((8, 8), (3, 3), fft=(9, 9))
+    assert rc == N.DH_EINVAL
+    with pytest.raises(ValueError):
+        N.check(rc)
+    rc, _ = _plan((8, 8), (4, 4), circular=1)
+    assert rc == N.DH_EINVAL
+    rc, _ = _plan((9, 9), (9, 9), circular=1)       # odd last axis: not supported
+    assert rc == N.DH_EUNSUPPORTED
+    with pytest.raises(NotImplementedError):
+        N.check(rc)
