@@ -11,10 +11,15 @@ PyTorch only owns device buffers and streams.
 from .base import DeconvolutionResult, MLXDeconvolutionResult
 from .composition import Compose, LinearOperator, compose
 from .core_operators import Crop, FractionalAreaDownsample, Pad
-from .linops import FFTConvolver, LinearFFTConvolver, fast_padded_shape
+from .linops import FFTConvolver, Hessian3D, LinearFFTConvolver, fast_padded_shape
+from .pdhg import HessianRegularizer, solve_pdhg_mlx, solve_pdhg_with_operator
 from .rl import RLResult, poisson_i_divergence, richardson_lucy_with_operator
 
 __all__ = [
+    "solve_pdhg_mlx",
+    "solve_pdhg_with_operator",
+    "HessianRegularizer",
+    "Hessian3D",
     "MLXDeconvolutionResult",
     "DeconvolutionResult",
     "RLResult",

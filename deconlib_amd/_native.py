@@ -57,6 +57,18 @@ class FuseArgs(C.Structure):
     ]
 
 
+class PdParams(C.Structure):
+    _fields_ = [
+        ("delta", C.c_double),
+        ("eta", C.c_double),
+        ("tol", C.c_double),
+        ("min_iter", C.c_int32),
+        ("patience", C.c_int32),
+    ]
+
+
+PD_NSLOTS = 16
+
 _vp, _i64, _int, _f, _d = C.c_void_p, C.c_int64, C.c_int, C.c_float, C.c_double
 
 
@@ -88,6 +100,15 @@ SYMBOLS = {
     "dh_box_copy": (_int, _vp, C.POINTER(_i64), _vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64),
                     C.POINTER(_i64), _vp),
     "dh_banded_apply": (_int, _vp, _vp, _i64, _i64, _i64, _i64, _vp, _vp, _int, _vp),
+    "dh_pd_partials": (_i64, _i64),
+    "dh_pd_y1": (_int, _vp, _vp, _vp, _vp, _i64, _vp, _f, _int, _vp),
+    "dh_pd_dual": (_int, _vp, _vp, _vp, _int, _int, _int, _vp, C.POINTER(_f), _f, _int, _vp),
+    "dh_pd_primal": (_int, _vp, _vp, _vp, _vp, _int, _int, _int, _vp, C.POINTER(_f), _vp, _vp),
+    "dh_pd_hess_sums": (_int, _vp, _vp, _int, _int, _int, C.POINTER(_f), _vp, _vp, _int, _vp, _vp),
+    "dh_pd_data_sums": (_int, _vp, _vp, _vp, _i64, _f, _int, _vp, _vp, _vp, _vp),
+    "dh_pd_scalars": (_int, _vp, _vp, _int, C.POINTER(PdParams), _vp),
+    "dh_hessian_forward": (_int, _vp, _vp, _int, _int, _int, C.POINTER(_f), _vp),
+    "dh_hessian_adjoint": (_int, _vp, _vp, _int, _int, _int, C.POINTER(_f), _vp),
     "dh_timing_enable": (_int, _int),
     "dh_timing_read": (_int, C.POINTER(_d), C.POINTER(_i64), _int),
 }
@@ -121,7 +142,7 @@ def check(rc: int, what: str = "") -> None:
 
 def call(name: str, *args) -> int:
     rc = fns[name](*args)
-    if isinstance(rc, int) and name not in ("dh_reduce_partials", "dh_abi_version"):
+    if isinstance(rc, int) and name not in ("dh_reduce_partials", "dh_abi_version", "dh_pd_partials"):
         check(rc, name)
     return rc
 

@@ -10,6 +10,7 @@
 #include "../../include/deconhip.h"
 #include "dh_fft.hpp"
 #include "dh_ops.hpp"
+#include "dh_pdhg.hpp"
 
 namespace {
 
@@ -389,6 +390,65 @@ int dh_conv_apply(const dh_conv_plan* p, int adjoint, const float* in, float* ou
     DH_TRY(hip_check(dh::launch_sum_partials(epi.partials, nblocks, reduce_out, st), "epilogue reduction"));
   }
   return DH_OK;
+}
+
+int64_t dh_pd_partials(int64_t n) { return dh::pd_partials_needed(n); }
+
+int dh_pd_y1(float* y1, const float* ax, const float* axp, const float* data, int64_t n, const double* scalars,
+             float background, int loss_type, void* stream) {
+  if (!y1 || !ax || !axp || !data || !scalars || n < 1) return fail(DH_EINVAL, "bad pd_y1 arguments");
+  if (loss_type != 0 && loss_type != 1) return fail(DH_EINVAL, "loss_type must be 0 (poisson) or 1 (gaussian)");
+  return hip_check(dh::pd_y1(y1, ax, axp, data, n, scalars, background, loss_type, (hipStream_t)stream), "pd_y1");
+}
+
+static bool dims_ok(int nz, int ny, int nx) { return nz >= 1 && ny >= 1 && nx >= 1; }
+
+int dh_pd_dual(float* y2, const float* x, const float* xp, int nz, int ny, int nx, const double* scalars,
+               const float* w, float alpha, int l12, void* stream) {
+  if (!y2 || !x || !xp || !scalars || !w || !dims_ok(nz, ny, nx)) return fail(DH_EINVAL, "bad pd_dual arguments");
+  return hip_check(dh::pd_dual(y2, x, xp, nz, ny, nx, scalars, w, alpha, l12, (hipStream_t)stream), "pd_dual");
+}
+
+int dh_pd_primal(float* x_new, const float* x, const float* g, const float* y2, int nz, int ny, int nx,
+                 double* scalars, const float* w, double* partials, void* stream) {
+  if (!x_new || !x || !g || !y2 || !scalars || !w || !partials || !dims_ok(nz, ny, nx))
+    return fail(DH_EINVAL, "bad pd_primal arguments");
+  return hip_check(dh::pd_primal(x_new, x, g, y2, nz, ny, nx, scalars, w, partials, (hipStream_t)stream),
+                   "pd_primal");
+}
+
+int dh_pd_hess_sums(const float* a, const float* b, int nz, int ny, int nx, const float* w, double* scalars,
+                    double* reg_loss_out, int reg_norm, double* partials, void* stream) {
+  if (!a || !b || !w || !scalars || !partials || !dims_ok(nz, ny, nx))
+    return fail(DH_EINVAL, "bad pd_hess_sums arguments");
+  if (reg_loss_out && reg_norm != 1 && reg_norm != 2) return fail(DH_EINVAL, "reg_norm must be 1 or 2");
+  return hip_check(dh::pd_hess_sums(a, b, nz, ny, nx, w, reg_loss_out ? reg_norm : 0, scalars, reg_loss_out, partials,
+                                    (hipStream_t)stream),
+                   "pd_hess_sums");
+}
+
+int dh_pd_data_sums(const float* a, const float* b, const float* data, int64_t n, float background, int loss_type,
+                    double* scalars, double* data_loss_out, double* partials, void* stream) {
+  if (!a || !b || !scalars || !partials || n < 1 || (data_loss_out && !data))
+    return fail(DH_EINVAL, "bad pd_data_sums arguments");
+  return hip_check(dh::pd_data_sums(a, b, data, n, background, data_loss_out ? 1 : 0, loss_type, scalars,
+                                    data_loss_out, partials, (hipStream_t)stream),
+                   "pd_data_sums");
+}
+
+int dh_pd_scalars(double* scalars, double* hist, int k, const dh_pd_params* params, void* stream) {
+  if (!scalars || !hist || !params || k < 0) return fail(DH_EINVAL, "bad pd_scalars arguments");
+  return hip_check(dh::pd_scalars(scalars, hist, k, *params, (hipStream_t)stream), "pd_scalars");
+}
+
+int dh_hessian_forward(const float* f, float* out, int nz, int ny, int nx, const float* w, void* stream) {
+  if (!f || !out || !w || !dims_ok(nz, ny, nx)) return fail(DH_EINVAL, "bad hessian_forward arguments");
+  return hip_check(dh::hess_forward(f, out, nz, ny, nx, w, (hipStream_t)stream), "hessian_forward");
+}
+
+int dh_hessian_adjoint(const float* y, float* out, int nz, int ny, int nx, const float* w, void* stream) {
+  if (!y || !out || !w || !dims_ok(nz, ny, nx)) return fail(DH_EINVAL, "bad hessian_adjoint arguments");
+  return hip_check(dh::hess_adjoint(y, out, nz, ny, nx, w, (hipStream_t)stream), "hessian_adjoint");
 }
 
 int dh_timing_enable(int on) {

@@ -1,0 +1,465 @@
+// PDHG (Malitsky–Pock) with Hessian regularization: stencil, prox and FP64
+// step-size kernels (pdhg_mlx.py:507-920, linops_mlx.py:182-262,
+// linops_core_mlx.py:42-80).
+//
+// Scalars (tau, sigma, theta, the convergence counter and the histories) live
+// in a float64 block on the device, so an iteration enqueues without any host
+// round trip.  Reductions are two-stage and deterministic.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "dh_pdhg.hpp"
+
+namespace dh {
+
+#define PD_LOOP(i, n) \
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
+
+constexpr int kPdThreads = 256;
+constexpr int kPdBlocks = 1024;
+
+__device__ static double pd_block_sum(double v, double* sh) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int i = 0; i < nw; ++i) r += sh[i];
+  }
+  __syncthreads();
+  return r;
+}
+
+// Sums NV partial vectors laid out [block][NV] into out[0..NV).
+__global__ void k_pd_finish(const double* __restrict__ part, int nblocks, int nv, double* out) {
+  __shared__ double sh[16];
+  for (int j = 0; j < nv; ++j) {
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) acc += part[(int64_t)b * nv + j];
+    double r = pd_block_sum(acc, sh);
+    if (threadIdx.x == 0) out[j] = r;
+  }
+}
+
+static hipError_t finish(const double* part, int nblocks, int nv, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_pd_finish, dim3(1), dim3(1024), 0, st, part, nblocks, nv, out);
+  return hipGetLastError();
+}
+
+hipError_t pd_move(double* sc, int from0, int to0, int from1, double* out1, hipStream_t st);
+
+static int pd_blocks(int64_t n) {
+  int64_t b = (n + kPdThreads - 1) / kPdThreads;
+  if (b > kPdBlocks) b = kPdBlocks;
+  return (int)(b < 1 ? 1 : b);
+}
+
+// ---------------------------------------------------------------------------
+// y1: data-term dual prox (pdhg_mlx.py:115-189), fp32 as the reference
+// ---------------------------------------------------------------------------
+
+__global__ void k_pd_y1(float* __restrict__ y1, const float* __restrict__ ax, const float* __restrict__ axp,
+                        const float* __restrict__ data, int64_t n, const double* __restrict__ sc, float bg,
+                        int loss_type) {
+  const double sigma = sc[PD_SIGMA], theta = sc[PD_THETA];
+  const float sf = (float)sigma, th = (float)theta;
+  const float one_pth = (float)(1.0 + theta);
+  const float sb = (float)(sigma * (double)bg), s4 = (float)(4.0 * sigma), s2 = (float)(2.0 * sigma);
+  const float inv1ps = (float)(1.0 + sigma);
+  PD_LOOP(i, n) {
+    // A xbar by linearity: (1 + theta) A x_k - theta A x_{k-1}
+    const float axb = one_pth * ax[i] - th * axp[i];
+    const float arg = y1[i] + sf * axb;
+    const float d = data[i];
+    float out;
+    if (loss_type == 0) {
+      const float c = 1.0f - arg - sb;
+      const float ds = fmaxf(d, 0.0f);
+      const float disc = c * c + s4 * ds;
+      const float den = sqrtf(disc) - c + 1e-12f;
+      float z = (s2 * ds) / den;
+      if (ds < 1e-6f) z = fmaxf(c, 1e-6f);
+      z = fmaxf(z, 1e-6f);
+      out = 1.0f - z;
+    } else {
+      out = (arg - sf * (d - bg)) / inv1ps;
+    }
+    y1[i] = out;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Hessian stencils with Neumann (replicate) boundaries
+// ---------------------------------------------------------------------------
+
+struct Dims {
+  int nz, ny, nx;
+};
+
+__device__ __forceinline__ int clampi(int v, int n) { return v < 0 ? 0 : (v >= n ? n - 1 : v); }
+
+// Six weighted components [zz, yy, xx, yz, xz, xy] of Hessian3D.forward at
+// (z, y, x) for f given by an accessor taking clamped coordinates.
+template <class F>
+__device__ __forceinline__ void hess6(const F& f, const Dims& d, int z, int y, int x, const float* w, float* h) {
+  const int zm = clampi(z - 1, d.nz), zp = clampi(z + 1, d.nz);
+  const int ym = clampi(y - 1, d.ny), yp = clampi(y + 1, d.ny);
+  const int xm = clampi(x - 1, d.nx), xp = clampi(x + 1, d.nx);
+  const float c = f(z, y, x);
+  // d2: fpad[:-2] - 2 * fpad[1:-1] + fpad[2:]
+  h[0] = (f(zm, y, x) - 2.0f * c) + f(zp, y, x);
+  h[1] = (f(z, ym, x) - 2.0f * c) + f(z, yp, x);
+  h[2] = (f(z, y, xm) - 2.0f * c) + f(z, y, xp);
+  // d1_cen: (fpad[2:] - fpad[:-2]) / 2, applied twice
+  const float dz_yp = (f(zp, yp, x) - f(zm, yp, x)) / 2.0f;
+  const float dz_ym = (f(zp, ym, x) - f(zm, ym, x)) / 2.0f;
+  h[3] = (dz_yp - dz_ym) / 2.0f;  // H_yz = d1_cen(Dz, y)
+  const float dz_xp = (f(zp, y, xp) - f(zm, y, xp)) / 2.0f;
+  const float dz_xm = (f(zp, y, xm) - f(zm, y, xm)) / 2.0f;
+  h[4] = (dz_xp - dz_xm) / 2.0f;  // H_xz = d1_cen(Dz, x)
+  const float dy_xp = (f(z, yp, xp) - f(z, ym, xp)) / 2.0f;
+  const float dy_xm = (f(z, yp, xm) - f(z, ym, xm)) / 2.0f;
+  h[5] = (dy_xp - dy_xm) / 2.0f;  // H_xy = d1_cen(Dy, x)
+#pragma unroll
+  for (int q = 0; q < 6; ++q) h[q] *= w[q];
+}
+
+// d1_cen_adj of g along one axis at index i (n samples):
+//   fpad = [-g0, g..., -g_{n-1}];  out[i] = (fpad[i] - fpad[i + 2]) / 2
+template <class G>
+__device__ __forceinline__ float d1adj(const G& g, int i, int n) {
+  const float l = (i == 0) ? -g(0) : g(i - 1);
+  const float r = (i == n - 1) ? -g(n - 1) : g(i + 1);
+  return (l - r) / 2.0f;
+}
+
+// d2 (self-adjoint) along one axis at index i with replicate padding.
+template <class G>
+__device__ __forceinline__ float d2at(const G& g, int i, int n) {
+  return (g(clampi(i - 1, n)) - 2.0f * g(i)) + g(clampi(i + 1, n));
+}
+
+// Hessian3D.adjoint at (z, y, x): hw(q, z, y, x) returns H[q] * w[q].
+template <class HW>
+__device__ __forceinline__ float hess_adj(const HW& hw, const Dims& d, int z, int y, int x) {
+  const float a0 = d2at([&](int i) { return hw(0, i, y, x); }, z, d.nz);
+  const float a1 = d2at([&](int i) { return hw(1, z, i, x); }, y, d.ny);
+  const float a2 = d2at([&](int i) { return hw(2, z, y, i); }, x, d.nx);
+  // d1_cen_adj(d1_cen_adj(Hw3, y), z)
+  const float a3 = d1adj([&](int iz) { return d1adj([&](int iy) { return hw(3, iz, iy, x); }, y, d.ny); }, z, d.nz);
+  // d1_cen_adj(d1_cen_adj(Hw4, x), z)
+  const float a4 = d1adj([&](int iz) { return d1adj([&](int ix) { return hw(4, iz, y, ix); }, x, d.nx); }, z, d.nz);
+  // d1_cen_adj(d1_cen_adj(Hw5, x), y)
+  const float a5 = d1adj([&](int iy) { return d1adj([&](int ix) { return hw(5, z, iy, ix); }, x, d.nx); }, y, d.ny);
+  return ((((a0 + a1) + a2) + a3) + a4) + a5;
+}
+
+struct W6 {
+  float w[6];
+};
+
+// y2 <- prox(y2 + sigma * H(xbar)), xbar = x + theta (x - xp)  (pdhg_mlx.py:729-734)
+__global__ void k_pd_dual(float* __restrict__ y2, const float* __restrict__ x, const float* __restrict__ xp,
+                          Dims d, const double* __restrict__ sc, W6 w, float alpha, int l12) {
+  const int64_t n = (int64_t)d.nz * d.ny * d.nx;
+  const float sf = (float)sc[PD_SIGMA], th = (float)sc[PD_THETA];
+  auto xbar = [&](int z, int y, int xx) {
+    const int64_t i = ((int64_t)z * d.ny + y) * d.nx + xx;
+    const float v = x[i];
+    return v + th * (v - xp[i]);
+  };
+  PD_LOOP(i, n) {
+    const int xx = (int)(i % d.nx);
+    const int y = (int)((i / d.nx) % d.ny);
+    const int z = (int)(i / ((int64_t)d.nx * d.ny));
+    float h[6];
+    hess6(xbar, d, z, y, xx, w.w, h);
+    float a[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) a[q] = y2[q * n + i] + sf * h[q];
+    if (!l12) {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) y2[q * n + i] = fminf(fmaxf(a[q], -alpha), alpha);
+    } else {
+      float ss = 0.0f;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) ss += a[q] * a[q];
+      const float nrm = sqrtf(ss + 1e-12f);
+      const float s = fmaxf(nrm / alpha, 1.0f);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) y2[q * n + i] = a[q] / s;
+    }
+  }
+}
+
+// xn = max(0, x - tau (g + H^T y2)); partial sums of (xn - x)^2, x^2, xn^2
+// (pdhg_mlx.py:741-742, 749-750, 805-806)
+__global__ void k_pd_primal(float* __restrict__ xn, const float* __restrict__ x, const float* __restrict__ g,
+                            const float* __restrict__ y2, Dims d, const double* __restrict__ sc, W6 w,
+                            double* __restrict__ part) {
+  __shared__ double sh[16];
+  const int64_t n = (int64_t)d.nz * d.ny * d.nx;
+  const float tf = (float)sc[PD_TAU];
+  double s_dx = 0.0, s_x = 0.0, s_xn = 0.0;
+  auto hw = [&](int q, int z, int y, int xx) {
+    return y2[q * n + ((int64_t)z * d.ny + y) * d.nx + xx] * w.w[q];
+  };
+  PD_LOOP(i, n) {
+    const int xx = (int)(i % d.nx);
+    const int y = (int)((i / d.nx) % d.ny);
+    const int z = (int)(i / ((int64_t)d.nx * d.ny));
+    const float lt = hess_adj(hw, d, z, y, xx);
+    const float grad = g[i] + lt;
+    const float xv = x[i];
+    const float v = fmaxf(xv - tf * grad, 0.0f);
+    xn[i] = v;
+    const float dv = v - xv;
+    s_dx += (double)(dv * dv);
+    s_x += (double)(xv * xv);
+    s_xn += (double)(v * v);
+  }
+  double r0 = pd_block_sum(s_dx, sh);
+  double r1 = pd_block_sum(s_x, sh);
+  double r2 = pd_block_sum(s_xn, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x * 3 + 0] = r0;
+    part[blockIdx.x * 3 + 1] = r1;
+    part[blockIdx.x * 3 + 2] = r2;
+  }
+}
+
+// sum |H a - H b|^2 (the reference's dL = L(x_new) - L(x)) and, if want_abs,
+// sum |H a| (the regularizer loss of a)
+__global__ void k_pd_hess_sums(const float* __restrict__ a, const float* __restrict__ b, Dims d, W6 w,
+                               int want_abs, double* __restrict__ part) {
+  __shared__ double sh[16];
+  const int64_t n = (int64_t)d.nz * d.ny * d.nx;
+  double s_sq = 0.0, s_abs = 0.0;
+  auto av = [&](int z, int y, int xx) { return a[((int64_t)z * d.ny + y) * d.nx + xx]; };
+  auto bv = [&](int z, int y, int xx) { return b[((int64_t)z * d.ny + y) * d.nx + xx]; };
+  PD_LOOP(i, n) {
+    const int xx = (int)(i % d.nx);
+    const int y = (int)((i / d.nx) % d.ny);
+    const int z = (int)(i / ((int64_t)d.nx * d.ny));
+    float ha[6], hb[6];
+    hess6(av, d, z, y, xx, w.w, ha);
+    hess6(bv, d, z, y, xx, w.w, hb);
+    float ss = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const float dd = ha[q] - hb[q];
+      s_sq += (double)(dd * dd);
+      if (want_abs == 1) s_abs += (double)fabsf(ha[q]);
+      ss += ha[q] * ha[q];
+    }
+    if (want_abs == 2) s_abs += (double)sqrtf(ss + 1e-12f);
+  }
+  double r0 = pd_block_sum(s_sq, sh);
+  double r1 = pd_block_sum(s_abs, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x * 2 + 0] = r0;
+    part[blockIdx.x * 2 + 1] = r1;
+  }
+}
+
+// Hessian3D.forward into out (6, nz, ny, nx) and Hessian3D.adjoint of y (6, ...)
+__global__ void k_hess_fwd(const float* __restrict__ f, float* __restrict__ out, Dims d, W6 w) {
+  const int64_t n = (int64_t)d.nz * d.ny * d.nx;
+  auto fv = [&](int z, int y, int xx) { return f[((int64_t)z * d.ny + y) * d.nx + xx]; };
+  PD_LOOP(i, n) {
+    const int xx = (int)(i % d.nx);
+    const int y = (int)((i / d.nx) % d.ny);
+    const int z = (int)(i / ((int64_t)d.nx * d.ny));
+    float h[6];
+    hess6(fv, d, z, y, xx, w.w, h);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) out[q * n + i] = h[q];
+  }
+}
+
+__global__ void k_hess_adj(const float* __restrict__ y2, float* __restrict__ out, Dims d, W6 w) {
+  const int64_t n = (int64_t)d.nz * d.ny * d.nx;
+  auto hw = [&](int q, int z, int y, int xx) {
+    return y2[q * n + ((int64_t)z * d.ny + y) * d.nx + xx] * w.w[q];
+  };
+  PD_LOOP(i, n) {
+    const int xx = (int)(i % d.nx);
+    const int y = (int)((i / d.nx) % d.ny);
+    const int z = (int)(i / ((int64_t)d.nx * d.ny));
+    out[i] = hess_adj(hw, d, z, y, xx);
+  }
+}
+
+// sum (a - b)^2 and, if want_loss, the Poisson (or Gaussian) data loss of a + bg
+// against data (pdhg_mlx.py:752-759, 860-872)
+__global__ void k_pd_data_sums(const float* __restrict__ a, const float* __restrict__ b,
+                               const float* __restrict__ data, int64_t n, float bg, int want_loss, int loss_type,
+                               double* __restrict__ part) {
+  __shared__ double sh[16];
+  double s_sq = 0.0, s_loss = 0.0;
+  PD_LOOP(i, n) {
+    const float av = a[i];
+    const float dd = av - b[i];
+    s_sq += (double)(dd * dd);
+    if (want_loss) {
+      const float fm = av + bg;
+      const float dv = data[i];
+      if (loss_type == 0) {
+        const float fms = fmaxf(fm, 1e-6f);
+        s_loss += (double)(dv * logf((dv + 1e-6f) / fms) - (dv - fm));
+      } else {
+        const float r = dv - fm;
+        s_loss += 0.5 * (double)(r * r);
+      }
+    }
+  }
+  double r0 = pd_block_sum(s_sq, sh);
+  double r1 = pd_block_sum(s_loss, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x * 2 + 0] = r0;
+    part[blockIdx.x * 2 + 1] = r1;
+  }
+}
+
+// Malitsky–Pock step-size update and convergence test in FP64 (pdhg_mlx.py:761-835).
+__global__ void k_pd_scalars(double* sc, double* hist, int k, PdParams pp) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double tau = sc[PD_TAU], sigma = sc[PD_SIGMA], theta = sc[PD_THETA], step = sc[PD_STEP];
+  const double dx2 = sc[PD_DX2], dK2 = sc[PD_DA2] + sc[PD_DL2];
+  const double ratio = dx2 / (2.0 * sigma * fmax(dK2, 1e-12));
+  const double tc = fmin(sqrt(1.0 + theta) * tau, pp.delta * sqrt(ratio));
+  double tn = (tc * sigma * dK2 > pp.delta * dx2) ? pp.eta * tc : tc;
+  tn = fmax(tn, step * 1e-4);
+  const double thn = tn / tau;
+  double sn = sigma / fmax(thn, 0.1);
+  sn = fmin(fmax(sn, step * 1e-4), step * 1e4);
+  const double rel = sqrt(dx2 / (fmax(sc[PD_XX], sc[PD_XNXN]) + 1e-12));
+  hist[3 * k + 0] = tn;
+  hist[3 * k + 1] = sn;
+  hist[3 * k + 2] = rel;
+  if (k >= pp.min_iter) {
+    if (rel < pp.tol) {
+      sc[PD_COUNT] += 1.0;
+      if (sc[PD_COUNT] >= pp.patience) {
+        sc[PD_DONE] = 1.0;
+        sc[PD_FINAL] = k + 1;
+        return;
+      }
+    } else {
+      sc[PD_COUNT] = 0.0;
+    }
+  }
+  sc[PD_TAU] = tn;
+  sc[PD_SIGMA] = sn;
+  sc[PD_THETA] = thn;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+
+hipError_t pd_y1(float* y1, const float* ax, const float* axp, const float* data, int64_t n, const double* sc,
+                 float bg, int loss_type, hipStream_t st) {
+  hipLaunchKernelGGL(k_pd_y1, dim3(pd_blocks(n)), dim3(kPdThreads), 0, st, y1, ax, axp, data, n, sc, bg, loss_type);
+  return hipGetLastError();
+}
+
+hipError_t pd_dual(float* y2, const float* x, const float* xp, int nz, int ny, int nx, const double* sc,
+                   const float* w, float alpha, int l12, hipStream_t st) {
+  Dims d{nz, ny, nx};
+  W6 ww;
+  for (int q = 0; q < 6; ++q) ww.w[q] = w[q];
+  const int64_t n = (int64_t)nz * ny * nx;
+  int64_t nb = (n + kPdThreads - 1) / kPdThreads;
+  if (nb > 65535LL * 32) nb = 65535LL * 32;
+  hipLaunchKernelGGL(k_pd_dual, dim3((unsigned)nb), dim3(kPdThreads), 0, st, y2, x, xp, d, sc, ww, alpha, l12);
+  return hipGetLastError();
+}
+
+hipError_t pd_primal(float* xn, const float* x, const float* g, const float* y2, int nz, int ny, int nx,
+                     double* sc, const float* w, double* part, hipStream_t st) {
+  Dims d{nz, ny, nx};
+  W6 ww;
+  for (int q = 0; q < 6; ++q) ww.w[q] = w[q];
+  const int64_t n = (int64_t)nz * ny * nx;
+  const int nb = pd_blocks(n);
+  hipLaunchKernelGGL(k_pd_primal, dim3(nb), dim3(kPdThreads), 0, st, xn, x, g, y2, d, sc, ww, part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return finish(part, nb, 3, sc + PD_DX2, st);  // DX2, XX, XNXN are consecutive
+}
+
+hipError_t pd_hess_sums(const float* a, const float* b, int nz, int ny, int nx, const float* w, int want_abs,
+                        double* sc, double* loss_out, double* part, hipStream_t st) {
+  Dims d{nz, ny, nx};
+  W6 ww;
+  for (int q = 0; q < 6; ++q) ww.w[q] = w[q];
+  const int64_t n = (int64_t)nz * ny * nx;
+  const int nb = pd_blocks(n);
+  hipLaunchKernelGGL(k_pd_hess_sums, dim3(nb), dim3(kPdThreads), 0, st, a, b, d, ww, want_abs, part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  e = finish(part, nb, 2, sc + PD_TMP0, st);
+  if (e != hipSuccess) return e;
+  // move: DL2 <- TMP0, loss_out <- TMP1
+  return pd_move(sc, PD_TMP0, PD_DL2, PD_TMP1, loss_out, st);
+}
+
+hipError_t pd_data_sums(const float* a, const float* b, const float* data, int64_t n, float bg, int want_loss,
+                        int loss_type, double* sc, double* loss_out, double* part, hipStream_t st) {
+  const int nb = pd_blocks(n);
+  hipLaunchKernelGGL(k_pd_data_sums, dim3(nb), dim3(kPdThreads), 0, st, a, b, data, n, bg, want_loss, loss_type,
+                     part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  e = finish(part, nb, 2, sc + PD_TMP0, st);
+  if (e != hipSuccess) return e;
+  return pd_move(sc, PD_TMP0, PD_DA2, PD_TMP1, loss_out, st);
+}
+
+__global__ void k_pd_move(double* sc, int from0, int to0, int from1, double* out1) {
+  if (threadIdx.x == 0) {
+    sc[to0] = sc[from0];
+    if (out1) out1[0] = sc[from1];
+  }
+}
+
+hipError_t pd_move(double* sc, int from0, int to0, int from1, double* out1, hipStream_t st) {
+  hipLaunchKernelGGL(k_pd_move, dim3(1), dim3(64), 0, st, sc, from0, to0, from1, out1);
+  return hipGetLastError();
+}
+
+hipError_t pd_scalars(double* sc, double* hist, int k, const PdParams& pp, hipStream_t st) {
+  hipLaunchKernelGGL(k_pd_scalars, dim3(1), dim3(64), 0, st, sc, hist, k, pp);
+  return hipGetLastError();
+}
+
+int64_t pd_partials_needed(int64_t n) { return (int64_t)pd_blocks(n) * 3 + 16; }
+
+static dim3 ew_blocks(int64_t n) {
+  int64_t nb = (n + kPdThreads - 1) / kPdThreads;
+  if (nb > 65535LL * 32) nb = 65535LL * 32;
+  return dim3((unsigned)nb);
+}
+
+hipError_t hess_forward(const float* f, float* out, int nz, int ny, int nx, const float* w, hipStream_t st) {
+  Dims d{nz, ny, nx};
+  W6 ww;
+  for (int q = 0; q < 6; ++q) ww.w[q] = w[q];
+  hipLaunchKernelGGL(k_hess_fwd, ew_blocks((int64_t)nz * ny * nx), dim3(kPdThreads), 0, st, f, out, d, ww);
+  return hipGetLastError();
+}
+
+hipError_t hess_adjoint(const float* y2, float* out, int nz, int ny, int nx, const float* w, hipStream_t st) {
+  Dims d{nz, ny, nx};
+  W6 ww;
+  for (int q = 0; q < 6; ++q) ww.w[q] = w[q];
+  hipLaunchKernelGGL(k_hess_adj, ew_blocks((int64_t)nz * ny * nx), dim3(kPdThreads), 0, st, y2, out, d, ww);
+  return hipGetLastError();
+}
+
+}  // namespace dh

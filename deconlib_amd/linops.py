@@ -50,6 +50,56 @@ def fast_padded_shape(signal_shape, kernel_shape, min_pad=None) -> Tuple[int, ..
     return tuple(out)
 
 
+SQRT2 = float(np.float32(np.sqrt(np.float32(2.0))))
+
+
+class Hessian3D:
+    """Six weighted second differences with Neumann boundaries (linops_mlx.py:182-262).
+
+    Components [zz, yy, xx, yz, xz, xy] weighted [r^2, 1, 1, r sqrt2, r sqrt2, sqrt2];
+    ``operator_norm_sq = 16 r^4 + 4 r^2 + 34``.  Forward and adjoint are the
+    libdeconhip stencil kernels.
+    """
+
+    def __init__(self, r: float = 1.0):
+        self.r = r
+        self.operator_norm_sq = 16.0 * (r ** 4) + 4.0 * (r ** 2) + 34.0
+        self._w = np.array([r * r, 1.0, 1.0, r * SQRT2, r * SQRT2, SQRT2], dtype=np.float32)
+
+    @classmethod
+    def from_spacing(cls, spacing):
+        dz, dy, dx = spacing
+        if abs(dy - dx) > 1e-2 * max(dy, dx):
+            raise ValueError(
+                f"Hessian3D.from_spacing assumes isotropic lateral sampling "
+                f"(dy == dx); got dy={dy}, dx={dx}. Construct Hessian3D(r=...) "
+                f"directly for anisotropic lateral spacing."
+            )
+        return cls(r=dy / dz)
+
+    def _wp(self):
+        return self._w.ctypes.data_as(C.POINTER(C.c_float))
+
+    def forward(self, f) -> torch.Tensor:
+        f = as_f32(f)
+        if f.dim() != 3:
+            raise ValueError("Hessian3D expects a 3-D volume")
+        out = empty_f32((6,) + tuple(f.shape))
+        N.call("dh_hessian_forward", f.data_ptr(), out.data_ptr(), *f.shape, self._wp(), N.stream_ptr())
+        return out
+
+    def adjoint(self, H) -> torch.Tensor:
+        H = as_f32(H)
+        if H.dim() != 4 or H.shape[0] != 6:
+            raise ValueError("Hessian3D.adjoint expects a (6, Z, Y, X) array")
+        out = empty_f32(tuple(H.shape[1:]))
+        N.call("dh_hessian_adjoint", H.data_ptr(), out.data_ptr(), *H.shape[1:], self._wp(), N.stream_ptr())
+        return out
+
+    def __call__(self, f):
+        return self.forward(f)
+
+
 class _ConvPlan:
     """Owns one native plan and its device buffers (twiddles, OTF, workspace)."""
 

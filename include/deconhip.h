@@ -129,6 +129,57 @@ int dh_rl_update(float* x, const float* corr, const float* sens, int64_t n, floa
 int dh_add_scalar(const float* in, const float* obs, float* out, int64_t n, float a,
                   double* partials_dev, double* reduce_out, void* stream);
 
+/* ---- PDHG with Hessian regularization (pdhg_mlx.py:507-920) ---- */
+/* float64 scalar block (caller-owned device array of DH_PD_NSLOTS doubles). */
+#define DH_PD_TAU 0
+#define DH_PD_SIGMA 1
+#define DH_PD_THETA 2
+#define DH_PD_STEP 3      /* 0.99 / sqrt(|A|^2 + |L|^2) */
+#define DH_PD_COUNT 4     /* consecutive iterations under tol */
+#define DH_PD_DONE 5      /* 1.0 once converged */
+#define DH_PD_FINAL 6     /* iteration count at convergence */
+#define DH_PD_DX2 7       /* sum (x_new - x)^2 */
+#define DH_PD_XX 8        /* sum x^2 */
+#define DH_PD_XNXN 9      /* sum x_new^2 */
+#define DH_PD_DA2 10      /* sum (A x_new - A x)^2 */
+#define DH_PD_DL2 11      /* sum (L x_new - L x)^2 */
+#define DH_PD_TMP0 12
+#define DH_PD_TMP1 13
+#define DH_PD_NSLOTS 16
+
+typedef struct {
+  double delta, eta, tol;
+  int32_t min_iter, patience;
+} dh_pd_params;
+
+/* Partials (doubles) the PDHG reductions need for n voxels. */
+int64_t dh_pd_partials(int64_t n);
+/* y1 <- prox_sigma F*(y1 + sigma ((1+theta) ax - theta axp))  (pdhg_mlx.py:115-189, 723-727);
+ * loss_type 0 = poisson, 1 = gaussian. */
+int dh_pd_y1(float* y1, const float* ax, const float* axp, const float* data, int64_t n, const double* scalars,
+             float background, int loss_type, void* stream);
+/* y2 (6, nz, ny, nx) <- prox(y2 + sigma H(x + theta (x - xp)))  with clip (l12 = 0) or the
+ * per-voxel L2 ball (l12 = 1) of radius alpha  (pdhg_mlx.py:729-734, 197-233).
+ * w = Hessian3D weights [r^2, 1, 1, r sqrt2, r sqrt2, sqrt2]. */
+int dh_pd_dual(float* y2, const float* x, const float* xp, int nz, int ny, int nx, const double* scalars,
+               const float* w, float alpha, int l12, void* stream);
+/* x_new = max(0, x - tau (g + H^T y2)); writes DX2, XX, XNXN  (pdhg_mlx.py:741-750, 805-806). */
+int dh_pd_primal(float* x_new, const float* x, const float* g, const float* y2, int nz, int ny, int nx,
+                 double* scalars, const float* w, double* partials, void* stream);
+/* DL2 = sum |H a - H b|^2; if reg_loss_out, *reg_loss_out = sum |H a| (reg_norm 1, "L1")
+ * or sum sqrt(sum_q (H a)_q^2 + 1e-12) (reg_norm 2, "L1_2")  (pdhg_mlx.py:874-882). */
+int dh_pd_hess_sums(const float* a, const float* b, int nz, int ny, int nx, const float* w, double* scalars,
+                    double* reg_loss_out, int reg_norm, double* partials, void* stream);
+/* DA2 = sum (a - b)^2; if data_loss_out, the data loss of a + background against data. */
+int dh_pd_data_sums(const float* a, const float* b, const float* data, int64_t n, float background,
+                    int loss_type, double* scalars, double* data_loss_out, double* partials, void* stream);
+/* Malitsky-Pock step update and convergence test for iteration k in float64
+ * (pdhg_mlx.py:761-835); hist[3k..3k+2] = tau_new, sigma_new, rel_change. */
+int dh_pd_scalars(double* scalars, double* hist, int k, const dh_pd_params* params, void* stream);
+/* Hessian3D.forward f -> out (6, nz, ny, nx) and adjoint y -> out (linops_mlx.py:224-259). */
+int dh_hessian_forward(const float* f, float* out, int nz, int ny, int nx, const float* w, void* stream);
+int dh_hessian_adjoint(const float* y, float* out, int nz, int ny, int nx, const float* w, void* stream);
+
 /* ---- measurement ---- */
 /* Pass ids for dh_timing_read. */
 #define DH_PASS_X_FWD 0
