@@ -87,6 +87,8 @@ SYMBOLS = {
     "dh_conv_plan_info": (_int, _vp, C.POINTER(PlanInfo)),
     "dh_conv_plan_bind": (_int, _vp, _vp, _vp),
     "dh_conv_plan_destroy": (_int, _vp),
+    "dh_conv_plan_set_tile": (_int, _vp, _int, _int),
+    "dh_conv_plan_get_tile": (_int, _vp, _int),
     "dh_otf_build": (_int, _vp, _vp, _int, _vp, _vp, _i64, C.POINTER(_d), _vp),
     "dh_conv_apply": (_int, _vp, _int, _vp, _vp, _vp, _vp, _i64, C.POINTER(FuseArgs), _vp),
     "dh_reduce_partials": (_i64, _i64),
@@ -123,6 +125,9 @@ if fns["dh_abi_version"]() != ABI_VERSION:
     )
 
 
+_VALUE_RETURNING = {"dh_reduce_partials", "dh_abi_version", "dh_pd_partials", "dh_conv_plan_get_tile"}
+
+
 class NativeError(RuntimeError):
     pass
 
@@ -142,7 +147,7 @@ def check(rc: int, what: str = "") -> None:
 
 def call(name: str, *args) -> int:
     rc = fns[name](*args)
-    if isinstance(rc, int) and name not in ("dh_reduce_partials", "dh_abi_version", "dh_pd_partials"):
+    if isinstance(rc, int) and name not in _VALUE_RETURNING:
         check(rc, name)
     return rc
 

@@ -130,6 +130,7 @@ struct dh_conv_plan {
   const float2* tw_dev = nullptr;
   int64_t b1_off = 0, b2_off = 0, part_off = 0, scal_off = 0, ws_bytes = 0;
   int64_t n_partials = 0;
+  int tile[DH_PASS_COUNT] = {0, 0, 0, 0, 0};
 };
 
 extern "C" {
@@ -195,9 +196,16 @@ int dh_conv_plan_create(int ndim, const int64_t* signal_shape, const int64_t* ke
     p->P[a] = c;
   }
   if (p->P[2] == 1) {
-    // 1-point x axis: treat as 2-point canvas is wrong for circular; reject
     delete p;
     return fail(DH_EUNSUPPORTED, "last axis of length 1 is not supported");
+  }
+  for (int a = 0; a < 3; ++a) {
+    // one line per workgroup holds at most 16 samples per thread, 1024 threads
+    const int line = (a == 2) ? p->P[2] / 2 : p->P[a];
+    if (line > 16384) {
+      delete p;
+      return fail(DH_EUNSUPPORTED, "canvas extent above 16384 (32768 on the last axis) is not supported");
+    }
   }
   for (int a = 0; a < 3; ++a) {
     if (circular) {
@@ -266,6 +274,27 @@ int dh_conv_plan_bind(dh_conv_plan* p, void* twiddles_dev, void* stream) {
   return DH_OK;
 }
 
+int dh_conv_plan_set_tile(dh_conv_plan* p, int pass, int tile) {
+  if (!p || pass < 0 || pass >= DH_PASS_COUNT || tile < 0) return fail(DH_EINVAL, "bad set_tile arguments");
+  if (tile > 0) {
+    const bool row = (pass == DH_PASS_X_FWD || pass == DH_PASS_X_INV);
+    const int n = row ? p->M : (pass == DH_PASS_Z ? p->P[0] : p->P[1]);
+    if (row ? !dh::row_tile_ok(n, tile) : !dh::col_tile_ok(n, tile))
+      return fail(DH_EINVAL, "tile does not fit the workgroup limits (1024 threads, 160 KiB LDS)");
+  }
+  p->tile[pass] = tile;
+  return DH_OK;
+}
+
+int dh_conv_plan_get_tile(const dh_conv_plan* p, int pass) {
+  if (!p || pass < 0 || pass >= DH_PASS_COUNT) return -1;
+  const bool row = (pass == DH_PASS_X_FWD || pass == DH_PASS_X_INV);
+  const int n = row ? p->M : (pass == DH_PASS_Z ? p->P[0] : p->P[1]);
+  if (row) return dh::row_launch_for(n, p->tile[pass]).R;
+  const int64_t ncols = (pass == DH_PASS_Z) ? (int64_t)p->P[1] * p->H : p->H;
+  return dh::col_launch_for(n, ncols, p->tile[pass]).L;
+}
+
 int dh_conv_plan_destroy(dh_conv_plan* p) {
   delete p;
   return DH_OK;
@@ -291,19 +320,19 @@ int dh_otf_build(const dh_conv_plan* p, const float* kernel_dev, int normalize, 
   }
   const int64_t H = p->H;
   DH_TRY(hip_check(dh::launch_x_r2c(p->xf, p->tw_dev + p->tw_xm, p->tw_dev + p->tw_xp, kernel_dev, B1,
-                                    (int64_t)p->k[0] * p->k[1], p->k[2], p->klo[2], p->khi[2], p->H, pro, st),
+                                    (int64_t)p->k[0] * p->k[1], p->k[2], p->klo[2], p->khi[2], p->H, pro, 0, st),
                    "otf x pass"));
   float2* zsrc = B1;
   if (p->P[1] > 1) {
     DH_TRY(hip_check(dh::launch_col_fft(-1, p->yf, p->tw_dev + p->tw_y, B1, B2, H, p->k[0], (int64_t)p->k[1] * H,
-                                        (int64_t)p->P[1] * H, H, H, p->klo[1], p->khi[1], p->P[1], 1.0f, st),
+                                        (int64_t)p->P[1] * H, H, H, p->klo[1], p->khi[1], p->P[1], 1.0f, 0, st),
                      "otf y pass"));
     zsrc = B2;
   }
   const double total = (double)p->P[0] * p->P[1] * p->P[2];
   const int64_t ncols = (int64_t)p->P[1] * H;
   DH_TRY(hip_check(dh::launch_col_fft(-1, p->zf, p->tw_dev + p->tw_z, zsrc, (float2*)otf_dev, ncols, 1, 0, 0,
-                                      ncols, ncols, p->klo[0], p->khi[0], p->P[0], (float)(1.0 / total), st),
+                                      ncols, ncols, p->klo[0], p->khi[0], p->P[0], (float)(1.0 / total), 0, st),
                    "otf z pass"));
   if (norm_sq_host) {
     DH_TRY(hip_check(dh::launch_cmax_abs2((const float2*)otf_dev, (int64_t)p->P[0] * ncols, part, scal + 1, st),
@@ -355,34 +384,34 @@ int dh_conv_apply(const dh_conv_plan* p, int adjoint, const float* in, float* ou
   const int64_t nrows = (int64_t)s0 * s1;
   DH_TRY(timing_begin(DH_PASS_X_FWD, st));
   DH_TRY(hip_check(dh::launch_x_r2c(p->xf, p->tw_dev + p->tw_xm, p->tw_dev + p->tw_xp, in, B1, nrows, s2, s2, 0,
-                                    p->H, pro, st),
+                                    p->H, pro, p->tile[DH_PASS_X_FWD], st),
                    "x forward pass"));
   DH_TRY(timing_end(st));
   float2* zb = B1;
   if (p->P[1] > 1) {
     DH_TRY(timing_begin(DH_PASS_Y_FWD, st));
     DH_TRY(hip_check(dh::launch_col_fft(-1, p->yf, p->tw_dev + p->tw_y, B1, B2, H, s0, (int64_t)s1 * H,
-                                        (int64_t)p->P[1] * H, H, H, s1, 0, p->P[1], 1.0f, st),
+                                        (int64_t)p->P[1] * H, H, H, s1, 0, p->P[1], 1.0f, p->tile[DH_PASS_Y_FWD], st),
                      "y forward pass"));
     DH_TRY(timing_end(st));
     zb = B2;
   }
   DH_TRY(timing_begin(DH_PASS_Z, st));
   DH_TRY(hip_check(dh::launch_z_conv(p->zf, p->zi, p->tw_dev + p->tw_z, zb, (const float2*)otf_dev,
-                                     (int64_t)p->P[1] * H, s0, s0, adjoint ? 1 : 0, st),
+                                     (int64_t)p->P[1] * H, s0, s0, adjoint ? 1 : 0, p->tile[DH_PASS_Z], st),
                    "z pass"));
   DH_TRY(timing_end(st));
   if (p->P[1] > 1) {
     DH_TRY(timing_begin(DH_PASS_Y_INV, st));
     DH_TRY(hip_check(dh::launch_col_fft(1, p->yi, p->tw_dev + p->tw_y, B2, B1, H, s0, (int64_t)p->P[1] * H,
-                                        (int64_t)s1 * H, H, H, p->P[1], 0, s1, 1.0f, st),
+                                        (int64_t)s1 * H, H, H, p->P[1], 0, s1, 1.0f, p->tile[DH_PASS_Y_INV], st),
                      "y inverse pass"));
     DH_TRY(timing_end(st));
   }
   int64_t nblocks = 0;
   DH_TRY(timing_begin(DH_PASS_X_INV, st));
   DH_TRY(hip_check(dh::launch_x_c2r(p->xi, p->tw_dev + p->tw_xm, p->tw_dev + p->tw_xp, B1, out, nrows, s2, p->H,
-                                    epi, &nblocks, st),
+                                    epi, p->tile[DH_PASS_X_INV], &nblocks, st),
                    "x inverse pass"));
   DH_TRY(timing_end(st));
   if (reduce_out) {

@@ -439,9 +439,37 @@ __global__ void k_sum_partials(const double* __restrict__ p, int64_t n, double* 
 
 static int threads_per_line(int n) { return (n + kEPT - 1) / kEPT; }
 
-ColLaunch col_launch_for(int n, int64_t ncols) {
+// Largest LDS a workgroup may take on gfx950 (160 KiB per CU).
+constexpr int64_t kMaxLds = 160 * 1024;
+
+static void allow_big_lds() {
+  static bool done = false;
+  if (done) return;
+  done = true;
+  (void)hipFuncSetAttribute((const void*)k_col_fft<-1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
+  (void)hipFuncSetAttribute((const void*)k_col_fft<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
+  (void)hipFuncSetAttribute((const void*)k_z_conv, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
+  (void)hipFuncSetAttribute((const void*)k_x_r2c, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
+  (void)hipFuncSetAttribute((const void*)k_x_c2r, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
+}
+
+bool col_tile_ok(int n, int L) {
+  const int T = threads_per_line(n);
+  return L >= 1 && L * T <= 1024 && (int64_t)L * n * 8 <= kMaxLds;
+}
+
+bool row_tile_ok(int m, int R) {
+  const int T = threads_per_line(m);
+  return R >= 1 && R * T <= 1024 && (int64_t)R * (m + 1) * 8 <= kMaxLds;
+}
+
+ColLaunch col_launch_for(int n, int64_t ncols, int tile) {
   ColLaunch c;
   c.T = threads_per_line(n);
+  if (tile > 0 && col_tile_ok(n, tile)) {
+    c.L = tile;
+    return c;
+  }
   int L = 1;
   // widest power-of-two column tile with <= 512 threads and <= 64 KiB of LDS
   while (L < 64 && (2 * L) * c.T <= 512 && (int64_t)(2 * L) * n * 8 <= 65536) L *= 2;
@@ -450,9 +478,13 @@ ColLaunch col_launch_for(int n, int64_t ncols) {
   return c;
 }
 
-RowLaunch row_launch_for(int m) {
+RowLaunch row_launch_for(int m, int tile) {
   RowLaunch r;
   r.T = threads_per_line(m);
+  if (tile > 0 && row_tile_ok(m, tile)) {
+    r.R = tile;
+    return r;
+  }
   int R = 1;
   while (R < 64 && (2 * R) * r.T <= 256 && (int64_t)(2 * R) * (m + 1) * 8 <= 65536) R *= 2;
   r.R = R;
@@ -461,8 +493,9 @@ RowLaunch row_launch_for(int m) {
 
 hipError_t launch_col_fft(int dir, const AxisSched& s, const float2* tw, const float2* src, float2* dst,
                           int64_t ncols, int nbatch, int64_t src_b, int64_t dst_b, int64_t src_rs,
-                          int64_t dst_rs, int lo, int hi, int n_out, float scale, hipStream_t st) {
-  ColLaunch c = col_launch_for(s.n, ncols);
+                          int64_t dst_rs, int lo, int hi, int n_out, float scale, int tile, hipStream_t st) {
+  allow_big_lds();
+  ColLaunch c = col_launch_for(s.n, ncols, tile);
   dim3 grid((unsigned)((ncols + c.L - 1) / c.L), (unsigned)nbatch);
   dim3 block(c.L * c.T);
   size_t shm = (size_t)s.n * c.L * sizeof(float2);
@@ -476,13 +509,14 @@ hipError_t launch_col_fft(int dir, const AxisSched& s, const float2* tw, const f
 }
 
 hipError_t launch_z_conv(const AxisSched& f, const AxisSched& iv, const float2* tw, float2* buf,
-                         const float2* otf, int64_t ncols, int lo, int n_out, int conj, hipStream_t st) {
+                         const float2* otf, int64_t ncols, int lo, int n_out, int conj, int tile, hipStream_t st) {
   if (f.n == 1) {
     dim3 grid((unsigned)((ncols + 255) / 256));
     hipLaunchKernelGGL(k_z_mul, grid, dim3(256), 0, st, buf, otf, ncols, conj);
     return hipGetLastError();
   }
-  ColLaunch c = col_launch_for(f.n, ncols);
+  allow_big_lds();
+  ColLaunch c = col_launch_for(f.n, ncols, tile);
   dim3 grid((unsigned)((ncols + c.L - 1) / c.L));
   dim3 block(c.L * c.T);
   size_t shm = (size_t)f.n * c.L * sizeof(float2);
@@ -491,9 +525,10 @@ hipError_t launch_z_conv(const AxisSched& f, const AxisSched& iv, const float2* 
 }
 
 hipError_t launch_x_r2c(const AxisSched& s, const float2* twm, const float2* twp, const float* src,
-                        float2* dst, int64_t nrows, int rowlen, int lo, int hi, int H, const Pro& pro,
+                        float2* dst, int64_t nrows, int rowlen, int lo, int hi, int H, const Pro& pro, int tile,
                         hipStream_t st) {
-  RowLaunch r = row_launch_for(s.n);
+  allow_big_lds();
+  RowLaunch r = row_launch_for(s.n, tile);
   dim3 grid((unsigned)((nrows + r.R - 1) / r.R));
   dim3 block(r.R * r.T);
   size_t shm = (size_t)r.R * (s.n + 1) * sizeof(float2);
@@ -503,9 +538,10 @@ hipError_t launch_x_r2c(const AxisSched& s, const float2* twm, const float2* twp
 }
 
 hipError_t launch_x_c2r(const AxisSched& s, const float2* twm, const float2* twp, const float2* src,
-                        float* dst, int64_t nrows, int s2, int H, const Epi& epi, int64_t* nblocks,
+                        float* dst, int64_t nrows, int s2, int H, const Epi& epi, int tile, int64_t* nblocks,
                         hipStream_t st) {
-  RowLaunch r = row_launch_for(s.n);
+  allow_big_lds();
+  RowLaunch r = row_launch_for(s.n, tile);
   dim3 grid((unsigned)((nrows + r.R - 1) / r.R));
   dim3 block(r.R * r.T);
   size_t shm = (size_t)r.R * (s.n + 1) * sizeof(float2);

@@ -48,8 +48,11 @@ struct RowLaunch {
   int T;        // threads per row
 };
 
-ColLaunch col_launch_for(int n, int64_t ncols);
-RowLaunch row_launch_for(int m);
+// tile = columns (col passes) or rows (row passes) per workgroup; 0 = heuristic.
+ColLaunch col_launch_for(int n, int64_t ncols, int tile);
+RowLaunch row_launch_for(int m, int tile);
+bool col_tile_ok(int n, int L);
+bool row_tile_ok(int m, int R);
 
 // Column FFT along a strided axis: for every batch b and column c,
 //   line e (canvas index) = src[b*src_b + srow(e)*rstride + c], where rows
@@ -58,25 +61,25 @@ RowLaunch row_launch_for(int m);
 hipError_t launch_col_fft(int dir, const AxisSched& s, const float2* tw, const float2* src,
                           float2* dst, int64_t ncols, int nbatch, int64_t src_b, int64_t dst_b,
                           int64_t src_rstride, int64_t dst_rstride, int lo, int hi, int n_out,
-                          float scale, hipStream_t st);
+                          float scale, int tile, hipStream_t st);
 
 // Fused Z pass: forward FFT (pruned input rows lo/hi), multiply by otf (or its
 // conjugate), inverse FFT, store rows [0, n_out).  In place on buf.
 hipError_t launch_z_conv(const AxisSched& f, const AxisSched& iv, const float2* tw, float2* buf,
-                         const float2* otf, int64_t ncols, int lo, int n_out, int conj,
+                         const float2* otf, int64_t ncols, int lo, int n_out, int conj, int tile,
                          hipStream_t st);
 
 // Forward real-to-complex X pass on rows of a real volume (row length
 // rowlen = lo + hi source samples mapped into a canvas of 2*m).
 hipError_t launch_x_r2c(const AxisSched& s, const float2* twm, const float2* twp, const float* src,
                         float2* dst, int64_t nrows, int rowlen, int lo, int hi, int H,
-                        const Pro& pro, hipStream_t st);
+                        const Pro& pro, int tile, hipStream_t st);
 
 // Inverse complex-to-real X pass; stores the first s2 samples with the epilogue.
 // Returns the number of workgroups (= partials written) through nblocks.
 hipError_t launch_x_c2r(const AxisSched& s, const float2* twm, const float2* twp, const float2* src,
-                        float* dst, int64_t nrows, int s2, int H, const Epi& epi, int64_t* nblocks,
-                        hipStream_t st);
+                        float* dst, int64_t nrows, int s2, int H, const Epi& epi, int tile,
+                        int64_t* nblocks, hipStream_t st);
 
 // Deterministic final reduction of n partials into out[0] (sum).
 hipError_t launch_sum_partials(const double* partials, int64_t n, double* out, hipStream_t st);

@@ -84,6 +84,12 @@ int dh_conv_plan_info(const dh_conv_plan* plan, dh_plan_info* info);
 /* Upload the twiddle tables into twiddles_dev (twiddle_bytes). */
 int dh_conv_plan_bind(dh_conv_plan* plan, void* twiddles_dev, void* stream);
 int dh_conv_plan_destroy(dh_conv_plan* plan);
+/* Launch shape of one pass (DH_PASS_*): columns per workgroup for the Y/Z
+ * passes, rows per workgroup for the X passes; 0 restores the built-in
+ * heuristic.  Semantics-free: used by the autotuner.  get_tile returns the
+ * value in effect. */
+int dh_conv_plan_set_tile(dh_conv_plan* plan, int pass, int tile);
+int dh_conv_plan_get_tile(const dh_conv_plan* plan, int pass);
 
 /* OTF = rfftn(pad_corner_origin_kernel(kernel / sum(kernel)))   (linops_mlx.py:408-411,
  * utils/padding.py:142-190), stored pre-scaled by 1/prod(canvas).  Writes
