@@ -11,6 +11,11 @@
 // first pass reads global memory and whose last pass writes global memory
 // directly (both coalesced: lanes walk consecutive columns or consecutive row
 // samples); the intermediate passes exchange through LDS.
+//
+// Each pass is a __device__ body over one tile (a group of columns or rows of
+// one volume).  The single-volume kernels run one volume per launch; the
+// batched kernels run the tiles of many volumes of different shapes in one
+// launch, finding each workgroup's volume from a prefix table.
 #include "dh_dft.hpp"
 #include "dh_fft.hpp"
 
@@ -20,6 +25,16 @@ template <int R>
 struct MBv {
   static constexpr int v = (kEPT + R - 1) / R + 1;
 };
+
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  Map the hardware block id to a logical one so that XCD x runs a
+// contiguous range of logical tiles: a bijection on [0, nb).
+constexpr int kNumXcd = 8;
+__device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t nb) {
+  const int64_t q = nb / kNumXcd, r = nb % kNumXcd;
+  const int64_t x = b % kNumXcd, i = b / kNumXcd;
+  return x * q + (x < r ? x : r) + i;
+}
 
 // One Stockham pass of radix R and span P over an n-point line.
 //   ld(e)    -> input element e
@@ -76,7 +91,8 @@ __device__ __forceinline__ void pass_any(int R, int n, int P, int t, int T, cons
 }
 
 // Runs all passes of schedule s: first pass loads with gld, last pass stores
-// with gst, the rest go through the LDS line accessor.
+// with gst, the rest go through the LDS line accessor.  Threads with t >= n
+// do no butterflies but take every barrier.
 template <int DIR, class GLd, class GSt, class LIdx>
 __device__ __forceinline__ void fft_line(const AxisSched& s, int t, int T, const float2* __restrict__ tw,
                                          float2* lds, LIdx lidx, GLd gld, GSt gst) {
@@ -104,24 +120,26 @@ __device__ __forceinline__ void fft_line(const AxisSched& s, int t, int T, const
   }
 }
 
+__device__ __forceinline__ int threads_per_line_d(int n) { return (n + kEPT - 1) / kEPT; }
+
 // ---------------------------------------------------------------------------
-// Column FFT (Y passes, OTF Z pass)
+// Column FFT body (Y passes, OTF Z pass): L columns of one batch plane
 // ---------------------------------------------------------------------------
 
 template <int DIR>
-__global__ void k_col_fft(AxisSched s, const float2* __restrict__ tw, const float2* __restrict__ src,
-                          float2* __restrict__ dst, int64_t ncols, int64_t src_b, int64_t dst_b,
-                          int64_t src_rs, int64_t dst_rs, int lo, int hi, int n_out, float scale,
-                          int L) {
-  extern __shared__ float2 lds[];
-  const int ln = threadIdx.x % L;
-  const int t = threadIdx.x / L;
-  const int T = blockDim.x / L;
-  const int64_t col = (int64_t)blockIdx.x * L + ln;
-  const bool ok = col < ncols;
+__device__ __forceinline__ void col_fft_body(const AxisSched& s, const float2* __restrict__ tw,
+                                             const float2* __restrict__ src, float2* __restrict__ dst,
+                                             int64_t ncols, int64_t src_rs, int64_t dst_rs, int lo, int hi,
+                                             int n_out, float scale, int64_t tile, int L, float2* lds) {
   const int n = s.n;
-  const float2* sb = src + (int64_t)blockIdx.y * src_b + col;
-  float2* db = dst + (int64_t)blockIdx.y * dst_b + col;
+  const int T = threads_per_line_d(n);
+  const int ln = threadIdx.x % L;
+  int t = threadIdx.x / L;
+  if (t >= T) t = n;  // idle thread
+  const int64_t col = tile * L + ln;
+  const bool ok = col < ncols;
+  const float2* sb = src + col;
+  float2* db = dst + col;
   const int shift = n - lo - hi;
   auto gld = [&](int e) -> float2 {
     int r = -1;
@@ -138,7 +156,7 @@ __global__ void k_col_fft(AxisSched s, const float2* __restrict__ tw, const floa
 }
 
 // ---------------------------------------------------------------------------
-// Fused Z pass: forward FFT -> x OTF -> inverse FFT, in place.
+// Fused Z body: forward FFT -> x OTF -> inverse FFT, in place, L columns.
 // ---------------------------------------------------------------------------
 
 template <int R>
@@ -192,37 +210,40 @@ __device__ __forceinline__ void z_mid_pass(int n, int P, int t, int T, const flo
   }
 }
 
-__global__ void k_z_conv(AxisSched f, AxisSched iv, const float2* __restrict__ tw, float2* buf,
-                         const float2* __restrict__ otf, int64_t ncols, int lo, int n_out, int conj,
-                         int L) {
-  extern __shared__ float2 lds[];
-  const int ln = threadIdx.x % L;
-  const int t = threadIdx.x / L;
-  const int T = blockDim.x / L;
-  const int64_t col = (int64_t)blockIdx.x * L + ln;
-  const bool ok = col < ncols;
+__device__ __forceinline__ void z_conv_body(const AxisSched& f, const AxisSched& iv, const float2* __restrict__ tw,
+                                            float2* buf, const float2* __restrict__ otf, int64_t ncols, int lo,
+                                            int n_out, int conj, int64_t tile, int L, float2* lds) {
   const int n = f.n;
+  const int T = threads_per_line_d(n);
+  const int ln = threadIdx.x % L;
+  int t = threadIdx.x / L;
+  if (t >= T) t = n;  // idle thread
+  const int64_t col = tile * L + ln;
+  const bool ok = col < ncols;
   float2* cb = buf + col;
   const int64_t rs = ncols;
   auto lidx = [&](int e) { return e * L + ln; };
   auto lld = [&](int e) { return lds[lidx(e)]; };
   auto lst = [&](int e, float2 v) { lds[lidx(e)] = v; };
-  auto gld = [&](int e) -> float2 {
-    return (ok && e < lo) ? cb[(int64_t)e * rs] : make_float2(0.f, 0.f);
-  };
+  auto gld = [&](int e) -> float2 { return (ok && e < lo) ? cb[(int64_t)e * rs] : make_float2(0.f, 0.f); };
   auto gst = [&](int e, float2 v) {
     if (ok && e < n_out) cb[(int64_t)e * rs] = v;
   };
+  if (n == 1) {  // the Z "FFT" is the identity: multiply by the OTF only
+    if (t == 0 && ok && lo > 0) {
+      const float2 v = cb[0], o = otf[col];
+      cb[0] = conj ? cmulc(v, o) : cmul(v, o);
+    }
+    return;
+  }
   const int np = f.np;
-  // forward passes except the last
-  for (int p = 0; p < np - 1; ++p) {
+  for (int p = 0; p < np - 1; ++p) {  // forward passes except the last
     const int R = f.radix[p], P = f.span[p];
     if (p == 0) pass_any<-1>(R, n, P, t, T, tw, gld, lst, false);
     else pass_any<-1>(R, n, P, t, T, tw, lld, lst, true);
     __syncthreads();
   }
-  // merged: last forward pass, x OTF, first inverse pass
-  {
+  {  // merged: last forward pass, x OTF, first inverse pass
     const int R = f.radix[np - 1], P = f.span[np - 1];
     const bool from_lds = np > 1, to_lds = np > 1;
     switch (R) {
@@ -236,8 +257,7 @@ __global__ void k_z_conv(AxisSched f, AxisSched iv, const float2* __restrict__ t
     }
     if (to_lds) __syncthreads();
   }
-  // remaining inverse passes
-  for (int p = 1; p < iv.np; ++p) {
+  for (int p = 1; p < iv.np; ++p) {  // remaining inverse passes
     const int R = iv.radix[p], P = iv.span[p];
     if (p == iv.np - 1) pass_any<1>(R, n, P, t, T, tw, lld, gst, false);
     else {
@@ -247,17 +267,8 @@ __global__ void k_z_conv(AxisSched f, AxisSched iv, const float2* __restrict__ t
   }
 }
 
-// P0 == 1: the Z "FFT" is the identity; multiply by the OTF only.
-__global__ void k_z_mul(float2* buf, const float2* __restrict__ otf, int64_t ncols, int conj) {
-  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < ncols) {
-    float2 v = buf[c], o = otf[c];
-    buf[c] = conj ? cmulc(v, o) : cmul(v, o);
-  }
-}
-
 // ---------------------------------------------------------------------------
-// X passes (rows): real <-> half spectrum through an m = P2/2 complex FFT.
+// X bodies (rows): real <-> half spectrum through an m = P2/2 complex FFT.
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ float apply_pro(const Pro& p, const float* src, int64_t idx) {
@@ -269,16 +280,22 @@ __device__ __forceinline__ float apply_pro(const Pro& p, const float* src, int64
   }
 }
 
-__global__ void k_x_r2c(AxisSched s, const float2* __restrict__ twm, const float2* __restrict__ twp,
-                        const float* __restrict__ src, float2* __restrict__ dst, int64_t nrows, int rowlen,
-                        int lo, int hi, int H, int RPB, Pro pro) {
-  extern __shared__ float2 lds[];
+// rows [row0, row0 + RPB) of the volume; tid -> (row, t)
+__device__ __forceinline__ void x_r2c_body(const AxisSched& s, const float2* __restrict__ twm,
+                                           const float2* __restrict__ twp, const float* __restrict__ src,
+                                           float2* __restrict__ dst, int64_t nrows, int rowlen, int lo, int hi,
+                                           int H, const Pro& pro, int64_t row0, int RPB, float2* lds) {
   const int m = s.n;
-  const int T = blockDim.x / RPB;
-  const int rl = threadIdx.x / T;
-  const int t = threadIdx.x % T;
-  const int64_t row = (int64_t)blockIdx.x * RPB + rl;
-  const bool ok = row < nrows;
+  const int T = threads_per_line_d(m);
+  int rl = threadIdx.x / T;
+  int t = threadIdx.x % T;
+  const bool active = rl < RPB;
+  if (!active) {
+    rl = 0;
+    t = m;
+  }
+  const int64_t row = row0 + rl;
+  const bool ok = active && row < nrows;
   const int P2 = 2 * m;
   const int shift = P2 - lo - hi;
   const float* sr = src + row * rowlen;
@@ -294,7 +311,8 @@ __global__ void k_x_r2c(AxisSched s, const float2* __restrict__ twm, const float
   auto lld = [&](int e) { return lrow[e]; };
   auto lst = [&](int e, float2 v) { lrow[e] = v; };
   if (s.np == 0) {
-    for (int e = t; e < m; e += T) lrow[e] = gld(e);
+    if (active)
+      for (int e = t; e < m; e += T) lrow[e] = gld(e);
   } else {
     for (int p = 0; p < s.np; ++p) {
       const int R = s.radix[p], P = s.span[p];
@@ -357,22 +375,30 @@ __device__ double block_sum(double v, double* scratch) {
   return r;
 }
 
-__global__ void k_x_c2r(AxisSched s, const float2* __restrict__ twm, const float2* __restrict__ twp,
-                        const float2* __restrict__ src, float* dst, int64_t nrows, int s2, int H, int RPB,
-                        Epi epi) {
-  extern __shared__ float2 lds[];
+// pidx: index of this workgroup's partial sum within the volume
+__device__ __forceinline__ void x_c2r_body(const AxisSched& s, const float2* __restrict__ twm,
+                                           const float2* __restrict__ twp, const float2* __restrict__ src,
+                                           float* dst, int64_t nrows, int s2, int H, const Epi& epi, int64_t row0,
+                                           int RPB, int64_t pidx, float2* lds) {
   const int m = s.n;
-  const int T = blockDim.x / RPB;
-  const int rl = threadIdx.x / T;
-  const int t = threadIdx.x % T;
-  const int64_t row = (int64_t)blockIdx.x * RPB + rl;
-  const bool ok = row < nrows;
+  const int T = threads_per_line_d(m);
+  int rl = threadIdx.x / T;
+  int t = threadIdx.x % T;
+  const bool active = rl < RPB;
+  if (!active) {
+    rl = 0;
+    t = m;
+  }
+  const int64_t row = row0 + rl;
+  const bool ok = active && row < nrows;
   float2* lrow = lds + rl * (m + 1);
-  if (ok) {
-    const float2* sr = src + row * H;
-    for (int k = t; k < H; k += T) lrow[k] = sr[k];
-  } else {
-    for (int k = t; k < H; k += T) lrow[k] = make_float2(0.f, 0.f);
+  if (active) {
+    if (ok) {
+      const float2* sr = src + row * H;
+      for (int k = t; k < H; k += T) lrow[k] = sr[k];
+    } else {
+      for (int k = t; k < H; k += T) lrow[k] = make_float2(0.f, 0.f);
+    }
   }
   __syncthreads();
   // Z'_q = (X_q + conj X_{m-q}) + i exp(+2 pi i q / P2) (X_q - conj X_{m-q});
@@ -421,8 +447,45 @@ __global__ void k_x_c2r(AxisSched s, const float2* __restrict__ twm, const float
   if (epi.partials) {
     __syncthreads();
     double tot = block_sum(acc, reinterpret_cast<double*>(lds));
-    if (threadIdx.x == 0) epi.partials[blockIdx.x] = tot;
+    if (threadIdx.x == 0) epi.partials[pidx] = tot;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Single-volume kernels
+// ---------------------------------------------------------------------------
+
+template <int DIR>
+__global__ void k_col_fft(AxisSched s, const float2* __restrict__ tw, const float2* __restrict__ src,
+                          float2* __restrict__ dst, int64_t ncols, int64_t src_b, int64_t dst_b, int64_t src_rs,
+                          int64_t dst_rs, int lo, int hi, int n_out, float scale, int L) {
+  extern __shared__ float2 lds[];
+  // XCD-aware tile order: consecutive column tiles (which share cache lines)
+  // go to the same XCD and its L2.
+  const int64_t nb = (int64_t)gridDim.x * gridDim.y;
+  const int64_t lin = xcd_remap((int64_t)blockIdx.y * gridDim.x + blockIdx.x, nb);
+  const int64_t tile = lin % gridDim.x, batch = lin / gridDim.x;
+  col_fft_body<DIR>(s, tw, src + batch * src_b, dst + batch * dst_b, ncols, src_rs, dst_rs, lo, hi, n_out, scale,
+                    tile, L, lds);
+}
+
+__global__ void k_z_conv(AxisSched f, AxisSched iv, const float2* __restrict__ tw, float2* buf,
+                         const float2* __restrict__ otf, int64_t ncols, int lo, int n_out, int conj, int L) {
+  extern __shared__ float2 lds[];
+  z_conv_body(f, iv, tw, buf, otf, ncols, lo, n_out, conj, xcd_remap(blockIdx.x, gridDim.x), L, lds);
+}
+
+__global__ void k_x_r2c(AxisSched s, const float2* __restrict__ twm, const float2* __restrict__ twp,
+                        const float* __restrict__ src, float2* __restrict__ dst, int64_t nrows, int rowlen, int lo,
+                        int hi, int H, int RPB, Pro pro) {
+  extern __shared__ float2 lds[];
+  x_r2c_body(s, twm, twp, src, dst, nrows, rowlen, lo, hi, H, pro, (int64_t)blockIdx.x * RPB, RPB, lds);
+}
+
+__global__ void k_x_c2r(AxisSched s, const float2* __restrict__ twm, const float2* __restrict__ twp,
+                        const float2* __restrict__ src, float* dst, int64_t nrows, int s2, int H, int RPB, Epi epi) {
+  extern __shared__ float2 lds[];
+  x_c2r_body(s, twm, twp, src, dst, nrows, s2, H, epi, (int64_t)blockIdx.x * RPB, RPB, blockIdx.x, lds);
 }
 
 __global__ void k_sum_partials(const double* __restrict__ p, int64_t n, double* out) {
@@ -434,10 +497,77 @@ __global__ void k_sum_partials(const double* __restrict__ p, int64_t n, double* 
 }
 
 // ---------------------------------------------------------------------------
+// Batched kernels: one launch per pass for many volumes of different shapes
+// ---------------------------------------------------------------------------
+
+// Volume of logical block b: prefix[v] <= b < prefix[v + 1].
+__device__ __forceinline__ int find_vol(const int64_t* __restrict__ prefix, int nvol, int64_t b) {
+  int lo = 0, hi = nvol - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void k_batch_pass(int pass, const BatchVol* __restrict__ vols, const int64_t* __restrict__ prefix,
+                             int nvol) {
+  extern __shared__ float2 lds[];
+  const int64_t nb = gridDim.x;
+  const int64_t b = xcd_remap(blockIdx.x, nb);
+  const int v = find_vol(prefix, nvol, b);
+  const BatchVol& d = vols[v];
+  const int64_t lb = b - prefix[v];
+  const int tile = d.tile[pass];
+  const int64_t H = d.H;
+  switch (pass) {
+    case PASS_X_FWD:
+      x_r2c_body(d.xf, d.tw_xm, d.tw_xp, d.in, d.B1, (int64_t)d.s0 * d.s1, d.s2, d.s2, 0, d.H, d.pro, lb * tile,
+                 tile, lds);
+      break;
+    case PASS_Y_FWD: {
+      const int64_t ntile = (H + tile - 1) / tile;
+      const int64_t z = lb / ntile;
+      col_fft_body<-1>(d.yf, d.tw_y, d.B1 + z * d.s1 * H, d.B2 + z * d.P1 * H, H, H, H, d.s1, 0, d.P1, 1.0f,
+                       lb % ntile, tile, lds);
+      break;
+    }
+    case PASS_Z:
+      z_conv_body(d.zf, d.zi, d.tw_z, d.P1 > 1 ? d.B2 : d.B1, d.otf, (int64_t)d.P1 * H, d.s0, d.s0, d.conj, lb, tile,
+                  lds);
+      break;
+    case PASS_Y_INV: {
+      const int64_t ntile = (H + tile - 1) / tile;
+      const int64_t z = lb / ntile;
+      col_fft_body<1>(d.yi, d.tw_y, d.B2 + z * d.P1 * H, d.B1 + z * d.s1 * H, H, H, H, d.P1, 0, d.s1, 1.0f,
+                      lb % ntile, tile, lds);
+      break;
+    }
+    default:
+      x_c2r_body(d.xi, d.tw_xm, d.tw_xp, d.B1, d.out, (int64_t)d.s0 * d.s1, d.s2, d.H, d.epi, lb * tile, tile, lb,
+                 lds);
+      break;
+  }
+}
+
+// Per-volume final sums of the X c2r partials: one workgroup per volume.
+__global__ void k_batch_sums(const BatchVol* __restrict__ vols, const int64_t* __restrict__ prefix) {
+  __shared__ double scratch[16];
+  const BatchVol& d = vols[blockIdx.x];
+  if (!d.epi.partials || !d.reduce_out) return;
+  const int64_t n = prefix[blockIdx.x + 1] - prefix[blockIdx.x];
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) acc += d.epi.partials[i];
+  double r = block_sum(acc, scratch);
+  if (threadIdx.x == 0) d.reduce_out[0] = r;
+}
+
+// ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
 
-static int threads_per_line(int n) { return (n + kEPT - 1) / kEPT; }
+int threads_per_line(int n) { return (n + kEPT - 1) / kEPT; }
 
 // Largest LDS a workgroup may take on gfx950 (160 KiB per CU).
 constexpr int64_t kMaxLds = 160 * 1024;
@@ -451,6 +581,7 @@ static void allow_big_lds() {
   (void)hipFuncSetAttribute((const void*)k_z_conv, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
   (void)hipFuncSetAttribute((const void*)k_x_r2c, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
   (void)hipFuncSetAttribute((const void*)k_x_c2r, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
+  (void)hipFuncSetAttribute((const void*)k_batch_pass, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
 }
 
 bool col_tile_ok(int n, int L) {
@@ -510,13 +641,8 @@ hipError_t launch_col_fft(int dir, const AxisSched& s, const float2* tw, const f
 
 hipError_t launch_z_conv(const AxisSched& f, const AxisSched& iv, const float2* tw, float2* buf,
                          const float2* otf, int64_t ncols, int lo, int n_out, int conj, int tile, hipStream_t st) {
-  if (f.n == 1) {
-    dim3 grid((unsigned)((ncols + 255) / 256));
-    hipLaunchKernelGGL(k_z_mul, grid, dim3(256), 0, st, buf, otf, ncols, conj);
-    return hipGetLastError();
-  }
   allow_big_lds();
-  ColLaunch c = col_launch_for(f.n, ncols, tile);
+  ColLaunch c = f.n == 1 ? ColLaunch{256, 1} : col_launch_for(f.n, ncols, tile);
   dim3 grid((unsigned)((ncols + c.L - 1) / c.L));
   dim3 block(c.L * c.T);
   size_t shm = (size_t)f.n * c.L * sizeof(float2);
@@ -553,6 +679,20 @@ hipError_t launch_x_c2r(const AxisSched& s, const float2* twm, const float2* twp
 
 hipError_t launch_sum_partials(const double* partials, int64_t n, double* out, hipStream_t st) {
   hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, st, partials, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_pass(int pass, const BatchVol* vols_dev, const int64_t* prefix_dev, int nvol,
+                             int64_t nblocks, int threads, size_t lds_bytes, hipStream_t st) {
+  allow_big_lds();
+  if (lds_bytes < 16 * sizeof(double)) lds_bytes = 16 * sizeof(double);
+  hipLaunchKernelGGL(k_batch_pass, dim3((unsigned)nblocks), dim3(threads), lds_bytes, st, pass, vols_dev,
+                     prefix_dev, nvol);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_sums(const BatchVol* vols_dev, const int64_t* prefix_dev, int nvol, hipStream_t st) {
+  hipLaunchKernelGGL(k_batch_sums, dim3(nvol), dim3(256), 0, st, vols_dev, prefix_dev);
   return hipGetLastError();
 }
 

@@ -84,4 +84,32 @@ hipError_t launch_x_c2r(const AxisSched& s, const float2* twm, const float2* twp
 // Deterministic final reduction of n partials into out[0] (sum).
 hipError_t launch_sum_partials(const double* partials, int64_t n, double* out, hipStream_t st);
 
+int threads_per_line(int n);
+
+// ---- batched (many volumes, one launch per pass) ----
+enum PassId { PASS_X_FWD = 0, PASS_Y_FWD = 1, PASS_Z = 2, PASS_Y_INV = 3, PASS_X_INV = 4, PASS_N = 5 };
+
+// Everything one volume's convolution application needs, resident on the device.
+struct BatchVol {
+  AxisSched xf, xi, yf, yi, zf, zi;
+  const float2* tw_z;
+  const float2* tw_y;
+  const float2* tw_xm;
+  const float2* tw_xp;
+  const float2* otf;
+  float2* B1;
+  float2* B2;
+  const float* in;
+  float* out;
+  Pro pro;
+  Epi epi;            // epi.partials: this volume's X c2r partials
+  double* reduce_out; // this volume's epilogue sum (may be null)
+  int s0, s1, s2, P1, H, conj;
+  int tile[PASS_N];   // columns (Y, Z) or rows (X) per workgroup
+};
+
+hipError_t launch_batch_pass(int pass, const BatchVol* vols_dev, const int64_t* prefix_dev, int nvol,
+                             int64_t nblocks, int threads, size_t lds_bytes, hipStream_t st);
+hipError_t launch_batch_sums(const BatchVol* vols_dev, const int64_t* prefix_dev, int nvol, hipStream_t st);
+
 }  // namespace dh

@@ -106,6 +106,29 @@ int dh_conv_apply(const dh_conv_plan* plan, int adjoint, const float* in, float*
                   const void* otf_dev, void* work_dev, int64_t work_bytes,
                   const dh_fuse_args* fuse, void* stream);
 
+/* ---- batched application: many plans (different shapes) in one launch per pass ---- */
+typedef struct dh_batch dh_batch;
+typedef struct {
+  const dh_conv_plan* plan;
+  const void* otf;
+  void* work;                /* this volume's own workspace (volumes run concurrently) */
+  int64_t work_bytes;
+  const float* in;
+  float* out;
+  dh_fuse_args fuse;         /* per-volume prologue/epilogue; reduce_out per volume */
+} dh_batch_item;
+/* Device descriptor bytes for up to max_items volumes (caller allocates). */
+int64_t dh_batch_desc_bytes(int max_items);
+/* Host object with pinned staging for the per-call descriptors. */
+int dh_batch_create(int max_items, dh_batch** out);
+int dh_batch_destroy(dh_batch* batch);
+/* Applies every item's convolution (adjoint == 0) or correlation (adjoint == 1)
+ * with its fused prologue/epilogue: 5 launches for the whole batch, each
+ * covering the tiles of all volumes (the same results as one dh_conv_apply per
+ * item).  Serving and batched RL call this once per half-iteration. */
+int dh_batch_conv_apply(dh_batch* batch, int nitems, const dh_batch_item* items, int adjoint, void* desc_dev,
+                        int64_t desc_bytes, void* stream);
+
 /* ---- reductions and elementwise steps (rl_mlx.py:123-150) ---- */
 #define DH_RED_SUM 0
 #define DH_RED_MAX 1
