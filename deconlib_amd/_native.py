@@ -57,6 +57,18 @@ class FuseArgs(C.Structure):
     ]
 
 
+class BatchItem(C.Structure):
+    _fields_ = [
+        ("plan", C.c_void_p),
+        ("otf", C.c_void_p),
+        ("work", C.c_void_p),
+        ("work_bytes", C.c_int64),
+        ("input", C.c_void_p),
+        ("out", C.c_void_p),
+        ("fuse", FuseArgs),
+    ]
+
+
 class PdParams(C.Structure):
     _fields_ = [
         ("delta", C.c_double),
@@ -111,6 +123,10 @@ SYMBOLS = {
     "dh_pd_scalars": (_int, _vp, _vp, _int, C.POINTER(PdParams), _vp),
     "dh_hessian_forward": (_int, _vp, _vp, _int, _int, _int, C.POINTER(_f), _vp),
     "dh_hessian_adjoint": (_int, _vp, _vp, _int, _int, _int, C.POINTER(_f), _vp),
+    "dh_batch_desc_bytes": (_i64, _int),
+    "dh_batch_create": (_int, _int, C.POINTER(_vp)),
+    "dh_batch_destroy": (_int, _vp),
+    "dh_batch_conv_apply": (_int, _vp, _int, C.POINTER(BatchItem), _int, _vp, _i64, _vp),
     "dh_timing_enable": (_int, _int),
     "dh_timing_read": (_int, C.POINTER(_d), C.POINTER(_i64), _int),
 }
@@ -125,7 +141,8 @@ if fns["dh_abi_version"]() != ABI_VERSION:
     )
 
 
-_VALUE_RETURNING = {"dh_reduce_partials", "dh_abi_version", "dh_pd_partials", "dh_conv_plan_get_tile"}
+_VALUE_RETURNING = {"dh_reduce_partials", "dh_abi_version", "dh_pd_partials", "dh_conv_plan_get_tile",
+                    "dh_batch_desc_bytes"}
 
 
 class NativeError(RuntimeError):

@@ -131,7 +131,113 @@ struct dh_conv_plan {
   int64_t b1_off = 0, b2_off = 0, part_off = 0, scal_off = 0, ws_bytes = 0;
   int64_t n_partials = 0;
   int tile[DH_PASS_COUNT] = {0, 0, 0, 0, 0};
+  bool ct = false;  // every axis length has compile-time specialised kernels
 };
+
+namespace {
+
+// ---- launch shapes of the specialised kernels ----
+constexpr int kCtMaxThreads = 512;
+constexpr int64_t kCtDefaultLds = 64 * 1024;
+constexpr int64_t kCtMaxLds = 160 * 1024;
+
+bool ct_col_tile_ok(int n, int L) {
+  return L >= 1 && L * dh::ct_threads(n) <= kCtMaxThreads && (int64_t)L * n * 8 <= kCtMaxLds;
+}
+bool ct_row_tile_ok(int m, int R) {
+  return R >= 1 && R * dh::ct_threads(m) <= kCtMaxThreads && (int64_t)R * (m + 1) * 8 <= kCtMaxLds;
+}
+// columns per workgroup: widest power of two within 512 threads and 64 KiB LDS
+int ct_col_tile(int n, int64_t ncols, int over) {
+  if (over > 0 && ct_col_tile_ok(n, over)) return over;
+  const int T = dh::ct_threads(n);
+  int L = 1;
+  while (L < 64 && 2 * L * T <= kCtMaxThreads && (int64_t)2 * L * n * 8 <= kCtDefaultLds) L *= 2;
+  while (L > 1 && L / 2 >= ncols) L /= 2;
+  return L;
+}
+int ct_row_tile(int m, int over) {
+  if (over > 0 && ct_row_tile_ok(m, over)) return over;
+  const int T = dh::ct_threads(m);
+  int R = 1;
+  while (R < 64 && 2 * R * T <= 256 && (int64_t)2 * R * (m + 1) * 8 <= kCtDefaultLds) R *= 2;
+  return R;
+}
+
+template <class A>
+void pack_one(dh::Pack<A>& pk, const A& a, int64_t nblk) {
+  pk.a[0] = a;
+  pk.prefix[0] = 0;
+  pk.prefix[1] = nblk;
+  pk.nvol = 1;
+}
+
+// One application with the specialised kernels.
+int ct_apply(const dh_conv_plan* p, int adjoint, const float* in, float* out, const void* otf, char* ws,
+             const dh::Pro& pro, const dh::Epi& epi, int64_t* xi_blocks, hipStream_t st) {
+  float2* B1 = (float2*)(ws + p->b1_off);
+  float2* B2 = (float2*)(ws + p->b2_off);
+  const int s0 = p->s[0], s1 = p->s[1], s2 = p->s[2];
+  const int64_t H = p->H;
+  const int64_t nrows = (int64_t)s0 * s1;
+  const float2* tw = p->tw_dev;
+  {
+    const int R = ct_row_tile(p->M, p->tile[DH_PASS_X_FWD]);
+    dh::Pack<dh::XfArgs> pk;
+    const int64_t nb = (nrows + R - 1) / R;
+    pack_one(pk, dh::XfArgs{in, B1, tw + p->tw_xm, tw + p->tw_xp, nrows, s2, s2, 0, (int)H, pro}, nb);
+    DH_TRY(timing_begin(DH_PASS_X_FWD, st));
+    DH_TRY(hip_check(dh::ct_launch(dh::CT_XF, p->M, -1, &pk, R, nb, st), "x forward pass"));
+    DH_TRY(timing_end(st));
+  }
+  float2* zb = B1;
+  if (p->P[1] > 1) {
+    const int L = ct_col_tile(p->P[1], H, p->tile[DH_PASS_Y_FWD]);
+    const int64_t nt = (H + L - 1) / L;
+    dh::Pack<dh::ColArgs> pk;
+    pack_one(pk, dh::ColArgs{B1, B2, tw + p->tw_y, H, (int64_t)s1 * H, (int64_t)p->P[1] * H, H, H, nt, s1, 0,
+                             p->P[1], 1.0f},
+             nt * s0);
+    DH_TRY(timing_begin(DH_PASS_Y_FWD, st));
+    DH_TRY(hip_check(dh::ct_launch(dh::CT_COL, p->P[1], -1, &pk, L, nt * s0, st), "y forward pass"));
+    DH_TRY(timing_end(st));
+    zb = B2;
+  }
+  {
+    const int64_t ncols = (int64_t)p->P[1] * H;
+    const int L = p->P[0] == 1 ? 256 : ct_col_tile(p->P[0], ncols, p->tile[DH_PASS_Z]);
+    const int64_t nb = (ncols + L - 1) / L;
+    dh::Pack<dh::ZArgs> pk;
+    pack_one(pk, dh::ZArgs{zb, (const float2*)otf, tw + p->tw_z, ncols, s0, s0, adjoint ? 1 : 0}, nb);
+    DH_TRY(timing_begin(DH_PASS_Z, st));
+    DH_TRY(hip_check(dh::ct_launch(dh::CT_Z, p->P[0], 0, &pk, L, nb, st), "z pass"));
+    DH_TRY(timing_end(st));
+  }
+  if (p->P[1] > 1) {
+    const int L = ct_col_tile(p->P[1], H, p->tile[DH_PASS_Y_INV]);
+    const int64_t nt = (H + L - 1) / L;
+    dh::Pack<dh::ColArgs> pk;
+    pack_one(pk, dh::ColArgs{B2, B1, tw + p->tw_y, H, (int64_t)p->P[1] * H, (int64_t)s1 * H, H, H, nt, p->P[1], 0,
+                             s1, 1.0f},
+             nt * s0);
+    DH_TRY(timing_begin(DH_PASS_Y_INV, st));
+    DH_TRY(hip_check(dh::ct_launch(dh::CT_COL, p->P[1], 1, &pk, L, nt * s0, st), "y inverse pass"));
+    DH_TRY(timing_end(st));
+  }
+  {
+    const int R = ct_row_tile(p->M, p->tile[DH_PASS_X_INV]);
+    const int64_t nb = (nrows + R - 1) / R;
+    dh::Pack<dh::XiArgs> pk;
+    pack_one(pk, dh::XiArgs{B1, out, tw + p->tw_xm, tw + p->tw_xp, nrows, s2, (int)H, epi}, nb);
+    DH_TRY(timing_begin(DH_PASS_X_INV, st));
+    DH_TRY(hip_check(dh::ct_launch(dh::CT_XI, p->M, 1, &pk, R, nb, st), "x inverse pass"));
+    DH_TRY(timing_end(st));
+    *xi_blocks = nb;
+  }
+  return DH_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -180,15 +286,25 @@ int dh_conv_plan_create(int ndim, const int64_t* signal_shape, const int64_t* ke
     p->fft[a] = f;
   }
   // Canvas this plan computes on: any canvas >= n + m - 1 gives the same
-  // linear convolution, so non-5-smooth or odd x extents are rounded up to
-  // the next even 5-smooth length (the real-to-complex pass needs even x).
+  // linear convolution.  Prefer a length with compile-time specialised
+  // kernels (at most 1.5x the 5-smooth minimum), else the next 5-smooth
+  // length; the x extent must be even (real-to-complex pass).
   for (int a = 0; a < 3; ++a) {
     int c = p->fft[a];
     if (!circular) {
-      c = next_smooth(c);
+      int sm = next_smooth(c);
       if (a == 2) {
-        while (c % 2 != 0 || !is_smooth(c)) ++c;
+        while (sm % 2 != 0 || !is_smooth(sm)) ++sm;
       }
+      int best = sm;
+      for (int cand = c; cand <= sm + sm / 2; ++cand) {
+        const bool ok = (a == 2) ? (cand % 2 == 0 && dh::ct_supported(cand / 2)) : dh::ct_supported(cand);
+        if (ok) {
+          best = cand;
+          break;
+        }
+      }
+      c = best;
     } else if (!is_smooth(c) || (a == 2 && c % 2 != 0 && c != 1)) {
       delete p;
       return fail(DH_EUNSUPPORTED, "circular plan needs 5-smooth extents and an even last axis");
@@ -218,6 +334,7 @@ int dh_conv_plan_create(int ndim, const int64_t* signal_shape, const int64_t* ke
   }
   p->M = p->P[2] / 2;
   p->H = p->M + 1;
+  p->ct = dh::ct_supported(p->P[0]) && dh::ct_supported(p->P[1]) && dh::ct_supported(p->M);
   if (!make_sched(p->P[0], p->zf, p->zi) || !make_sched(p->P[1], p->yf, p->yi) ||
       !make_sched(p->M, p->xf, p->xi)) {
     delete p;
@@ -279,8 +396,9 @@ int dh_conv_plan_set_tile(dh_conv_plan* p, int pass, int tile) {
   if (tile > 0) {
     const bool row = (pass == DH_PASS_X_FWD || pass == DH_PASS_X_INV);
     const int n = row ? p->M : (pass == DH_PASS_Z ? p->P[0] : p->P[1]);
-    if (row ? !dh::row_tile_ok(n, tile) : !dh::col_tile_ok(n, tile))
-      return fail(DH_EINVAL, "tile does not fit the workgroup limits (1024 threads, 160 KiB LDS)");
+    const bool ok = p->ct ? (row ? ct_row_tile_ok(n, tile) : ct_col_tile_ok(n, tile))
+                          : (row ? dh::row_tile_ok(n, tile) : dh::col_tile_ok(n, tile));
+    if (!ok) return fail(DH_EINVAL, "tile does not fit the workgroup limits (threads, 160 KiB LDS)");
   }
   p->tile[pass] = tile;
   return DH_OK;
@@ -290,8 +408,9 @@ int dh_conv_plan_get_tile(const dh_conv_plan* p, int pass) {
   if (!p || pass < 0 || pass >= DH_PASS_COUNT) return -1;
   const bool row = (pass == DH_PASS_X_FWD || pass == DH_PASS_X_INV);
   const int n = row ? p->M : (pass == DH_PASS_Z ? p->P[0] : p->P[1]);
-  if (row) return dh::row_launch_for(n, p->tile[pass]).R;
   const int64_t ncols = (pass == DH_PASS_Z) ? (int64_t)p->P[1] * p->H : p->H;
+  if (p->ct) return row ? ct_row_tile(n, p->tile[pass]) : ct_col_tile(n, ncols, p->tile[pass]);
+  if (row) return dh::row_launch_for(n, p->tile[pass]).R;
   return dh::col_launch_for(n, ncols, p->tile[pass]).L;
 }
 
@@ -319,21 +438,50 @@ int dh_otf_build(const dh_conv_plan* p, const float* kernel_dev, int normalize, 
     pro.dptr = scal;
   }
   const int64_t H = p->H;
-  DH_TRY(hip_check(dh::launch_x_r2c(p->xf, p->tw_dev + p->tw_xm, p->tw_dev + p->tw_xp, kernel_dev, B1,
-                                    (int64_t)p->k[0] * p->k[1], p->k[2], p->klo[2], p->khi[2], p->H, pro, 0, st),
-                   "otf x pass"));
-  float2* zsrc = B1;
-  if (p->P[1] > 1) {
-    DH_TRY(hip_check(dh::launch_col_fft(-1, p->yf, p->tw_dev + p->tw_y, B1, B2, H, p->k[0], (int64_t)p->k[1] * H,
-                                        (int64_t)p->P[1] * H, H, H, p->klo[1], p->khi[1], p->P[1], 1.0f, 0, st),
-                     "otf y pass"));
-    zsrc = B2;
-  }
   const double total = (double)p->P[0] * p->P[1] * p->P[2];
   const int64_t ncols = (int64_t)p->P[1] * H;
-  DH_TRY(hip_check(dh::launch_col_fft(-1, p->zf, p->tw_dev + p->tw_z, zsrc, (float2*)otf_dev, ncols, 1, 0, 0,
-                                      ncols, ncols, p->klo[0], p->khi[0], p->P[0], (float)(1.0 / total), 0, st),
-                   "otf z pass"));
+  const int64_t krows = (int64_t)p->k[0] * p->k[1];
+  float2* zsrc = B1;
+  if (p->ct) {
+    const float2* tw = p->tw_dev;
+    const int R = ct_row_tile(p->M, 0);
+    const int64_t nbx = (krows + R - 1) / R;
+    dh::Pack<dh::XfArgs> px;
+    pack_one(px, dh::XfArgs{kernel_dev, B1, tw + p->tw_xm, tw + p->tw_xp, krows, p->k[2], p->klo[2], p->khi[2],
+                            (int)H, pro},
+             nbx);
+    DH_TRY(hip_check(dh::ct_launch(dh::CT_XF, p->M, -1, &px, R, nbx, st), "otf x pass"));
+    if (p->P[1] > 1) {
+      const int L = ct_col_tile(p->P[1], H, 0);
+      const int64_t nt = (H + L - 1) / L;
+      dh::Pack<dh::ColArgs> py;
+      pack_one(py, dh::ColArgs{B1, B2, tw + p->tw_y, H, (int64_t)p->k[1] * H, (int64_t)p->P[1] * H, H, H, nt,
+                               p->klo[1], p->khi[1], p->P[1], 1.0f},
+               nt * p->k[0]);
+      DH_TRY(hip_check(dh::ct_launch(dh::CT_COL, p->P[1], -1, &py, L, nt * p->k[0], st), "otf y pass"));
+      zsrc = B2;
+    }
+    const int L = ct_col_tile(p->P[0], ncols, 0);
+    const int64_t nt = (ncols + L - 1) / L;
+    dh::Pack<dh::ColArgs> pz;
+    pack_one(pz, dh::ColArgs{zsrc, (float2*)otf_dev, tw + p->tw_z, ncols, 0, 0, ncols, ncols, nt, p->klo[0],
+                             p->khi[0], p->P[0], (float)(1.0 / total)},
+             nt);
+    DH_TRY(hip_check(dh::ct_launch(dh::CT_COL, p->P[0], -1, &pz, L, nt, st), "otf z pass"));
+  } else {
+    DH_TRY(hip_check(dh::launch_x_r2c(p->xf, p->tw_dev + p->tw_xm, p->tw_dev + p->tw_xp, kernel_dev, B1, krows,
+                                      p->k[2], p->klo[2], p->khi[2], p->H, pro, 0, st),
+                     "otf x pass"));
+    if (p->P[1] > 1) {
+      DH_TRY(hip_check(dh::launch_col_fft(-1, p->yf, p->tw_dev + p->tw_y, B1, B2, H, p->k[0], (int64_t)p->k[1] * H,
+                                          (int64_t)p->P[1] * H, H, H, p->klo[1], p->khi[1], p->P[1], 1.0f, 0, st),
+                       "otf y pass"));
+      zsrc = B2;
+    }
+    DH_TRY(hip_check(dh::launch_col_fft(-1, p->zf, p->tw_dev + p->tw_z, zsrc, (float2*)otf_dev, ncols, 1, 0, 0,
+                                        ncols, ncols, p->klo[0], p->khi[0], p->P[0], (float)(1.0 / total), 0, st),
+                     "otf z pass"));
+  }
   if (norm_sq_host) {
     DH_TRY(hip_check(dh::launch_cmax_abs2((const float2*)otf_dev, (int64_t)p->P[0] * ncols, part, scal + 1, st),
                      "otf norm"));
@@ -377,6 +525,15 @@ int dh_conv_apply(const dh_conv_plan* p, int adjoint, const float* in, float* ou
   }
   if (!in && pro.kind != DH_PRO_ONES) return fail(DH_EINVAL, "null input");
   hipStream_t st = (hipStream_t)stream;
+  if (p->ct) {
+    int64_t nb = 0;
+    DH_TRY(ct_apply(p, adjoint, in, out, otf_dev, ws, pro, epi, &nb, st));
+    if (reduce_out) {
+      if (nb > p->n_partials) return fail(DH_EWORKSPACE, "partials area too small");
+      DH_TRY(hip_check(dh::launch_sum_partials(epi.partials, nb, reduce_out, st), "epilogue reduction"));
+    }
+    return DH_OK;
+  }
   float2* B1 = (float2*)(ws + p->b1_off);
   float2* B2 = (float2*)(ws + p->b2_off);
   const int s0 = p->s[0], s1 = p->s[1], s2 = p->s[2];
@@ -547,6 +704,129 @@ int fill_fuse(const dh_fuse_args* fuse, dh::Pro& pro, dh::Epi& epi, double** red
   return DH_OK;
 }
 
+// Batched application with the specialised kernels: per pass, the volumes are
+// grouped by transform length and launch shape, and each group of up to kPack
+// volumes is one launch with its descriptors in the kernel arguments.
+template <class A>
+struct Group {
+  int n = 0, tile = 0;
+  std::vector<A> args;
+  std::vector<int64_t> nblk;
+};
+
+template <class A>
+int launch_groups(int kind, int dir, std::vector<Group<A>>& groups, int pass, hipStream_t st) {
+  for (auto& g : groups) {
+    for (size_t s = 0; s < g.args.size(); s += dh::kPack) {
+      dh::Pack<A> pk;
+      pk.nvol = (int)std::min<size_t>(dh::kPack, g.args.size() - s);
+      pk.prefix[0] = 0;
+      for (int v = 0; v < pk.nvol; ++v) {
+        pk.a[v] = g.args[s + v];
+        pk.prefix[v + 1] = pk.prefix[v] + g.nblk[s + v];
+      }
+      if (pk.prefix[pk.nvol] == 0) continue;
+      DH_TRY(timing_begin(pass, st));
+      DH_TRY(hip_check(dh::ct_launch(kind, g.n, dir, &pk, g.tile, pk.prefix[pk.nvol], st), "batched pass"));
+      DH_TRY(timing_end(st));
+    }
+  }
+  return DH_OK;
+}
+
+template <class A>
+void add_to_group(std::vector<Group<A>>& groups, int n, int tile, const A& a, int64_t nblk) {
+  for (auto& g : groups) {
+    if (g.n == n && g.tile == tile) {
+      g.args.push_back(a);
+      g.nblk.push_back(nblk);
+      return;
+    }
+  }
+  Group<A> g;
+  g.n = n;
+  g.tile = tile;
+  g.args.push_back(a);
+  g.nblk.push_back(nblk);
+  groups.push_back(std::move(g));
+}
+
+int ct_batch_apply(int nitems, const dh_batch_item* items, int adjoint, hipStream_t st) {
+  std::vector<Group<dh::XfArgs>> gxf;
+  std::vector<Group<dh::ColArgs>> gyf, gyi;
+  std::vector<Group<dh::ZArgs>> gz;
+  std::vector<Group<dh::XiArgs>> gxi;
+  struct Red {
+    double* partials;
+    int64_t n;
+    double* out;
+  };
+  std::vector<Red> reds;
+  for (int i = 0; i < nitems; ++i) {
+    const dh_batch_item& it = items[i];
+    const dh_conv_plan* p = it.plan;
+    if (!it.otf || !it.work || !it.out) return fail(DH_EINVAL, "null batch item field");
+    if (!p->tw_dev) return fail(DH_EINVAL, "plan is not bound (dh_conv_plan_bind)");
+    if (it.work_bytes < p->ws_bytes) return fail(DH_EWORKSPACE, "workspace too small");
+    dh::Pro pro;
+    dh::Epi epi;
+    double* red = nullptr;
+    DH_TRY(fill_fuse(&it.fuse, pro, epi, &red));
+    if (!it.in && pro.kind != DH_PRO_ONES) return fail(DH_EINVAL, "null input");
+    char* ws = (char*)it.work;
+    float2* B1 = (float2*)(ws + p->b1_off);
+    float2* B2 = (float2*)(ws + p->b2_off);
+    const float2* tw = p->tw_dev;
+    const int s0 = p->s[0], s1 = p->s[1], s2 = p->s[2];
+    const int64_t H = p->H, nrows = (int64_t)s0 * s1;
+    {
+      const int R = ct_row_tile(p->M, p->tile[DH_PASS_X_FWD]);
+      add_to_group(gxf, p->M, R, dh::XfArgs{it.in, B1, tw + p->tw_xm, tw + p->tw_xp, nrows, s2, s2, 0, (int)H, pro},
+                   (nrows + R - 1) / R);
+    }
+    float2* zb = B1;
+    if (p->P[1] > 1) {
+      const int L = ct_col_tile(p->P[1], H, p->tile[DH_PASS_Y_FWD]);
+      const int64_t nt = (H + L - 1) / L;
+      add_to_group(gyf, p->P[1], L,
+                   dh::ColArgs{B1, B2, tw + p->tw_y, H, (int64_t)s1 * H, (int64_t)p->P[1] * H, H, H, nt, s1, 0,
+                               p->P[1], 1.0f},
+                   nt * s0);
+      const int Li = ct_col_tile(p->P[1], H, p->tile[DH_PASS_Y_INV]);
+      const int64_t nti = (H + Li - 1) / Li;
+      add_to_group(gyi, p->P[1], Li,
+                   dh::ColArgs{B2, B1, tw + p->tw_y, H, (int64_t)p->P[1] * H, (int64_t)s1 * H, H, H, nti, p->P[1],
+                               0, s1, 1.0f},
+                   nti * s0);
+      zb = B2;
+    }
+    {
+      const int64_t ncols = (int64_t)p->P[1] * H;
+      const int L = p->P[0] == 1 ? 256 : ct_col_tile(p->P[0], ncols, p->tile[DH_PASS_Z]);
+      add_to_group(gz, p->P[0], L, dh::ZArgs{zb, (const float2*)it.otf, tw + p->tw_z, ncols, s0, s0, adjoint ? 1 : 0},
+                   (ncols + L - 1) / L);
+    }
+    {
+      const int R = ct_row_tile(p->M, p->tile[DH_PASS_X_INV]);
+      const int64_t nb = (nrows + R - 1) / R;
+      if (red) {
+        if (nb > p->n_partials) return fail(DH_EWORKSPACE, "partials area too small");
+        epi.partials = (double*)(ws + p->part_off);
+        reds.push_back(Red{epi.partials, nb, red});
+      }
+      add_to_group(gxi, p->M, R, dh::XiArgs{B1, it.out, tw + p->tw_xm, tw + p->tw_xp, nrows, s2, (int)H, epi}, nb);
+    }
+  }
+  DH_TRY(launch_groups(dh::CT_XF, -1, gxf, DH_PASS_X_FWD, st));
+  DH_TRY(launch_groups(dh::CT_COL, -1, gyf, DH_PASS_Y_FWD, st));
+  DH_TRY(launch_groups(dh::CT_Z, 0, gz, DH_PASS_Z, st));
+  DH_TRY(launch_groups(dh::CT_COL, 1, gyi, DH_PASS_Y_INV, st));
+  DH_TRY(launch_groups(dh::CT_XI, 1, gxi, DH_PASS_X_INV, st));
+  for (const Red& r : reds)
+    DH_TRY(hip_check(dh::launch_sum_partials(r.partials, r.n, r.out, st), "batched epilogue reduction"));
+  return DH_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -588,6 +868,9 @@ int dh_batch_conv_apply(dh_batch* b, int nitems, const dh_batch_item* items, int
   if (nitems > b->max_items) return fail(DH_EINVAL, "more items than the batch was created for");
   if (desc_bytes < dh_batch_desc_bytes(b->max_items)) return fail(DH_EWORKSPACE, "descriptor buffer too small");
   hipStream_t st = (hipStream_t)stream;
+  bool all_ct = true;
+  for (int i = 0; i < nitems; ++i) all_ct = all_ct && items[i].plan && items[i].plan->ct;
+  if (all_ct) return ct_batch_apply(nitems, items, adjoint, st);
   // the staging buffers may still feed the previous upload
   if (b->pending) DH_TRY(hip_check(hipEventSynchronize(b->uploaded), "batch staging wait"));
   b->pending = false;

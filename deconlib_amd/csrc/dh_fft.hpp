@@ -86,6 +86,59 @@ hipError_t launch_sum_partials(const double* partials, int64_t n, double* out, h
 
 int threads_per_line(int n);
 
+// ---- compile-time specialised passes (dh_fftct*.hpp): per-volume arguments ----
+struct ColArgs {  // Y forward/inverse, OTF Z: columns of (batch, n, ncols) planes
+  const float2* src;
+  float2* dst;
+  const float2* tw;
+  int64_t ncols, src_b, dst_b, src_rs, dst_rs, ntiles;
+  int lo, hi, n_out;
+  float scale;
+};
+struct ZArgs {  // fused Z: forward, x OTF, inverse, in place
+  float2* buf;
+  const float2* otf;
+  const float2* tw;
+  int64_t ncols;
+  int lo, n_out, conj;
+};
+struct XfArgs {  // X real-to-complex rows
+  const float* src;
+  float2* dst;
+  const float2* twm;
+  const float2* twp;
+  int64_t nrows;
+  int rowlen, lo, hi, H;
+  Pro pro;
+};
+struct XiArgs {  // X complex-to-real rows with the epilogue
+  const float2* src;
+  float* dst;
+  const float2* twm;
+  const float2* twp;
+  int64_t nrows;
+  int s2, H;
+  Epi epi;
+};
+
+// Up to kPack volumes of one transform length per launch, passed by value.
+constexpr int kPack = 16;
+template <class A>
+struct Pack {
+  A a[kPack];
+  int64_t prefix[kPack + 1];  // first logical block of each volume
+  int nvol;
+};
+
+enum CtKind { CT_COL = 0, CT_Z = 1, CT_XF = 2, CT_XI = 3 };
+// True if length n has specialised kernels.
+bool ct_supported(int n);
+// Threads per line of the specialised kernels for length n.
+int ct_threads(int n);
+// Launches kind for length n; pk points to the matching Pack<...>.
+// tile = columns (CT_COL, CT_Z) or rows (CT_XF, CT_XI) per workgroup.
+hipError_t ct_launch(int kind, int n, int dir, const void* pk, int tile, int64_t nblocks, hipStream_t st);
+
 // ---- batched (many volumes, one launch per pass) ----
 enum PassId { PASS_X_FWD = 0, PASS_Y_FWD = 1, PASS_Z = 2, PASS_Y_INV = 3, PASS_X_INV = 4, PASS_N = 5 };
 

@@ -1,0 +1,140 @@
+// Compile-time specialised FFT passes: one instantiation per transform length.
+//
+// For a length N the radix schedule, the span of every pass, the threads per
+// line T = N / R_max and each thread's butterfly count are constants, so every
+// register array is indexed statically and the kernel carries exactly the
+// registers its passes need (the runtime-length kernels in dh_fft.hip keep
+// every radix's code live at once).  Bodies are shared by the single-volume
+// and batched entry points.
+#pragma once
+#include "dh_dft.hpp"
+#include "dh_fft.hpp"
+
+namespace dh {
+namespace ct {
+
+struct Sched {
+  int np;
+  int r[12];
+};
+
+// 16s first, then 8/4/2, then 5s and 3s (the same rule as the runtime plans).
+constexpr Sched make_sched(int n) {
+  Sched s{0, {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
+  int m = n, c2 = 0, c3 = 0, c5 = 0;
+  while (m % 2 == 0) m /= 2, ++c2;
+  while (m % 3 == 0) m /= 3, ++c3;
+  while (m % 5 == 0) m /= 5, ++c5;
+  int n16 = c2 / 4, rem = c2 % 4;
+  if (rem == 1 && n16 > 0) {
+    --n16;
+    for (int i = 0; i < n16; ++i) s.r[s.np++] = 16;
+    s.r[s.np++] = 8;
+    s.r[s.np++] = 4;
+  } else {
+    for (int i = 0; i < n16; ++i) s.r[s.np++] = 16;
+    if (rem == 3) s.r[s.np++] = 8;
+    if (rem == 2) s.r[s.np++] = 4;
+    if (rem == 1) s.r[s.np++] = 2;
+  }
+  for (int i = 0; i < c5; ++i) s.r[s.np++] = 5;
+  for (int i = 0; i < c3; ++i) s.r[s.np++] = 3;
+  return s;
+}
+
+constexpr Sched reversed(Sched s) {
+  Sched o{s.np, {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
+  for (int i = 0; i < s.np; ++i) o.r[i] = s.r[s.np - 1 - i];
+  return o;
+}
+
+constexpr int span_of(Sched s, int p) {
+  int v = 1;
+  for (int i = 0; i < p; ++i) v *= s.r[i];
+  return v;
+}
+
+constexpr int rmax(Sched s) {
+  int v = 1;
+  for (int i = 0; i < s.np; ++i) v = s.r[i] > v ? s.r[i] : v;
+  return v;
+}
+
+template <int N>
+struct Geo {
+  static constexpr Sched F = make_sched(N);
+  static constexpr Sched I = reversed(make_sched(N));
+  static constexpr int T = N / rmax(make_sched(N));  // threads per line
+};
+
+// One Stockham pass with everything compile-time; t < T.
+template <int N, int T, int R, int P, int DIR, class Ld, class St>
+__device__ __forceinline__ void pass(int t, const float2* __restrict__ tw, Ld ld, St st, bool sync_mid) {
+  constexpr int NB = N / R;
+  constexpr int M = (NB + T - 1) / T;
+  float2 u[M][R];
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    const int i = t + m * T;
+    if ((M * T == NB) || i < NB) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) u[m][r] = ld(i + r * NB);
+    }
+  }
+  if (sync_mid) __syncthreads();
+  constexpr int step = N / (P * R);
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    const int i = t + m * T;
+    if ((M * T == NB) || i < NB) {
+      const int k = i % P;
+      if (P > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          float2 w = tw[r * k * step];
+          if (DIR > 0) w.y = -w.y;
+          u[m][r] = cmul(u[m][r], w);
+        }
+      }
+      Dft<R, DIR>::run(u[m]);
+      const int j = (i / P) * P * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) st(j + r * P, u[m][r]);
+    }
+  }
+}
+
+// Passes [p, S.np) of schedule S; the first uses gld, the last gst, the rest
+// LDS.  LAST_LDS: gst writes the same LDS the last pass reads, so that pass
+// needs the barrier between its loads and stores.
+template <int N, bool INV, int DIR, int p, bool LAST_LDS = false, class GLd, class GSt, class LLd, class LSt>
+__device__ __forceinline__ void passes(int t, const float2* __restrict__ tw, GLd gld, GSt gst, LLd lld, LSt lst,
+                                       bool first_from_global) {
+  constexpr Sched S = INV ? Geo<N>::I : Geo<N>::F;
+  if constexpr (p < S.np) {
+    constexpr int R = S.r[p];
+    constexpr int P = span_of(S, p);
+    constexpr int T = Geo<N>::T;
+    constexpr bool last = (p == S.np - 1);
+    const bool first = (p == 0) && first_from_global;
+    if (first && last) pass<N, T, R, P, DIR>(t, tw, gld, gst, false);
+    else if (first) pass<N, T, R, P, DIR>(t, tw, gld, lst, false);
+    else if (last) pass<N, T, R, P, DIR>(t, tw, lld, gst, LAST_LDS);
+    else pass<N, T, R, P, DIR>(t, tw, lld, lst, true);
+    if (!last) __syncthreads();
+    passes<N, INV, DIR, p + 1, LAST_LDS>(t, tw, gld, gst, lld, lst, first_from_global);
+  }
+}
+
+// Full line transform: global -> (LDS ...) -> global.
+template <int N, int DIR, class GLd, class GSt, class LLd, class LSt>
+__device__ __forceinline__ void line(int t, const float2* __restrict__ tw, GLd gld, GSt gst, LLd lld, LSt lst) {
+  if constexpr (N == 1) {
+    if (t == 0) gst(0, gld(0));
+  } else {
+    passes<N, false, DIR, 0>(t, tw, gld, gst, lld, lst, true);
+  }
+}
+
+}  // namespace ct
+}  // namespace dh

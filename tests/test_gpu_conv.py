@@ -29,6 +29,8 @@ SHAPES = [
     ((250,), (31,)),                   # 1-D
     ((3, 5, 6), (3, 5, 6)),            # tiny
     ((40, 96, 96), (40, 96, 96)),      # canvas 80x192x192 (radix 3 and 5 on every axis)
+    ((7000,), (3001,)),                # canvas 10000: no specialised length, runtime-length kernels
+    ((5000, 6), (1201, 3)),            # y canvas 6250: runtime-length column pass
 ]
 
 
