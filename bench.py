@@ -60,6 +60,64 @@ def make_inputs(torch):
     return psf, obs, blur
 
 
+def bench_pdhg(torch, N, obs, blur, steps, warmup):
+    """PDHG-Hessian (alpha 1e-3, L1, Poisson) iterations on the same volume and operator."""
+    from deconlib_amd.pdhg import _PDState, _nonneg
+
+    x0 = _nonneg(torch.from_numpy(obs).cuda())
+    st = _PDState(obs, blur, x0, 1e-3, warmup + steps, background=0.0, r=1.0, tol=0.0)
+    for k in range(warmup):
+        st.step(k, False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(warmup, warmup + steps):
+        st.step(k, False)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    vox = int(np.prod(obs.shape))
+    return {
+        "metric": "voxel-iters/s (PDHG-Hessian) at 64x512x512 fp32",
+        "value": vox * steps / dt,
+        "ms_per_step": dt * 1e3 / steps,
+        "steps": steps,
+        "warmup": warmup,
+        "finite": bool(torch.isfinite(st.x).all().item()),
+        "tau_last": float(st.hist[3 * (warmup + steps - 1)].item()),
+    }
+
+
+def bench_serving(torch, n_jobs, rate, seed=1003):
+    """Serving sample: Poisson arrivals of mixed RL (100 it) / PDHG-Hessian (200 it) jobs,
+    sizes log-uniform 16x64x64 .. 48x256x256; continuous batching vs the plain one-job loop."""
+    from deconlib_amd.serving import DeconvServer, latency_stats, run_plain
+    from deconlib_amd.synth import serving_requests
+
+    from deconlib_amd import LinearFFTConvolver
+
+    def gpu_blur(psf, obj):
+        return LinearFFTConvolver(psf, obj.shape).forward(obj.astype(np.float32)).cpu().numpy()
+
+    t0 = time.time()
+    reqs = serving_requests(n_jobs, rate, seed, blur=gpu_blur)
+    log(f"serving: {n_jobs} requests synthesised in {time.time() - t0:.1f}s")
+    warm = serving_requests(2, 1000.0, seed + 1, blur=gpu_blur)
+    DeconvServer().run(warm)
+    srv = DeconvServer()
+    t0 = time.perf_counter()
+    done = srv.run(reqs)
+    makespan = time.perf_counter() - t0
+    stats = latency_stats(done, makespan)
+    plain_s = run_plain(reqs)
+    vox_it = sum(int(np.prod(r.observed.shape)) * r.num_iter for r in reqs)
+    stats.update({
+        "workload": f"{n_jobs} jobs, Poisson arrivals {rate}/s, 70% RL 100 it / 30% PDHG-Hessian 200 it, "
+                    "volumes log-uniform 16x64x64..48x256x256, per-job aberrated PSFs",
+        "plain_loop_voxel_iters_per_s": vox_it / plain_s,
+        "speedup_vs_plain_loop": (stats["voxel_iters_per_s"] * plain_s / vox_it) if vox_it else None,
+    })
+    return stats
+
+
 def cpu_baseline(psf, obs, iters):
     """The oracle's RL iteration (fp32 NumPy/SciPy restatement) timed on the host cores."""
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
@@ -98,8 +156,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-iters", type=int, default=2)
+    ap.add_argument("--cpu-iters", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serving-jobs", type=int, default=40)
+    ap.add_argument("--serving-rate", type=float, default=40.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
@@ -142,6 +202,12 @@ def main():
     z = per_pass["z_conv"]
     log(f"per-pass: {json.dumps(per_pass)}")
     ok = bool(torch.isfinite(st.x).all().item())
+    pdhg = bench_pdhg(torch, N, obs, blur, args.steps, args.warmup)
+    log(f"pdhg: {json.dumps(pdhg)}")
+    serving = None
+    if args.serving_jobs > 0:
+        serving = bench_serving(torch, args.serving_jobs, args.serving_rate)
+        log(f"serving: {json.dumps(serving)}")
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -180,6 +246,8 @@ def main():
             "avg_ms": z["avg_ms"],
         },
         "passes": per_pass,
+        "pdhg": pdhg,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

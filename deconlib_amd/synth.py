@@ -51,6 +51,48 @@ def rl_workload(shape=(64, 256, 256), n_blobs=200, seed=1000, lateral=0.085, axi
     return psf.astype(np.float32), obj.astype(np.float32), obs
 
 
+def aberrated_psf(shape, rng: np.random.Generator, lateral=0.085, axial=0.1):
+    """Config-0 optics with IndexMismatch depth ~ U[0, 20] um and Zernike spherical ~ U[-0.5, 0.5] rad."""
+    from .psf import IndexMismatch, ZernikeAberration
+
+    ab = [IndexMismatch(depth=float(rng.uniform(0.0, 20.0))),
+          ZernikeAberration({ZernikeAberration.SPHERICAL: float(rng.uniform(-0.5, 0.5))})]
+    return widefield_psf(shape, lateral, axial, CONFIG0_OPTICS, aberrations=ab)
+
+
+def serving_requests(n_jobs: int, rate_per_s: float, seed: int, z_range=(16, 48), xy_range=(64, 256),
+                     rl_fraction=0.7, rl_iters=100, pdhg_iters=200, density=200 / (64 * 256 * 256),
+                     blur=None):
+    """Poisson arrivals of mixed RL / PDHG-Hessian jobs with log-uniform volume sizes.
+
+    Sizes are rounded to multiples of 8; each job has its own aberrated PSF,
+    a blob object at the config-0 blob density and a Poisson observation.
+    ``blur(psf, obj)`` computes the noiseless image (default: host float64
+    FFT).  Returns a list of ``serving.Request``.
+    """
+    from .serving import Request
+
+    blur = blur or host_linear_blur
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for j in range(n_jobs):
+        t += float(rng.exponential(1.0 / rate_per_s))
+        z = int(round(np.exp(rng.uniform(np.log(z_range[0]), np.log(z_range[1]))) / 8)) * 8
+        xy = int(round(np.exp(rng.uniform(np.log(xy_range[0]), np.log(xy_range[1]))) / 8)) * 8
+        shape = (max(z, 8), max(xy, 8), max(xy, 8))
+        psf = aberrated_psf(shape, rng)
+        obj = blob_volume(shape, max(1, int(round(density * np.prod(shape)))), rng)
+        clean = np.maximum(blur(psf, obj), 0.0)
+        obs = rng.poisson(clean).astype(np.float32)
+        if rng.uniform() < rl_fraction:
+            out.append(Request(j, t, "rl", obs, psf, num_iter=rl_iters))
+        else:
+            alpha = float(np.exp(rng.uniform(np.log(1e-4), np.log(1e-2))))
+            out.append(Request(j, t, "pdhg", obs, psf, num_iter=pdhg_iters, alpha=alpha))
+    return out
+
+
 def host_linear_blur(psf: np.ndarray, obj: np.ndarray) -> np.ndarray:
     """Wrap-free convolution of obj with a corner-origin psf (float64, numpy FFT)."""
     s = obj.shape
