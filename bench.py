@@ -158,6 +158,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-iters", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--serving-jobs", type=int, default=40)
     ap.add_argument("--serving-rate", type=float, default=40.0)
     args = ap.parse_args()
@@ -172,6 +173,13 @@ def main():
     from deconlib_amd.rl import _RLState
 
     psf, obs, blur = make_inputs(torch)
+    tiles = {}
+    if not args.no_tune:
+        from deconlib_amd.autotune import autotune
+
+        t0 = time.time()
+        tiles = autotune(blur, verbose=True)
+        log(f"autotune {time.time() - t0:.1f}s: {tiles}")
     st = _RLState(obs, blur, background=0.0, num_iter=args.warmup + args.steps + 1, eval_interval=10)
     for k in range(args.warmup):
         st.step(k)
@@ -232,6 +240,7 @@ def main():
             "volume": list(SHAPE),
             "canvas": list(P),
             "solver": "richardson_lucy_with_operator (fused)",
+            "tiles": {N.PASS_NAMES[p]: t for p, t in tiles.items()},
             "finite": ok,
         },
         "roofline": {

@@ -21,7 +21,7 @@ if not _LIB_PATH.exists():
 
 lib = C.CDLL(str(_LIB_PATH))
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 DH_OK, DH_EINVAL, DH_EHIP, DH_EWORKSPACE, DH_EUNSUPPORTED = 0, -1, -2, -3, -4
 
@@ -114,7 +114,7 @@ SYMBOLS = {
     "dh_box_copy": (_int, _vp, C.POINTER(_i64), _vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64),
                     C.POINTER(_i64), _vp),
     "dh_banded_apply": (_int, _vp, _vp, _i64, _i64, _i64, _i64, _vp, _vp, _int, _vp),
-    "dh_pd_partials": (_i64, _i64),
+    "dh_pd_partials": (_i64, _int, _int, _int),
     "dh_pd_y1": (_int, _vp, _vp, _vp, _vp, _i64, _vp, _f, _int, _vp),
     "dh_pd_dual": (_int, _vp, _vp, _vp, _int, _int, _int, _vp, C.POINTER(_f), _f, _int, _vp),
     "dh_pd_primal": (_int, _vp, _vp, _vp, _vp, _int, _int, _int, _vp, C.POINTER(_f), _vp, _vp),
@@ -123,10 +123,7 @@ SYMBOLS = {
     "dh_pd_scalars": (_int, _vp, _vp, _int, C.POINTER(PdParams), _vp),
     "dh_hessian_forward": (_int, _vp, _vp, _int, _int, _int, C.POINTER(_f), _vp),
     "dh_hessian_adjoint": (_int, _vp, _vp, _int, _int, _int, C.POINTER(_f), _vp),
-    "dh_batch_desc_bytes": (_i64, _int),
-    "dh_batch_create": (_int, _int, C.POINTER(_vp)),
-    "dh_batch_destroy": (_int, _vp),
-    "dh_batch_conv_apply": (_int, _vp, _int, C.POINTER(BatchItem), _int, _vp, _i64, _vp),
+    "dh_batch_conv_apply": (_int, _int, C.POINTER(BatchItem), _int, _vp),
     "dh_timing_enable": (_int, _int),
     "dh_timing_read": (_int, C.POINTER(_d), C.POINTER(_i64), _int),
 }
@@ -141,8 +138,7 @@ if fns["dh_abi_version"]() != ABI_VERSION:
     )
 
 
-_VALUE_RETURNING = {"dh_reduce_partials", "dh_abi_version", "dh_pd_partials", "dh_conv_plan_get_tile",
-                    "dh_batch_desc_bytes"}
+_VALUE_RETURNING = {"dh_reduce_partials", "dh_abi_version", "dh_pd_partials", "dh_conv_plan_get_tile"}
 
 
 class NativeError(RuntimeError):

@@ -75,11 +75,6 @@ class BatchRunner:
 
     def __init__(self, max_items: int = 256):
         self.max_items = int(max_items)
-        h = C.c_void_p()
-        N.call("dh_batch_create", self.max_items, C.byref(h))
-        self._h = h
-        nbytes = int(N.fns["dh_batch_desc_bytes"](self.max_items))
-        self.desc = torch.empty(nbytes, dtype=torch.uint8, device=device())
 
     def step(self, jobs: List[RLJob]) -> None:
         """One RL iteration for every job in ``jobs`` (all not yet done)."""
@@ -88,16 +83,9 @@ class BatchRunner:
             chunk = live[s:s + self.max_items]
             for forward in (True, False):
                 arr = (N.BatchItem * len(chunk))(*[j._item(forward) for j in chunk])
-                N.call("dh_batch_conv_apply", self._h, len(chunk), arr, 0 if forward else 1,
-                       self.desc.data_ptr(), self.desc.numel(), N.stream_ptr())
+                N.call("dh_batch_conv_apply", len(chunk), arr, 0 if forward else 1, N.stream_ptr())
         for j in live:
             j.k += 1
-
-    def __del__(self):
-        h = getattr(self, "_h", None)
-        if h is not None and h.value:
-            N.fns["dh_batch_destroy"](h)
-            self._h = None
 
 
 def batched_richardson_lucy(jobs: List[RLJob], runner: Optional[BatchRunner] = None) -> List[RLResult]:

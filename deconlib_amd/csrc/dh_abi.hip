@@ -164,6 +164,13 @@ int ct_row_tile(int m, int over) {
   return R;
 }
 
+bool aligned8(const void* p) { return ((uintptr_t)p & 7) == 0; }
+// float2 row access is valid when every row starts on an 8-byte boundary
+int xf_vec2(const float* src, int rowlen) { return (src && rowlen % 2 == 0 && aligned8(src)) ? 1 : 0; }
+int xi_vec2(const float* dst, int s2, const dh::Epi& e) {
+  return (s2 % 2 == 0 && aligned8(dst) && aligned8(e.aux0) && aligned8(e.aux1)) ? 1 : 0;
+}
+
 template <class A>
 void pack_one(dh::Pack<A>& pk, const A& a, int64_t nblk) {
   pk.a[0] = a;
@@ -185,7 +192,8 @@ int ct_apply(const dh_conv_plan* p, int adjoint, const float* in, float* out, co
     const int R = ct_row_tile(p->M, p->tile[DH_PASS_X_FWD]);
     dh::Pack<dh::XfArgs> pk;
     const int64_t nb = (nrows + R - 1) / R;
-    pack_one(pk, dh::XfArgs{in, B1, tw + p->tw_xm, tw + p->tw_xp, nrows, s2, s2, 0, (int)H, pro}, nb);
+    pack_one(pk, dh::XfArgs{in, B1, tw + p->tw_xm, tw + p->tw_xp, nrows, s2, s2, 0, (int)H, pro, xf_vec2(in, s2)},
+             nb);
     DH_TRY(timing_begin(DH_PASS_X_FWD, st));
     DH_TRY(hip_check(dh::ct_launch(dh::CT_XF, p->M, -1, &pk, R, nb, st), "x forward pass"));
     DH_TRY(timing_end(st));
@@ -228,7 +236,8 @@ int ct_apply(const dh_conv_plan* p, int adjoint, const float* in, float* out, co
     const int R = ct_row_tile(p->M, p->tile[DH_PASS_X_INV]);
     const int64_t nb = (nrows + R - 1) / R;
     dh::Pack<dh::XiArgs> pk;
-    pack_one(pk, dh::XiArgs{B1, out, tw + p->tw_xm, tw + p->tw_xp, nrows, s2, (int)H, epi}, nb);
+    pack_one(pk, dh::XiArgs{B1, out, tw + p->tw_xm, tw + p->tw_xp, nrows, s2, (int)H, epi, xi_vec2(out, s2, epi)},
+             nb);
     DH_TRY(timing_begin(DH_PASS_X_INV, st));
     DH_TRY(hip_check(dh::ct_launch(dh::CT_XI, p->M, 1, &pk, R, nb, st), "x inverse pass"));
     DH_TRY(timing_end(st));
@@ -448,7 +457,7 @@ int dh_otf_build(const dh_conv_plan* p, const float* kernel_dev, int normalize, 
     const int64_t nbx = (krows + R - 1) / R;
     dh::Pack<dh::XfArgs> px;
     pack_one(px, dh::XfArgs{kernel_dev, B1, tw + p->tw_xm, tw + p->tw_xp, krows, p->k[2], p->klo[2], p->khi[2],
-                            (int)H, pro},
+                            (int)H, pro, 0},
              nbx);
     DH_TRY(hip_check(dh::ct_launch(dh::CT_XF, p->M, -1, &px, R, nbx, st), "otf x pass"));
     if (p->P[1] > 1) {
@@ -578,7 +587,10 @@ int dh_conv_apply(const dh_conv_plan* p, int adjoint, const float* in, float* ou
   return DH_OK;
 }
 
-int64_t dh_pd_partials(int64_t n) { return dh::pd_partials_needed(n); }
+int64_t dh_pd_partials(int nz, int ny, int nx) {
+  if (nz < 1 || ny < 1 || nx < 1) return 0;
+  return dh::pd_partials_needed(nz, ny, nx);
+}
 
 int dh_pd_y1(float* y1, const float* ax, const float* axp, const float* data, int64_t n, const double* scalars,
              float background, int loss_type, void* stream) {
@@ -664,17 +676,7 @@ int dh_timing_read(double* ms, int64_t* counts, int reset) {
   return DH_OK;
 }
 
-// ---- batched application over many plans ----
-
 }  // extern "C"
-
-struct dh_batch {
-  int max_items = 0;
-  dh::BatchVol* host_vols = nullptr;  // pinned staging
-  int64_t* host_prefix = nullptr;     // pinned staging, PASS_N * (max_items + 1)
-  hipEvent_t uploaded = nullptr;
-  bool pending = false;
-};
 
 namespace {
 
@@ -781,7 +783,8 @@ int ct_batch_apply(int nitems, const dh_batch_item* items, int adjoint, hipStrea
     const int64_t H = p->H, nrows = (int64_t)s0 * s1;
     {
       const int R = ct_row_tile(p->M, p->tile[DH_PASS_X_FWD]);
-      add_to_group(gxf, p->M, R, dh::XfArgs{it.in, B1, tw + p->tw_xm, tw + p->tw_xp, nrows, s2, s2, 0, (int)H, pro},
+      add_to_group(gxf, p->M, R,
+                   dh::XfArgs{it.in, B1, tw + p->tw_xm, tw + p->tw_xp, nrows, s2, s2, 0, (int)H, pro, xf_vec2(it.in, s2)},
                    (nrows + R - 1) / R);
     }
     float2* zb = B1;
@@ -814,7 +817,9 @@ int ct_batch_apply(int nitems, const dh_batch_item* items, int adjoint, hipStrea
         epi.partials = (double*)(ws + p->part_off);
         reds.push_back(Red{epi.partials, nb, red});
       }
-      add_to_group(gxi, p->M, R, dh::XiArgs{B1, it.out, tw + p->tw_xm, tw + p->tw_xp, nrows, s2, (int)H, epi}, nb);
+      add_to_group(gxi, p->M, R,
+                   dh::XiArgs{B1, it.out, tw + p->tw_xm, tw + p->tw_xp, nrows, s2, (int)H, epi, xi_vec2(it.out, s2, epi)},
+                   nb);
     }
   }
   DH_TRY(launch_groups(dh::CT_XF, -1, gxf, DH_PASS_X_FWD, st));
@@ -831,137 +836,22 @@ int ct_batch_apply(int nitems, const dh_batch_item* items, int adjoint, hipStrea
 
 extern "C" {
 
-int64_t dh_batch_desc_bytes(int max_items) {
-  if (max_items < 1) return 0;
-  return (int64_t)max_items * (int64_t)sizeof(dh::BatchVol) + (int64_t)dh::PASS_N * (max_items + 1) * 8;
-}
-
-int dh_batch_create(int max_items, dh_batch** out) {
-  if (!out || max_items < 1) return fail(DH_EINVAL, "bad batch_create arguments");
-  auto* b = new dh_batch();
-  b->max_items = max_items;
-  if (hipHostMalloc((void**)&b->host_vols, sizeof(dh::BatchVol) * max_items, hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void**)&b->host_prefix, 8 * dh::PASS_N * (max_items + 1), hipHostMallocDefault) != hipSuccess ||
-      hipEventCreateWithFlags(&b->uploaded, hipEventDisableTiming) != hipSuccess) {
-    if (b->host_vols) (void)hipHostFree(b->host_vols);
-    if (b->host_prefix) (void)hipHostFree(b->host_prefix);
-    delete b;
-    return fail(DH_EHIP, "pinned staging allocation failed");
-  }
-  *out = b;
-  return DH_OK;
-}
-
-int dh_batch_destroy(dh_batch* b) {
-  if (!b) return DH_OK;
-  if (b->pending) (void)hipEventSynchronize(b->uploaded);
-  (void)hipEventDestroy(b->uploaded);
-  (void)hipHostFree(b->host_vols);
-  (void)hipHostFree(b->host_prefix);
-  delete b;
-  return DH_OK;
-}
-
-int dh_batch_conv_apply(dh_batch* b, int nitems, const dh_batch_item* items, int adjoint, void* desc_dev,
-                        int64_t desc_bytes, void* stream) {
-  if (!b || !items || !desc_dev || nitems < 1) return fail(DH_EINVAL, "bad batch arguments");
-  if (nitems > b->max_items) return fail(DH_EINVAL, "more items than the batch was created for");
-  if (desc_bytes < dh_batch_desc_bytes(b->max_items)) return fail(DH_EWORKSPACE, "descriptor buffer too small");
+int dh_batch_conv_apply(int nitems, const dh_batch_item* items, int adjoint, void* stream) {
+  if (!items || nitems < 1) return fail(DH_EINVAL, "bad batch arguments");
+  for (int i = 0; i < nitems; ++i)
+    if (!items[i].plan) return fail(DH_EINVAL, "null plan in batch item");
   hipStream_t st = (hipStream_t)stream;
-  bool all_ct = true;
-  for (int i = 0; i < nitems; ++i) all_ct = all_ct && items[i].plan && items[i].plan->ct;
-  if (all_ct) return ct_batch_apply(nitems, items, adjoint, st);
-  // the staging buffers may still feed the previous upload
-  if (b->pending) DH_TRY(hip_check(hipEventSynchronize(b->uploaded), "batch staging wait"));
-  b->pending = false;
-  const int stride = b->max_items + 1;
-  int threads[dh::PASS_N] = {0, 0, 0, 0, 0};
-  int64_t lds[dh::PASS_N] = {0, 0, 0, 0, 0};
-  for (int p = 0; p < dh::PASS_N; ++p) b->host_prefix[p * stride] = 0;
+  // specialised lengths: grouped launches; the rest: one application each
+  std::vector<dh_batch_item> cts;
   for (int i = 0; i < nitems; ++i) {
     const dh_batch_item& it = items[i];
-    const dh_conv_plan* p = it.plan;
-    if (!p || !it.otf || !it.work || !it.out) return fail(DH_EINVAL, "null batch item field");
-    if (!p->tw_dev) return fail(DH_EINVAL, "plan is not bound (dh_conv_plan_bind)");
-    if (it.work_bytes < p->ws_bytes) return fail(DH_EWORKSPACE, "workspace too small");
-    dh::BatchVol& d = b->host_vols[i];
-    double* red = nullptr;
-    DH_TRY(fill_fuse(&it.fuse, d.pro, d.epi, &red));
-    if (!it.in && d.pro.kind != DH_PRO_ONES) return fail(DH_EINVAL, "null input");
-    char* ws = (char*)it.work;
-    d.xf = p->xf;
-    d.xi = p->xi;
-    d.yf = p->yf;
-    d.yi = p->yi;
-    d.zf = p->zf;
-    d.zi = p->zi;
-    d.tw_z = p->tw_dev + p->tw_z;
-    d.tw_y = p->tw_dev + p->tw_y;
-    d.tw_xm = p->tw_dev + p->tw_xm;
-    d.tw_xp = p->tw_dev + p->tw_xp;
-    d.otf = (const float2*)it.otf;
-    d.B1 = (float2*)(ws + p->b1_off);
-    d.B2 = (float2*)(ws + p->b2_off);
-    d.in = it.in;
-    d.out = it.out;
-    d.epi.partials = red ? (double*)(ws + p->part_off) : nullptr;
-    d.reduce_out = red;
-    d.s0 = p->s[0];
-    d.s1 = p->s[1];
-    d.s2 = p->s[2];
-    d.P1 = p->P[1];
-    d.H = p->H;
-    d.conj = adjoint ? 1 : 0;
-    const int64_t nrows = (int64_t)p->s[0] * p->s[1];
-    const int64_t H = p->H;
-    int64_t nblk[dh::PASS_N];
-    for (int q = 0; q < dh::PASS_N; ++q) {
-      const bool row = (q == dh::PASS_X_FWD || q == dh::PASS_X_INV);
-      if (row) {
-        dh::RowLaunch r = dh::row_launch_for(p->M, p->tile[q]);
-        d.tile[q] = r.R;
-        nblk[q] = (nrows + r.R - 1) / r.R;
-        threads[q] = std::max(threads[q], r.R * r.T);
-        lds[q] = std::max<int64_t>(lds[q], (int64_t)r.R * (p->M + 1) * 8);
-      } else if (q == dh::PASS_Z) {
-        const int64_t ncols = (int64_t)p->P[1] * H;
-        dh::ColLaunch c = p->P[0] == 1 ? dh::ColLaunch{256, 1} : dh::col_launch_for(p->P[0], ncols, p->tile[q]);
-        d.tile[q] = c.L;
-        nblk[q] = (ncols + c.L - 1) / c.L;
-        threads[q] = std::max(threads[q], c.L * c.T);
-        lds[q] = std::max<int64_t>(lds[q], (int64_t)c.L * p->P[0] * 8);
-      } else {
-        dh::ColLaunch c = dh::col_launch_for(p->P[1], H, p->tile[q]);
-        d.tile[q] = c.L;
-        nblk[q] = p->P[1] > 1 ? ((H + c.L - 1) / c.L) * p->s[0] : 0;
-        if (nblk[q]) {
-          threads[q] = std::max(threads[q], c.L * c.T);
-          lds[q] = std::max<int64_t>(lds[q], (int64_t)c.L * p->P[1] * 8);
-        }
-      }
-      b->host_prefix[q * stride + i + 1] = b->host_prefix[q * stride + i] + nblk[q];
+    if (it.plan->ct) {
+      cts.push_back(it);
+    } else {
+      DH_TRY(dh_conv_apply(it.plan, adjoint, it.in, it.out, it.otf, it.work, it.work_bytes, &it.fuse, stream));
     }
   }
-  char* dd = (char*)desc_dev;
-  dh::BatchVol* vols_dev = (dh::BatchVol*)dd;
-  int64_t* prefix_dev = (int64_t*)(dd + (int64_t)b->max_items * sizeof(dh::BatchVol));
-  DH_TRY(hip_check(hipMemcpyAsync(vols_dev, b->host_vols, sizeof(dh::BatchVol) * nitems, hipMemcpyHostToDevice, st),
-                   "batch descriptor upload"));
-  DH_TRY(hip_check(hipMemcpyAsync(prefix_dev, b->host_prefix, 8 * dh::PASS_N * stride, hipMemcpyHostToDevice, st),
-                   "batch prefix upload"));
-  DH_TRY(hip_check(hipEventRecord(b->uploaded, st), "batch upload event"));
-  b->pending = true;
-  for (int q = 0; q < dh::PASS_N; ++q) {
-    const int64_t total = b->host_prefix[q * stride + nitems];
-    if (total == 0) continue;
-    DH_TRY(timing_begin(q, st));
-    DH_TRY(hip_check(dh::launch_batch_pass(q, vols_dev, prefix_dev + q * stride, nitems, total, threads[q],
-                                           (size_t)lds[q], st),
-                     "batched pass"));
-    DH_TRY(timing_end(st));
-  }
-  DH_TRY(hip_check(dh::launch_batch_sums(vols_dev, prefix_dev + dh::PASS_X_INV * stride, nitems, st),
-                   "batched sums"));
+  if (!cts.empty()) DH_TRY(ct_batch_apply((int)cts.size(), cts.data(), adjoint, st));
   return DH_OK;
 }
 

@@ -12,10 +12,9 @@
 // directly (both coalesced: lanes walk consecutive columns or consecutive row
 // samples); the intermediate passes exchange through LDS.
 //
-// Each pass is a __device__ body over one tile (a group of columns or rows of
-// one volume).  The single-volume kernels run one volume per launch; the
-// batched kernels run the tiles of many volumes of different shapes in one
-// launch, finding each workgroup's volume from a prefix table.
+// These runtime-length kernels cover canvases without a compile-time
+// specialised length (dh_fftct*.hpp); each pass is a __device__ body over one
+// tile (a group of columns or rows of one volume).
 #include "dh_dft.hpp"
 #include "dh_fft.hpp"
 
@@ -497,73 +496,6 @@ __global__ void k_sum_partials(const double* __restrict__ p, int64_t n, double* 
 }
 
 // ---------------------------------------------------------------------------
-// Batched kernels: one launch per pass for many volumes of different shapes
-// ---------------------------------------------------------------------------
-
-// Volume of logical block b: prefix[v] <= b < prefix[v + 1].
-__device__ __forceinline__ int find_vol(const int64_t* __restrict__ prefix, int nvol, int64_t b) {
-  int lo = 0, hi = nvol - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (prefix[mid] <= b) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
-
-__global__ void k_batch_pass(int pass, const BatchVol* __restrict__ vols, const int64_t* __restrict__ prefix,
-                             int nvol) {
-  extern __shared__ float2 lds[];
-  const int64_t nb = gridDim.x;
-  const int64_t b = xcd_remap(blockIdx.x, nb);
-  const int v = find_vol(prefix, nvol, b);
-  const BatchVol& d = vols[v];
-  const int64_t lb = b - prefix[v];
-  const int tile = d.tile[pass];
-  const int64_t H = d.H;
-  switch (pass) {
-    case PASS_X_FWD:
-      x_r2c_body(d.xf, d.tw_xm, d.tw_xp, d.in, d.B1, (int64_t)d.s0 * d.s1, d.s2, d.s2, 0, d.H, d.pro, lb * tile,
-                 tile, lds);
-      break;
-    case PASS_Y_FWD: {
-      const int64_t ntile = (H + tile - 1) / tile;
-      const int64_t z = lb / ntile;
-      col_fft_body<-1>(d.yf, d.tw_y, d.B1 + z * d.s1 * H, d.B2 + z * d.P1 * H, H, H, H, d.s1, 0, d.P1, 1.0f,
-                       lb % ntile, tile, lds);
-      break;
-    }
-    case PASS_Z:
-      z_conv_body(d.zf, d.zi, d.tw_z, d.P1 > 1 ? d.B2 : d.B1, d.otf, (int64_t)d.P1 * H, d.s0, d.s0, d.conj, lb, tile,
-                  lds);
-      break;
-    case PASS_Y_INV: {
-      const int64_t ntile = (H + tile - 1) / tile;
-      const int64_t z = lb / ntile;
-      col_fft_body<1>(d.yi, d.tw_y, d.B2 + z * d.P1 * H, d.B1 + z * d.s1 * H, H, H, H, d.P1, 0, d.s1, 1.0f,
-                      lb % ntile, tile, lds);
-      break;
-    }
-    default:
-      x_c2r_body(d.xi, d.tw_xm, d.tw_xp, d.B1, d.out, (int64_t)d.s0 * d.s1, d.s2, d.H, d.epi, lb * tile, tile, lb,
-                 lds);
-      break;
-  }
-}
-
-// Per-volume final sums of the X c2r partials: one workgroup per volume.
-__global__ void k_batch_sums(const BatchVol* __restrict__ vols, const int64_t* __restrict__ prefix) {
-  __shared__ double scratch[16];
-  const BatchVol& d = vols[blockIdx.x];
-  if (!d.epi.partials || !d.reduce_out) return;
-  const int64_t n = prefix[blockIdx.x + 1] - prefix[blockIdx.x];
-  double acc = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) acc += d.epi.partials[i];
-  double r = block_sum(acc, scratch);
-  if (threadIdx.x == 0) d.reduce_out[0] = r;
-}
-
-// ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
 
@@ -581,7 +513,6 @@ static void allow_big_lds() {
   (void)hipFuncSetAttribute((const void*)k_z_conv, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
   (void)hipFuncSetAttribute((const void*)k_x_r2c, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
   (void)hipFuncSetAttribute((const void*)k_x_c2r, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
-  (void)hipFuncSetAttribute((const void*)k_batch_pass, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
 }
 
 bool col_tile_ok(int n, int L) {
@@ -679,20 +610,6 @@ hipError_t launch_x_c2r(const AxisSched& s, const float2* twm, const float2* twp
 
 hipError_t launch_sum_partials(const double* partials, int64_t n, double* out, hipStream_t st) {
   hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, st, partials, n, out);
-  return hipGetLastError();
-}
-
-hipError_t launch_batch_pass(int pass, const BatchVol* vols_dev, const int64_t* prefix_dev, int nvol,
-                             int64_t nblocks, int threads, size_t lds_bytes, hipStream_t st) {
-  allow_big_lds();
-  if (lds_bytes < 16 * sizeof(double)) lds_bytes = 16 * sizeof(double);
-  hipLaunchKernelGGL(k_batch_pass, dim3((unsigned)nblocks), dim3(threads), lds_bytes, st, pass, vols_dev,
-                     prefix_dev, nvol);
-  return hipGetLastError();
-}
-
-hipError_t launch_batch_sums(const BatchVol* vols_dev, const int64_t* prefix_dev, int nvol, hipStream_t st) {
-  hipLaunchKernelGGL(k_batch_sums, dim3(nvol), dim3(256), 0, st, vols_dev, prefix_dev);
   return hipGetLastError();
 }
 

@@ -110,6 +110,7 @@ struct XfArgs {  // X real-to-complex rows
   int64_t nrows;
   int rowlen, lo, hi, H;
   Pro pro;
+  int vec2;  // rows start 8-byte aligned and hi == 0: load sample pairs as float2
 };
 struct XiArgs {  // X complex-to-real rows with the epilogue
   const float2* src;
@@ -119,6 +120,7 @@ struct XiArgs {  // X complex-to-real rows with the epilogue
   int64_t nrows;
   int s2, H;
   Epi epi;
+  int vec2;  // output and epilogue operands 8-byte aligned per row: float2 accesses
 };
 
 // Up to kPack volumes of one transform length per launch, passed by value.
@@ -139,30 +141,8 @@ int ct_threads(int n);
 // tile = columns (CT_COL, CT_Z) or rows (CT_XF, CT_XI) per workgroup.
 hipError_t ct_launch(int kind, int n, int dir, const void* pk, int tile, int64_t nblocks, hipStream_t st);
 
-// ---- batched (many volumes, one launch per pass) ----
+// ---- pass ids ----
 enum PassId { PASS_X_FWD = 0, PASS_Y_FWD = 1, PASS_Z = 2, PASS_Y_INV = 3, PASS_X_INV = 4, PASS_N = 5 };
 
-// Everything one volume's convolution application needs, resident on the device.
-struct BatchVol {
-  AxisSched xf, xi, yf, yi, zf, zi;
-  const float2* tw_z;
-  const float2* tw_y;
-  const float2* tw_xm;
-  const float2* tw_xp;
-  const float2* otf;
-  float2* B1;
-  float2* B2;
-  const float* in;
-  float* out;
-  Pro pro;
-  Epi epi;            // epi.partials: this volume's X c2r partials
-  double* reduce_out; // this volume's epilogue sum (may be null)
-  int s0, s1, s2, P1, H, conj;
-  int tile[PASS_N];   // columns (Y, Z) or rows (X) per workgroup
-};
-
-hipError_t launch_batch_pass(int pass, const BatchVol* vols_dev, const int64_t* prefix_dev, int nvol,
-                             int64_t nblocks, int threads, size_t lds_bytes, hipStream_t st);
-hipError_t launch_batch_sums(const BatchVol* vols_dev, const int64_t* prefix_dev, int nvol, hipStream_t st);
 
 }  // namespace dh

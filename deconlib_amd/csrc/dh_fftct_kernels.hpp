@@ -61,6 +61,38 @@ __device__ __forceinline__ float epi_apply(const Epi& e, float v, int64_t idx, d
   }
 }
 
+// epi_apply on the sample pair (idx, idx + 1) with float2 operand loads;
+// the pair is accumulated in the same order as two epi_apply calls.
+__device__ __forceinline__ float2 epi_apply2(const Epi& e, float2 v, int64_t idx, double& acc) {
+  switch (e.kind) {
+    case EPI_FLOOR: return make_float2(fmaxf(v.x, e.a), fmaxf(v.y, e.a));
+    case EPI_RL_RATIO: {
+      const float2 o = *reinterpret_cast<const float2*>(e.aux0 + idx);
+      const float m0 = v.x + e.a, m1 = v.y + e.a;
+      if (e.partials) {
+        acc += (double)idiv(o.x, m0);
+        acc += (double)idiv(o.y, m1);
+      }
+      return make_float2(o.x / fmaxf(m0, e.b), o.y / fmaxf(m1, e.b));
+    }
+    case EPI_RL_UPDATE: {
+      const float2 xv = *reinterpret_cast<const float2*>(e.aux0 + idx);
+      const float2 sv = *reinterpret_cast<const float2*>(e.aux1 + idx);
+      return make_float2(fmaxf(xv.x * (v.x / sv.x), e.b), fmaxf(xv.y * (v.y / sv.y), e.b));
+    }
+    case EPI_ADD: {
+      const float m0 = v.x + e.a, m1 = v.y + e.a;
+      if (e.partials && e.aux0) {
+        const float2 o = *reinterpret_cast<const float2*>(e.aux0 + idx);
+        acc += (double)idiv(o.x, m0);
+        acc += (double)idiv(o.y, m1);
+      }
+      return make_float2(m0, m1);
+    }
+    default: return v;
+  }
+}
+
 __device__ __forceinline__ double bsum(double v, double* scratch) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -232,7 +264,22 @@ __device__ __forceinline__ void xf_body(const XfArgs& a, int64_t lb, int RPB, fl
     if (!ok || c < 0) return 0.0f;
     return pro_apply(a.pro, sr, c);
   };
-  auto gld = [&](int e) -> float2 { return make_float2(sample(2 * e), sample(2 * e + 1)); };
+  const bool vec2 = a.vec2 != 0;
+  auto gld = [&](int e) -> float2 {
+    if (vec2 && ok && 2 * e + 1 < lo) {
+      const float2 v = *reinterpret_cast<const float2*>(sr + 2 * e);
+      switch (a.pro.kind) {
+        case PRO_ONES: return make_float2(1.0f, 1.0f);
+        case PRO_SUB_FLOOR: return make_float2(fmaxf(v.x - a.pro.a, a.pro.b), fmaxf(v.y - a.pro.a, a.pro.b));
+        case PRO_DIV_DEV: {
+          const float s = (float)(*a.pro.dptr);
+          return make_float2(v.x / s, v.y / s);
+        }
+        default: return v;
+      }
+    }
+    return make_float2(sample(2 * e), sample(2 * e + 1));
+  };
   auto lld = [&](int e) { return lrow[e]; };
   auto lst = [&](int e, float2 v) { lrow[e] = v; };
   // all passes end in LDS (natural order), then the real-to-half-spectrum step
@@ -299,9 +346,14 @@ __device__ __forceinline__ void xi_body(const XiArgs& a, int64_t lb, int RPB, fl
   double acc = 0.0;
   float* drow = a.dst + row * s2;
   const int64_t base = row * s2;
+  const bool vec2 = a.vec2 != 0;
   auto gst = [&](int e, float2 v) {
     if (!ok) return;
     const int q0 = 2 * e, q1 = 2 * e + 1;
+    if (vec2 && q1 < s2) {
+      *reinterpret_cast<float2*>(drow + q0) = epi_apply2(a.epi, v, base + q0, acc);
+      return;
+    }
     if (q0 < s2) drow[q0] = epi_apply(a.epi, v.x, base + q0, acc);
     if (q1 < s2) drow[q1] = epi_apply(a.epi, v.y, base + q1, acc);
   };

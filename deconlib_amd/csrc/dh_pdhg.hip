@@ -9,6 +9,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "dh_pdhg.hpp"
 
 namespace dh {
@@ -162,107 +164,208 @@ struct W6 {
   float w[6];
 };
 
+// ---------------------------------------------------------------------------
+// LDS-tiled stencils: a workgroup owns a TY x TX column of the volume over ZC
+// consecutive planes and slides a three-plane window (with a one-voxel halo,
+// filled from clamped coordinates = the Neumann boundary) down z, so each
+// voxel of x, x_prev and y2 is read from HBM about once.
+// ---------------------------------------------------------------------------
+
+constexpr int kTX = 64, kTY = 8, kZC = 16;
+constexpr int kPW = kTX + 2, kPH = kTY + 2, kPlane = kPW * kPH;
+constexpr int kTileThreads = kTX * kTY;
+
+struct Tile {
+  int x0, y0, z0, z1;  // z1 = one past the last plane of this workgroup
+  int tx, ty, x, y;
+  bool in;             // (y, x) inside the volume
+};
+
+__device__ __forceinline__ Tile tile_of(const Dims& d) {
+  Tile t;
+  t.x0 = blockIdx.x * kTX;
+  t.y0 = blockIdx.y * kTY;
+  t.z0 = blockIdx.z * kZC;
+  t.z1 = min(t.z0 + kZC, d.nz);
+  t.tx = threadIdx.x % kTX;
+  t.ty = threadIdx.x / kTX;
+  t.x = t.x0 + t.tx;
+  t.y = t.y0 + t.ty;
+  t.in = t.x < d.nx && t.y < d.ny;
+  return t;
+}
+
+// Fills one halo'd plane: dst[ly * kPW + lx] = f(global index at clamped (z, y, x)).
+template <class F>
+__device__ __forceinline__ void fill_plane(float* dst, const Dims& d, const Tile& t, int z, F f) {
+  const int zc = clampi(z, d.nz);
+  for (int i = threadIdx.x; i < kPlane; i += kTileThreads) {
+    const int ly = i / kPW, lx = i - ly * kPW;
+    const int gy = clampi(t.y0 + ly - 1, d.ny), gx = clampi(t.x0 + lx - 1, d.nx);
+    dst[i] = f(((int64_t)zc * d.ny + gy) * d.nx + gx);
+  }
+}
+
+__device__ __forceinline__ int slot_of(int z, int z0) { return (z - z0 + 1) % 3; }
+
 // y2 <- prox(y2 + sigma * H(xbar)), xbar = x + theta (x - xp)  (pdhg_mlx.py:729-734)
-__global__ void k_pd_dual(float* __restrict__ y2, const float* __restrict__ x, const float* __restrict__ xp,
-                          Dims d, const double* __restrict__ sc, W6 w, float alpha, int l12) {
+__global__ void __launch_bounds__(kTileThreads) k_pd_dual(float* __restrict__ y2, const float* __restrict__ x,
+                                                          const float* __restrict__ xp, Dims d,
+                                                          const double* __restrict__ sc, W6 w, float alpha, int l12) {
+  __shared__ float pl[3 * kPlane];
+  const Tile t = tile_of(d);
   const int64_t n = (int64_t)d.nz * d.ny * d.nx;
   const float sf = (float)sc[PD_SIGMA], th = (float)sc[PD_THETA];
-  auto xbar = [&](int z, int y, int xx) {
-    const int64_t i = ((int64_t)z * d.ny + y) * d.nx + xx;
+  auto xbar = [&](int64_t i) {
     const float v = x[i];
     return v + th * (v - xp[i]);
   };
-  PD_LOOP(i, n) {
-    const int xx = (int)(i % d.nx);
-    const int y = (int)((i / d.nx) % d.ny);
-    const int z = (int)(i / ((int64_t)d.nx * d.ny));
-    float h[6];
-    hess6(xbar, d, z, y, xx, w.w, h);
-    float a[6];
+  fill_plane(pl + slot_of(t.z0 - 1, t.z0) * kPlane, d, t, t.z0 - 1, xbar);
+  fill_plane(pl + slot_of(t.z0, t.z0) * kPlane, d, t, t.z0, xbar);
+  for (int z = t.z0; z < t.z1; ++z) {
+    fill_plane(pl + slot_of(z + 1, t.z0) * kPlane, d, t, z + 1, xbar);
+    __syncthreads();
+    if (t.in) {
+      auto f = [&](int zz, int yy, int xx) {
+        return pl[slot_of(zz, t.z0) * kPlane + (yy - t.y0 + 1) * kPW + (xx - t.x0 + 1)];
+      };
+      float h[6];
+      hess6(f, d, z, t.y, t.x, w.w, h);
+      const int64_t i = ((int64_t)z * d.ny + t.y) * d.nx + t.x;
+      float a[6];
 #pragma unroll
-    for (int q = 0; q < 6; ++q) a[q] = y2[q * n + i] + sf * h[q];
-    if (!l12) {
+      for (int q = 0; q < 6; ++q) a[q] = y2[q * n + i] + sf * h[q];
+      if (!l12) {
 #pragma unroll
-      for (int q = 0; q < 6; ++q) y2[q * n + i] = fminf(fmaxf(a[q], -alpha), alpha);
-    } else {
-      float ss = 0.0f;
+        for (int q = 0; q < 6; ++q) y2[q * n + i] = fminf(fmaxf(a[q], -alpha), alpha);
+      } else {
+        float ss = 0.0f;
 #pragma unroll
-      for (int q = 0; q < 6; ++q) ss += a[q] * a[q];
-      const float nrm = sqrtf(ss + 1e-12f);
-      const float s = fmaxf(nrm / alpha, 1.0f);
+        for (int q = 0; q < 6; ++q) ss += a[q] * a[q];
+        const float nrm = sqrtf(ss + 1e-12f);
+        const float s = fmaxf(nrm / alpha, 1.0f);
 #pragma unroll
-      for (int q = 0; q < 6; ++q) y2[q * n + i] = a[q] / s;
+        for (int q = 0; q < 6; ++q) y2[q * n + i] = a[q] / s;
+      }
     }
+    __syncthreads();
   }
 }
 
 // xn = max(0, x - tau (g + H^T y2)); partial sums of (xn - x)^2, x^2, xn^2
-// (pdhg_mlx.py:741-742, 749-750, 805-806)
-__global__ void k_pd_primal(float* __restrict__ xn, const float* __restrict__ x, const float* __restrict__ g,
-                            const float* __restrict__ y2, Dims d, const double* __restrict__ sc, W6 w,
-                            double* __restrict__ part) {
+// (pdhg_mlx.py:741-742, 749-750, 805-806).  Components 0 (zz), 3 (yz) and
+// 4 (xz) need z-neighbours (three-plane windows); 1, 2 and 5 only the current plane.
+__global__ void __launch_bounds__(kTileThreads) k_pd_primal(float* __restrict__ xn, const float* __restrict__ x,
+                                                            const float* __restrict__ g,
+                                                            const float* __restrict__ y2, Dims d,
+                                                            const double* __restrict__ sc, W6 w,
+                                                            double* __restrict__ part) {
+  __shared__ float win[3][3 * kPlane];  // components 0, 3, 4
+  __shared__ float cur[3][kPlane];      // components 1, 2, 5 at the current plane
   __shared__ double sh[16];
+  const Tile t = tile_of(d);
   const int64_t n = (int64_t)d.nz * d.ny * d.nx;
   const float tf = (float)sc[PD_TAU];
   double s_dx = 0.0, s_x = 0.0, s_xn = 0.0;
-  auto hw = [&](int q, int z, int y, int xx) {
-    return y2[q * n + ((int64_t)z * d.ny + y) * d.nx + xx] * w.w[q];
+  const int wq[3] = {0, 3, 4}, cq[3] = {1, 2, 5};
+  auto comp = [&](int q) {
+    const float* src = y2 + q * n;
+    const float wv = w.w[q];
+    return [src, wv](int64_t i) { return src[i] * wv; };
   };
-  PD_LOOP(i, n) {
-    const int xx = (int)(i % d.nx);
-    const int y = (int)((i / d.nx) % d.ny);
-    const int z = (int)(i / ((int64_t)d.nx * d.ny));
-    const float lt = hess_adj(hw, d, z, y, xx);
-    const float grad = g[i] + lt;
-    const float xv = x[i];
-    const float v = fmaxf(xv - tf * grad, 0.0f);
-    xn[i] = v;
-    const float dv = v - xv;
-    s_dx += (double)(dv * dv);
-    s_x += (double)(xv * xv);
-    s_xn += (double)(v * v);
+  for (int j = 0; j < 3; ++j) {
+    fill_plane(&win[j][slot_of(t.z0 - 1, t.z0) * kPlane], d, t, t.z0 - 1, comp(wq[j]));
+    fill_plane(&win[j][slot_of(t.z0, t.z0) * kPlane], d, t, t.z0, comp(wq[j]));
   }
+  for (int z = t.z0; z < t.z1; ++z) {
+    for (int j = 0; j < 3; ++j) {
+      fill_plane(&win[j][slot_of(z + 1, t.z0) * kPlane], d, t, z + 1, comp(wq[j]));
+      fill_plane(cur[j], d, t, z, comp(cq[j]));
+    }
+    __syncthreads();
+    if (t.in) {
+      auto hw = [&](int q, int zz, int yy, int xx) {
+        const int o = (yy - t.y0 + 1) * kPW + (xx - t.x0 + 1);
+        switch (q) {
+          case 0: return win[0][slot_of(zz, t.z0) * kPlane + o];
+          case 3: return win[1][slot_of(zz, t.z0) * kPlane + o];
+          case 4: return win[2][slot_of(zz, t.z0) * kPlane + o];
+          case 1: return cur[0][o];
+          case 2: return cur[1][o];
+          default: return cur[2][o];
+        }
+      };
+      const float lt = hess_adj(hw, d, z, t.y, t.x);
+      const int64_t i = ((int64_t)z * d.ny + t.y) * d.nx + t.x;
+      const float grad = g[i] + lt;
+      const float xv = x[i];
+      const float v = fmaxf(xv - tf * grad, 0.0f);
+      xn[i] = v;
+      const float dv = v - xv;
+      s_dx += (double)(dv * dv);
+      s_x += (double)(xv * xv);
+      s_xn += (double)(v * v);
+    }
+    __syncthreads();
+  }
+  const int64_t b = ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
   double r0 = pd_block_sum(s_dx, sh);
   double r1 = pd_block_sum(s_x, sh);
   double r2 = pd_block_sum(s_xn, sh);
   if (threadIdx.x == 0) {
-    part[blockIdx.x * 3 + 0] = r0;
-    part[blockIdx.x * 3 + 1] = r1;
-    part[blockIdx.x * 3 + 2] = r2;
+    part[b * 3 + 0] = r0;
+    part[b * 3 + 1] = r1;
+    part[b * 3 + 2] = r2;
   }
 }
 
 // sum |H a - H b|^2 (the reference's dL = L(x_new) - L(x)) and, if want_abs,
-// sum |H a| (the regularizer loss of a)
-__global__ void k_pd_hess_sums(const float* __restrict__ a, const float* __restrict__ b, Dims d, W6 w,
-                               int want_abs, double* __restrict__ part) {
+// sum |H a| (1, "L1") or sum sqrt(sum_q (H a)_q^2 + 1e-12) (2, "L1_2")
+__global__ void __launch_bounds__(kTileThreads) k_pd_hess_sums(const float* __restrict__ a,
+                                                               const float* __restrict__ b, Dims d, W6 w,
+                                                               int want_abs, double* __restrict__ part) {
+  __shared__ float pa[3 * kPlane], pb[3 * kPlane];
   __shared__ double sh[16];
-  const int64_t n = (int64_t)d.nz * d.ny * d.nx;
+  const Tile t = tile_of(d);
   double s_sq = 0.0, s_abs = 0.0;
-  auto av = [&](int z, int y, int xx) { return a[((int64_t)z * d.ny + y) * d.nx + xx]; };
-  auto bv = [&](int z, int y, int xx) { return b[((int64_t)z * d.ny + y) * d.nx + xx]; };
-  PD_LOOP(i, n) {
-    const int xx = (int)(i % d.nx);
-    const int y = (int)((i / d.nx) % d.ny);
-    const int z = (int)(i / ((int64_t)d.nx * d.ny));
-    float ha[6], hb[6];
-    hess6(av, d, z, y, xx, w.w, ha);
-    hess6(bv, d, z, y, xx, w.w, hb);
-    float ss = 0.0f;
+  auto av = [&](int64_t i) { return a[i]; };
+  auto bv = [&](int64_t i) { return b[i]; };
+  fill_plane(pa + slot_of(t.z0 - 1, t.z0) * kPlane, d, t, t.z0 - 1, av);
+  fill_plane(pa + slot_of(t.z0, t.z0) * kPlane, d, t, t.z0, av);
+  fill_plane(pb + slot_of(t.z0 - 1, t.z0) * kPlane, d, t, t.z0 - 1, bv);
+  fill_plane(pb + slot_of(t.z0, t.z0) * kPlane, d, t, t.z0, bv);
+  for (int z = t.z0; z < t.z1; ++z) {
+    fill_plane(pa + slot_of(z + 1, t.z0) * kPlane, d, t, z + 1, av);
+    fill_plane(pb + slot_of(z + 1, t.z0) * kPlane, d, t, z + 1, bv);
+    __syncthreads();
+    if (t.in) {
+      auto fa = [&](int zz, int yy, int xx) {
+        return pa[slot_of(zz, t.z0) * kPlane + (yy - t.y0 + 1) * kPW + (xx - t.x0 + 1)];
+      };
+      auto fb = [&](int zz, int yy, int xx) {
+        return pb[slot_of(zz, t.z0) * kPlane + (yy - t.y0 + 1) * kPW + (xx - t.x0 + 1)];
+      };
+      float ha[6], hb[6];
+      hess6(fa, d, z, t.y, t.x, w.w, ha);
+      hess6(fb, d, z, t.y, t.x, w.w, hb);
+      float ss = 0.0f;
 #pragma unroll
-    for (int q = 0; q < 6; ++q) {
-      const float dd = ha[q] - hb[q];
-      s_sq += (double)(dd * dd);
-      if (want_abs == 1) s_abs += (double)fabsf(ha[q]);
-      ss += ha[q] * ha[q];
+      for (int q = 0; q < 6; ++q) {
+        const float dd = ha[q] - hb[q];
+        s_sq += (double)(dd * dd);
+        if (want_abs == 1) s_abs += (double)fabsf(ha[q]);
+        ss += ha[q] * ha[q];
+      }
+      if (want_abs == 2) s_abs += (double)sqrtf(ss + 1e-12f);
     }
-    if (want_abs == 2) s_abs += (double)sqrtf(ss + 1e-12f);
+    __syncthreads();
   }
+  const int64_t blk = ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
   double r0 = pd_block_sum(s_sq, sh);
   double r1 = pd_block_sum(s_abs, sh);
   if (threadIdx.x == 0) {
-    part[blockIdx.x * 2 + 0] = r0;
-    part[blockIdx.x * 2 + 1] = r1;
+    part[blk * 2 + 0] = r0;
+    part[blk * 2 + 1] = r1;
   }
 }
 
@@ -368,15 +471,21 @@ hipError_t pd_y1(float* y1, const float* ax, const float* axp, const float* data
   return hipGetLastError();
 }
 
+static dim3 tile_grid(int nz, int ny, int nx) {
+  return dim3((unsigned)((nx + kTX - 1) / kTX), (unsigned)((ny + kTY - 1) / kTY), (unsigned)((nz + kZC - 1) / kZC));
+}
+
+static int tile_blocks(int nz, int ny, int nx) {
+  const dim3 g = tile_grid(nz, ny, nx);
+  return (int)(g.x * g.y * g.z);
+}
+
 hipError_t pd_dual(float* y2, const float* x, const float* xp, int nz, int ny, int nx, const double* sc,
                    const float* w, float alpha, int l12, hipStream_t st) {
   Dims d{nz, ny, nx};
   W6 ww;
   for (int q = 0; q < 6; ++q) ww.w[q] = w[q];
-  const int64_t n = (int64_t)nz * ny * nx;
-  int64_t nb = (n + kPdThreads - 1) / kPdThreads;
-  if (nb > 65535LL * 32) nb = 65535LL * 32;
-  hipLaunchKernelGGL(k_pd_dual, dim3((unsigned)nb), dim3(kPdThreads), 0, st, y2, x, xp, d, sc, ww, alpha, l12);
+  hipLaunchKernelGGL(k_pd_dual, tile_grid(nz, ny, nx), dim3(kTileThreads), 0, st, y2, x, xp, d, sc, ww, alpha, l12);
   return hipGetLastError();
 }
 
@@ -385,12 +494,10 @@ hipError_t pd_primal(float* xn, const float* x, const float* g, const float* y2,
   Dims d{nz, ny, nx};
   W6 ww;
   for (int q = 0; q < 6; ++q) ww.w[q] = w[q];
-  const int64_t n = (int64_t)nz * ny * nx;
-  const int nb = pd_blocks(n);
-  hipLaunchKernelGGL(k_pd_primal, dim3(nb), dim3(kPdThreads), 0, st, xn, x, g, y2, d, sc, ww, part);
+  hipLaunchKernelGGL(k_pd_primal, tile_grid(nz, ny, nx), dim3(kTileThreads), 0, st, xn, x, g, y2, d, sc, ww, part);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  return finish(part, nb, 3, sc + PD_DX2, st);  // DX2, XX, XNXN are consecutive
+  return finish(part, tile_blocks(nz, ny, nx), 3, sc + PD_DX2, st);  // DX2, XX, XNXN are consecutive
 }
 
 hipError_t pd_hess_sums(const float* a, const float* b, int nz, int ny, int nx, const float* w, int want_abs,
@@ -398,12 +505,10 @@ hipError_t pd_hess_sums(const float* a, const float* b, int nz, int ny, int nx, 
   Dims d{nz, ny, nx};
   W6 ww;
   for (int q = 0; q < 6; ++q) ww.w[q] = w[q];
-  const int64_t n = (int64_t)nz * ny * nx;
-  const int nb = pd_blocks(n);
-  hipLaunchKernelGGL(k_pd_hess_sums, dim3(nb), dim3(kPdThreads), 0, st, a, b, d, ww, want_abs, part);
+  hipLaunchKernelGGL(k_pd_hess_sums, tile_grid(nz, ny, nx), dim3(kTileThreads), 0, st, a, b, d, ww, want_abs, part);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  e = finish(part, nb, 2, sc + PD_TMP0, st);
+  e = finish(part, tile_blocks(nz, ny, nx), 2, sc + PD_TMP0, st);
   if (e != hipSuccess) return e;
   // move: DL2 <- TMP0, loss_out <- TMP1
   return pd_move(sc, PD_TMP0, PD_DL2, PD_TMP1, loss_out, st);
@@ -438,7 +543,10 @@ hipError_t pd_scalars(double* sc, double* hist, int k, const PdParams& pp, hipSt
   return hipGetLastError();
 }
 
-int64_t pd_partials_needed(int64_t n) { return (int64_t)pd_blocks(n) * 3 + 16; }
+int64_t pd_partials_needed(int nz, int ny, int nx) {
+  const int64_t n = (int64_t)nz * ny * nx;
+  return 3 * std::max<int64_t>(pd_blocks(n), tile_blocks(nz, ny, nx)) + 16;
+}
 
 static dim3 ew_blocks(int64_t n) {
   int64_t nb = (n + kPdThreads - 1) / kPdThreads;

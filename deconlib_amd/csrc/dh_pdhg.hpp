@@ -39,7 +39,7 @@ hipError_t pd_data_sums(const float* a, const float* b, const float* data, int64
                         int loss_type, double* sc, double* loss_out, double* part, hipStream_t st);
 hipError_t pd_move(double* sc, int from0, int to0, int from1, double* out1, hipStream_t st);
 hipError_t pd_scalars(double* sc, double* hist, int k, const PdParams& pp, hipStream_t st);
-int64_t pd_partials_needed(int64_t n);
+int64_t pd_partials_needed(int nz, int ny, int nx);
 hipError_t hess_forward(const float* f, float* out, int nz, int ny, int nx, const float* w, hipStream_t st);
 hipError_t hess_adjoint(const float* y2, float* out, int nz, int ny, int nx, const float* w, hipStream_t st);
 

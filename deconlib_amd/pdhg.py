@@ -153,8 +153,9 @@ class _PDState:
         self.sc = torch.from_numpy(sc).to(dev)
         self.hist = torch.zeros(3 * max(self.num_iter, 1), dtype=torch.float64, device=dev)
         self.loss = torch.zeros(2 * (self.num_iter + 1), dtype=torch.float64, device=dev)
-        n = max(int(np.prod(self.S)), self.data.numel())
-        self.part = torch.empty(int(N.fns["dh_pd_partials"](n)), dtype=torch.float64, device=dev)
+        if self.data.numel() > int(np.prod(self.S)):
+            raise ValueError("the data domain must not be larger than the reconstruction domain")
+        self.part = torch.empty(int(N.fns["dh_pd_partials"](*self.S)), dtype=torch.float64, device=dev)
 
     def _fwd(self, x, out):
         if self._plan is not None:

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define DH_ABI_VERSION 1
+#define DH_ABI_VERSION 2
 
 #define DH_OK 0
 #define DH_EINVAL (-1)
@@ -106,8 +106,7 @@ int dh_conv_apply(const dh_conv_plan* plan, int adjoint, const float* in, float*
                   const void* otf_dev, void* work_dev, int64_t work_bytes,
                   const dh_fuse_args* fuse, void* stream);
 
-/* ---- batched application: many plans (different shapes) in one launch per pass ---- */
-typedef struct dh_batch dh_batch;
+/* ---- batched application: many plans (different shapes), few launches ---- */
 typedef struct {
   const dh_conv_plan* plan;
   const void* otf;
@@ -117,17 +116,13 @@ typedef struct {
   float* out;
   dh_fuse_args fuse;         /* per-volume prologue/epilogue; reduce_out per volume */
 } dh_batch_item;
-/* Device descriptor bytes for up to max_items volumes (caller allocates). */
-int64_t dh_batch_desc_bytes(int max_items);
-/* Host object with pinned staging for the per-call descriptors. */
-int dh_batch_create(int max_items, dh_batch** out);
-int dh_batch_destroy(dh_batch* batch);
 /* Applies every item's convolution (adjoint == 0) or correlation (adjoint == 1)
- * with its fused prologue/epilogue: 5 launches for the whole batch, each
- * covering the tiles of all volumes (the same results as one dh_conv_apply per
- * item).  Serving and batched RL call this once per half-iteration. */
-int dh_batch_conv_apply(dh_batch* batch, int nitems, const dh_batch_item* items, int adjoint, void* desc_dev,
-                        int64_t desc_bytes, void* stream);
+ * with its fused prologue/epilogue, with the same results as one dh_conv_apply
+ * per item.  Items whose canvas has specialised kernels are grouped per pass by
+ * transform length: each group of up to 16 volumes is one launch, descriptors
+ * in the kernel arguments.  Serving and batched RL call this once per
+ * half-iteration. */
+int dh_batch_conv_apply(int nitems, const dh_batch_item* items, int adjoint, void* stream);
 
 /* ---- reductions and elementwise steps (rl_mlx.py:123-150) ---- */
 #define DH_RED_SUM 0
@@ -181,8 +176,9 @@ typedef struct {
   int32_t min_iter, patience;
 } dh_pd_params;
 
-/* Partials (doubles) the PDHG reductions need for n voxels. */
-int64_t dh_pd_partials(int64_t n);
+/* Partials (doubles) the PDHG reductions need for an (nz, ny, nx) volume whose
+ * data domain has at most as many voxels. */
+int64_t dh_pd_partials(int nz, int ny, int nx);
 /* y1 <- prox_sigma F*(y1 + sigma ((1+theta) ax - theta axp))  (pdhg_mlx.py:115-189, 723-727);
  * loss_type 0 = poisson, 1 = gaussian. */
 int dh_pd_y1(float* y1, const float* ax, const float* axp, const float* data, int64_t n, const double* scalars,
