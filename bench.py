@@ -159,6 +159,7 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--serving-jobs", type=int, default=40)
     ap.add_argument("--serving-rate", type=float, default=40.0)
     args = ap.parse_args()
@@ -212,6 +213,14 @@ def main():
     ok = bool(torch.isfinite(st.x).all().item())
     pdhg = bench_pdhg(torch, N, obs, blur, args.steps, args.warmup)
     log(f"pdhg: {json.dumps(pdhg)}")
+    if args.copy_probe:
+        # a plain device copy of known size: the reference point for the HBM counters
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")  # 256 MiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy probe: 3 x 268435456-byte device-to-device copies")
     serving = None
     if args.serving_jobs > 0:
         serving = bench_serving(torch, args.serving_jobs, args.serving_rate)

@@ -31,7 +31,9 @@ def _key(plan) -> str:
 def _time_passes(op, x, reps: int):
     plan = op._plan
     out = empty_f32(x.shape)
-    plan.apply(x, False, out)
+    for _ in range(2):
+        plan.apply(x, False, out)
+        plan.apply(x, True, out)
     torch.cuda.synchronize()
     N.call("dh_timing_read", None, None, 1)
     N.call("dh_timing_enable", 1)
@@ -48,7 +50,7 @@ def _time_passes(op, x, reps: int):
     return [ms[i] / cnt[i] if cnt[i] else float("inf") for i in range(5)]
 
 
-def autotune(op, reps: int = 3, candidates=(1, 2, 4, 8, 16, 32, 64), cache_file: Optional[str] = None,
+def autotune(op, reps: int = 10, candidates=(1, 2, 4, 8, 16, 32, 64), cache_file: Optional[str] = None,
              verbose: bool = False) -> Dict[int, int]:
     """Picks the fastest tile for every pass of ``op`` (a LinearFFTConvolver/FFTConvolver)."""
     plan = op._plan

@@ -52,6 +52,23 @@ static hipError_t finish(const double* part, int nblocks, int nv, double* out, h
   return hipGetLastError();
 }
 
+// Sums two partial vectors [block][2] into *out0 and, if out1, *out1.
+__global__ void k_pd_finish2(const double* __restrict__ part, int nblocks, double* out0, double* out1) {
+  __shared__ double sh[16];
+  for (int j = 0; j < 2; ++j) {
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) acc += part[(int64_t)b * 2 + j];
+    double r = pd_block_sum(acc, sh);
+    double* o = j == 0 ? out0 : out1;
+    if (threadIdx.x == 0 && o) o[0] = r;
+  }
+}
+
+static hipError_t finish2(const double* part, int nblocks, double* out0, double* out1, hipStream_t st) {
+  hipLaunchKernelGGL(k_pd_finish2, dim3(1), dim3(1024), 0, st, part, nblocks, out0, out1);
+  return hipGetLastError();
+}
+
 hipError_t pd_move(double* sc, int from0, int to0, int from1, double* out1, hipStream_t st);
 
 static int pd_blocks(int64_t n) {
@@ -181,11 +198,25 @@ struct Tile {
   bool in;             // (y, x) inside the volume
 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs; remap so that each XCD
+// (and its L2) runs a contiguous x-then-y range of tiles, whose halos overlap.
+__device__ __forceinline__ void tile_coords(int& bx, int& by, int& bz) {
+  const int64_t nb = (int64_t)gridDim.x * gridDim.y * gridDim.z;
+  const int64_t b = ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+  const int64_t q = nb / 8, r = nb % 8, x = b % 8, i = b / 8;
+  const int64_t l = x * q + (x < r ? x : r) + i;
+  bx = (int)(l % gridDim.x);
+  by = (int)((l / gridDim.x) % gridDim.y);
+  bz = (int)(l / ((int64_t)gridDim.x * gridDim.y));
+}
+
 __device__ __forceinline__ Tile tile_of(const Dims& d) {
   Tile t;
-  t.x0 = blockIdx.x * kTX;
-  t.y0 = blockIdx.y * kTY;
-  t.z0 = blockIdx.z * kZC;
+  int bx, by, bz;
+  tile_coords(bx, by, bz);
+  t.x0 = bx * kTX;
+  t.y0 = by * kTY;
+  t.z0 = bz * kZC;
   t.z1 = min(t.z0 + kZC, d.nz);
   t.tx = threadIdx.x % kTX;
   t.ty = threadIdx.x / kTX;
@@ -208,44 +239,87 @@ __device__ __forceinline__ void fill_plane(float* dst, const Dims& d, const Tile
 
 __device__ __forceinline__ int slot_of(int z, int z0) { return (z - z0 + 1) % 3; }
 
+// Each thread stages at most two samples of every halo'd plane (kPlane <=
+// 2 * kTileThreads).  Their in-plane offsets are fixed, so the next plane's
+// loads are issued into registers before the current plane is computed and
+// land in LDS after it (software pipelining of the z sweep).
+struct PlanePos {
+  int64_t off0, off1;
+  bool has1;
+  __device__ PlanePos(const Dims& d, const Tile& t) {
+    const int i0 = threadIdx.x, i1 = threadIdx.x + kTileThreads;
+    const int ly0 = i0 / kPW, lx0 = i0 - ly0 * kPW;
+    off0 = (int64_t)clampi(t.y0 + ly0 - 1, d.ny) * d.nx + clampi(t.x0 + lx0 - 1, d.nx);
+    has1 = i1 < kPlane;
+    const int ly1 = has1 ? i1 / kPW : 0, lx1 = has1 ? i1 - ly1 * kPW : 0;
+    off1 = (int64_t)clampi(t.y0 + ly1 - 1, d.ny) * d.nx + clampi(t.x0 + lx1 - 1, d.nx);
+  }
+  __device__ __forceinline__ int64_t base(const Dims& d, int z) const {
+    return (int64_t)clampi(z, d.nz) * d.ny * d.nx;
+  }
+  __device__ __forceinline__ void load(const float* __restrict__ src, int64_t b, float& v0, float& v1) const {
+    v0 = src[b + off0];
+    v1 = has1 ? src[b + off1] : 0.0f;
+  }
+  __device__ __forceinline__ void store(float* dst, float v0, float v1) const {
+    dst[threadIdx.x] = v0;
+    if (has1) dst[threadIdx.x + kTileThreads] = v1;
+  }
+};
+
+__device__ __forceinline__ float lds_at(const float* planes, int slot, const Tile& t, int yy, int xx) {
+  return planes[slot * kPlane + (yy - t.y0 + 1) * kPW + (xx - t.x0 + 1)];
+}
+
 // y2 <- prox(y2 + sigma * H(xbar)), xbar = x + theta (x - xp)  (pdhg_mlx.py:729-734)
 __global__ void __launch_bounds__(kTileThreads) k_pd_dual(float* __restrict__ y2, const float* __restrict__ x,
                                                           const float* __restrict__ xp, Dims d,
                                                           const double* __restrict__ sc, W6 w, float alpha, int l12) {
   __shared__ float pl[3 * kPlane];
   const Tile t = tile_of(d);
+  const PlanePos pp(d, t);
   const int64_t n = (int64_t)d.nz * d.ny * d.nx;
   const float sf = (float)sc[PD_SIGMA], th = (float)sc[PD_THETA];
-  auto xbar = [&](int64_t i) {
-    const float v = x[i];
-    return v + th * (v - xp[i]);
-  };
-  fill_plane(pl + slot_of(t.z0 - 1, t.z0) * kPlane, d, t, t.z0 - 1, xbar);
-  fill_plane(pl + slot_of(t.z0, t.z0) * kPlane, d, t, t.z0, xbar);
+  auto xbar = [&](float v, float vp) { return v + th * (v - vp); };
+  float a0, a1, b0, b1;
+  for (int zz = t.z0 - 1; zz <= t.z0; ++zz) {
+    const int64_t bz = pp.base(d, zz);
+    pp.load(x, bz, a0, a1);
+    pp.load(xp, bz, b0, b1);
+    pp.store(pl + slot_of(zz, t.z0) * kPlane, xbar(a0, b0), xbar(a1, b1));
+  }
+  {
+    const int64_t bz = pp.base(d, t.z0 + 1);
+    pp.load(x, bz, a0, a1);
+    pp.load(xp, bz, b0, b1);
+  }
   for (int z = t.z0; z < t.z1; ++z) {
-    fill_plane(pl + slot_of(z + 1, t.z0) * kPlane, d, t, z + 1, xbar);
+    pp.store(pl + slot_of(z + 1, t.z0) * kPlane, xbar(a0, b0), xbar(a1, b1));
     __syncthreads();
+    if (z + 1 < t.z1) {
+      const int64_t bz = pp.base(d, z + 2);
+      pp.load(x, bz, a0, a1);
+      pp.load(xp, bz, b0, b1);
+    }
     if (t.in) {
-      auto f = [&](int zz, int yy, int xx) {
-        return pl[slot_of(zz, t.z0) * kPlane + (yy - t.y0 + 1) * kPW + (xx - t.x0 + 1)];
-      };
+      auto f = [&](int zz, int yy, int xx) { return lds_at(pl, slot_of(zz, t.z0), t, yy, xx); };
       float h[6];
       hess6(f, d, z, t.y, t.x, w.w, h);
       const int64_t i = ((int64_t)z * d.ny + t.y) * d.nx + t.x;
-      float a[6];
+      float av[6];
 #pragma unroll
-      for (int q = 0; q < 6; ++q) a[q] = y2[q * n + i] + sf * h[q];
+      for (int q = 0; q < 6; ++q) av[q] = y2[q * n + i] + sf * h[q];
       if (!l12) {
 #pragma unroll
-        for (int q = 0; q < 6; ++q) y2[q * n + i] = fminf(fmaxf(a[q], -alpha), alpha);
+        for (int q = 0; q < 6; ++q) y2[q * n + i] = fminf(fmaxf(av[q], -alpha), alpha);
       } else {
         float ss = 0.0f;
 #pragma unroll
-        for (int q = 0; q < 6; ++q) ss += a[q] * a[q];
+        for (int q = 0; q < 6; ++q) ss += av[q] * av[q];
         const float nrm = sqrtf(ss + 1e-12f);
         const float s = fmaxf(nrm / alpha, 1.0f);
 #pragma unroll
-        for (int q = 0; q < 6; ++q) y2[q * n + i] = a[q] / s;
+        for (int q = 0; q < 6; ++q) y2[q * n + i] = av[q] / s;
       }
     }
     __syncthreads();
@@ -264,25 +338,45 @@ __global__ void __launch_bounds__(kTileThreads) k_pd_primal(float* __restrict__ 
   __shared__ float cur[3][kPlane];      // components 1, 2, 5 at the current plane
   __shared__ double sh[16];
   const Tile t = tile_of(d);
+  const PlanePos pp(d, t);
   const int64_t n = (int64_t)d.nz * d.ny * d.nx;
   const float tf = (float)sc[PD_TAU];
   double s_dx = 0.0, s_x = 0.0, s_xn = 0.0;
-  const int wq[3] = {0, 3, 4}, cq[3] = {1, 2, 5};
-  auto comp = [&](int q) {
-    const float* src = y2 + q * n;
-    const float wv = w.w[q];
-    return [src, wv](int64_t i) { return src[i] * wv; };
-  };
-  for (int j = 0; j < 3; ++j) {
-    fill_plane(&win[j][slot_of(t.z0 - 1, t.z0) * kPlane], d, t, t.z0 - 1, comp(wq[j]));
-    fill_plane(&win[j][slot_of(t.z0, t.z0) * kPlane], d, t, t.z0, comp(wq[j]));
+  const float* wsrc[3] = {y2, y2 + 3 * n, y2 + 4 * n};
+  const float* csrc[3] = {y2 + 1 * n, y2 + 2 * n, y2 + 5 * n};
+  const float ww[3] = {w.w[0], w.w[3], w.w[4]}, cw[3] = {w.w[1], w.w[2], w.w[5]};
+  float wv[3][2], cv[3][2];
+  for (int zz = t.z0 - 1; zz <= t.z0; ++zz) {
+    const int64_t bz = pp.base(d, zz);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      pp.load(wsrc[j], bz, wv[j][0], wv[j][1]);
+      pp.store(&win[j][slot_of(zz, t.z0) * kPlane], wv[j][0] * ww[j], wv[j][1] * ww[j]);
+    }
+  }
+  {
+    const int64_t bn = pp.base(d, t.z0 + 1), bc = pp.base(d, t.z0);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      pp.load(wsrc[j], bn, wv[j][0], wv[j][1]);
+      pp.load(csrc[j], bc, cv[j][0], cv[j][1]);
+    }
   }
   for (int z = t.z0; z < t.z1; ++z) {
+#pragma unroll
     for (int j = 0; j < 3; ++j) {
-      fill_plane(&win[j][slot_of(z + 1, t.z0) * kPlane], d, t, z + 1, comp(wq[j]));
-      fill_plane(cur[j], d, t, z, comp(cq[j]));
+      pp.store(&win[j][slot_of(z + 1, t.z0) * kPlane], wv[j][0] * ww[j], wv[j][1] * ww[j]);
+      pp.store(cur[j], cv[j][0] * cw[j], cv[j][1] * cw[j]);
     }
     __syncthreads();
+    if (z + 1 < t.z1) {
+      const int64_t bn = pp.base(d, z + 2), bc = pp.base(d, z + 1);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        pp.load(wsrc[j], bn, wv[j][0], wv[j][1]);
+        pp.load(csrc[j], bc, cv[j][0], cv[j][1]);
+      }
+    }
     if (t.in) {
       auto hw = [&](int q, int zz, int yy, int xx) {
         const int o = (yy - t.y0 + 1) * kPW + (xx - t.x0 + 1);
@@ -319,44 +413,56 @@ __global__ void __launch_bounds__(kTileThreads) k_pd_primal(float* __restrict__ 
   }
 }
 
-// sum |H a - H b|^2 (the reference's dL = L(x_new) - L(x)) and, if want_abs,
-// sum |H a| (1, "L1") or sum sqrt(sum_q (H a)_q^2 + 1e-12) (2, "L1_2")
+// sum |H(a - b)|^2 (dL of the step-size rule; H is linear, so this is the
+// reference's |L x_new - L x|^2 up to rounding) and, if want_abs, the
+// regularizer loss of a: sum |H a| (1, "L1") or sum sqrt(sum_q (H a)_q^2 + 1e-12) (2, "L1_2")
 __global__ void __launch_bounds__(kTileThreads) k_pd_hess_sums(const float* __restrict__ a,
                                                                const float* __restrict__ b, Dims d, W6 w,
                                                                int want_abs, double* __restrict__ part) {
-  __shared__ float pa[3 * kPlane], pb[3 * kPlane];
+  __shared__ float pdl[3 * kPlane], pa[3 * kPlane];
   __shared__ double sh[16];
   const Tile t = tile_of(d);
+  const PlanePos pp(d, t);
   double s_sq = 0.0, s_abs = 0.0;
-  auto av = [&](int64_t i) { return a[i]; };
-  auto bv = [&](int64_t i) { return b[i]; };
-  fill_plane(pa + slot_of(t.z0 - 1, t.z0) * kPlane, d, t, t.z0 - 1, av);
-  fill_plane(pa + slot_of(t.z0, t.z0) * kPlane, d, t, t.z0, av);
-  fill_plane(pb + slot_of(t.z0 - 1, t.z0) * kPlane, d, t, t.z0 - 1, bv);
-  fill_plane(pb + slot_of(t.z0, t.z0) * kPlane, d, t, t.z0, bv);
+  float a0, a1, b0, b1;
+  for (int zz = t.z0 - 1; zz <= t.z0; ++zz) {
+    const int64_t bz = pp.base(d, zz);
+    pp.load(a, bz, a0, a1);
+    pp.load(b, bz, b0, b1);
+    pp.store(pdl + slot_of(zz, t.z0) * kPlane, a0 - b0, a1 - b1);
+    if (want_abs) pp.store(pa + slot_of(zz, t.z0) * kPlane, a0, a1);
+  }
+  {
+    const int64_t bz = pp.base(d, t.z0 + 1);
+    pp.load(a, bz, a0, a1);
+    pp.load(b, bz, b0, b1);
+  }
   for (int z = t.z0; z < t.z1; ++z) {
-    fill_plane(pa + slot_of(z + 1, t.z0) * kPlane, d, t, z + 1, av);
-    fill_plane(pb + slot_of(z + 1, t.z0) * kPlane, d, t, z + 1, bv);
+    pp.store(pdl + slot_of(z + 1, t.z0) * kPlane, a0 - b0, a1 - b1);
+    if (want_abs) pp.store(pa + slot_of(z + 1, t.z0) * kPlane, a0, a1);
     __syncthreads();
+    if (z + 1 < t.z1) {
+      const int64_t bz = pp.base(d, z + 2);
+      pp.load(a, bz, a0, a1);
+      pp.load(b, bz, b0, b1);
+    }
     if (t.in) {
-      auto fa = [&](int zz, int yy, int xx) {
-        return pa[slot_of(zz, t.z0) * kPlane + (yy - t.y0 + 1) * kPW + (xx - t.x0 + 1)];
-      };
-      auto fb = [&](int zz, int yy, int xx) {
-        return pb[slot_of(zz, t.z0) * kPlane + (yy - t.y0 + 1) * kPW + (xx - t.x0 + 1)];
-      };
-      float ha[6], hb[6];
-      hess6(fa, d, z, t.y, t.x, w.w, ha);
-      hess6(fb, d, z, t.y, t.x, w.w, hb);
-      float ss = 0.0f;
+      auto fd = [&](int zz, int yy, int xx) { return lds_at(pdl, slot_of(zz, t.z0), t, yy, xx); };
+      float h[6];
+      hess6(fd, d, z, t.y, t.x, w.w, h);
 #pragma unroll
-      for (int q = 0; q < 6; ++q) {
-        const float dd = ha[q] - hb[q];
-        s_sq += (double)(dd * dd);
-        if (want_abs == 1) s_abs += (double)fabsf(ha[q]);
-        ss += ha[q] * ha[q];
+      for (int q = 0; q < 6; ++q) s_sq += (double)(h[q] * h[q]);
+      if (want_abs) {
+        auto fa = [&](int zz, int yy, int xx) { return lds_at(pa, slot_of(zz, t.z0), t, yy, xx); };
+        hess6(fa, d, z, t.y, t.x, w.w, h);
+        float ss = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+          if (want_abs == 1) s_abs += (double)fabsf(h[q]);
+          ss += h[q] * h[q];
+        }
+        if (want_abs == 2) s_abs += (double)sqrtf(ss + 1e-12f);
       }
-      if (want_abs == 2) s_abs += (double)sqrtf(ss + 1e-12f);
     }
     __syncthreads();
   }
@@ -508,10 +614,7 @@ hipError_t pd_hess_sums(const float* a, const float* b, int nz, int ny, int nx, 
   hipLaunchKernelGGL(k_pd_hess_sums, tile_grid(nz, ny, nx), dim3(kTileThreads), 0, st, a, b, d, ww, want_abs, part);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  e = finish(part, tile_blocks(nz, ny, nx), 2, sc + PD_TMP0, st);
-  if (e != hipSuccess) return e;
-  // move: DL2 <- TMP0, loss_out <- TMP1
-  return pd_move(sc, PD_TMP0, PD_DL2, PD_TMP1, loss_out, st);
+  return finish2(part, tile_blocks(nz, ny, nx), sc + PD_DL2, loss_out, st);
 }
 
 hipError_t pd_data_sums(const float* a, const float* b, const float* data, int64_t n, float bg, int want_loss,
@@ -521,9 +624,7 @@ hipError_t pd_data_sums(const float* a, const float* b, const float* data, int64
                      part);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  e = finish(part, nb, 2, sc + PD_TMP0, st);
-  if (e != hipSuccess) return e;
-  return pd_move(sc, PD_TMP0, PD_DA2, PD_TMP1, loss_out, st);
+  return finish2(part, nb, sc + PD_DA2, loss_out, st);
 }
 
 __global__ void k_pd_move(double* sc, int from0, int to0, int from1, double* out1) {
