@@ -30,18 +30,30 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def conv_pass_bytes(S, P):
-    """Algorithmic HBM bytes of the five passes of one pruned convolution (complex64 = 8 B)."""
+def conv_pass_bytes(S, P, epi_operand_bytes=0):
+    """Algorithmic HBM bytes of the five passes of one pruned convolution (complex64 = 8 B).
+
+    ``epi_operand_bytes``: bytes per output voxel the fused epilogue of the X inverse
+    pass reads besides the spectrum (RL: obs in the forward, x and the sensitivity in
+    the adjoint, 6 B on average per launch)."""
     s0, s1, s2 = S
     p0, p1, p2 = P
     H = p2 // 2 + 1
+    V = s0 * s1 * s2
     return {
-        "x_fwd": 4 * s0 * s1 * s2 + 8 * s0 * s1 * H,
+        "x_fwd": 4 * V + 8 * s0 * s1 * H,
         "y_fwd": 8 * s0 * s1 * H + 8 * s0 * p1 * H,
         "z_conv": 8 * s0 * p1 * H + 8 * p0 * p1 * H + 8 * s0 * p1 * H,
         "y_inv": 8 * s0 * p1 * H + 8 * s0 * s1 * H,
-        "x_inv": 8 * s0 * s1 * H + 4 * s0 * s1 * s2,
+        "x_inv": 8 * s0 * s1 * H + 4 * V + epi_operand_bytes * V,
     }
+
+
+def rl_iteration_bytes(S, P):
+    """Algorithmic HBM bytes of one fused RL iteration: two convolutions plus the
+    epilogue operands (forward: obs; adjoint: x, sensitivity)."""
+    core = sum(conv_pass_bytes(S, P).values())
+    return 2 * core + 12 * int(np.prod(S))
 
 
 def make_inputs(torch):
@@ -156,12 +168,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-iters", type=int, default=12)
+    ap.add_argument("--cpu-iters", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--tune-cache", default=None, help="JSON file the autotuner reads/writes")
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--serving-jobs", type=int, default=40)
-    ap.add_argument("--serving-rate", type=float, default=40.0)
+    ap.add_argument("--serving-rate", type=float, default=1000.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
@@ -179,7 +192,7 @@ def main():
         from deconlib_amd.autotune import autotune
 
         t0 = time.time()
-        tiles = autotune(blur, verbose=True)
+        tiles = autotune(blur, verbose=True, cache_file=args.tune_cache)
         log(f"autotune {time.time() - t0:.1f}s: {tiles}")
     st = _RLState(obs, blur, background=0.0, num_iter=args.warmup + args.steps + 1, eval_interval=10)
     for k in range(args.warmup):
@@ -202,7 +215,8 @@ def main():
     vox = int(np.prod(SHAPE))
     value = vox * args.steps / dt
     P = blur.canvas_shape
-    pb = conv_pass_bytes(SHAPE, P)
+    pb = conv_pass_bytes(SHAPE, P, epi_operand_bytes=6)
+    it_bytes = rl_iteration_bytes(SHAPE, P)
     per_pass = {}
     for i, name in enumerate(N.PASS_NAMES):
         if cnt[i]:
@@ -264,6 +278,8 @@ def main():
             "avg_ms": z["avg_ms"],
         },
         "passes": per_pass,
+        "iteration": {"algorithmic_bytes": it_bytes,
+                      "hbm_frac": it_bytes / (dt / args.steps) / (PEAK_HBM_GBS * 1e9)},
         "pdhg": pdhg,
         "serving": serving,
         "cpu_baseline": cpu,

@@ -21,13 +21,14 @@ if not _LIB_PATH.exists():
 
 lib = C.CDLL(str(_LIB_PATH))
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 DH_OK, DH_EINVAL, DH_EHIP, DH_EWORKSPACE, DH_EUNSUPPORTED = 0, -1, -2, -3, -4
 
 PRO_NONE, PRO_ONES, PRO_SUB_FLOOR = 0, 1, 2
 EPI_NONE, EPI_FLOOR, EPI_RL_RATIO, EPI_RL_UPDATE, EPI_ADD = 0, 1, 2, 3, 4
 RED_SUM, RED_MAX, RED_SUMSQ = 0, 1, 2
+TILE_PERSISTENT = 0x10000  # dh_conv_plan_set_tile flag: persistent grid with next-tile prefetch
 
 
 class PlanInfo(C.Structure):
@@ -40,6 +41,7 @@ class PlanInfo(C.Structure):
         ("otf_bytes", C.c_int64),
         ("work_bytes", C.c_int64),
         ("twiddle_bytes", C.c_int64),
+        ("spectrum_block", C.c_int32),
     ]
 
 

@@ -2,8 +2,9 @@
 
 For one operator, each of the five passes is timed (HIP events around the
 pass, ``dh_timing_*``) for every feasible workgroup tile (columns per
-workgroup for the Y and Z passes, rows per workgroup for the X passes) and
-the fastest is kept on the plan.  The choice is semantics-free: every tile
+workgroup for the Y and Z passes, rows per workgroup for the X passes), each
+launched either one workgroup per tile or as a persistent grid, and the
+fastest is kept on the plan.  The choice is semantics-free: every launch shape
 computes the same values.  Results are cached per (canvas, signal) shape in
 memory and, optionally, in a JSON file.
 """
@@ -21,6 +22,10 @@ from . import _native as N
 from ._tensor import empty_f32
 
 _CACHE: Dict[str, Dict[int, int]] = {}
+
+
+def _name(tile: int) -> str:
+    return f"{tile & 0xFFFF}p" if tile & N.TILE_PERSISTENT else str(tile)
 
 
 def _key(plan) -> str:
@@ -68,9 +73,10 @@ def autotune(op, reps: int = 10, candidates=(1, 2, 4, 8, 16, 32, 64), cache_file
     for p in range(5):
         timings = {}
         for t in candidates:
-            if N.fns["dh_conv_plan_set_tile"](plan._h, p, t) != 0:
-                continue
-            timings[t] = _time_passes(op, x, reps)[p]
+            for v in (t, t | N.TILE_PERSISTENT):
+                if N.fns["dh_conv_plan_set_tile"](plan._h, p, v) != 0:
+                    continue
+                timings[v] = _time_passes(op, x, reps)[p]
         N.call("dh_conv_plan_set_tile", plan._h, p, 0)
         if timings:
             choice = min(timings, key=timings.get)
@@ -80,8 +86,8 @@ def autotune(op, reps: int = 10, candidates=(1, 2, 4, 8, 16, 32, 64), cache_file
                 best[p] = choice
             if verbose:
                 print(f"[autotune] pass {N.PASS_NAMES[p]}: " +
-                      ", ".join(f"{t}:{v * 1e3:.1f}us" for t, v in sorted(timings.items())) +
-                      f" default {default * 1e3:.1f}us -> {best.get(p, 'default')}")
+                      ", ".join(f"{_name(t)}:{v * 1e3:.1f}us" for t, v in sorted(timings.items())) +
+                      f" default {default * 1e3:.1f}us -> {_name(best[p]) if p in best else 'default'}")
     _CACHE[key] = best
     if cache_file:
         data = {}

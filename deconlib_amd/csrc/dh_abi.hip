@@ -141,18 +141,30 @@ constexpr int kCtMaxThreads = 512;
 constexpr int64_t kCtDefaultLds = 64 * 1024;
 constexpr int64_t kCtMaxLds = 160 * 1024;
 
-bool ct_col_tile_ok(int n, int L) {
-  return L >= 1 && L * dh::ct_threads(n) <= kCtMaxThreads && (int64_t)L * n * 8 <= kCtMaxLds;
+// Column kernels: the fused Z pass (LDS n*L), a plain-layout column pass and a
+// blocked-layout (Y) column pass, whose tile is at most one block or whole blocks.
+enum ColKind { COL_Z = 0, COL_PLAIN = 1, COL_BLOCKED = 2 };
+
+int64_t col_lds_bytes(int n, int L, int kind) {
+  return 8 * (kind == COL_Z ? (int64_t)n * L : dh::col_lds_elems(n, L));
 }
+bool ct_col_tile_ok(int n, int L, int kind) {
+  if (L < 1 || L * dh::ct_threads(n) > kCtMaxThreads || col_lds_bytes(n, L, kind) > kCtMaxLds) return false;
+  return kind != COL_BLOCKED || L <= dh::kBlk || L % dh::kBlk == 0;
+}
+// rows per workgroup: a power of two (the blocked row walk relies on it)
 bool ct_row_tile_ok(int m, int R) {
-  return R >= 1 && R * dh::ct_threads(m) <= kCtMaxThreads && (int64_t)R * (m + 1) * 8 <= kCtMaxLds;
+  return R >= 1 && (R & (R - 1)) == 0 && R * dh::ct_threads(m) <= kCtMaxThreads &&
+         dh::xrow_lds_elems(m, R) * 8 <= kCtMaxLds;
 }
 // columns per workgroup: widest power of two within 512 threads and 64 KiB LDS
-int ct_col_tile(int n, int64_t ncols, int over) {
-  if (over > 0 && ct_col_tile_ok(n, over)) return over;
+// (one block for the blocked layout)
+int ct_col_tile(int n, int64_t ncols, int over, int kind) {
+  if (over > 0 && ct_col_tile_ok(n, over, kind)) return over;
   const int T = dh::ct_threads(n);
+  const int cap = kind == COL_BLOCKED ? dh::kBlk : 64;
   int L = 1;
-  while (L < 64 && 2 * L * T <= kCtMaxThreads && (int64_t)2 * L * n * 8 <= kCtDefaultLds) L *= 2;
+  while (L < cap && 2 * L * T <= kCtMaxThreads && col_lds_bytes(n, 2 * L, kind) <= kCtDefaultLds) L *= 2;
   while (L > 1 && L / 2 >= ncols) L /= 2;
   return L;
 }
@@ -160,8 +172,20 @@ int ct_row_tile(int m, int over) {
   if (over > 0 && ct_row_tile_ok(m, over)) return over;
   const int T = dh::ct_threads(m);
   int R = 1;
-  while (R < 64 && 2 * R * T <= 256 && (int64_t)2 * R * (m + 1) * 8 <= kCtDefaultLds) R *= 2;
+  while (R < 64 && 2 * R * T <= 256 && dh::xrow_lds_elems(m, 2 * R) * 8 <= kCtDefaultLds) R *= 2;
   return R;
+}
+
+static_assert(DH_TILE_PERSISTENT == dh::kTilePersistent, "tile flag mismatch");
+
+// Tile override of a pass (0 = heuristic) and whether it launches persistently
+// (by default only the X inverse pass, whose epilogue operands make a second
+// dependent memory round trip per tile that the next-tile prefetch hides).
+int tile_override(const dh_conv_plan* p, int pass) { return p->tile[pass] & dh::kTileMask; }
+int persist_flag(const dh_conv_plan* p, int pass) {
+  const int v = p->tile[pass];
+  const bool on = v > 0 ? (v & dh::kTilePersistent) != 0 : pass == DH_PASS_X_INV;
+  return on ? dh::kTilePersistent : 0;
 }
 
 bool aligned8(const void* p) { return ((uintptr_t)p & 7) == 0; }
@@ -179,70 +203,80 @@ void pack_one(dh::Pack<A>& pk, const A& a, int64_t nblk) {
   pk.nvol = 1;
 }
 
+// The five pass descriptors of one application with the specialised kernels,
+// with the launch tile and workgroup count of each.  The X/Y spectra in the
+// workspace use the blocked layout.
+struct CtPasses {
+  dh::XfArgs xf;
+  dh::ColArgs yf, yi;
+  dh::ZArgs z;
+  dh::XiArgs xi;
+  int t_xf = 0, t_yf = 0, t_z = 0, t_yi = 0, t_xi = 0;
+  int64_t n_xf = 0, n_yf = 0, n_z = 0, n_yi = 0, n_xi = 0;
+  bool has_y = false;
+};
+
+CtPasses ct_passes(const dh_conv_plan* p, int adjoint, const float* in, float* out, const void* otf, char* ws,
+                   const dh::Pro& pro, const dh::Epi& epi) {
+  CtPasses c;
+  float2* B1 = (float2*)(ws + p->b1_off);
+  float2* B2 = (float2*)(ws + p->b2_off);
+  const int s0 = p->s[0], s1 = p->s[1], s2 = p->s[2], P1 = p->P[1];
+  const int64_t H = p->H, nrows = (int64_t)s0 * s1;
+  const float2* tw = p->tw_dev;
+  c.t_xf = ct_row_tile(p->M, tile_override(p, DH_PASS_X_FWD));
+  c.n_xf = (nrows + c.t_xf - 1) / c.t_xf;
+  c.t_xf |= persist_flag(p, DH_PASS_X_FWD);
+  c.xf = dh::XfArgs{in, B1, tw + p->tw_xm, tw + p->tw_xp, nrows, s2, s2, 0, (int)H, pro, xf_vec2(in, s2),
+                    dh::kBlk, s1};
+  c.has_y = P1 > 1;
+  if (c.has_y) {
+    c.t_yf = ct_col_tile(P1, H, tile_override(p, DH_PASS_Y_FWD), COL_BLOCKED);
+    const int64_t nt = (H + c.t_yf - 1) / c.t_yf;
+    c.n_yf = nt * s0;
+    c.t_yf |= persist_flag(p, DH_PASS_Y_FWD);
+    c.yf = dh::ColArgs{B1, B2, tw + p->tw_y, H, (int64_t)s1 * H, (int64_t)P1 * H, s1, P1, nt, s1, 0, P1, 1.0f,
+                       dh::kBlk};
+    c.t_yi = ct_col_tile(P1, H, tile_override(p, DH_PASS_Y_INV), COL_BLOCKED);
+    const int64_t nti = (H + c.t_yi - 1) / c.t_yi;
+    c.n_yi = nti * s0;
+    c.t_yi |= persist_flag(p, DH_PASS_Y_INV);
+    c.yi = dh::ColArgs{B2, B1, tw + p->tw_y, H, (int64_t)P1 * H, (int64_t)s1 * H, P1, s1, nti, P1, 0, s1, 1.0f,
+                       dh::kBlk};
+  }
+  const int64_t ncols = (int64_t)P1 * H;
+  c.t_z = p->P[0] == 1 ? 256 : ct_col_tile(p->P[0], ncols, tile_override(p, DH_PASS_Z), COL_Z);
+  c.n_z = (ncols + c.t_z - 1) / c.t_z;
+  c.t_z |= persist_flag(p, DH_PASS_Z);
+  c.z = dh::ZArgs{c.has_y ? B2 : B1, (const float2*)otf, tw + p->tw_z, ncols, s0, s0, adjoint ? 1 : 0};
+  c.t_xi = ct_row_tile(p->M, tile_override(p, DH_PASS_X_INV));
+  c.n_xi = (nrows + c.t_xi - 1) / c.t_xi;
+  c.t_xi |= persist_flag(p, DH_PASS_X_INV);
+  c.xi = dh::XiArgs{B1, out, tw + p->tw_xm, tw + p->tw_xp, nrows, s2, (int)H, epi, xi_vec2(out, s2, epi),
+                    dh::kBlk, s1};
+  return c;
+}
+
+template <class A>
+int launch_one(int kind, int n, int dir, const A& a, int tile, int64_t nb, int pass, const char* what,
+               hipStream_t st) {
+  dh::Pack<A> pk;
+  pack_one(pk, a, nb);
+  DH_TRY(timing_begin(pass, st));
+  DH_TRY(hip_check(dh::ct_launch(kind, n, dir, &pk, tile, nb, st), what));
+  return timing_end(st);
+}
+
 // One application with the specialised kernels.
 int ct_apply(const dh_conv_plan* p, int adjoint, const float* in, float* out, const void* otf, char* ws,
              const dh::Pro& pro, const dh::Epi& epi, int64_t* xi_blocks, hipStream_t st) {
-  float2* B1 = (float2*)(ws + p->b1_off);
-  float2* B2 = (float2*)(ws + p->b2_off);
-  const int s0 = p->s[0], s1 = p->s[1], s2 = p->s[2];
-  const int64_t H = p->H;
-  const int64_t nrows = (int64_t)s0 * s1;
-  const float2* tw = p->tw_dev;
-  {
-    const int R = ct_row_tile(p->M, p->tile[DH_PASS_X_FWD]);
-    dh::Pack<dh::XfArgs> pk;
-    const int64_t nb = (nrows + R - 1) / R;
-    pack_one(pk, dh::XfArgs{in, B1, tw + p->tw_xm, tw + p->tw_xp, nrows, s2, s2, 0, (int)H, pro, xf_vec2(in, s2)},
-             nb);
-    DH_TRY(timing_begin(DH_PASS_X_FWD, st));
-    DH_TRY(hip_check(dh::ct_launch(dh::CT_XF, p->M, -1, &pk, R, nb, st), "x forward pass"));
-    DH_TRY(timing_end(st));
-  }
-  float2* zb = B1;
-  if (p->P[1] > 1) {
-    const int L = ct_col_tile(p->P[1], H, p->tile[DH_PASS_Y_FWD]);
-    const int64_t nt = (H + L - 1) / L;
-    dh::Pack<dh::ColArgs> pk;
-    pack_one(pk, dh::ColArgs{B1, B2, tw + p->tw_y, H, (int64_t)s1 * H, (int64_t)p->P[1] * H, H, H, nt, s1, 0,
-                             p->P[1], 1.0f},
-             nt * s0);
-    DH_TRY(timing_begin(DH_PASS_Y_FWD, st));
-    DH_TRY(hip_check(dh::ct_launch(dh::CT_COL, p->P[1], -1, &pk, L, nt * s0, st), "y forward pass"));
-    DH_TRY(timing_end(st));
-    zb = B2;
-  }
-  {
-    const int64_t ncols = (int64_t)p->P[1] * H;
-    const int L = p->P[0] == 1 ? 256 : ct_col_tile(p->P[0], ncols, p->tile[DH_PASS_Z]);
-    const int64_t nb = (ncols + L - 1) / L;
-    dh::Pack<dh::ZArgs> pk;
-    pack_one(pk, dh::ZArgs{zb, (const float2*)otf, tw + p->tw_z, ncols, s0, s0, adjoint ? 1 : 0}, nb);
-    DH_TRY(timing_begin(DH_PASS_Z, st));
-    DH_TRY(hip_check(dh::ct_launch(dh::CT_Z, p->P[0], 0, &pk, L, nb, st), "z pass"));
-    DH_TRY(timing_end(st));
-  }
-  if (p->P[1] > 1) {
-    const int L = ct_col_tile(p->P[1], H, p->tile[DH_PASS_Y_INV]);
-    const int64_t nt = (H + L - 1) / L;
-    dh::Pack<dh::ColArgs> pk;
-    pack_one(pk, dh::ColArgs{B2, B1, tw + p->tw_y, H, (int64_t)p->P[1] * H, (int64_t)s1 * H, H, H, nt, p->P[1], 0,
-                             s1, 1.0f},
-             nt * s0);
-    DH_TRY(timing_begin(DH_PASS_Y_INV, st));
-    DH_TRY(hip_check(dh::ct_launch(dh::CT_COL, p->P[1], 1, &pk, L, nt * s0, st), "y inverse pass"));
-    DH_TRY(timing_end(st));
-  }
-  {
-    const int R = ct_row_tile(p->M, p->tile[DH_PASS_X_INV]);
-    const int64_t nb = (nrows + R - 1) / R;
-    dh::Pack<dh::XiArgs> pk;
-    pack_one(pk, dh::XiArgs{B1, out, tw + p->tw_xm, tw + p->tw_xp, nrows, s2, (int)H, epi, xi_vec2(out, s2, epi)},
-             nb);
-    DH_TRY(timing_begin(DH_PASS_X_INV, st));
-    DH_TRY(hip_check(dh::ct_launch(dh::CT_XI, p->M, 1, &pk, R, nb, st), "x inverse pass"));
-    DH_TRY(timing_end(st));
-    *xi_blocks = nb;
-  }
+  const CtPasses c = ct_passes(p, adjoint, in, out, otf, ws, pro, epi);
+  DH_TRY(launch_one(dh::CT_XF, p->M, -1, c.xf, c.t_xf, c.n_xf, DH_PASS_X_FWD, "x forward pass", st));
+  if (c.has_y) DH_TRY(launch_one(dh::CT_COL, p->P[1], -1, c.yf, c.t_yf, c.n_yf, DH_PASS_Y_FWD, "y forward pass", st));
+  DH_TRY(launch_one(dh::CT_Z, p->P[0], 0, c.z, c.t_z, c.n_z, DH_PASS_Z, "z pass", st));
+  if (c.has_y) DH_TRY(launch_one(dh::CT_COL, p->P[1], 1, c.yi, c.t_yi, c.n_yi, DH_PASS_Y_INV, "y inverse pass", st));
+  DH_TRY(launch_one(dh::CT_XI, p->M, 1, c.xi, c.t_xi, c.n_xi, DH_PASS_X_INV, "x inverse pass", st));
+  *xi_blocks = c.n_xi;
   return DH_OK;
 }
 
@@ -386,6 +420,7 @@ int dh_conv_plan_info(const dh_conv_plan* p, dh_plan_info* info) {
   info->otf_bytes = (int64_t)p->P[0] * p->P[1] * p->H * 8;
   info->work_bytes = p->ws_bytes;
   info->twiddle_bytes = (int64_t)p->tw_host.size() * 8;
+  info->spectrum_block = p->ct ? dh::kBlk : 0;
   return DH_OK;
 }
 
@@ -405,8 +440,13 @@ int dh_conv_plan_set_tile(dh_conv_plan* p, int pass, int tile) {
   if (tile > 0) {
     const bool row = (pass == DH_PASS_X_FWD || pass == DH_PASS_X_INV);
     const int n = row ? p->M : (pass == DH_PASS_Z ? p->P[0] : p->P[1]);
-    const bool ok = p->ct ? (row ? ct_row_tile_ok(n, tile) : ct_col_tile_ok(n, tile))
-                          : (row ? dh::row_tile_ok(n, tile) : dh::col_tile_ok(n, tile));
+    const int kind = pass == DH_PASS_Z ? COL_Z : COL_BLOCKED;
+    const int t = tile & dh::kTileMask;
+    if (tile & ~(dh::kTileMask | dh::kTilePersistent)) return fail(DH_EINVAL, "unknown tile flags");
+    if ((tile & dh::kTilePersistent) && !p->ct)
+      return fail(DH_EINVAL, "persistent launches need a canvas with specialised kernels");
+    const bool ok = p->ct ? (row ? ct_row_tile_ok(n, t) : ct_col_tile_ok(n, t, kind))
+                          : (row ? dh::row_tile_ok(n, t) : dh::col_tile_ok(n, t));
     if (!ok) return fail(DH_EINVAL, "tile does not fit the workgroup limits (threads, 160 KiB LDS)");
   }
   p->tile[pass] = tile;
@@ -418,7 +458,10 @@ int dh_conv_plan_get_tile(const dh_conv_plan* p, int pass) {
   const bool row = (pass == DH_PASS_X_FWD || pass == DH_PASS_X_INV);
   const int n = row ? p->M : (pass == DH_PASS_Z ? p->P[0] : p->P[1]);
   const int64_t ncols = (pass == DH_PASS_Z) ? (int64_t)p->P[1] * p->H : p->H;
-  if (p->ct) return row ? ct_row_tile(n, p->tile[pass]) : ct_col_tile(n, ncols, p->tile[pass]);
+  if (p->ct)
+    return persist_flag(p, pass) |
+           (row ? ct_row_tile(n, tile_override(p, pass))
+                : ct_col_tile(n, ncols, tile_override(p, pass), pass == DH_PASS_Z ? COL_Z : COL_BLOCKED));
   if (row) return dh::row_launch_for(n, p->tile[pass]).R;
   return dh::col_launch_for(n, ncols, p->tile[pass]).L;
 }
@@ -457,24 +500,25 @@ int dh_otf_build(const dh_conv_plan* p, const float* kernel_dev, int normalize, 
     const int64_t nbx = (krows + R - 1) / R;
     dh::Pack<dh::XfArgs> px;
     pack_one(px, dh::XfArgs{kernel_dev, B1, tw + p->tw_xm, tw + p->tw_xp, krows, p->k[2], p->klo[2], p->khi[2],
-                            (int)H, pro, 0},
+                            (int)H, pro, 0, dh::kBlk, p->k[1]},
              nbx);
     DH_TRY(hip_check(dh::ct_launch(dh::CT_XF, p->M, -1, &px, R, nbx, st), "otf x pass"));
     if (p->P[1] > 1) {
-      const int L = ct_col_tile(p->P[1], H, 0);
+      const int L = ct_col_tile(p->P[1], H, 0, COL_BLOCKED);
       const int64_t nt = (H + L - 1) / L;
       dh::Pack<dh::ColArgs> py;
-      pack_one(py, dh::ColArgs{B1, B2, tw + p->tw_y, H, (int64_t)p->k[1] * H, (int64_t)p->P[1] * H, H, H, nt,
-                               p->klo[1], p->khi[1], p->P[1], 1.0f},
+      pack_one(py, dh::ColArgs{B1, B2, tw + p->tw_y, H, (int64_t)p->k[1] * H, (int64_t)p->P[1] * H, p->k[1],
+                               p->P[1], nt, p->klo[1], p->khi[1], p->P[1], 1.0f, dh::kBlk},
                nt * p->k[0]);
       DH_TRY(hip_check(dh::ct_launch(dh::CT_COL, p->P[1], -1, &py, L, nt * p->k[0], st), "otf y pass"));
       zsrc = B2;
     }
-    const int L = ct_col_tile(p->P[0], ncols, 0);
+    // the Z transform treats each (blocked) plane as a flat row of ncols columns
+    const int L = ct_col_tile(p->P[0], ncols, 0, COL_PLAIN);
     const int64_t nt = (ncols + L - 1) / L;
     dh::Pack<dh::ColArgs> pz;
     pack_one(pz, dh::ColArgs{zsrc, (float2*)otf_dev, tw + p->tw_z, ncols, 0, 0, ncols, ncols, nt, p->klo[0],
-                             p->khi[0], p->P[0], (float)(1.0 / total)},
+                             p->khi[0], p->P[0], (float)(1.0 / total), 0},
              nt);
     DH_TRY(hip_check(dh::ct_launch(dh::CT_COL, p->P[0], -1, &pz, L, nt, st), "otf z pass"));
   } else {
@@ -776,51 +820,19 @@ int ct_batch_apply(int nitems, const dh_batch_item* items, int adjoint, hipStrea
     DH_TRY(fill_fuse(&it.fuse, pro, epi, &red));
     if (!it.in && pro.kind != DH_PRO_ONES) return fail(DH_EINVAL, "null input");
     char* ws = (char*)it.work;
-    float2* B1 = (float2*)(ws + p->b1_off);
-    float2* B2 = (float2*)(ws + p->b2_off);
-    const float2* tw = p->tw_dev;
-    const int s0 = p->s[0], s1 = p->s[1], s2 = p->s[2];
-    const int64_t H = p->H, nrows = (int64_t)s0 * s1;
-    {
-      const int R = ct_row_tile(p->M, p->tile[DH_PASS_X_FWD]);
-      add_to_group(gxf, p->M, R,
-                   dh::XfArgs{it.in, B1, tw + p->tw_xm, tw + p->tw_xp, nrows, s2, s2, 0, (int)H, pro, xf_vec2(it.in, s2)},
-                   (nrows + R - 1) / R);
+    if (red) epi.partials = (double*)(ws + p->part_off);
+    const CtPasses c = ct_passes(p, adjoint, it.in, it.out, it.otf, ws, pro, epi);
+    if (red) {
+      if (c.n_xi > p->n_partials) return fail(DH_EWORKSPACE, "partials area too small");
+      reds.push_back(Red{epi.partials, c.n_xi, red});
     }
-    float2* zb = B1;
-    if (p->P[1] > 1) {
-      const int L = ct_col_tile(p->P[1], H, p->tile[DH_PASS_Y_FWD]);
-      const int64_t nt = (H + L - 1) / L;
-      add_to_group(gyf, p->P[1], L,
-                   dh::ColArgs{B1, B2, tw + p->tw_y, H, (int64_t)s1 * H, (int64_t)p->P[1] * H, H, H, nt, s1, 0,
-                               p->P[1], 1.0f},
-                   nt * s0);
-      const int Li = ct_col_tile(p->P[1], H, p->tile[DH_PASS_Y_INV]);
-      const int64_t nti = (H + Li - 1) / Li;
-      add_to_group(gyi, p->P[1], Li,
-                   dh::ColArgs{B2, B1, tw + p->tw_y, H, (int64_t)p->P[1] * H, (int64_t)s1 * H, H, H, nti, p->P[1],
-                               0, s1, 1.0f},
-                   nti * s0);
-      zb = B2;
+    add_to_group(gxf, p->M, c.t_xf, c.xf, c.n_xf);
+    if (c.has_y) {
+      add_to_group(gyf, p->P[1], c.t_yf, c.yf, c.n_yf);
+      add_to_group(gyi, p->P[1], c.t_yi, c.yi, c.n_yi);
     }
-    {
-      const int64_t ncols = (int64_t)p->P[1] * H;
-      const int L = p->P[0] == 1 ? 256 : ct_col_tile(p->P[0], ncols, p->tile[DH_PASS_Z]);
-      add_to_group(gz, p->P[0], L, dh::ZArgs{zb, (const float2*)it.otf, tw + p->tw_z, ncols, s0, s0, adjoint ? 1 : 0},
-                   (ncols + L - 1) / L);
-    }
-    {
-      const int R = ct_row_tile(p->M, p->tile[DH_PASS_X_INV]);
-      const int64_t nb = (nrows + R - 1) / R;
-      if (red) {
-        if (nb > p->n_partials) return fail(DH_EWORKSPACE, "partials area too small");
-        epi.partials = (double*)(ws + p->part_off);
-        reds.push_back(Red{epi.partials, nb, red});
-      }
-      add_to_group(gxi, p->M, R,
-                   dh::XiArgs{B1, it.out, tw + p->tw_xm, tw + p->tw_xp, nrows, s2, (int)H, epi, xi_vec2(it.out, s2, epi)},
-                   nb);
-    }
+    add_to_group(gz, p->P[0], c.t_z, c.z, c.n_z);
+    add_to_group(gxi, p->M, c.t_xi, c.xi, c.n_xi);
   }
   DH_TRY(launch_groups(dh::CT_XF, -1, gxf, DH_PASS_X_FWD, st));
   DH_TRY(launch_groups(dh::CT_COL, -1, gyf, DH_PASS_Y_FWD, st));

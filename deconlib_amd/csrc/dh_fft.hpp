@@ -87,13 +87,34 @@ hipError_t launch_sum_partials(const double* partials, int64_t n, double* out, h
 int threads_per_line(int n);
 
 // ---- compile-time specialised passes (dh_fftct*.hpp): per-volume arguments ----
+//
+// Blocked spectrum layout (the X/Y spectra of specialised plans and their
+// OTF): a plane of `rows` rows and H columns is stored as ceil(H / kBlk)
+// column blocks one after another.  Block b holds columns [b*kBlk, b*kBlk+w_b),
+// w_b = min(kBlk, H - b*kBlk), row-major inside the block, so element (r, c)
+// sits at b*kBlk*rows + r*w_b + (c - b*kBlk).  The plane stays dense (rows*H
+// elements); a Y pass reads kBlk columns of consecutive rows as one contiguous
+// run, and the X passes write/read kBlk samples of several rows contiguously.
+constexpr int kBlk = 8;
+
+// LDS elements of the specialised kernels: one column workgroup of L lines of
+// length n (8 spare slots every 16 line elements), and one X row of a length-m
+// transform (one spare slot every 32 elements, indices 0..m used).
+constexpr int64_t col_lds_elems(int n, int L) { return (int64_t)n * L + 8 * (n >> 4) + 8; }
+constexpr int xrow_stride(int m) { return m + (m >> 5) + 1; }
+// X workgroup of R rows: the rows, then the twiddle tables (m + m + 1 entries).
+constexpr int64_t xrow_lds_elems(int m, int R) { return (int64_t)R * xrow_stride(m) + 2 * m + 1; }
+
 struct ColArgs {  // Y forward/inverse, OTF Z: columns of (batch, n, ncols) planes
   const float2* src;
   float2* dst;
   const float2* tw;
+  // blk == 0: element (row r, column c) at r*src_rs + c (dst_rs for dst);
+  // blk  > 0: blocked layout, src_rs/dst_rs = rows of the source/destination plane
   int64_t ncols, src_b, dst_b, src_rs, dst_rs, ntiles;
   int lo, hi, n_out;
   float scale;
+  int blk;
 };
 struct ZArgs {  // fused Z: forward, x OTF, inverse, in place
   float2* buf;
@@ -103,7 +124,7 @@ struct ZArgs {  // fused Z: forward, x OTF, inverse, in place
   int lo, n_out, conj;
 };
 struct XfArgs {  // X real-to-complex rows
-  const float* src;
+  const float* src;  // may be null with PRO_ONES, which never reads it
   float2* dst;
   const float2* twm;
   const float2* twp;
@@ -111,6 +132,8 @@ struct XfArgs {  // X real-to-complex rows
   int rowlen, lo, hi, H;
   Pro pro;
   int vec2;  // rows start 8-byte aligned and hi == 0: load sample pairs as float2
+  int blk;   // 0: spectrum rows at row*H; > 0: blocked layout with prow rows per plane
+  int prow;
 };
 struct XiArgs {  // X complex-to-real rows with the epilogue
   const float2* src;
@@ -121,6 +144,8 @@ struct XiArgs {  // X complex-to-real rows with the epilogue
   int s2, H;
   Epi epi;
   int vec2;  // output and epilogue operands 8-byte aligned per row: float2 accesses
+  int blk;   // as XfArgs
+  int prow;
 };
 
 // Up to kPack volumes of one transform length per launch, passed by value.
@@ -138,7 +163,11 @@ bool ct_supported(int n);
 // Threads per line of the specialised kernels for length n.
 int ct_threads(int n);
 // Launches kind for length n; pk points to the matching Pack<...>.
-// tile = columns (CT_COL, CT_Z) or rows (CT_XF, CT_XI) per workgroup.
+// tile = columns (CT_COL, CT_Z) or rows (CT_XF, CT_XI) per workgroup, plus
+// kTilePersistent for a persistent grid (one workgroup per resident slot, each
+// walking tiles with a next-tile prefetch) instead of one workgroup per tile.
+constexpr int kTilePersistent = 0x10000;
+constexpr int kTileMask = 0xFFFF;
 hipError_t ct_launch(int kind, int n, int dir, const void* pk, int tile, int64_t nblocks, hipStream_t st);
 
 // ---- pass ids ----

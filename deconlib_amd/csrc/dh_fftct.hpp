@@ -67,12 +67,66 @@ struct Geo {
   static constexpr int T = N / rmax(make_sched(N));  // threads per line
 };
 
-// One Stockham pass with everything compile-time; t < T.
-template <int N, int T, int R, int P, int DIR, class Ld, class St>
-__device__ __forceinline__ void pass(int t, const float2* __restrict__ tw, Ld ld, St st, bool sync_mid) {
+// u[r] *= w^r for r = 1..R-1, w = tw[b] (conjugated when DIR > 0), where
+// tw[r*b] = w^r.  Loads w, w^2, w^4, w^8 from the table and forms the other
+// powers with at most three complex products, so a radix-16 butterfly issues 4
+// table loads instead of 15.
+template <int R, int DIR>
+__device__ __forceinline__ void twiddle(float2 (&u)[R], const float2* __restrict__ tw, int b) {
+  auto ld = [&](int r) {
+    float2 w = tw[r * b];
+    if (DIR > 0) w.y = -w.y;
+    return w;
+  };
+  if constexpr (R == 2) {
+    u[1] = cmul(u[1], ld(1));
+  } else if constexpr (R == 3) {
+    u[1] = cmul(u[1], ld(1));
+    u[2] = cmul(u[2], ld(2));
+  } else if constexpr (R == 4 || R == 5) {
+    const float2 w1 = ld(1), w2 = ld(2);
+    u[1] = cmul(u[1], w1);
+    u[2] = cmul(u[2], w2);
+    u[3] = cmul(u[3], cmul(w1, w2));
+    if constexpr (R == 5) u[4] = cmul(u[4], ld(4));
+  } else if constexpr (R == 8 || R == 16) {
+    const float2 w1 = ld(1), w2 = ld(2), w4 = ld(4);
+    const float2 w3 = cmul(w1, w2), w5 = cmul(w1, w4), w6 = cmul(w2, w4), w7 = cmul(w3, w4);
+    u[1] = cmul(u[1], w1);
+    u[2] = cmul(u[2], w2);
+    u[3] = cmul(u[3], w3);
+    u[4] = cmul(u[4], w4);
+    u[5] = cmul(u[5], w5);
+    u[6] = cmul(u[6], w6);
+    u[7] = cmul(u[7], w7);
+    if constexpr (R == 16) {
+      const float2 w8 = ld(8);
+      u[8] = cmul(u[8], w8);
+      u[9] = cmul(u[9], cmul(w1, w8));
+      u[10] = cmul(u[10], cmul(w2, w8));
+      u[11] = cmul(u[11], cmul(w3, w8));
+      u[12] = cmul(u[12], cmul(w4, w8));
+      u[13] = cmul(u[13], cmul(w5, w8));
+      u[14] = cmul(u[14], cmul(w6, w8));
+      u[15] = cmul(u[15], cmul(w7, w8));
+    }
+  } else {
+#pragma unroll
+    for (int r = 1; r < R; ++r) u[r] = cmul(u[r], ld(r));
+  }
+}
+
+// Butterflies per thread of a radix-R pass over N points with T threads.
+template <int N, int T, int R>
+struct Bfly {
+  static constexpr int M = (N / R + T - 1) / T;
+};
+
+// Register operands of one Stockham pass: u[m][r] = element i + r*N/R, i = t + m*T.
+template <int N, int T, int R, class Ld>
+__device__ __forceinline__ void pass_load(float2 (&u)[(N / R + T - 1) / T][R], int t, Ld ld) {
   constexpr int NB = N / R;
-  constexpr int M = (NB + T - 1) / T;
-  float2 u[M][R];
+  constexpr int M = Bfly<N, T, R>::M;
 #pragma unroll
   for (int m = 0; m < M; ++m) {
     const int i = t + m * T;
@@ -81,27 +135,36 @@ __device__ __forceinline__ void pass(int t, const float2* __restrict__ tw, Ld ld
       for (int r = 0; r < R; ++r) u[m][r] = ld(i + r * NB);
     }
   }
-  if (sync_mid) __syncthreads();
+}
+
+// Twiddles, butterflies and the Stockham-ordered stores of one pass.
+template <int N, int T, int R, int P, int DIR, class St>
+__device__ __forceinline__ void pass_store(float2 (&u)[(N / R + T - 1) / T][R], int t, const float2* __restrict__ tw,
+                                           St st) {
+  constexpr int NB = N / R;
+  constexpr int M = Bfly<N, T, R>::M;
   constexpr int step = N / (P * R);
 #pragma unroll
   for (int m = 0; m < M; ++m) {
     const int i = t + m * T;
     if ((M * T == NB) || i < NB) {
       const int k = i % P;
-      if (P > 1) {
-#pragma unroll
-        for (int r = 1; r < R; ++r) {
-          float2 w = tw[r * k * step];
-          if (DIR > 0) w.y = -w.y;
-          u[m][r] = cmul(u[m][r], w);
-        }
-      }
+      if (P > 1) twiddle<R, DIR>(u[m], tw, k * step);
       Dft<R, DIR>::run(u[m]);
       const int j = (i / P) * P * R + k;
 #pragma unroll
       for (int r = 0; r < R; ++r) st(j + r * P, u[m][r]);
     }
   }
+}
+
+// One Stockham pass with everything compile-time; t < T.
+template <int N, int T, int R, int P, int DIR, class Ld, class St>
+__device__ __forceinline__ void pass(int t, const float2* __restrict__ tw, Ld ld, St st, bool sync_mid) {
+  float2 u[Bfly<N, T, R>::M][R];
+  pass_load<N, T, R>(u, t, ld);
+  if (sync_mid) __syncthreads();
+  pass_store<N, T, R, P, DIR>(u, t, tw, st);
 }
 
 // Passes [p, S.np) of schedule S; the first uses gld, the last gst, the rest

@@ -5,6 +5,8 @@
 // so a single volume needs no descriptor upload and a batch of volumes of
 // one length is a single launch per pass.
 #pragma once
+#include <mutex>
+
 #include "dh_fftct.hpp"
 
 namespace dh {
@@ -108,39 +110,113 @@ __device__ __forceinline__ double bsum(double v, double* scratch) {
 }
 
 // ---- column pass (Y forward / inverse, OTF Z) ----
-template <int N, int DIR>
-__device__ __forceinline__ void col_body(const ColArgs& a, int64_t lb, int L, float2* lds) {
-  constexpr int T = Geo<N>::T;
-  const int ln = threadIdx.x % L;
-  const int t = threadIdx.x / L;
+// LDS slot of line element e, column ln: 8 extra slots every 16 elements spread
+// a wave's (rows x columns) accesses over the banks.
+__device__ __forceinline__ int col_lds(int e, int ln, int L) { return e * L + ln + 8 * (e >> 4); }
+
+// Hides the lane coordinates from loop-invariant code motion in the persistent
+// loops below: otherwise every pass's LDS and twiddle addresses are hoisted out
+// of the tile loop and stay live, which doubles the register count.  Emits no
+// instruction.
+__device__ __forceinline__ void opaque(int& a, int& b) { asm volatile("" : "+v"(a), "+v"(b)); }
+__device__ __forceinline__ void opaque(int& a) { asm volatile("" : "+v"(a)); }
+
+// Addressing of one lane's column in one tile (logical block lb of a volume).
+struct ColLane {
+  const float2* sb;
+  float2* db;
+  int64_t srs, drs;
+  bool ok;
+};
+
+__device__ __forceinline__ ColLane col_lane(const ColArgs& a, int64_t lb, int L, int ln) {
   const int64_t tile = lb % a.ntiles, batch = lb / a.ntiles;
   const int64_t col = tile * L + ln;
-  const bool ok = col < a.ncols;
-  const float2* sb = a.src + batch * a.src_b + col;
-  float2* db = a.dst + batch * a.dst_b + col;
-  const int lo = a.lo, hi = a.hi, shift = N - a.lo - a.hi;
-  auto gld = [&](int e) -> float2 {
-    int r = -1;
-    if (e < lo) r = e;
-    else if (e >= N - hi) r = e - shift;
-    if (!ok || r < 0) return make_float2(0.f, 0.f);
-    return sb[(int64_t)r * a.src_rs];
-  };
-  auto gst = [&](int e, float2 v) {
-    if (ok && e < a.n_out) db[(int64_t)e * a.dst_rs] = make_float2(v.x * a.scale, v.y * a.scale);
-  };
-  auto lld = [&](int e) { return lds[e * L + ln]; };
-  auto lst = [&](int e, float2 v) { lds[e * L + ln] = v; };
-  (void)T;
-  line<N, DIR>(t, a.tw, gld, gst, lld, lst);
+  int64_t soff = col, doff = col, srs = a.src_rs, drs = a.dst_rs;
+  if (a.blk > 0) {
+    const int64_t b = col / kBlk, j = col - b * kBlk;
+    const int64_t w = a.ncols - b * kBlk < kBlk ? a.ncols - b * kBlk : kBlk;
+    soff = b * kBlk * a.src_rs + j;
+    doff = b * kBlk * a.dst_rs + j;
+    srs = drs = w;
+  }
+  return ColLane{a.src + batch * a.src_b + soff, a.dst + batch * a.dst_b + doff, srs, drs, col < a.ncols};
 }
 
+// Line element e of the zero-padded canvas: rows [0, lo) and [N-hi, N) hold
+// the source rows 0..lo+hi-1, the rest is zero.
+template <int N>
+__device__ __forceinline__ float2 col_ld(const ColArgs& a, const ColLane& c, int e) {
+  int r = -1;
+  if (e < a.lo) r = e;
+  else if (e >= N - a.hi) r = e - (N - a.lo - a.hi);
+  if (!c.ok || r < 0) return make_float2(0.f, 0.f);
+  return c.sb[(int64_t)r * c.srs];
+}
+
+// Persistent column pass: each workgroup walks tiles b = blockIdx.x,
+// blockIdx.x + gridDim.x, ... and loads the next tile's first-pass operands
+// into registers while the current tile's remaining passes run.
 template <int N, int DIR>
 __global__ void __launch_bounds__(kMaxThreads) k_col(Pack<ColArgs> pk, int L) {
   extern __shared__ float2 lds[];
-  const int64_t b = xcd_remap(blockIdx.x, gridDim.x);
-  const int v = pick(pk, b);
-  col_body<N, DIR>(pk.a[v], b - pk.prefix[v], L, lds);
+  constexpr int T = Geo<N>::T;
+  constexpr Sched S = Geo<N>::F;
+  constexpr int R0 = S.np > 0 ? S.r[0] : 1;
+  // Blocked layout: a wave takes kBlk columns x consecutive line elements (one
+  // contiguous run per access); plain layout: L consecutive columns.
+  const int G = (pk.a[0].blk > 0 && L > kBlk) ? kBlk : L;
+  const int thr = threadIdx.x;
+  const int ln0 = thr % G + G * (thr / (G * T));
+  const int t0 = (thr / G) % T;
+  const int64_t total = pk.prefix[pk.nvol];
+  int64_t b = blockIdx.x;
+  if (b >= total) return;
+  if constexpr (N == 1) {  // length-1 transform: scaled copy
+    for (; b < total; b += gridDim.x) {
+      const int v = pick(pk, b);
+      const ColArgs& a = pk.a[v];
+      const ColLane c = col_lane(a, b - pk.prefix[v], L, ln0);
+      const float2 x = col_ld<N>(a, c, 0);
+      if (c.ok && a.n_out > 0) c.db[0] = make_float2(x.x * a.scale, x.y * a.scale);
+    }
+    return;
+  }
+  float2 u[Bfly<N, T, R0>::M][R0];
+  auto fetch = [&](int64_t bb) {
+    int ln = ln0, t = t0;
+    opaque(ln, t);
+    const int v = pick(pk, bb);
+    const ColArgs& a = pk.a[v];
+    const ColLane c = col_lane(a, bb - pk.prefix[v], L, ln);
+    pass_load<N, T, R0>(u, t, [&](int e) { return col_ld<N>(a, c, e); });
+  };
+  fetch(b);
+  for (;;) {
+    int ln = ln0, t = t0;
+    opaque(ln, t);
+    auto lld = [&](int e) { return lds[col_lds(e, ln, L)]; };
+    auto lst = [&](int e, float2 v) { lds[col_lds(e, ln, L)] = v; };
+    const int v = pick(pk, b);
+    const ColArgs& a = pk.a[v];
+    const ColLane c = col_lane(a, b - pk.prefix[v], L, ln);
+    auto gst = [&](int e, float2 val) {
+      if (c.ok && e < a.n_out) c.db[(int64_t)e * c.drs] = make_float2(val.x * a.scale, val.y * a.scale);
+    };
+    const int64_t bn = b + gridDim.x;
+    if constexpr (S.np == 1) {
+      pass_store<N, T, R0, 1, DIR>(u, t, a.tw, gst);
+      if (bn < total) fetch(bn);
+    } else {
+      pass_store<N, T, R0, 1, DIR>(u, t, a.tw, lst);
+      if (bn < total) fetch(bn);
+      __syncthreads();
+      passes<N, false, DIR, 1>(t, a.tw, lld, gst, lld, lst, false);
+    }
+    if (bn >= total) break;
+    b = bn;
+    __syncthreads();  // the next tile's first pass overwrites the LDS lines
+  }
 }
 
 // ---- fused Z pass: forward FFT, x OTF, inverse FFT, in place ----
@@ -172,10 +248,7 @@ __device__ __forceinline__ void z_mid(int t, const float2* __restrict__ tw, floa
     const int i = t + m * T;
     if ((M * T == NB) || i < NB) {
       const int k = i % P;
-      if (P > 1) {
-#pragma unroll
-        for (int r = 1; r < R; ++r) u[m][r] = cmul(u[m][r], tw[r * k * step]);
-      }
+      if (P > 1) twiddle<R, -1>(u[m], tw, k * step);
       Dft<R, -1>::run(u[m]);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
@@ -207,9 +280,9 @@ __device__ __forceinline__ void z_fwd_passes(int t, const float2* __restrict__ t
 }
 
 template <int N>
-__device__ __forceinline__ void z_body(const ZArgs& a, int64_t lb, int L, float2* lds) {
-  const int ln = threadIdx.x % L;
-  const int t = threadIdx.x / L;
+__device__ __forceinline__ void z_body(const ZArgs& a, int64_t lb, int L, float2* lds, int thr) {
+  const int ln = thr % L;
+  const int t = thr / L;
   const int64_t col = lb * L + ln;
   const bool ok = col < a.ncols;
   float2* cb = a.buf + col;
@@ -237,101 +310,313 @@ __device__ __forceinline__ void z_body(const ZArgs& a, int64_t lb, int L, float2
   }
 }
 
+// Persistent fused Z pass (same tile walk and next-tile prefetch as k_col);
+// lengths with a single radix pass work straight from global memory.
 template <int N>
 __global__ void __launch_bounds__(kMaxThreads) k_z(Pack<ZArgs> pk, int L) {
   extern __shared__ float2 lds[];
-  const int64_t b = xcd_remap(blockIdx.x, gridDim.x);
-  const int v = pick(pk, b);
-  z_body<N>(pk.a[v], b - pk.prefix[v], L, lds);
+  constexpr Sched F = Geo<N>::F;
+  const int64_t total = pk.prefix[pk.nvol];
+  int64_t b = blockIdx.x;
+  if constexpr (F.np < 2) {
+    for (; b < total; b += gridDim.x) {
+      int thr = threadIdx.x;
+      opaque(thr);
+      const int v = pick(pk, b);
+      z_body<N>(pk.a[v], b - pk.prefix[v], L, lds, thr);
+    }
+  } else {
+    // The next tile's operands are prefetched unless the middle pass (radix
+    // 16: 16 data + 16 OTF operands per butterfly) leaves no registers for them.
+    constexpr bool kPrefetch = F.r[F.np - 1] <= 8;
+    constexpr int T = Geo<N>::T;
+    constexpr int R0 = F.r[0];
+    const int ln0 = threadIdx.x % L;
+    const int t0 = threadIdx.x / L;
+    if (b >= total) return;
+    float2 u[Bfly<N, T, R0>::M][R0];
+    auto fetch = [&](int64_t bb) {
+      int ln = ln0, t = t0;
+      opaque(ln, t);
+      const int v = pick(pk, bb);
+      const ZArgs& a = pk.a[v];
+      const int64_t col = (bb - pk.prefix[v]) * L + ln;
+      const bool ok = col < a.ncols;
+      const float2* cb = a.buf + col;
+      pass_load<N, T, R0>(u, t, [&](int e) {
+        return (ok && e < a.lo) ? cb[(int64_t)e * a.ncols] : make_float2(0.f, 0.f);
+      });
+    };
+    fetch(b);
+    for (;;) {
+      int ln = ln0, t = t0;
+      opaque(ln, t);
+      auto lld = [&](int e) { return lds[e * L + ln]; };
+      auto lst = [&](int e, float2 v) { lds[e * L + ln] = v; };
+      const int v = pick(pk, b);
+      const ZArgs& a = pk.a[v];
+      const int64_t col = (b - pk.prefix[v]) * L + ln;
+      const bool ok = col < a.ncols;
+      float2* cb = a.buf + col;
+      const int64_t rs = a.ncols;
+      auto gst = [&](int e, float2 val) {
+        if (ok && e < a.n_out) cb[(int64_t)e * rs] = val;
+      };
+      pass_store<N, T, R0, 1, -1>(u, t, a.tw, lst);
+      const int64_t bn = b + gridDim.x;
+      if (kPrefetch && bn < total) fetch(bn);
+      __syncthreads();
+      z_fwd_passes<N, 1>(t, a.tw, lld, lld, lst);
+      z_mid<N, F.r[F.np - 1], span_of(F, F.np - 1)>(t, a.tw, lds, L, ln, cb, rs, a.lo, ok, a.otf, col, a.conj, cb,
+                                                    a.n_out);
+      __syncthreads();
+      passes<N, true, 1, 1>(t, a.tw, lld, gst, lld, lst, false);
+      if (bn >= total) break;
+      b = bn;
+      if (!kPrefetch) fetch(b);
+      __syncthreads();
+    }
+  }
+}
+
+// ---- X rows: LDS layout and the blocked spectrum row walk ----
+// LDS slot of row element e: one spare slot every 32 elements, so Stockham
+// stores at stride R do not fall on one bank.
+__device__ __forceinline__ int xrow_lds(int e) { return e + (e >> 5); }
+
+// Visits every (row rl, column k < H) of a workgroup's RPB rows (RPB a power of
+// two) in blocked-layout order: a wave covers kBlk columns x consecutive rows,
+// one contiguous run.  fn(rl, k, global offset of (row, k), row valid).
+template <class Fn>
+__device__ __forceinline__ void blocked_rows(int64_t row0, int RPB, int64_t nrows, int H, int prow, Fn fn) {
+  const int lg = __builtin_ctz(RPB);
+  const int nbk = (H + kBlk - 1) / kBlk;
+  const int total = nbk * RPB * kBlk;
+  const int64_t z0 = row0 / prow;
+  const int y0 = (int)(row0 - z0 * prow);
+  for (int f = threadIdx.x; f < total; f += blockDim.x) {
+    const int j = f & (kBlk - 1);
+    const int rl = (f / kBlk) & (RPB - 1);
+    const int b = f >> (lg + 3);
+    const int k = b * kBlk + j;
+    if (k >= H) continue;
+    int64_t z = z0;
+    int y = y0 + rl;
+    if (y >= prow) {
+      z += y / prow;
+      y %= prow;
+    }
+    const int w = (b == nbk - 1) ? H - b * kBlk : kBlk;
+    const int64_t off = z * prow * (int64_t)H + (int64_t)b * kBlk * prow + (int64_t)y * w + j;
+    fn(rl, k, off, row0 + rl < nrows);
+  }
 }
 
 // ---- X forward: real rows -> half spectrum (M-point complex FFT) ----
+// Complex sample e (real samples 2e, 2e+1 of the zero-padded row of length 2M,
+// prologue applied) of row `row` of volume a.
+// PRO_DIV_DEV (the OTF build's kernel / sum) is applied afterwards by xf_div.
 template <int M>
-__device__ __forceinline__ void xf_body(const XfArgs& a, int64_t lb, int RPB, float2* lds) {
-  constexpr int T = Geo<M>::T;
-  const int rl = threadIdx.x / T;
-  const int t = threadIdx.x % T;
-  const int64_t row = lb * RPB + rl;
-  const bool ok = row < a.nrows;
+__device__ __forceinline__ float2 xf_ld(const XfArgs& a, int64_t row, int e) {
   constexpr int P2 = 2 * M;
-  const int lo = a.lo, hi = a.hi, shift = P2 - a.lo - a.hi;
+  const bool ok = row < a.nrows;
+  const int kind = a.pro.kind;
+  // PRO_ONES never reads the source (which may then be null)
+  const bool read = kind != PRO_ONES;
   const float* sr = a.src + row * a.rowlen;
-  float2* lrow = lds + rl * (M + 1);
-  auto sample = [&](int q) -> float {
-    int c = -1;
-    if (q < lo) c = q;
-    else if (q >= P2 - hi) c = q - shift;
-    if (!ok || c < 0) return 0.0f;
-    return pro_apply(a.pro, sr, c);
-  };
-  const bool vec2 = a.vec2 != 0;
-  auto gld = [&](int e) -> float2 {
-    if (vec2 && ok && 2 * e + 1 < lo) {
-      const float2 v = *reinterpret_cast<const float2*>(sr + 2 * e);
-      switch (a.pro.kind) {
-        case PRO_ONES: return make_float2(1.0f, 1.0f);
-        case PRO_SUB_FLOOR: return make_float2(fmaxf(v.x - a.pro.a, a.pro.b), fmaxf(v.y - a.pro.a, a.pro.b));
-        case PRO_DIV_DEV: {
-          const float s = (float)(*a.pro.dptr);
-          return make_float2(v.x / s, v.y / s);
-        }
-        default: return v;
-      }
-    }
-    return make_float2(sample(2 * e), sample(2 * e + 1));
-  };
-  auto lld = [&](int e) { return lrow[e]; };
-  auto lst = [&](int e, float2 v) { lrow[e] = v; };
-  // all passes end in LDS (natural order), then the real-to-half-spectrum step
-  if constexpr (M == 1) {
-    if (t == 0) lrow[0] = gld(0);
+  float2 raw = make_float2(0.f, 0.f);
+  bool ok0, ok1;
+  if (a.vec2) {  // hi == 0, even row length, 8-byte aligned rows
+    ok0 = ok1 = ok && 2 * e + 1 < a.lo;
+    if (ok0 && read) raw = *reinterpret_cast<const float2*>(sr + 2 * e);
   } else {
-    passes<M, false, -1, 0, true>(t, a.twm, gld, lst, lld, lst, true);
+    auto src_of = [&](int q) {
+      if (q < a.lo) return q;
+      if (q >= P2 - a.hi) return q - (P2 - a.lo - a.hi);
+      return -1;
+    };
+    const int c0 = src_of(2 * e), c1 = src_of(2 * e + 1);
+    ok0 = ok && c0 >= 0;
+    ok1 = ok && c1 >= 0;
+    if (ok0 && read) raw.x = sr[c0];
+    if (ok1 && read) raw.y = sr[c1];
   }
-  __syncthreads();
-  if (ok) {
-    float2* dr = a.dst + row * a.H;
-    for (int k = t; k <= M; k += T) {
-      const float2 zk = lrow[k == M ? 0 : k];
-      const float2 zm = lrow[k == 0 ? 0 : M - k];
-      const float2 zmc = make_float2(zm.x, -zm.y);
-      const float2 E = make_float2(0.5f * (zk.x + zmc.x), 0.5f * (zk.y + zmc.y));
-      const float2 D = make_float2(zk.x - zmc.x, zk.y - zmc.y);
-      const float2 O = make_float2(0.5f * D.y, -0.5f * D.x);  // -i/2 * D
-      dr[k] = cadd(E, cmul(a.twp[k], O));
-    }
-  }
+  auto pro = [&](float x, bool valid) {
+    const float y = kind == PRO_ONES ? 1.0f : (kind == PRO_SUB_FLOOR ? fmaxf(x - a.pro.a, a.pro.b) : x);
+    return valid ? y : 0.0f;
+  };
+  return make_float2(pro(raw.x, ok0), pro(raw.y, ok1));
 }
 
+// PRO_DIV_DEV on a thread's first-pass operands (zero padding stays zero).
+template <int N, int T, int R>
+__device__ __forceinline__ void xf_div(float2 (&u)[(N / R + T - 1) / T][R], const Pro& p) {
+  if (p.kind != PRO_DIV_DEV) return;
+  const float s = (float)(*p.dptr);
+#pragma unroll
+  for (int m = 0; m < (N / R + T - 1) / T; ++m)
+#pragma unroll
+    for (int r = 0; r < R; ++r) u[m][r] = make_float2(u[m][r].x / s, u[m][r].y / s);
+}
+
+// Spectrum sample k of a row whose M-point transform z sits in LDS at lr.
+template <int M>
+__device__ __forceinline__ float2 xf_post(const float2* lr, const float2* twp, int k) {
+  const float2 zk = lr[xrow_lds(k == M ? 0 : k)];
+  const float2 zm = lr[xrow_lds(k == 0 ? 0 : M - k)];
+  const float2 zmc = make_float2(zm.x, -zm.y);
+  const float2 E = make_float2(0.5f * (zk.x + zmc.x), 0.5f * (zk.y + zmc.y));
+  const float2 D = make_float2(zk.x - zmc.x, zk.y - zmc.y);
+  const float2 O = make_float2(0.5f * D.y, -0.5f * D.x);  // -i/2 * D
+  return cadd(E, cmul(twp[k], O));
+}
+
+// Persistent: workgroups walk tiles of RPB rows, prefetching the next tile's
+// samples while the current tile's passes and stores run.  Both twiddle tables
+// sit in LDS after the rows (one length per launch, so one copy per workgroup).
 template <int M>
 __global__ void __launch_bounds__(kMaxThreads) k_xf(Pack<XfArgs> pk, int RPB) {
   extern __shared__ float2 lds[];
-  const int64_t b = blockIdx.x;
-  const int v = pick(pk, b);
-  xf_body<M>(pk.a[v], b - pk.prefix[v], RPB, lds);
+  constexpr int T = Geo<M>::T;
+  constexpr Sched S = Geo<M>::F;
+  constexpr int R0 = S.np > 0 ? S.r[0] : 1;
+  constexpr int RS = xrow_stride(M);
+  const int rl0 = threadIdx.x / T;
+  const int t0 = threadIdx.x % T;
+  const int64_t total = pk.prefix[pk.nvol];
+  int64_t b = blockIdx.x;
+  if (b >= total) return;
+  float2* ltw = lds + RPB * RS;
+  for (int i = threadIdx.x; i < 2 * M + 1; i += blockDim.x) ltw[i] = i < M ? pk.a[0].twm[i] : pk.a[0].twp[i - M];
+  const float2* twm = ltw;
+  const float2* twp = ltw + M;
+  float2 u[Bfly<M, T, R0>::M][R0];
+  auto fetch = [&](int64_t bb) {
+    int rl = rl0, t = t0;
+    opaque(rl, t);
+    const int v = pick(pk, bb);
+    const XfArgs& a = pk.a[v];
+    const int64_t row = (bb - pk.prefix[v]) * RPB + rl;
+    if constexpr (M == 1) {
+      u[0][0] = xf_ld<M>(a, row, 0);
+    } else {
+      pass_load<M, T, R0>(u, t, [&](int e) { return xf_ld<M>(a, row, e); });
+    }
+    xf_div<M, T, R0>(u, a.pro);
+  };
+  fetch(b);
+  for (;;) {
+    int rl = rl0, t = t0;
+    opaque(rl, t);
+    float2* lrow = lds + rl * RS;
+    auto lld = [&](int e) { return lrow[xrow_lds(e)]; };
+    auto lst = [&](int e, float2 v) { lrow[xrow_lds(e)] = v; };
+    const int v = pick(pk, b);
+    const XfArgs& a = pk.a[v];
+    const int64_t lb = b - pk.prefix[v];
+    const int64_t row = lb * RPB + rl;
+    if constexpr (M == 1) {
+      if (t == 0) lrow[0] = u[0][0];
+    } else {
+      pass_store<M, T, R0, 1, -1>(u, t, twm, lst);
+    }
+    const int64_t bn = b + gridDim.x;
+    if (bn < total) fetch(bn);
+    __syncthreads();
+    // remaining passes, in LDS (the last one reads and writes the row: LAST_LDS)
+    passes<M, false, -1, 1, true>(t, twm, lld, lst, lld, lst, false);
+    __syncthreads();
+    if (a.blk > 0) {
+      blocked_rows(lb * RPB, RPB, a.nrows, a.H, a.prow, [&](int r, int k, int64_t off, bool valid) {
+        if (valid) a.dst[off] = xf_post<M>(lds + r * RS, twp, k);
+      });
+    } else if (row < a.nrows) {
+      float2* dr = a.dst + row * a.H;
+      for (int k = t; k <= M; k += T) dr[k] = xf_post<M>(lrow, twp, k);
+    }
+    if (bn >= total) break;
+    b = bn;
+    __syncthreads();
+  }
 }
 
 // ---- X inverse: half spectrum -> cropped real rows with the fused epilogue ----
+// Elements of the H = M+1 spectrum samples of a tile's rows one thread loads:
+// ceil((M + kBlk) / T) covers both the blocked walk (at most ceil(nbk*kBlk/T)
+// visits) and plain rows (ceil(H/T)).
 template <int M>
-__device__ __forceinline__ void xi_body(const XiArgs& a, int64_t lb, int RPB, float2* lds) {
+struct XiEpt {
+  static constexpr int value = (M + kBlk + Geo<M>::T - 1) / Geo<M>::T;
+};
+
+// Loads (q) or places into LDS (!load) this thread's spectrum samples of the
+// tile whose rows start at row0: visit i of the blocked walk, or sample
+// k = t + i*T of the thread's own row for the plain layout.
+template <int M>
+__device__ __forceinline__ void xi_tile(const XiArgs& a, int64_t row0, int RPB,
+                                        float2 (&q)[(M + kBlk + Geo<M>::T - 1) / Geo<M>::T], float2* lds, bool load,
+                                        int thr) {
   constexpr int T = Geo<M>::T;
-  const int rl = threadIdx.x / T;
-  const int t = threadIdx.x % T;
+  constexpr int RS = xrow_stride(M);
+  const int H = a.H;
+  if (a.blk > 0) {
+    const int lg = __builtin_ctz(RPB);
+    const int nbk = (H + kBlk - 1) / kBlk;
+    const int total = nbk * RPB * kBlk;
+    const int64_t z0 = row0 / a.prow;
+    const int y0 = (int)(row0 - z0 * a.prow);
+#pragma unroll
+    for (int i = 0; i < XiEpt<M>::value; ++i) {
+      const int f = thr + i * blockDim.x;
+      const int j = f & (kBlk - 1);
+      const int r = (f / kBlk) & (RPB - 1);
+      const int bk = f >> (lg + 3);
+      const int k = bk * kBlk + j;
+      if (f >= total || k >= H) continue;
+      if (!load) {
+        lds[r * RS + xrow_lds(k)] = q[i];
+        continue;
+      }
+      float2 val = make_float2(0.f, 0.f);
+      if (row0 + r < a.nrows) {
+        int64_t z = z0;
+        int y = y0 + r;
+        if (y >= a.prow) {
+          z += y / a.prow;
+          y %= a.prow;
+        }
+        const int w = (bk == nbk - 1) ? H - bk * kBlk : kBlk;
+        val = a.src[z * a.prow * (int64_t)H + (int64_t)bk * kBlk * a.prow + (int64_t)y * w + j];
+      }
+      q[i] = val;
+    }
+  } else {
+    const int rl = thr / T, t = thr % T;
+    const int64_t row = row0 + rl;
+#pragma unroll
+    for (int i = 0; i < XiEpt<M>::value; ++i) {
+      const int k = t + i * T;
+      if (k >= H) continue;
+      if (!load) lds[rl * RS + xrow_lds(k)] = q[i];
+      else q[i] = row < a.nrows ? a.src[row * H + k] : make_float2(0.f, 0.f);
+    }
+  }
+}
+
+template <int M>
+__device__ __forceinline__ void xi_body(const XiArgs& a, int64_t lb, int RPB, float2* lds, const float2* twm,
+                                        const float2* twp, int thr) {
+  constexpr int T = Geo<M>::T;
+  const int rl = thr / T;
+  const int t = thr % T;
   const int64_t row = lb * RPB + rl;
   const bool ok = row < a.nrows;
-  const int H = a.H, s2 = a.s2;
-  float2* lrow = lds + rl * (M + 1);
-  if (ok) {
-    const float2* sr = a.src + row * H;
-    for (int k = t; k < H; k += T) lrow[k] = sr[k];
-  } else {
-    for (int k = t; k < H; k += T) lrow[k] = make_float2(0.f, 0.f);
-  }
-  __syncthreads();
-  const float2* twp = a.twp;
+  const int s2 = a.s2;
+  constexpr int RS = xrow_stride(M);
+  float2* lrow = lds + rl * RS;
   auto pre = [&](int q) -> float2 {
-    float2 xq = lrow[q];
-    float2 xm = lrow[M - q];
+    float2 xq = lrow[xrow_lds(q)];
+    float2 xm = lrow[xrow_lds(M - q)];
     if (q == 0) {
       xq.y = 0.f;
       xm.y = 0.f;
@@ -357,8 +642,8 @@ __device__ __forceinline__ void xi_body(const XiArgs& a, int64_t lb, int RPB, fl
     if (q0 < s2) drow[q0] = epi_apply(a.epi, v.x, base + q0, acc);
     if (q1 < s2) drow[q1] = epi_apply(a.epi, v.y, base + q1, acc);
   };
-  auto lld = [&](int e) { return lrow[e]; };
-  auto lst = [&](int e, float2 v) { lrow[e] = v; };
+  auto lld = [&](int e) { return lrow[xrow_lds(e)]; };
+  auto lst = [&](int e, float2 v) { lrow[xrow_lds(e)] = v; };
   if constexpr (M == 1) {
     float2 v = make_float2(0.f, 0.f);
     if (t == 0) v = pre(0);
@@ -369,11 +654,11 @@ __device__ __forceinline__ void xi_body(const XiArgs& a, int64_t lb, int RPB, fl
     constexpr Sched S = Geo<M>::F;
     constexpr int R0 = S.r[0];
     if constexpr (S.np == 1) {
-      pass<M, T, R0, 1, 1>(t, a.twm, pre, gst, true);
+      pass<M, T, R0, 1, 1>(t, twm, pre, gst, true);
     } else {
-      pass<M, T, R0, 1, 1>(t, a.twm, pre, lst, true);
+      pass<M, T, R0, 1, 1>(t, twm, pre, lst, true);
       __syncthreads();
-      passes<M, false, 1, 1>(t, a.twm, pre, gst, lld, lst, false);
+      passes<M, false, 1, 1>(t, twm, pre, gst, lld, lst, false);
     }
   }
   if (a.epi.partials) {
@@ -383,46 +668,126 @@ __device__ __forceinline__ void xi_body(const XiArgs& a, int64_t lb, int RPB, fl
   }
 }
 
+// Persistent: the next tile's spectrum samples are loaded into registers while
+// the current tile is transformed; twiddles in LDS as for k_xf.
 template <int M>
 __global__ void __launch_bounds__(kMaxThreads) k_xi(Pack<XiArgs> pk, int RPB) {
   extern __shared__ float2 lds[];
-  const int64_t b = blockIdx.x;
-  const int v = pick(pk, b);
-  xi_body<M>(pk.a[v], b - pk.prefix[v], RPB, lds);
+  constexpr int RS = xrow_stride(M);
+  const int64_t total = pk.prefix[pk.nvol];
+  int64_t b = blockIdx.x;
+  if (b >= total) return;
+  float2* ltw = lds + RPB * RS;
+  for (int i = threadIdx.x; i < 2 * M + 1; i += blockDim.x) ltw[i] = i < M ? pk.a[0].twm[i] : pk.a[0].twp[i - M];
+  float2 q[XiEpt<M>::value];
+  {
+    int thr = threadIdx.x;
+    opaque(thr);
+    const int v = pick(pk, b);
+    xi_tile<M>(pk.a[v], (b - pk.prefix[v]) * RPB, RPB, q, lds, true, thr);
+  }
+  for (;;) {
+    int thr = threadIdx.x;
+    opaque(thr);
+    const int v = pick(pk, b);
+    const XiArgs& a = pk.a[v];
+    const int64_t lb = b - pk.prefix[v];
+    xi_tile<M>(a, lb * RPB, RPB, q, lds, false, thr);
+    const int64_t bn = b + gridDim.x;
+    if (bn < total) {
+      int thr2 = threadIdx.x;
+      opaque(thr2);
+      const int vn = pick(pk, bn);
+      xi_tile<M>(pk.a[vn], (bn - pk.prefix[vn]) * RPB, RPB, q, lds, true, thr2);
+    }
+    __syncthreads();
+    xi_body<M>(a, lb, RPB, lds, ltw, ltw + M, thr);
+    if (bn >= total) break;
+    b = bn;
+    __syncthreads();
+  }
 }
 
 // ---- host launchers (explicitly instantiated per length) ----
+// Compute units of the current device.
+inline int device_cus() {
+  static int cache[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+  if (cache[dev] == 0) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n < 1) n = 1;
+    cache[dev] = n;
+  }
+  return cache[dev];
+}
+
+// Workgroups of a persistent launch: as many as the device holds at once
+// (occupancy x compute units), at most nblocks.
+inline unsigned persistent_grid(const void* kernel, int threads, size_t shm, int64_t nblocks) {
+  struct Entry {
+    const void* k;
+    int threads;
+    size_t shm;
+    int per;
+  };
+  static std::mutex mu;
+  static Entry cache[128];
+  static int n = 0;
+  int per = 0;
+  {
+    std::lock_guard<std::mutex> g(mu);
+    for (int i = 0; i < n && !per; ++i)
+      if (cache[i].k == kernel && cache[i].threads == threads && cache[i].shm == shm) per = cache[i].per;
+    if (!per) {
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, shm) != hipSuccess || per < 1) per = 1;
+      if (n < 128) cache[n++] = Entry{kernel, threads, shm, per};
+    }
+  }
+  const int64_t cap = (int64_t)per * device_cus();
+  return (unsigned)(nblocks < cap ? nblocks : cap);
+}
+
 template <int N>
 struct Launch {
   static constexpr int T = Geo<N>::T;
-  static hipError_t col(int dir, const Pack<ColArgs>& pk, int L, int64_t nblocks, hipStream_t st) {
-    const size_t shm = (size_t)N * L * sizeof(float2);
+  // one workgroup per tile, or a persistent grid
+  static unsigned grid(const void* k, int threads, size_t shm, int64_t nblocks, bool persist) {
+    return persist ? persistent_grid(k, threads, shm, nblocks) : (unsigned)nblocks;
+  }
+  static hipError_t col(int dir, const Pack<ColArgs>& pk, int L, int64_t nblocks, bool persist, hipStream_t st) {
+    const size_t shm = (size_t)col_lds_elems(N, L) * sizeof(float2);
     if (dir < 0) {
-      (void)hipFuncSetAttribute((const void*)k_col<N, -1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-      hipLaunchKernelGGL((k_col<N, -1>), dim3((unsigned)nblocks), dim3(L * T), shm, st, pk, L);
+      const void* k = (const void*)k_col<N, -1>;
+      (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+      hipLaunchKernelGGL((k_col<N, -1>), dim3(grid(k, L * T, shm, nblocks, persist)), dim3(L * T), shm, st, pk, L);
     } else {
-      (void)hipFuncSetAttribute((const void*)k_col<N, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-      hipLaunchKernelGGL((k_col<N, 1>), dim3((unsigned)nblocks), dim3(L * T), shm, st, pk, L);
+      const void* k = (const void*)k_col<N, 1>;
+      (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+      hipLaunchKernelGGL((k_col<N, 1>), dim3(grid(k, L * T, shm, nblocks, persist)), dim3(L * T), shm, st, pk, L);
     }
     return hipGetLastError();
   }
-  static hipError_t z(const Pack<ZArgs>& pk, int L, int64_t nblocks, hipStream_t st) {
+  static hipError_t z(const Pack<ZArgs>& pk, int L, int64_t nblocks, bool persist, hipStream_t st) {
     const size_t shm = (size_t)N * L * sizeof(float2);
-    (void)hipFuncSetAttribute((const void*)k_z<N>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    hipLaunchKernelGGL((k_z<N>), dim3((unsigned)nblocks), dim3(L * T), shm, st, pk, L);
+    const void* k = (const void*)k_z<N>;
+    (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    hipLaunchKernelGGL((k_z<N>), dim3(grid(k, L * T, shm, nblocks, persist)), dim3(L * T), shm, st, pk, L);
     return hipGetLastError();
   }
-  static hipError_t xf(const Pack<XfArgs>& pk, int R, int64_t nblocks, hipStream_t st) {
-    const size_t shm = (size_t)R * (N + 1) * sizeof(float2);
-    (void)hipFuncSetAttribute((const void*)k_xf<N>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    hipLaunchKernelGGL((k_xf<N>), dim3((unsigned)nblocks), dim3(R * T), shm, st, pk, R);
+  static hipError_t xf(const Pack<XfArgs>& pk, int R, int64_t nblocks, bool persist, hipStream_t st) {
+    const size_t shm = (size_t)xrow_lds_elems(N, R) * sizeof(float2);
+    const void* k = (const void*)k_xf<N>;
+    (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    hipLaunchKernelGGL((k_xf<N>), dim3(grid(k, R * T, shm, nblocks, persist)), dim3(R * T), shm, st, pk, R);
     return hipGetLastError();
   }
-  static hipError_t xi(const Pack<XiArgs>& pk, int R, int64_t nblocks, hipStream_t st) {
-    size_t shm = (size_t)R * (N + 1) * sizeof(float2);
+  static hipError_t xi(const Pack<XiArgs>& pk, int R, int64_t nblocks, bool persist, hipStream_t st) {
+    size_t shm = (size_t)xrow_lds_elems(N, R) * sizeof(float2);
     if (shm < 16 * sizeof(double)) shm = 16 * sizeof(double);
-    (void)hipFuncSetAttribute((const void*)k_xi<N>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    hipLaunchKernelGGL((k_xi<N>), dim3((unsigned)nblocks), dim3(R * T), shm, st, pk, R);
+    const void* k = (const void*)k_xi<N>;
+    (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    hipLaunchKernelGGL((k_xi<N>), dim3(grid(k, R * T, shm, nblocks, persist)), dim3(R * T), shm, st, pk, R);
     return hipGetLastError();
   }
 };

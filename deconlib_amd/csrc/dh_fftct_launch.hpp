@@ -7,11 +7,13 @@ namespace ct {
 
 template <int N>
 hipError_t launch_kind(int kind, int dir, const void* pk, int tile, int64_t nblocks, hipStream_t st) {
+  const int t = tile & kTileMask;
+  const bool persist = (tile & kTilePersistent) != 0;
   switch (kind) {
-    case CT_COL: return Launch<N>::col(dir, *(const Pack<ColArgs>*)pk, tile, nblocks, st);
-    case CT_Z: return Launch<N>::z(*(const Pack<ZArgs>*)pk, tile, nblocks, st);
-    case CT_XF: return Launch<N>::xf(*(const Pack<XfArgs>*)pk, tile, nblocks, st);
-    default: return Launch<N>::xi(*(const Pack<XiArgs>*)pk, tile, nblocks, st);
+    case CT_COL: return Launch<N>::col(dir, *(const Pack<ColArgs>*)pk, t, nblocks, persist, st);
+    case CT_Z: return Launch<N>::z(*(const Pack<ZArgs>*)pk, t, nblocks, persist, st);
+    case CT_XF: return Launch<N>::xf(*(const Pack<XfArgs>*)pk, t, nblocks, persist, st);
+    default: return Launch<N>::xi(*(const Pack<XiArgs>*)pk, t, nblocks, persist, st);
   }
 }
 

@@ -136,7 +136,16 @@ class _ConvPlan:
         """Unscaled complex64 OTF (P0, P1, P2//2+1), as the reference stores it."""
         p = self.info.canvas_shape
         h = p[2] // 2 + 1
-        z = self.otf_buf.view(torch.complex64).reshape(p[0], p[1], h)
+        flat = self.otf_buf.view(torch.complex64).reshape(p[0], p[1] * h)
+        b = int(self.info.spectrum_block)
+        if b <= 0 or b >= h:
+            z = flat.reshape(p[0], p[1], h)
+        else:
+            # column blocks of width b (the last one narrower), each (P1, width) row-major
+            nfull = (h - 1) // b
+            full = flat[:, :nfull * b * p[1]].reshape(p[0], nfull, p[1], b).permute(0, 2, 1, 3)
+            last = flat[:, nfull * b * p[1]:].reshape(p[0], p[1], h - nfull * b)
+            z = torch.cat([full.reshape(p[0], p[1], nfull * b), last], dim=2)
         return z * float(p[0] * p[1] * p[2])
 
     def __del__(self):

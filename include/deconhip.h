@@ -15,7 +15,9 @@
  *     thread-local message for the last failure.
  *   - Volumes are C-contiguous float32 (z, y, x); 1-D and 2-D inputs are
  *     promoted to 3-D with leading unit axes.  Spectra are interleaved
- *     complex64 laid out (P0, P1, P2/2+1).
+ *     complex64 (P0, P1, P2/2+1) planes; within a plane the
+ *     (P1, H = P2/2+1) samples are stored in the order given by
+ *     dh_plan_info.spectrum_block (see there).
  */
 #ifndef DECONHIP_H
 #define DECONHIP_H
@@ -26,7 +28,7 @@
 extern "C" {
 #endif
 
-#define DH_ABI_VERSION 2
+#define DH_ABI_VERSION 3
 
 #define DH_OK 0
 #define DH_EINVAL (-1)
@@ -45,6 +47,12 @@ typedef struct {
   int64_t otf_bytes;           /* complex64 (P0, P1, P2/2+1)                */
   int64_t work_bytes;          /* scratch for the pruned passes + partials  */
   int64_t twiddle_bytes;       /* device twiddle tables                     */
+  /* Order of the H = P2/2+1 columns of each OTF plane.  0: row-major (P1, H).
+   * B > 0: column blocks of width B, block b holding columns
+   * [bB, bB + w_b), w_b = min(B, H - bB), each block row-major (P1, w_b) and
+   * the blocks one after another; sample (y, c) sits at
+   * bB*P1 + y*w_b + (c - bB), b = c / B.  Planes are dense either way. */
+  int32_t spectrum_block;
 } dh_plan_info;
 
 /* Fused prologue applied to each input sample of the forward FFT pass
@@ -85,9 +93,12 @@ int dh_conv_plan_info(const dh_conv_plan* plan, dh_plan_info* info);
 int dh_conv_plan_bind(dh_conv_plan* plan, void* twiddles_dev, void* stream);
 int dh_conv_plan_destroy(dh_conv_plan* plan);
 /* Launch shape of one pass (DH_PASS_*): columns per workgroup for the Y/Z
- * passes, rows per workgroup for the X passes; 0 restores the built-in
- * heuristic.  Semantics-free: used by the autotuner.  get_tile returns the
- * value in effect. */
+ * passes, rows per workgroup for the X passes (a power of two), optionally
+ * plus DH_TILE_PERSISTENT for a persistent grid (one workgroup per resident
+ * slot walking the tiles, prefetching the next tile's operands) instead of one
+ * workgroup per tile; 0 restores the built-in heuristic.  Semantics-free: used
+ * by the autotuner.  get_tile returns the value in effect. */
+#define DH_TILE_PERSISTENT 0x10000
 int dh_conv_plan_set_tile(dh_conv_plan* plan, int pass, int tile);
 int dh_conv_plan_get_tile(const dh_conv_plan* plan, int pass);
 

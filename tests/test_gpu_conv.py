@@ -133,3 +133,32 @@ def test_layout_operators(gpu):
         np.testing.assert_allclose(fx, od.forward(x), rtol=2e-6, atol=1e-6)
         z = rng.random(fx.shape).astype(np.float32)
         np.testing.assert_allclose(d.adjoint(z).cpu().numpy(), od.adjoint(z), rtol=2e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("sshape,kshape", [((16, 32, 32), (16, 32, 32)), ((10, 15, 9), (4, 6, 8)),
+                                           ((12, 40, 64), (7, 21, 33))])
+def test_launch_shapes_are_bit_identical(gpu, sshape, kshape):
+    """Every tile size and the persistent grid compute the same values in the
+    same order: results are bit-identical to the default launch shapes."""
+    from deconlib_amd import LinearFFTConvolver
+    from deconlib_amd import _native as N
+
+    print(f"launch shapes {sshape} * {kshape}", flush=True)
+    rng = np.random.default_rng(12)
+    k = (rng.random(kshape) + 0.05).astype(np.float32)
+    x = torch.from_numpy(rng.standard_normal(sshape).astype(np.float32)).cuda()
+    op = LinearFFTConvolver(k, sshape)
+    ref_f = op.forward(x).cpu().numpy()
+    ref_a = op.adjoint(x).cpu().numpy()
+    h = op._plan._h
+    tried = 0
+    for p in range(5):
+        for t in (1, 2, 4, 8, 16, 32):
+            for v in (t, t | N.TILE_PERSISTENT):
+                if N.fns["dh_conv_plan_set_tile"](h, p, v) != 0:
+                    continue
+                tried += 1
+                assert np.array_equal(op.forward(x).cpu().numpy(), ref_f), (p, v)
+                assert np.array_equal(op.adjoint(x).cpu().numpy(), ref_a), (p, v)
+        N.call("dh_conv_plan_set_tile", h, p, 0)
+    assert tried >= 10
