@@ -24,6 +24,10 @@ SHAPE = (64, 512, 512)
 LATERAL, AXIAL = 0.085, 0.1
 N_BLOBS = 800          # config-0 blob density (200 per 64x256x256) on a 64x512x512 volume
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per k_z<128> launch at this workload, from rocprofv3 FETCH_SIZE (x2) and
+# WRITE_SIZE (x1), both scaled by what they read for a device copy of known size
+# (profiles/fft_passes_64x512x512_hbm_counters.md): (394075.8 * 2 + 262656.0) KiB.
+Z_TRAFFIC_BYTES = int((394075.8 * 2 + 262656.0) * 1024)
 
 
 def log(msg):
@@ -273,7 +277,9 @@ def main():
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
             "frac": z["gbs"] / PEAK_HBM_GBS,
-            "traffic": None,
+            "traffic": Z_TRAFFIC_BYTES,
+            "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated "
+                              "(profiles/fft_passes_64x512x512_hbm_counters.md)",
             "algorithmic_bytes": pb["z_conv"],
             "avg_ms": z["avg_ms"],
         },
