@@ -102,6 +102,38 @@ def bench_pdhg(torch, N, obs, blur, steps, warmup):
     }
 
 
+def bench_host_to_host(torch, psf, obs, iters=50):
+    """One whole RL solve from host arrays to a host array: operator construction (OTF
+    build), PCIe upload, `iters` iterations, PCIe download.  Reported beside `value`
+    (which times device-resident iterations only)."""
+    from deconlib_amd import LinearFFTConvolver, richardson_lucy_with_operator
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    op = LinearFFTConvolver(psf, obs.shape)
+    res = richardson_lucy_with_operator(obs, op, num_iter=iters)
+    out = res.restored.cpu().numpy()
+    dt = time.perf_counter() - t0
+    x = torch.from_numpy(obs).pin_memory()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    d = x.cuda(non_blocking=True)
+    torch.cuda.synchronize()
+    up = time.perf_counter() - t1
+    t1 = time.perf_counter()
+    d.cpu()
+    down = time.perf_counter() - t1
+    vox = int(np.prod(obs.shape))
+    return {
+        "iterations": iters,
+        "seconds": dt,
+        "voxel_iters_per_s": vox * iters / dt,
+        "h2d_GBps_pinned": obs.nbytes / up / 1e9,
+        "d2h_GBps": obs.nbytes / down / 1e9,
+        "finite": bool(np.isfinite(out).all()),
+    }
+
+
 def bench_serving(torch, n_jobs, rate, seed=1003):
     """Serving sample: Poisson arrivals of mixed RL (100 it) / PDHG-Hessian (200 it) jobs,
     sizes log-uniform 16x64x64 .. 48x256x256; continuous batching vs the plain one-job loop."""
@@ -231,6 +263,8 @@ def main():
     ok = bool(torch.isfinite(st.x).all().item())
     pdhg = bench_pdhg(torch, N, obs, blur, args.steps, args.warmup)
     log(f"pdhg: {json.dumps(pdhg)}")
+    host = bench_host_to_host(torch, psf, obs)
+    log(f"host-to-host solve: {json.dumps(host)}")
     if args.copy_probe:
         # a plain device copy of known size: the reference point for the HBM counters
         src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")  # 256 MiB
@@ -287,6 +321,7 @@ def main():
         "iteration": {"algorithmic_bytes": it_bytes,
                       "hbm_frac": it_bytes / (dt / args.steps) / (PEAK_HBM_GBS * 1e9)},
         "pdhg": pdhg,
+        "host_to_host": host,
         "serving": serving,
         "cpu_baseline": cpu,
     }
