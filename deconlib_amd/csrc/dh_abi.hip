@@ -9,6 +9,7 @@
 
 #include "../../include/deconhip.h"
 #include "dh_fft.hpp"
+#include "dh_fftct.hpp"
 #include "dh_ops.hpp"
 #include "dh_pdhg.hpp"
 
@@ -88,6 +89,23 @@ void push_table(std::vector<float2>& tw, int n, int count) {
   for (int m = 0; m < count; ++m) {
     const double a = -2.0 * M_PI * (double)m / (double)n;
     tw.push_back(make_float2((float)std::cos(a), (float)std::sin(a)));
+  }
+}
+
+// Per-pass twiddle rows of the specialised kernels for schedule s: pass q
+// (radix R, span P > 1) gets tw_pows(R) rows of P entries, row j entry k =
+// exp(-2*pi*i * 2^j * k / (P*R)) (the layout dh::ct::twiddle reads).
+void push_pass_tables(std::vector<float2>& tw, const dh::ct::Sched& s) {
+  int span = 1;
+  for (int q = 0; q < s.np; ++q) {
+    const int R = s.r[q];
+    if (span > 1)
+      for (int j = 0; j < dh::ct::tw_pows(R); ++j)
+        for (int k = 0; k < span; ++k) {
+          const double a = -2.0 * M_PI * (double)((1 << j) * k) / (double)(span * R);
+          tw.push_back(make_float2((float)std::cos(a), (float)std::sin(a)));
+        }
+    span *= R;
   }
 }
 
@@ -384,14 +402,24 @@ int dh_conv_plan_create(int ndim, const int64_t* signal_shape, const int64_t* ke
     return fail(DH_EUNSUPPORTED, "canvas extents must be 5-smooth");
   }
   // twiddle tables
+  // specialised kernels: per-pass rows (Z: forward then inverse schedule);
+  // runtime-length kernels: one table of n entries per axis
   p->tw_z = 0;
-  push_table(p->tw_host, p->P[0], p->P[0]);
+  if (p->ct) {
+    push_pass_tables(p->tw_host, dh::ct::make_sched(p->P[0]));
+    push_pass_tables(p->tw_host, dh::ct::reversed(dh::ct::make_sched(p->P[0])));
+  } else {
+    push_table(p->tw_host, p->P[0], p->P[0]);
+  }
   p->tw_y = (int64_t)p->tw_host.size();
-  push_table(p->tw_host, p->P[1], p->P[1]);
+  if (p->ct) push_pass_tables(p->tw_host, dh::ct::make_sched(p->P[1]));
+  else push_table(p->tw_host, p->P[1], p->P[1]);
   p->tw_xm = (int64_t)p->tw_host.size();
-  push_table(p->tw_host, p->M, p->M);
+  if (p->ct) push_pass_tables(p->tw_host, dh::ct::make_sched(p->M));
+  else push_table(p->tw_host, p->M, p->M);
   p->tw_xp = (int64_t)p->tw_host.size();
   push_table(p->tw_host, p->P[2], p->M + 1);
+  if (p->tw_host.empty()) p->tw_host.push_back(make_float2(1.f, 0.f));
   // workspace layout
   const int64_t H = p->H;
   const int64_t rows1 = std::max<int64_t>((int64_t)p->s[0] * p->s[1], (int64_t)p->k[0] * p->k[1]);

@@ -67,30 +67,47 @@ struct Geo {
   static constexpr int T = N / rmax(make_sched(N));  // threads per line
 };
 
-// u[r] *= w^r for r = 1..R-1, w = tw[b] (conjugated when DIR > 0), where
-// tw[r*b] = w^r.  Loads w, w^2, w^4, w^8 from the table and forms the other
-// powers with at most three complex products, so a radix-16 butterfly issues 4
-// table loads instead of 15.
-template <int R, int DIR>
-__device__ __forceinline__ void twiddle(float2 (&u)[R], const float2* __restrict__ tw, int b) {
-  auto ld = [&](int r) {
-    float2 w = tw[r * b];
+// ---- per-pass twiddle tables ----
+// Pass p of a schedule (radix R, span P > 1) owns tw_pows(R) rows of P entries:
+// row j, entry k = w(k)^(2^j), w(k) = exp(-2*pi*i*k / (P*R)).  A butterfly loads
+// w, w^2, w^4, w^8 (as its radix needs) at index k, so lanes with consecutive k
+// read consecutive entries (no LDS bank conflicts when the table sits in LDS),
+// and forms the other powers with at most three complex products.
+constexpr int tw_pows(int R) { return R == 16 ? 4 : (R == 8 || R == 5) ? 3 : (R == 4 || R == 3) ? 2 : 1; }
+
+// Entries before pass p's rows (schedule order).
+constexpr int tw_pass_offset(Sched s, int p) {
+  int off = 0, span = 1;
+  for (int q = 0; q < p; ++q) {
+    if (span > 1) off += tw_pows(s.r[q]) * span;
+    span *= s.r[q];
+  }
+  return off;
+}
+constexpr int tw_table_size(Sched s) { return tw_pass_offset(s, s.np); }
+
+// u[r] *= w^r for r = 1..R-1 (w conjugated when DIR > 0) from pass rows tw, index k.
+template <int R, int DIR, int P>
+__device__ __forceinline__ void twiddle(float2 (&u)[R], const float2* __restrict__ tw, int k) {
+  auto ld = [&](int j) {
+    float2 w = tw[j * P + k];
     if (DIR > 0) w.y = -w.y;
     return w;
   };
   if constexpr (R == 2) {
-    u[1] = cmul(u[1], ld(1));
+    u[1] = cmul(u[1], ld(0));
   } else if constexpr (R == 3) {
-    u[1] = cmul(u[1], ld(1));
-    u[2] = cmul(u[2], ld(2));
+    u[1] = cmul(u[1], ld(0));
+    u[2] = cmul(u[2], ld(1));
   } else if constexpr (R == 4 || R == 5) {
-    const float2 w1 = ld(1), w2 = ld(2);
+    const float2 w1 = ld(0), w2 = ld(1);
     u[1] = cmul(u[1], w1);
     u[2] = cmul(u[2], w2);
     u[3] = cmul(u[3], cmul(w1, w2));
-    if constexpr (R == 5) u[4] = cmul(u[4], ld(4));
-  } else if constexpr (R == 8 || R == 16) {
-    const float2 w1 = ld(1), w2 = ld(2), w4 = ld(4);
+    if constexpr (R == 5) u[4] = cmul(u[4], ld(2));
+  } else {
+    static_assert(R == 8 || R == 16, "radix");
+    const float2 w1 = ld(0), w2 = ld(1), w4 = ld(2);
     const float2 w3 = cmul(w1, w2), w5 = cmul(w1, w4), w6 = cmul(w2, w4), w7 = cmul(w3, w4);
     u[1] = cmul(u[1], w1);
     u[2] = cmul(u[2], w2);
@@ -100,7 +117,7 @@ __device__ __forceinline__ void twiddle(float2 (&u)[R], const float2* __restrict
     u[6] = cmul(u[6], w6);
     u[7] = cmul(u[7], w7);
     if constexpr (R == 16) {
-      const float2 w8 = ld(8);
+      const float2 w8 = ld(3);
       u[8] = cmul(u[8], w8);
       u[9] = cmul(u[9], cmul(w1, w8));
       u[10] = cmul(u[10], cmul(w2, w8));
@@ -110,9 +127,6 @@ __device__ __forceinline__ void twiddle(float2 (&u)[R], const float2* __restrict
       u[14] = cmul(u[14], cmul(w6, w8));
       u[15] = cmul(u[15], cmul(w7, w8));
     }
-  } else {
-#pragma unroll
-    for (int r = 1; r < R; ++r) u[r] = cmul(u[r], ld(r));
   }
 }
 
@@ -137,19 +151,19 @@ __device__ __forceinline__ void pass_load(float2 (&u)[(N / R + T - 1) / T][R], i
   }
 }
 
-// Twiddles, butterflies and the Stockham-ordered stores of one pass.
+// Twiddles, butterflies and the Stockham-ordered stores of one pass; tw = the
+// pass's table rows (unused when P == 1).
 template <int N, int T, int R, int P, int DIR, class St>
 __device__ __forceinline__ void pass_store(float2 (&u)[(N / R + T - 1) / T][R], int t, const float2* __restrict__ tw,
                                            St st) {
   constexpr int NB = N / R;
   constexpr int M = Bfly<N, T, R>::M;
-  constexpr int step = N / (P * R);
 #pragma unroll
   for (int m = 0; m < M; ++m) {
     const int i = t + m * T;
     if ((M * T == NB) || i < NB) {
       const int k = i % P;
-      if (P > 1) twiddle<R, DIR>(u[m], tw, k * step);
+      if constexpr (P > 1) twiddle<R, DIR, P>(u[m], tw, k);
       Dft<R, DIR>::run(u[m]);
       const int j = (i / P) * P * R + k;
 #pragma unroll
@@ -170,6 +184,7 @@ __device__ __forceinline__ void pass(int t, const float2* __restrict__ tw, Ld ld
 // Passes [p, S.np) of schedule S; the first uses gld, the last gst, the rest
 // LDS.  LAST_LDS: gst writes the same LDS the last pass reads, so that pass
 // needs the barrier between its loads and stores.
+// tw = the schedule's tables (pass rows in schedule order).
 template <int N, bool INV, int DIR, int p, bool LAST_LDS = false, class GLd, class GSt, class LLd, class LSt>
 __device__ __forceinline__ void passes(int t, const float2* __restrict__ tw, GLd gld, GSt gst, LLd lld, LSt lst,
                                        bool first_from_global) {
@@ -180,10 +195,12 @@ __device__ __forceinline__ void passes(int t, const float2* __restrict__ tw, GLd
     constexpr int T = Geo<N>::T;
     constexpr bool last = (p == S.np - 1);
     const bool first = (p == 0) && first_from_global;
-    if (first && last) pass<N, T, R, P, DIR>(t, tw, gld, gst, false);
-    else if (first) pass<N, T, R, P, DIR>(t, tw, gld, lst, false);
-    else if (last) pass<N, T, R, P, DIR>(t, tw, lld, gst, LAST_LDS);
-    else pass<N, T, R, P, DIR>(t, tw, lld, lst, true);
+    constexpr int off = tw_pass_offset(S, p);
+    const float2* twp = tw + off;
+    if (first && last) pass<N, T, R, P, DIR>(t, twp, gld, gst, false);
+    else if (first) pass<N, T, R, P, DIR>(t, twp, gld, lst, false);
+    else if (last) pass<N, T, R, P, DIR>(t, twp, lld, gst, LAST_LDS);
+    else pass<N, T, R, P, DIR>(t, twp, lld, lst, true);
     if (!last) __syncthreads();
     passes<N, INV, DIR, p + 1, LAST_LDS>(t, tw, gld, gst, lld, lst, first_from_global);
   }

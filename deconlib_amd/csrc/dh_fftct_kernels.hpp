@@ -220,6 +220,10 @@ __global__ void __launch_bounds__(kMaxThreads) k_col(Pack<ColArgs> pk, int L) {
 }
 
 // ---- fused Z pass: forward FFT, x OTF, inverse FFT, in place ----
+// Entries of the forward-schedule tables; the inverse schedule's rows follow them.
+template <int N>
+constexpr int kTwF = tw_table_size(Geo<N>::F);
+
 template <int N, int R, int P>
 __device__ __forceinline__ void z_mid(int t, const float2* __restrict__ tw, float2* lds, int L, int ln,
                                       const float2* cb, int64_t rs, int lo, bool ok, const float2* __restrict__ otf,
@@ -242,13 +246,14 @@ __device__ __forceinline__ void z_mid(int t, const float2* __restrict__ tw, floa
     }
   }
   if (from_lds) __syncthreads();
-  constexpr int step = N / (P * R);
+  constexpr int off = tw_pass_offset(Geo<N>::F, Geo<N>::F.np - 1);
+  const float2* twp = tw + off;
 #pragma unroll
   for (int m = 0; m < M; ++m) {
     const int i = t + m * T;
     if ((M * T == NB) || i < NB) {
       const int k = i % P;
-      if (P > 1) twiddle<R, -1>(u[m], tw, k * step);
+      if constexpr (P > 1) twiddle<R, -1, P>(u[m], twp, k);
       Dft<R, -1>::run(u[m]);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
@@ -272,8 +277,10 @@ __device__ __forceinline__ void z_fwd_passes(int t, const float2* __restrict__ t
   if constexpr (p < S.np - 1) {
     constexpr int R = S.r[p];
     constexpr int P = span_of(S, p);
-    if (p == 0) pass<N, Geo<N>::T, R, P, -1>(t, tw, gld, lst, false);
-    else pass<N, Geo<N>::T, R, P, -1>(t, tw, lld, lst, true);
+    constexpr int off = tw_pass_offset(S, p);
+    const float2* twp = tw + off;
+    if (p == 0) pass<N, Geo<N>::T, R, P, -1>(t, twp, gld, lst, false);
+    else pass<N, Geo<N>::T, R, P, -1>(t, twp, lld, lst, true);
     __syncthreads();
     z_fwd_passes<N, p + 1>(t, tw, gld, lld, lst);
   }
@@ -305,7 +312,7 @@ __device__ __forceinline__ void z_body(const ZArgs& a, int64_t lb, int L, float2
     z_mid<N, F.r[F.np - 1], span_of(F, F.np - 1)>(t, a.tw, lds, L, ln, cb, rs, lo, ok, a.otf, col, a.conj, cb, n_out);
     if constexpr (F.np > 1) {
       __syncthreads();
-      passes<N, true, 1, 1>(t, a.tw, gld, gst, lld, lst, false);
+      passes<N, true, 1, 1>(t, a.tw + kTwF<N>, gld, gst, lld, lst, false);
     }
   }
 }
@@ -370,7 +377,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_z(Pack<ZArgs> pk, int L) {
       z_mid<N, F.r[F.np - 1], span_of(F, F.np - 1)>(t, a.tw, lds, L, ln, cb, rs, a.lo, ok, a.otf, col, a.conj, cb,
                                                     a.n_out);
       __syncthreads();
-      passes<N, true, 1, 1>(t, a.tw, lld, gst, lld, lst, false);
+      passes<N, true, 1, 1>(t, a.tw + kTwF<N>, lld, gst, lld, lst, false);
       if (bn >= total) break;
       b = bn;
       if (!kPrefetch) fetch(b);
@@ -409,6 +416,19 @@ __device__ __forceinline__ void blocked_rows(int64_t row0, int RPB, int64_t nrow
     const int w = (b == nbk - 1) ? H - b * kBlk : kBlk;
     const int64_t off = z * prow * (int64_t)H + (int64_t)b * kBlk * prow + (int64_t)y * w + j;
     fn(rl, k, off, row0 + rl < nrows);
+  }
+}
+
+// Both twiddle tables of an X workgroup in LDS after its rows: the pass tables
+// of the length-M transform at [0, tw_table_size), the M+1 split twiddles at
+// [M, 2M].  One transform length per launch, so one copy serves every volume.
+template <int M>
+__device__ __forceinline__ void stage_x_tables(float2* ltw, const float2* twm, const float2* twp) {
+  constexpr int TWM = tw_table_size(Geo<M>::F);
+  static_assert(TWM <= M, "the pass tables fit the LDS slot reserved for them");
+  for (int i = threadIdx.x; i < 2 * M + 1; i += blockDim.x) {
+    if (i < TWM) ltw[i] = twm[i];
+    else if (i >= M) ltw[i] = twp[i - M];
   }
 }
 
@@ -487,7 +507,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_xf(Pack<XfArgs> pk, int RPB) {
   int64_t b = blockIdx.x;
   if (b >= total) return;
   float2* ltw = lds + RPB * RS;
-  for (int i = threadIdx.x; i < 2 * M + 1; i += blockDim.x) ltw[i] = i < M ? pk.a[0].twm[i] : pk.a[0].twp[i - M];
+  stage_x_tables<M>(ltw, pk.a[0].twm, pk.a[0].twp);
   const float2* twm = ltw;
   const float2* twp = ltw + M;
   float2 u[Bfly<M, T, R0>::M][R0];
@@ -678,7 +698,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_xi(Pack<XiArgs> pk, int RPB) {
   int64_t b = blockIdx.x;
   if (b >= total) return;
   float2* ltw = lds + RPB * RS;
-  for (int i = threadIdx.x; i < 2 * M + 1; i += blockDim.x) ltw[i] = i < M ? pk.a[0].twm[i] : pk.a[0].twp[i - M];
+  stage_x_tables<M>(ltw, pk.a[0].twm, pk.a[0].twp);
   float2 q[XiEpt<M>::value];
   {
     int thr = threadIdx.x;
