@@ -71,6 +71,23 @@ class BatchItem(C.Structure):
     ]
 
 
+class RLItem(C.Structure):
+    _fields_ = [
+        ("plan", C.c_void_p),
+        ("otf", C.c_void_p),
+        ("work", C.c_void_p),
+        ("work_bytes", C.c_int64),
+        ("x", C.c_void_p),
+        ("obs", C.c_void_p),
+        ("sens", C.c_void_p),
+        ("background", C.c_float),
+        ("eps", C.c_float),
+        ("reduce_out", C.c_void_p),
+        ("spectrum_ready", C.c_int32),
+        ("keep_spectrum", C.c_int32),
+    ]
+
+
 class PdParams(C.Structure):
     _fields_ = [
         ("delta", C.c_double),
@@ -126,6 +143,8 @@ SYMBOLS = {
     "dh_hessian_forward": (_int, _vp, _vp, _int, _int, _int, C.POINTER(_f), _vp),
     "dh_hessian_adjoint": (_int, _vp, _vp, _int, _int, _int, C.POINTER(_f), _vp),
     "dh_batch_conv_apply": (_int, _int, C.POINTER(BatchItem), _int, _vp),
+    "dh_rl_iteration": (_int, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _f, _f, _vp, _int, _int, _vp),
+    "dh_batch_rl_iteration": (_int, _int, C.POINTER(RLItem), _vp),
     "dh_timing_enable": (_int, _int),
     "dh_timing_read": (_int, C.POINTER(_d), C.POINTER(_i64), _int),
 }

@@ -1,13 +1,12 @@
 """Batched Richardson–Lucy: many volumes of different shapes advanced in lockstep.
 
-Each job keeps its own device-resident state (x, sensitivity, ratio buffer,
-plan workspace, OTF).  One iteration of the whole batch is two calls of
-``dh_batch_conv_apply``: the forward convolutions of every job (ratio fused in
-the epilogue) and then every adjoint (multiplicative update fused).  Inside
-each call the volumes are grouped by transform length, so a pass over the
-whole batch is a handful of launches, not one per volume.  Per-job results
-are bit-identical to running the job alone (same kernels, same shapes, same
-reduction order).
+Each job keeps its own device-resident state (x, sensitivity, plan workspace,
+OTF).  One iteration of the whole batch is one call of ``dh_batch_rl_iteration``
+(the fused iteration of every job, stage by stage, volumes grouped by transform
+length, so a stage over the whole batch is a handful of launches, not one per
+volume); jobs whose canvas lacks specialised kernels use two calls of
+``dh_batch_conv_apply`` instead.  Per-job results are bit-identical to running
+the job alone (same kernels, same shapes, same reduction order).
 """
 
 from __future__ import annotations
@@ -61,6 +60,24 @@ class RLJob:
                            work_bytes=int(plan.info.work_bytes), input=src.data_ptr(), out=dst.data_ptr(),
                            fuse=f)
 
+    def _rl_item(self) -> N.RLItem:
+        """This job's next fused iteration (dh_rl_iteration semantics)."""
+        st = self.state
+        plan = st.plan
+        k = self.k
+        red = st.slot(k - 1) if (k > 0 and st.wants_loss(k - 1)) else None
+        keep = k + 1 < self.num_iter
+        ready = st._spec_gen is not None and plan.gen == st._spec_gen
+        return N.RLItem(plan=plan._h, otf=plan.otf_buf.data_ptr(), work=plan.work.data_ptr(),
+                        work_bytes=int(plan.info.work_bytes), x=st.x.data_ptr(), obs=st.obs.data_ptr(),
+                        sens=st.sens.data_ptr(), background=st.bg, eps=_EPS, reduce_out=red,
+                        spectrum_ready=int(ready), keep_spectrum=int(keep))
+
+    def _after_fused(self) -> None:
+        st = self.state
+        st.plan.gen += 1
+        st._spec_gen = st.plan.gen if self.k + 1 < self.num_iter else None
+
     def finish(self) -> RLResult:
         st = self.state
         k_last = self.k - 1
@@ -77,10 +94,25 @@ class BatchRunner:
         self.max_items = int(max_items)
 
     def step(self, jobs: List[RLJob]) -> None:
-        """One RL iteration for every job in ``jobs`` (all not yet done)."""
+        """One RL iteration for every job in ``jobs`` (all not yet done).
+
+        Jobs on canvases with specialised kernels advance through the fused batched
+        iteration (ratio kept on chip, each job's X spectrum carried between calls);
+        the others through a batched forward and a batched adjoint application.
+        """
         live = [j for j in jobs if not j.done]
-        for s in range(0, len(live), self.max_items):
-            chunk = live[s:s + self.max_items]
+        fused = [j for j in live if j.state.fused_iter]
+        plain = [j for j in live if not j.state.fused_iter]
+        for s in range(0, len(fused), self.max_items):
+            chunk = fused[s:s + self.max_items]
+            arr = (N.RLItem * len(chunk))(*[j._rl_item() for j in chunk])
+            N.call("dh_batch_rl_iteration", len(chunk), arr, N.stream_ptr())
+            for j in chunk:
+                j._after_fused()
+        for j in plain:
+            j.state.plan.gen += 1  # the batched launches overwrite each plan's workspace
+        for s in range(0, len(plain), self.max_items):
+            chunk = plain[s:s + self.max_items]
             for forward in (True, False):
                 arr = (N.BatchItem * len(chunk))(*[j._item(forward) for j in chunk])
                 N.call("dh_batch_conv_apply", len(chunk), arr, 0 if forward else 1, N.stream_ptr())

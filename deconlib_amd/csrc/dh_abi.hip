@@ -270,7 +270,7 @@ CtPasses ct_passes(const dh_conv_plan* p, int adjoint, const float* in, float* o
   c.t_xi = ct_row_tile(p->M, tile_override(p, DH_PASS_X_INV));
   c.n_xi = (nrows + c.t_xi - 1) / c.t_xi;
   c.t_xi |= persist_flag(p, DH_PASS_X_INV);
-  c.xi = dh::XiArgs{B1, out, tw + p->tw_xm, tw + p->tw_xp, nrows, s2, (int)H, epi, xi_vec2(out, s2, epi),
+  c.xi = dh::XiArgs{B1, out, nullptr, tw + p->tw_xm, tw + p->tw_xp, nrows, s2, (int)H, epi, xi_vec2(out, s2, epi),
                     dh::kBlk, s1};
   return c;
 }
@@ -298,11 +298,57 @@ int ct_apply(const dh_conv_plan* p, int adjoint, const float* in, float* out, co
   return DH_OK;
 }
 
+// One fused RL iteration with the specialised kernels (see dh_rl_iteration).
+int ct_rl_iteration(const dh_conv_plan* p, const void* otf, char* ws, float* x, const float* obs, const float* sens,
+                    float bg, float eps, double* reduce_out, int spectrum_ready, int keep_spectrum, hipStream_t st) {
+  float2* B1 = (float2*)(ws + p->b1_off);
+  const dh::Pro none{dh::PRO_NONE, 0.f, 0.f, nullptr};
+  dh::Epi ratio{dh::EPI_RL_RATIO, bg, eps, obs, nullptr, nullptr};
+  if (reduce_out) ratio.partials = (double*)(ws + p->part_off);
+  const dh::Epi update{dh::EPI_RL_UPDATE, 0.f, eps, x, sens, nullptr};
+  // forward convolution of x; its inverse X pass computes the ratio and runs the
+  // adjoint's forward X pass on it (the ratio never reaches HBM)
+  CtPasses f = ct_passes(p, 0, x, nullptr, otf, ws, none, ratio);
+  f.xi.spec = B1;
+  if (!spectrum_ready) DH_TRY(launch_one(dh::CT_XF, p->M, -1, f.xf, f.t_xf, f.n_xf, DH_PASS_X_FWD, "x forward pass", st));
+  if (f.has_y) DH_TRY(launch_one(dh::CT_COL, p->P[1], -1, f.yf, f.t_yf, f.n_yf, DH_PASS_Y_FWD, "y forward pass", st));
+  DH_TRY(launch_one(dh::CT_Z, p->P[0], 0, f.z, f.t_z, f.n_z, DH_PASS_Z, "z pass", st));
+  if (f.has_y) DH_TRY(launch_one(dh::CT_COL, p->P[1], 1, f.yi, f.t_yi, f.n_yi, DH_PASS_Y_INV, "y inverse pass", st));
+  if (reduce_out && f.n_xi > p->n_partials) return fail(DH_EWORKSPACE, "partials area too small");
+  DH_TRY(launch_one(dh::CT_XR, p->M, 1, f.xi, f.t_xi, f.n_xi, DH_PASS_X_INV, "x round trip (ratio)", st));
+  // adjoint (correlation) of the ratio; its inverse X pass applies the update in
+  // place and, when asked, leaves the forward X spectrum of the new x in B1
+  CtPasses a = ct_passes(p, 1, nullptr, x, otf, ws, none, update);
+  if (a.has_y) DH_TRY(launch_one(dh::CT_COL, p->P[1], -1, a.yf, a.t_yf, a.n_yf, DH_PASS_Y_FWD, "y forward pass", st));
+  DH_TRY(launch_one(dh::CT_Z, p->P[0], 0, a.z, a.t_z, a.n_z, DH_PASS_Z, "z pass", st));
+  if (a.has_y) DH_TRY(launch_one(dh::CT_COL, p->P[1], 1, a.yi, a.t_yi, a.n_yi, DH_PASS_Y_INV, "y inverse pass", st));
+  if (keep_spectrum) {
+    a.xi.spec = B1;
+    DH_TRY(launch_one(dh::CT_XR, p->M, 1, a.xi, a.t_xi, a.n_xi, DH_PASS_X_INV, "x round trip (update)", st));
+  } else {
+    DH_TRY(launch_one(dh::CT_XI, p->M, 1, a.xi, a.t_xi, a.n_xi, DH_PASS_X_INV, "x inverse pass", st));
+  }
+  if (reduce_out)
+    DH_TRY(hip_check(dh::launch_sum_partials(ratio.partials, f.n_xi, reduce_out, st), "epilogue reduction"));
+  return DH_OK;
+}
+
 }  // namespace
 
 extern "C" {
 
 int dh_abi_version(void) { return DH_ABI_VERSION; }
+
+int dh_rl_iteration(const dh_conv_plan* p, const void* otf_dev, void* work_dev, int64_t work_bytes, float* x,
+                    const float* obs, const float* sens, float background, float eps, double* reduce_out,
+                    int spectrum_ready, int keep_spectrum, void* stream) {
+  if (!p || !otf_dev || !work_dev || !x || !obs || !sens) return fail(DH_EINVAL, "null argument");
+  if (!p->tw_dev) return fail(DH_EINVAL, "plan is not bound (dh_conv_plan_bind)");
+  if (work_bytes < p->ws_bytes) return fail(DH_EWORKSPACE, "workspace too small");
+  if (!p->ct) return fail(DH_EUNSUPPORTED, "the fused RL iteration needs a canvas with specialised kernels");
+  return ct_rl_iteration(p, otf_dev, (char*)work_dev, x, obs, sens, background, eps, reduce_out, spectrum_ready,
+                         keep_spectrum, (hipStream_t)stream);
+}
 
 const char* dh_last_error(void) { return g_err.c_str(); }
 
@@ -872,9 +918,83 @@ int ct_batch_apply(int nitems, const dh_batch_item* items, int adjoint, hipStrea
   return DH_OK;
 }
 
+// Batched fused RL iterations: the stages of dh_rl_iteration, each stage one
+// grouped launch per transform length over every item.
+int ct_batch_rl(int nitems, const dh_rl_item* items, hipStream_t st) {
+  std::vector<Group<dh::XfArgs>> gxf;
+  std::vector<Group<dh::ColArgs>> gyf1, gyi1, gyf2, gyi2;
+  std::vector<Group<dh::ZArgs>> gz1, gz2;
+  std::vector<Group<dh::XiArgs>> gr1, gr2, gi2;
+  struct Red {
+    double* partials;
+    int64_t n;
+    double* out;
+  };
+  std::vector<Red> reds;
+  for (int i = 0; i < nitems; ++i) {
+    const dh_rl_item& it = items[i];
+    const dh_conv_plan* p = it.plan;
+    char* ws = (char*)it.work;
+    float2* B1 = (float2*)(ws + p->b1_off);
+    const dh::Pro none{dh::PRO_NONE, 0.f, 0.f, nullptr};
+    dh::Epi ratio{dh::EPI_RL_RATIO, it.background, it.eps, it.obs, nullptr, nullptr};
+    if (it.reduce_out) ratio.partials = (double*)(ws + p->part_off);
+    const dh::Epi update{dh::EPI_RL_UPDATE, 0.f, it.eps, it.x, it.sens, nullptr};
+    CtPasses f = ct_passes(p, 0, it.x, nullptr, it.otf, ws, none, ratio);
+    f.xi.spec = B1;
+    CtPasses a = ct_passes(p, 1, nullptr, it.x, it.otf, ws, none, update);
+    if (it.reduce_out) {
+      if (f.n_xi > p->n_partials) return fail(DH_EWORKSPACE, "partials area too small");
+      reds.push_back(Red{ratio.partials, f.n_xi, it.reduce_out});
+    }
+    if (!it.spectrum_ready) add_to_group(gxf, p->M, f.t_xf, f.xf, f.n_xf);
+    if (f.has_y) {
+      add_to_group(gyf1, p->P[1], f.t_yf, f.yf, f.n_yf);
+      add_to_group(gyi1, p->P[1], f.t_yi, f.yi, f.n_yi);
+      add_to_group(gyf2, p->P[1], a.t_yf, a.yf, a.n_yf);
+      add_to_group(gyi2, p->P[1], a.t_yi, a.yi, a.n_yi);
+    }
+    add_to_group(gz1, p->P[0], f.t_z, f.z, f.n_z);
+    add_to_group(gz2, p->P[0], a.t_z, a.z, a.n_z);
+    add_to_group(gr1, p->M, f.t_xi, f.xi, f.n_xi);
+    if (it.keep_spectrum) {
+      a.xi.spec = B1;
+      add_to_group(gr2, p->M, a.t_xi, a.xi, a.n_xi);
+    } else {
+      add_to_group(gi2, p->M, a.t_xi, a.xi, a.n_xi);
+    }
+  }
+  DH_TRY(launch_groups(dh::CT_XF, -1, gxf, DH_PASS_X_FWD, st));
+  DH_TRY(launch_groups(dh::CT_COL, -1, gyf1, DH_PASS_Y_FWD, st));
+  DH_TRY(launch_groups(dh::CT_Z, 0, gz1, DH_PASS_Z, st));
+  DH_TRY(launch_groups(dh::CT_COL, 1, gyi1, DH_PASS_Y_INV, st));
+  DH_TRY(launch_groups(dh::CT_XR, 1, gr1, DH_PASS_X_INV, st));
+  DH_TRY(launch_groups(dh::CT_COL, -1, gyf2, DH_PASS_Y_FWD, st));
+  DH_TRY(launch_groups(dh::CT_Z, 0, gz2, DH_PASS_Z, st));
+  DH_TRY(launch_groups(dh::CT_COL, 1, gyi2, DH_PASS_Y_INV, st));
+  DH_TRY(launch_groups(dh::CT_XR, 1, gr2, DH_PASS_X_INV, st));
+  DH_TRY(launch_groups(dh::CT_XI, 1, gi2, DH_PASS_X_INV, st));
+  for (const Red& r : reds)
+    DH_TRY(hip_check(dh::launch_sum_partials(r.partials, r.n, r.out, st), "batched epilogue reduction"));
+  return DH_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int dh_batch_rl_iteration(int nitems, const dh_rl_item* items, void* stream) {
+  if (!items || nitems < 1) return fail(DH_EINVAL, "bad batch arguments");
+  for (int i = 0; i < nitems; ++i) {
+    const dh_rl_item& it = items[i];
+    if (!it.plan || !it.otf || !it.work || !it.x || !it.obs || !it.sens)
+      return fail(DH_EINVAL, "null field in RL batch item");
+    if (!it.plan->tw_dev) return fail(DH_EINVAL, "plan is not bound (dh_conv_plan_bind)");
+    if (it.work_bytes < it.plan->ws_bytes) return fail(DH_EWORKSPACE, "workspace too small");
+    if (!it.plan->ct) return fail(DH_EUNSUPPORTED, "the fused RL iteration needs a canvas with specialised kernels");
+  }
+  return ct_batch_rl(nitems, items, (hipStream_t)stream);
+}
 
 int dh_batch_conv_apply(int nitems, const dh_batch_item* items, int adjoint, void* stream) {
   if (!items || nitems < 1) return fail(DH_EINVAL, "bad batch arguments");

@@ -137,7 +137,8 @@ struct XfArgs {  // X real-to-complex rows
 };
 struct XiArgs {  // X complex-to-real rows with the epilogue
   const float2* src;
-  float* dst;
+  float* dst;        // cropped real rows (may be null in a round trip: not stored)
+  float2* spec;      // round trip only: forward X spectrum of the epilogue's output (blocked)
   const float2* twm;
   const float2* twp;
   int64_t nrows;
@@ -157,7 +158,9 @@ struct Pack {
   int nvol;
 };
 
-enum CtKind { CT_COL = 0, CT_Z = 1, CT_XF = 2, CT_XI = 3 };
+// CT_XR: X round trip = inverse X pass with the epilogue, whose output rows are
+// transformed again by the forward X pass (XiArgs.spec) without leaving the CU.
+enum CtKind { CT_COL = 0, CT_Z = 1, CT_XF = 2, CT_XI = 3, CT_XR = 4 };
 // True if length n has specialised kernels.
 bool ct_supported(int n);
 // Threads per line of the specialised kernels for length n.

@@ -623,7 +623,11 @@ __device__ __forceinline__ void xi_tile(const XiArgs& a, int64_t row0, int RPB,
   }
 }
 
-template <int M>
+// RT: round trip.  The epilogue's output rows (zero beyond s2) are written to the
+// LDS row instead of (or besides) a.dst, transformed by the forward X pass in
+// LDS, and the spectrum stored to a.spec; bit-identical to an X inverse pass
+// writing the rows followed by an X forward pass reading them.
+template <int M, bool RT>
 __device__ __forceinline__ void xi_body(const XiArgs& a, int64_t lb, int RPB, float2* lds, const float2* twm,
                                         const float2* twp, int thr) {
   constexpr int T = Geo<M>::T;
@@ -649,18 +653,28 @@ __device__ __forceinline__ void xi_body(const XiArgs& a, int64_t lb, int RPB, fl
     return make_float2(A.x - wb.y, A.y + wb.x);
   };
   double acc = 0.0;
-  float* drow = a.dst + row * s2;
+  float* drow = a.dst ? a.dst + row * s2 : nullptr;
   const int64_t base = row * s2;
   const bool vec2 = a.vec2 != 0;
   auto gst = [&](int e, float2 v) {
-    if (!ok) return;
     const int q0 = 2 * e, q1 = 2 * e + 1;
-    if (vec2 && q1 < s2) {
-      *reinterpret_cast<float2*>(drow + q0) = epi_apply2(a.epi, v, base + q0, acc);
-      return;
+    float2 o = make_float2(0.f, 0.f);
+    if (ok) {
+      if (vec2 && q1 < s2) {
+        o = epi_apply2(a.epi, v, base + q0, acc);
+        if (!RT || drow) *reinterpret_cast<float2*>(drow + q0) = o;
+      } else {
+        if (q0 < s2) {
+          o.x = epi_apply(a.epi, v.x, base + q0, acc);
+          if (!RT || drow) drow[q0] = o.x;
+        }
+        if (q1 < s2) {
+          o.y = epi_apply(a.epi, v.y, base + q1, acc);
+          if (!RT || drow) drow[q1] = o.y;
+        }
+      }
     }
-    if (q0 < s2) drow[q0] = epi_apply(a.epi, v.x, base + q0, acc);
-    if (q1 < s2) drow[q1] = epi_apply(a.epi, v.y, base + q1, acc);
+    if constexpr (RT) lrow[xrow_lds(e)] = o;
   };
   auto lld = [&](int e) { return lrow[xrow_lds(e)]; };
   auto lst = [&](int e, float2 v) { lrow[xrow_lds(e)] = v; };
@@ -678,7 +692,29 @@ __device__ __forceinline__ void xi_body(const XiArgs& a, int64_t lb, int RPB, fl
     } else {
       pass<M, T, R0, 1, 1>(t, twm, pre, lst, true);
       __syncthreads();
-      passes<M, false, 1, 1>(t, twm, pre, gst, lld, lst, false);
+      // a round trip's stores go to the LDS row the last pass reads: LAST_LDS
+      passes<M, false, 1, 1, RT>(t, twm, pre, gst, lld, lst, false);
+    }
+  }
+  if constexpr (RT) {
+    // fresh lane coordinates: keeps the forward phase's addresses from being
+    // computed (and held in registers) during the inverse phase
+    int thr2 = thr;
+    opaque(thr2);
+    const int t2 = thr2 % T;
+    float2* lrow2 = lds + (thr2 / T) * RS;
+    auto lld2 = [&](int e) { return lrow2[xrow_lds(e)]; };
+    auto lst2 = [&](int e, float2 v) { lrow2[xrow_lds(e)] = v; };
+    __syncthreads();
+    passes<M, false, -1, 0, true>(t2, twm, lld2, lst2, lld2, lst2, false);
+    __syncthreads();
+    if (a.blk > 0) {
+      blocked_rows(lb * RPB, RPB, a.nrows, a.H, a.prow, [&](int r, int k, int64_t off, bool valid) {
+        if (valid) a.spec[off] = xf_post<M>(lds + r * RS, twp, k);
+      });
+    } else if (ok) {
+      float2* sr = a.spec + row * a.H;
+      for (int k = t2; k <= M; k += T) sr[k] = xf_post<M>(lrow2, twp, k);
     }
   }
   if (a.epi.partials) {
@@ -690,7 +726,7 @@ __device__ __forceinline__ void xi_body(const XiArgs& a, int64_t lb, int RPB, fl
 
 // Persistent: the next tile's spectrum samples are loaded into registers while
 // the current tile is transformed; twiddles in LDS as for k_xf.
-template <int M>
+template <int M, bool RT>
 __global__ void __launch_bounds__(kMaxThreads) k_xi(Pack<XiArgs> pk, int RPB) {
   extern __shared__ float2 lds[];
   constexpr int RS = xrow_stride(M);
@@ -700,6 +736,23 @@ __global__ void __launch_bounds__(kMaxThreads) k_xi(Pack<XiArgs> pk, int RPB) {
   float2* ltw = lds + RPB * RS;
   stage_x_tables<M>(ltw, pk.a[0].twm, pk.a[0].twp);
   float2 q[XiEpt<M>::value];
+  if constexpr (RT) {
+    // the round trip holds two transforms' registers: load each tile at its start
+    // rather than keeping a prefetched tile live across the body
+    for (; b < total; b += gridDim.x) {
+      int thr = threadIdx.x;
+      opaque(thr);
+      const int v = pick(pk, b);
+      const XiArgs& a = pk.a[v];
+      const int64_t lb = b - pk.prefix[v];
+      xi_tile<M>(a, lb * RPB, RPB, q, lds, true, thr);
+      xi_tile<M>(a, lb * RPB, RPB, q, lds, false, thr);
+      __syncthreads();
+      xi_body<M, RT>(a, lb, RPB, lds, ltw, ltw + M, thr);
+      __syncthreads();
+    }
+    return;
+  }
   {
     int thr = threadIdx.x;
     opaque(thr);
@@ -721,7 +774,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_xi(Pack<XiArgs> pk, int RPB) {
       xi_tile<M>(pk.a[vn], (bn - pk.prefix[vn]) * RPB, RPB, q, lds, true, thr2);
     }
     __syncthreads();
-    xi_body<M>(a, lb, RPB, lds, ltw, ltw + M, thr);
+    xi_body<M, RT>(a, lb, RPB, lds, ltw, ltw + M, thr);
     if (bn >= total) break;
     b = bn;
     __syncthreads();
@@ -805,9 +858,17 @@ struct Launch {
   static hipError_t xi(const Pack<XiArgs>& pk, int R, int64_t nblocks, bool persist, hipStream_t st) {
     size_t shm = (size_t)xrow_lds_elems(N, R) * sizeof(float2);
     if (shm < 16 * sizeof(double)) shm = 16 * sizeof(double);
-    const void* k = (const void*)k_xi<N>;
+    const void* k = (const void*)k_xi<N, false>;
     (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    hipLaunchKernelGGL((k_xi<N>), dim3(grid(k, R * T, shm, nblocks, persist)), dim3(R * T), shm, st, pk, R);
+    hipLaunchKernelGGL((k_xi<N, false>), dim3(grid(k, R * T, shm, nblocks, persist)), dim3(R * T), shm, st, pk, R);
+    return hipGetLastError();
+  }
+  static hipError_t xr(const Pack<XiArgs>& pk, int R, int64_t nblocks, bool persist, hipStream_t st) {
+    size_t shm = (size_t)xrow_lds_elems(N, R) * sizeof(float2);
+    if (shm < 16 * sizeof(double)) shm = 16 * sizeof(double);
+    const void* k = (const void*)k_xi<N, true>;
+    (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    hipLaunchKernelGGL((k_xi<N, true>), dim3(grid(k, R * T, shm, nblocks, persist)), dim3(R * T), shm, st, pk, R);
     return hipGetLastError();
   }
 };

@@ -13,7 +13,8 @@ hipError_t launch_kind(int kind, int dir, const void* pk, int tile, int64_t nblo
     case CT_COL: return Launch<N>::col(dir, *(const Pack<ColArgs>*)pk, t, nblocks, persist, st);
     case CT_Z: return Launch<N>::z(*(const Pack<ZArgs>*)pk, t, nblocks, persist, st);
     case CT_XF: return Launch<N>::xf(*(const Pack<XfArgs>*)pk, t, nblocks, persist, st);
-    default: return Launch<N>::xi(*(const Pack<XiArgs>*)pk, t, nblocks, persist, st);
+    case CT_XI: return Launch<N>::xi(*(const Pack<XiArgs>*)pk, t, nblocks, persist, st);
+    default: return Launch<N>::xr(*(const Pack<XiArgs>*)pk, t, nblocks, persist, st);
   }
 }
 

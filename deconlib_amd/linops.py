@@ -124,9 +124,13 @@ class _ConvPlan:
                self.work.data_ptr(), int(info.work_bytes), C.byref(nsq), N.stream_ptr())
         self.norm_sq = float(nsq.value)
         self.canvas_shape = tuple(int(v) for v in info.canvas_shape)[3 - nd:]
+        # bumped by everything that writes the workspace, so a caller that left a
+        # spectrum there (dh_rl_iteration keep_spectrum) can tell whether it survives
+        self.gen = 0
 
     def apply(self, x: Optional[torch.Tensor], adjoint: bool, out: torch.Tensor,
               fuse: Optional[N.FuseArgs] = None) -> torch.Tensor:
+        self.gen += 1
         N.call("dh_conv_apply", self._h, int(adjoint), N.ptr(x), out.data_ptr(),
                self.otf_buf.data_ptr(), self.work.data_ptr(), int(self.info.work_bytes),
                C.byref(fuse) if fuse is not None else None, N.stream_ptr())

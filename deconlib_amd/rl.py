@@ -82,6 +82,9 @@ class _RLState:
         self.loss_slots = torch.zeros(n_slots, dtype=torch.float64, device=dev)
         self.slot_of = {}
         self.n_obs = self.obs.numel()
+        # one-call fused iteration (ratio never in HBM) on specialised canvases
+        self.fused_iter = self.plan is not None and int(self.plan.info.spectrum_block) > 0
+        self._spec_gen = None
         if self.plan is not None:
             self._init_fused(init, shape)
         else:
@@ -144,7 +147,16 @@ class _RLState:
         """x_k -> x_{k+1}; sums the loss of x_k when iteration k-1 asked for one."""
         red = self.slot(k - 1) if (k > 0 and self.wants_loss(k - 1)) else None
         st = N.stream_ptr()
-        if self.plan is not None:
+        if self.fused_iter:
+            plan = self.plan
+            keep = k + 1 < self.num_iter
+            ready = self._spec_gen is not None and plan.gen == self._spec_gen
+            N.call("dh_rl_iteration", plan._h, plan.otf_buf.data_ptr(), plan.work.data_ptr(),
+                   int(plan.info.work_bytes), self.x.data_ptr(), self.obs.data_ptr(), self.sens.data_ptr(),
+                   self.bg, _EPS, red, int(ready), int(keep), st)
+            plan.gen += 1
+            self._spec_gen = plan.gen if keep else None
+        elif self.plan is not None:
             f = N.FuseArgs(epilogue=N.EPI_RL_RATIO, epi_a=self.bg, epi_b=_EPS,
                            aux0=self.obs.data_ptr(), reduce_out=red)
             self.plan.apply(self.x, False, self.ratio, f)

@@ -117,6 +117,22 @@ int dh_conv_apply(const dh_conv_plan* plan, int adjoint, const float* in, float*
                   const void* otf_dev, void* work_dev, int64_t work_bytes,
                   const dh_fuse_args* fuse, void* stream);
 
+/* One Richardson-Lucy iteration (the compiled step _rl_step, rl_mlx.py:146-150):
+ *   x <- max(x * A^T(obs / max(A x + background, eps)) / sens, eps)
+ * with sens = A^T 1 on the support and +inf off it (dh_rl_support).  The ratio never
+ * reaches HBM: the forward convolution's inverse X pass computes it and runs the
+ * adjoint's forward X pass on it in the same kernel.  reduce_out (device, may be NULL)
+ * receives sum over voxels of the I-divergence of obs against A x + background for
+ * the input x (rl_mlx.py:58-80, unnormalised).  spectrum_ready != 0: the workspace still
+ * holds the forward X spectrum of x, left by the previous call on this workspace with
+ * keep_spectrum != 0 and nothing run on the workspace since.  keep_spectrum != 0: the
+ * update pass also leaves the forward X spectrum of the new x there.  Results are
+ * bit-identical to two dh_conv_apply calls with DH_EPI_RL_RATIO / DH_EPI_RL_UPDATE.
+ * DH_EUNSUPPORTED for canvases without specialised kernels. */
+int dh_rl_iteration(const dh_conv_plan* plan, const void* otf_dev, void* work_dev, int64_t work_bytes,
+                    float* x, const float* obs, const float* sens, float background, float eps,
+                    double* reduce_out, int spectrum_ready, int keep_spectrum, void* stream);
+
 /* ---- batched application: many plans (different shapes), few launches ---- */
 typedef struct {
   const dh_conv_plan* plan;
@@ -134,6 +150,26 @@ typedef struct {
  * in the kernel arguments.  Serving and batched RL call this once per
  * half-iteration. */
 int dh_batch_conv_apply(int nitems, const dh_batch_item* items, int adjoint, void* stream);
+
+/* Batched fused RL iterations: one dh_rl_iteration per item (each item its own plan,
+ * OTF, workspace and state; any mix of shapes), run stage by stage with the items
+ * grouped per transform length, so a batch iteration is a few launches per stage.
+ * Each item's result is bit-identical to dh_rl_iteration on it alone. */
+typedef struct {
+  const dh_conv_plan* plan;
+  const void* otf;
+  void* work;
+  int64_t work_bytes;
+  float* x;
+  const float* obs;
+  const float* sens;
+  float background;
+  float eps;
+  double* reduce_out;          /* device, may be NULL */
+  int32_t spectrum_ready;
+  int32_t keep_spectrum;
+} dh_rl_item;
+int dh_batch_rl_iteration(int nitems, const dh_rl_item* items, void* stream);
 
 /* ---- reductions and elementwise steps (rl_mlx.py:123-150) ---- */
 #define DH_RED_SUM 0
