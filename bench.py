@@ -1,7 +1,7 @@
 """Benchmark: Richardson–Lucy voxel-iterations/s at 64x512x512 float32 on one MI355X.
 
-A step is one RL iteration (forward convolution with the fused ratio
-epilogue + adjoint correlation with the fused multiplicative update) on a
+A step is one RL iteration (forward convolution, ratio, adjoint correlation,
+multiplicative update: one dh_rl_iteration call, the ratio kept on chip) on a
 64x512x512 volume with a full-size widefield PSF (FFT canvas 128x1024x1024).
 Inputs are resident in HBM before the timed region starts.
 
@@ -53,11 +53,25 @@ def conv_pass_bytes(S, P, epi_operand_bytes=0):
     }
 
 
-def rl_iteration_bytes(S, P):
-    """Algorithmic HBM bytes of one fused RL iteration: two convolutions plus the
-    epilogue operands (forward: obs; adjoint: x, sensitivity)."""
+def rl_iteration_bytes(S, P, fused=True):
+    """Algorithmic HBM bytes of one RL iteration.
+
+    Two-application form: two convolutions plus the epilogue operands (forward: obs;
+    adjoint: x, sensitivity).  Fused form (dh_rl_iteration): the ratio is never
+    written or read back and the updated x is transformed where it is produced, so
+    12 B per voxel less (ratio write + read, x re-read)."""
     core = sum(conv_pass_bytes(S, P).values())
-    return 2 * core + 12 * int(np.prod(S))
+    V = int(np.prod(S))
+    return 2 * core + 12 * V - (12 * V if fused else 0)
+
+
+def round_trip_bytes(S, P):
+    """Mean algorithmic bytes of the two X round trips of a fused RL iteration:
+    spectrum in and out (8 B x s0*s1*H each) plus obs (ratio) or x, sensitivity in
+    and x out (update): 16*s0*s1*H + 8*V on average."""
+    s0, s1, _ = S
+    H = P[2] // 2 + 1
+    return 16 * s0 * s1 * H + 8 * int(np.prod(S))
 
 
 def make_inputs(torch):
@@ -252,12 +266,15 @@ def main():
     value = vox * args.steps / dt
     P = blur.canvas_shape
     pb = conv_pass_bytes(SHAPE, P, epi_operand_bytes=6)
-    it_bytes = rl_iteration_bytes(SHAPE, P)
+    if st.fused_iter:  # pass 4 is the X round trip (inverse + epilogue + forward)
+        pb["x_inv"] = round_trip_bytes(SHAPE, P)
+    it_bytes = rl_iteration_bytes(SHAPE, P, fused=st.fused_iter)
     per_pass = {}
     for i, name in enumerate(N.PASS_NAMES):
         if cnt[i]:
             avg = ms[i] / cnt[i]
-            per_pass[name] = {"avg_ms": avg, "gbs": pb[name] / (avg * 1e-3) / 1e9, "launches": int(cnt[i])}
+            key = "x_round_trip" if (name == "x_inv" and st.fused_iter) else name
+            per_pass[key] = {"avg_ms": avg, "gbs": pb[name] / (avg * 1e-3) / 1e9, "launches": int(cnt[i])}
     z = per_pass["z_conv"]
     log(f"per-pass: {json.dumps(per_pass)}")
     ok = bool(torch.isfinite(st.x).all().item())
@@ -300,7 +317,8 @@ def main():
                         f"{P[0]}x{P[1]}x{P[2]}, {N_BLOBS} Gaussian blobs + 10, Poisson",
             "volume": list(SHAPE),
             "canvas": list(P),
-            "solver": "richardson_lucy_with_operator (fused)",
+            "solver": "richardson_lucy_with_operator (fused iteration, dh_rl_iteration)"
+                      if st.fused_iter else "richardson_lucy_with_operator (two fused applications)",
             "tiles": {N.PASS_NAMES[p]: t for p, t in tiles.items()},
             "finite": ok,
         },

@@ -157,6 +157,7 @@ namespace {
 // ---- launch shapes of the specialised kernels ----
 constexpr int kCtMaxThreads = 512;
 constexpr int64_t kCtDefaultLds = 64 * 1024;
+constexpr int64_t kCtRowDefaultLds = 96 * 1024;
 constexpr int64_t kCtMaxLds = 160 * 1024;
 
 // Column kernels: the fused Z pass (LDS n*L), a plain-layout column pass and a
@@ -175,14 +176,18 @@ bool ct_row_tile_ok(int m, int R) {
   return R >= 1 && (R & (R - 1)) == 0 && R * dh::ct_threads(m) <= kCtMaxThreads &&
          dh::xrow_lds_elems(m, R) * 8 <= kCtMaxLds;
 }
-// columns per workgroup: widest power of two within 512 threads and 64 KiB LDS
-// (one block for the blocked layout)
+// Default launch shapes (what the autotuner settles on for the 64x512x512
+// workload, canvas 128x1024x1024): columns per workgroup = the widest power of
+// two within 64 KiB LDS and 512 threads (256 for the fused Z pass), one block for
+// the blocked Y layout; rows per workgroup = the widest power of two within 512
+// threads and 96 KiB LDS.
 int ct_col_tile(int n, int64_t ncols, int over, int kind) {
   if (over > 0 && ct_col_tile_ok(n, over, kind)) return over;
   const int T = dh::ct_threads(n);
   const int cap = kind == COL_BLOCKED ? dh::kBlk : 64;
+  const int max_threads = kind == COL_Z ? 256 : kCtMaxThreads;
   int L = 1;
-  while (L < cap && 2 * L * T <= kCtMaxThreads && col_lds_bytes(n, 2 * L, kind) <= kCtDefaultLds) L *= 2;
+  while (L < cap && 2 * L * T <= max_threads && col_lds_bytes(n, 2 * L, kind) <= kCtDefaultLds) L *= 2;
   while (L > 1 && L / 2 >= ncols) L /= 2;
   return L;
 }
@@ -190,19 +195,20 @@ int ct_row_tile(int m, int over) {
   if (over > 0 && ct_row_tile_ok(m, over)) return over;
   const int T = dh::ct_threads(m);
   int R = 1;
-  while (R < 64 && 2 * R * T <= 256 && dh::xrow_lds_elems(m, 2 * R) * 8 <= kCtDefaultLds) R *= 2;
+  while (R < 64 && 2 * R * T <= kCtMaxThreads && dh::xrow_lds_elems(m, 2 * R) * 8 <= kCtRowDefaultLds) R *= 2;
   return R;
 }
 
 static_assert(DH_TILE_PERSISTENT == dh::kTilePersistent, "tile flag mismatch");
 
 // Tile override of a pass (0 = heuristic) and whether it launches persistently
-// (by default only the X inverse pass, whose epilogue operands make a second
-// dependent memory round trip per tile that the next-tile prefetch hides).
+// (by default the X passes: one-time per-workgroup set-up (the twiddle tables in
+// LDS) and, for the X inverse, a second dependent memory round trip per tile for
+// the epilogue operands, which the next-tile prefetch hides).
 int tile_override(const dh_conv_plan* p, int pass) { return p->tile[pass] & dh::kTileMask; }
 int persist_flag(const dh_conv_plan* p, int pass) {
   const int v = p->tile[pass];
-  const bool on = v > 0 ? (v & dh::kTilePersistent) != 0 : pass == DH_PASS_X_INV;
+  const bool on = v > 0 ? (v & dh::kTilePersistent) != 0 : (pass == DH_PASS_X_INV || pass == DH_PASS_X_FWD);
   return on ? dh::kTilePersistent : 0;
 }
 

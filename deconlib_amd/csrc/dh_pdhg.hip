@@ -293,6 +293,14 @@ __global__ void __launch_bounds__(kTileThreads) k_pd_dual(float* __restrict__ y2
     pp.load(x, bz, a0, a1);
     pp.load(xp, bz, b0, b1);
   }
+  // this voxel's y2 components, loaded one plane ahead
+  const int64_t pstride = (int64_t)d.ny * d.nx;
+  float yc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (t.in) {
+    const int64_t i0 = ((int64_t)t.z0 * d.ny + t.y) * d.nx + t.x;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) yc[q] = y2[q * n + i0];
+  }
   for (int z = t.z0; z < t.z1; ++z) {
     pp.store(pl + slot_of(z + 1, t.z0) * kPlane, xbar(a0, b0), xbar(a1, b1));
     __syncthreads();
@@ -301,14 +309,19 @@ __global__ void __launch_bounds__(kTileThreads) k_pd_dual(float* __restrict__ y2
       pp.load(x, bz, a0, a1);
       pp.load(xp, bz, b0, b1);
     }
+    const int64_t i = ((int64_t)z * d.ny + t.y) * d.nx + t.x;
+    float yn[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (t.in && z + 1 < t.z1) {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) yn[q] = y2[q * n + i + pstride];
+    }
     if (t.in) {
       auto f = [&](int zz, int yy, int xx) { return lds_at(pl, slot_of(zz, t.z0), t, yy, xx); };
       float h[6];
       hess6(f, d, z, t.y, t.x, w.w, h);
-      const int64_t i = ((int64_t)z * d.ny + t.y) * d.nx + t.x;
       float av[6];
 #pragma unroll
-      for (int q = 0; q < 6; ++q) av[q] = y2[q * n + i] + sf * h[q];
+      for (int q = 0; q < 6; ++q) av[q] = yc[q] + sf * h[q];
       if (!l12) {
 #pragma unroll
         for (int q = 0; q < 6; ++q) y2[q * n + i] = fminf(fmaxf(av[q], -alpha), alpha);
@@ -322,6 +335,8 @@ __global__ void __launch_bounds__(kTileThreads) k_pd_dual(float* __restrict__ y2
         for (int q = 0; q < 6; ++q) y2[q * n + i] = av[q] / s;
       }
     }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) yc[q] = yn[q];
     __syncthreads();
   }
 }
@@ -329,7 +344,7 @@ __global__ void __launch_bounds__(kTileThreads) k_pd_dual(float* __restrict__ y2
 // xn = max(0, x - tau (g + H^T y2)); partial sums of (xn - x)^2, x^2, xn^2
 // (pdhg_mlx.py:741-742, 749-750, 805-806).  Components 0 (zz), 3 (yz) and
 // 4 (xz) need z-neighbours (three-plane windows); 1, 2 and 5 only the current plane.
-__global__ void __launch_bounds__(kTileThreads) k_pd_primal(float* __restrict__ xn, const float* __restrict__ x,
+__global__ void __launch_bounds__(kTileThreads, 6) k_pd_primal(float* __restrict__ xn, const float* __restrict__ x,
                                                             const float* __restrict__ g,
                                                             const float* __restrict__ y2, Dims d,
                                                             const double* __restrict__ sc, W6 w,
@@ -362,6 +377,13 @@ __global__ void __launch_bounds__(kTileThreads) k_pd_primal(float* __restrict__ 
       pp.load(csrc[j], bc, cv[j][0], cv[j][1]);
     }
   }
+  // this voxel's gradient and iterate, loaded one plane ahead
+  float gc = 0.f, xc = 0.f;
+  if (t.in) {
+    const int64_t i0 = ((int64_t)t.z0 * d.ny + t.y) * d.nx + t.x;
+    gc = g[i0];
+    xc = x[i0];
+  }
   for (int z = t.z0; z < t.z1; ++z) {
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
@@ -377,6 +399,12 @@ __global__ void __launch_bounds__(kTileThreads) k_pd_primal(float* __restrict__ 
         pp.load(csrc[j], bc, cv[j][0], cv[j][1]);
       }
     }
+    float gn = 0.f, xnext = 0.f;
+    if (t.in && z + 1 < t.z1) {
+      const int64_t i1 = ((int64_t)(z + 1) * d.ny + t.y) * d.nx + t.x;
+      gn = g[i1];
+      xnext = x[i1];
+    }
     if (t.in) {
       auto hw = [&](int q, int zz, int yy, int xx) {
         const int o = (yy - t.y0 + 1) * kPW + (xx - t.x0 + 1);
@@ -391,8 +419,8 @@ __global__ void __launch_bounds__(kTileThreads) k_pd_primal(float* __restrict__ 
       };
       const float lt = hess_adj(hw, d, z, t.y, t.x);
       const int64_t i = ((int64_t)z * d.ny + t.y) * d.nx + t.x;
-      const float grad = g[i] + lt;
-      const float xv = x[i];
+      const float grad = gc + lt;
+      const float xv = xc;
       const float v = fmaxf(xv - tf * grad, 0.0f);
       xn[i] = v;
       const float dv = v - xv;
@@ -400,6 +428,8 @@ __global__ void __launch_bounds__(kTileThreads) k_pd_primal(float* __restrict__ 
       s_x += (double)(xv * xv);
       s_xn += (double)(v * v);
     }
+    gc = gn;
+    xc = xnext;
     __syncthreads();
   }
   const int64_t b = ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
