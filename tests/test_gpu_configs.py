@@ -1,0 +1,159 @@
+"""The workload configurations at their full sizes.
+
+Where the float32 oracle finishes in seconds the GPU result is compared with it
+on the same inputs; at every size the tests check properties that do not depend
+on the size:
+
+* RL (64x256x256, canvas 128x512x512): one-iteration parity with the oracle
+  (relative L2 <= 1e-5); over 50 iterations the I-divergence decreases
+  monotonically (EM) and the flux identity sum(A^T 1 * x_new) = sum(observed)
+  holds (exact for bg = 0 while A x > eps);
+* binned PDHG-Hessian (object 128x512x512, data 64x256x256, canvas
+  256x1024x1024): adjoint identity of the composed operator, iterates finite and
+  non-negative, objective decreasing;
+* batched RL over the batched configuration's uneven shapes (z in {16..64},
+  y = x in {128..512}, per-volume aberrated PSFs): every volume's batched result
+  is bit-identical to its solo result;
+* the large single volume (256x2048x2048, canvas 512x4096x4096): adjoint
+  identity of the convolution and one PDHG-Hessian iteration at full size
+  (skipped when the device has less than 200 GB free).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import deconv as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _gpu_observation(psf, obj, seed, scale_to=None):
+    from deconlib_amd import LinearFFTConvolver
+
+    clean = LinearFFTConvolver(psf, obj.shape).forward(obj.astype(np.float32)).cpu().numpy()
+    clean = np.maximum(clean.astype(np.float64), 0.0)
+    if scale_to is not None:
+        clean *= scale_to / clean.max()
+    return np.random.default_rng(seed).poisson(clean).astype(np.float32)
+
+
+def test_rl_config_full_size(gpu):
+    from deconlib_amd import LinearFFTConvolver, richardson_lucy_with_operator
+    from deconlib_amd.psf import widefield_psf
+    from deconlib_amd.synth import CONFIG0_OPTICS, blob_volume
+
+    shape = (64, 256, 256)
+    print(f"RL at {shape}", flush=True)
+    psf = widefield_psf(shape, 0.085, 0.1, CONFIG0_OPTICS).astype(np.float32)
+    obj = blob_volume(shape, 200, np.random.default_rng(1000))
+    obs = _gpu_observation(psf, obj, 1000)
+    op = LinearFFTConvolver(psf, shape)
+    assert op.canvas_shape == (128, 512, 512)
+    one = richardson_lucy_with_operator(obs, op, num_iter=1)
+    ref = O.richardson_lucy(obs, O.LinearFFTConvolver(psf, shape), num_iter=1)
+    assert rel_l2(one.restored.cpu().numpy(), ref.restored) <= 1e-5
+    res = richardson_lucy_with_operator(obs, op, num_iter=50)
+    x = res.restored
+    assert bool(torch.isfinite(x).all()) and float(x.min()) >= 0.0
+    h = res.loss_history
+    assert len(h) == 6 and all(b < a for a, b in zip(h, h[1:]))
+    sens = op.adjoint(torch.ones(shape, device="cuda")).double()
+    flux = float((sens * x.double()).sum())
+    assert abs(flux - float(obs.astype(np.float64).sum())) <= 1e-4 * float(obs.sum())
+
+
+def test_binned_pdhg_config_full_size(gpu):
+    from deconlib_amd import (Crop, FractionalAreaDownsample, LinearFFTConvolver, compose,
+                              solve_pdhg_mlx)
+    from deconlib_amd.psf import widefield_psf
+    from deconlib_amd.synth import CONFIG0_OPTICS, blob_volume
+
+    hi, lo = (128, 512, 512), (64, 256, 256)
+    print(f"binned PDHG {hi} -> {lo}", flush=True)
+    psf = widefield_psf(hi, 0.0425, 0.05, CONFIG0_OPTICS).astype(np.float32)
+    obj = blob_volume(hi, 2000, np.random.default_rng(1001)).astype(np.float32)
+    conv = LinearFFTConvolver(psf, hi)
+    assert conv.canvas_shape == (256, 1024, 1024)
+    A = compose(Crop(lo, lo), FractionalAreaDownsample(2), conv)
+    clean = np.maximum(A.forward(torch.from_numpy(obj).cuda()).double().cpu().numpy(), 0.0)
+    obs = np.random.default_rng(1001).poisson(clean * (500.0 / clean.max())).astype(np.float32)
+    # adjoint identity of the composed operator at full size
+    g = torch.Generator(device="cuda").manual_seed(5)
+    xr = torch.rand(hi, device="cuda", generator=g)
+    yr = torch.rand(lo, device="cuda", generator=g)
+    lhs = float((A.forward(xr).double() * yr.double()).sum())
+    rhs = float((xr.double() * A.adjoint(yr).double()).sum())
+    assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+    res = solve_pdhg_mlx(obs, psf, alpha=1e-3, regularization="hessian", norm="L1", num_iter=20,
+                         bin_factors=2, tol=0.0)
+    x = res.restored
+    assert tuple(x.shape) == hi and res.iterations == 20
+    assert bool(torch.isfinite(x).all()) and float(x.min()) >= 0.0
+    assert res.loss_history[-1] < res.loss_history[0]
+    assert all(t > 0 for t in res.tau_history) and all(s > 0 for s in res.sigma_history)
+
+
+def test_batched_config_shapes(gpu):
+    from deconlib_amd import LinearFFTConvolver, richardson_lucy_with_operator
+    from deconlib_amd.batch import RLJob, batched_richardson_lucy
+    from deconlib_amd.synth import aberrated_psf, blob_volume
+
+    rng = np.random.default_rng(1002)
+    inputs = []
+    for _ in range(8):
+        z = int(rng.choice([16, 32, 48, 64]))
+        xy = int(rng.choice([128, 192, 256, 384, 512]))
+        shape = (z, xy, xy)
+        psf = aberrated_psf(shape, rng).astype(np.float32)
+        obj = blob_volume(shape, max(1, round(200 * np.prod(shape) / 4194304)), rng)
+        inputs.append((psf, _gpu_observation(psf, obj, int(rng.integers(1 << 30)))))
+    print("batched shapes", [o.shape for _, o in inputs], flush=True)
+    alone = [richardson_lucy_with_operator(obs, LinearFFTConvolver(psf, obs.shape), num_iter=4)
+             for psf, obs in inputs]
+    mixed = batched_richardson_lucy([RLJob(obs, LinearFFTConvolver(psf, obs.shape), num_iter=4)
+                                     for psf, obs in inputs])
+    for a, m in zip(alone, mixed):
+        assert bool(torch.isfinite(m.restored).all())
+        assert torch.equal(a.restored, m.restored)
+        assert a.loss_history == m.loss_history
+
+
+def test_large_volume_full_size(gpu):
+    from deconlib_amd import LinearFFTConvolver
+    from deconlib_amd.pdhg import _PDState, _nonneg
+
+    free, _ = torch.cuda.mem_get_info()
+    if free < 200e9:
+        pytest.skip(f"needs 200 GB free device memory, {free / 1e9:.0f} GB free")
+    shape = (256, 2048, 2048)
+    print(f"large volume {shape}", flush=True)
+    # separable Gaussian PSF (sigma 3 planes, 1.5 px), corner origin, built on the device
+    axes = []
+    for n, s in zip(shape, (3.0, 1.5, 1.5)):
+        i = torch.arange(n, device="cuda", dtype=torch.float32)
+        d = torch.minimum(i, n - i)
+        axes.append(torch.exp(-0.5 * (d / s) ** 2))
+    psf = axes[0][:, None, None] * axes[1][None, :, None] * axes[2][None, None, :]
+    op = LinearFFTConvolver(psf, shape)
+    del psf
+    assert op.canvas_shape == (512, 4096, 4096)
+    g = torch.Generator(device="cuda").manual_seed(7)
+    x = torch.rand(shape, device="cuda", generator=g)
+    y = torch.rand(shape, device="cuda", generator=g)
+    lhs = float(torch.dot(op.forward(x).view(-1).double(), y.view(-1).double()))
+    rhs = float(torch.dot(x.view(-1).double(), op.adjoint(y).view(-1).double()))
+    assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+    data = (op.forward(x) * 100.0).round_()
+    del x, y
+    st = _PDState(data, op, _nonneg(op.adjoint(data)), 5e-4, 1, r=1.0, tol=0.0)
+    st.step(0, True)
+    torch.cuda.synchronize()
+    assert bool(torch.isfinite(st.x).all()) and float(st.x.min()) >= 0.0
+    assert np.isfinite(st.loss_value(0))
