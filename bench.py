@@ -148,6 +148,48 @@ def bench_host_to_host(torch, psf, obs, iters=50):
     }
 
 
+def bench_batched_small(torch, n_vol=32, shape=(16, 64, 64), iters=50, seed=1005):
+    """Many small RL volumes (canvas 32x128x128): the plain one-job-at-a-time loop
+    against the batched fused iteration, eager and with the steady-state iteration
+    replayed from a HIP graph (launch overhead dominates at this size)."""
+    from deconlib_amd import LinearFFTConvolver, richardson_lucy_with_operator
+    from deconlib_amd.batch import RLJob, batched_richardson_lucy
+    from deconlib_amd.synth import blob_volume
+
+    rng = np.random.default_rng(seed)
+    z = np.minimum(np.arange(shape[0]), shape[0] - np.arange(shape[0]))[:, None, None]
+    y = np.minimum(np.arange(shape[1]), shape[1] - np.arange(shape[1]))[None, :, None]
+    x = np.minimum(np.arange(shape[2]), shape[2] - np.arange(shape[2]))[None, None, :]
+    psf = np.exp(-0.5 * ((z / 2.0) ** 2 + (y / 1.5) ** 2 + (x / 1.5) ** 2)).astype(np.float32)
+    vols = []
+    for _ in range(n_vol):
+        obj = blob_volume(shape, 6, rng).astype(np.float32)
+        clean = LinearFFTConvolver(psf, shape).forward(obj).cpu().numpy()
+        vols.append(rng.poisson(np.maximum(clean, 0.0)).astype(np.float32))
+    vox_it = n_vol * int(np.prod(shape)) * iters
+
+    def timed(fn):
+        fn()  # warm-up (plans, occupancy queries, graph capture path)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        return vox_it / (time.perf_counter() - t0)
+
+    ops = [LinearFFTConvolver(psf, shape) for _ in vols]
+    plain = timed(lambda: [richardson_lucy_with_operator(v, o, num_iter=iters) for v, o in zip(vols, ops)])
+    eager = timed(lambda: batched_richardson_lucy([RLJob(v, o, num_iter=iters) for v, o in zip(vols, ops)]))
+    graph = timed(lambda: batched_richardson_lucy([RLJob(v, o, num_iter=iters) for v, o in zip(vols, ops)],
+                                                  graphs=True))
+    return {
+        "workload": f"{n_vol} volumes {shape[0]}x{shape[1]}x{shape[2]}, RL {iters} it (canvas 32x128x128)",
+        "plain_loop_voxel_iters_per_s": plain,
+        "batched_voxel_iters_per_s": eager,
+        "batched_graph_voxel_iters_per_s": graph,
+        "graph_speedup_vs_plain_loop": graph / plain,
+    }
+
+
 def bench_serving(torch, n_jobs, rate, seed=1003):
     """Serving sample: Poisson arrivals of mixed RL (100 it) / PDHG-Hessian (200 it) jobs,
     sizes log-uniform 16x64x64 .. 48x256x256; continuous batching vs the plain one-job loop."""
@@ -282,6 +324,8 @@ def main():
     log(f"pdhg: {json.dumps(pdhg)}")
     host = bench_host_to_host(torch, psf, obs)
     log(f"host-to-host solve: {json.dumps(host)}")
+    small = bench_batched_small(torch)
+    log(f"batched small volumes: {json.dumps(small)}")
     if args.copy_probe:
         # a plain device copy of known size: the reference point for the HBM counters
         src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")  # 256 MiB
@@ -340,6 +384,7 @@ def main():
                       "hbm_frac": it_bytes / (dt / args.steps) / (PEAK_HBM_GBS * 1e9)},
         "pdhg": pdhg,
         "host_to_host": host,
+        "batched_small": small,
         "serving": serving,
         "cpu_baseline": cpu,
     }

@@ -78,6 +78,15 @@ class RLJob:
         st.plan.gen += 1
         st._spec_gen = st.plan.gen if self.k + 1 < self.num_iter else None
 
+    def _steady(self) -> bool:
+        """True when this job's next fused iteration is the steady-state call: its X
+        spectrum is resident and kept for the following iteration, and no loss is
+        due (the call's arguments are then the same every iteration)."""
+        st = self.state
+        k = self.k
+        return (st.fused_iter and k >= 1 and k + 1 < self.num_iter and not st.wants_loss(k - 1)
+                and st._spec_gen is not None and st.plan.gen == st._spec_gen)
+
     def finish(self) -> RLResult:
         st = self.state
         k_last = self.k - 1
@@ -88,10 +97,47 @@ class RLJob:
 
 
 class BatchRunner:
-    """Advances a changing set of RL jobs one iteration at a time (continuous batching)."""
+    """Advances a changing set of RL jobs one iteration at a time (continuous batching).
 
-    def __init__(self, max_items: int = 256):
+    With ``graphs=True`` the steady-state iteration of an unchanged set of jobs
+    (every job's spectrum resident, no loss due) is captured once as a HIP graph
+    (``torch.cuda.CUDAGraph``) and replayed: one graph launch per iteration instead
+    of every grouped kernel launch and its host-side argument packing.  Other
+    iterations (first, loss-evaluation, last) and any change of the job set run
+    eagerly; a replay executes the same kernels on the same buffers, so results are
+    bit-identical to eager stepping.
+    """
+
+    def __init__(self, max_items: int = 256, graphs: bool = False):
         self.max_items = int(max_items)
+        self.graphs = bool(graphs)
+        self._graph = None
+        self._graph_jobs = None
+        self.replays = 0
+        self.captures = 0
+
+    def _try_graph(self, fused: List[RLJob]) -> bool:
+        """Runs the iteration of ``fused`` as a graph replay when possible."""
+        if not self.graphs or not fused or len(fused) > self.max_items or not all(j._steady() for j in fused):
+            return False
+        key = tuple(id(j) for j in fused)
+        if self._graph_jobs != key:
+            arr = (N.RLItem * len(fused))(*[j._rl_item() for j in fused])
+            g = torch.cuda.CUDAGraph()
+            try:
+                with torch.cuda.graph(g):
+                    N.call("dh_batch_rl_iteration", len(fused), arr, N.stream_ptr())
+            except Exception:  # capture unsupported here: keep stepping eagerly
+                self.graphs = False
+                self._graph = self._graph_jobs = None
+                return False
+            self._graph, self._graph_jobs = g, key
+            self.captures += 1
+        self._graph.replay()
+        self.replays += 1
+        for j in fused:
+            j._after_fused()
+        return True
 
     def step(self, jobs: List[RLJob]) -> None:
         """One RL iteration for every job in ``jobs`` (all not yet done).
@@ -103,6 +149,8 @@ class BatchRunner:
         live = [j for j in jobs if not j.done]
         fused = [j for j in live if j.state.fused_iter]
         plain = [j for j in live if not j.state.fused_iter]
+        if self._try_graph(fused):
+            fused = []
         for s in range(0, len(fused), self.max_items):
             chunk = fused[s:s + self.max_items]
             arr = (N.RLItem * len(chunk))(*[j._rl_item() for j in chunk])
@@ -120,9 +168,11 @@ class BatchRunner:
             j.k += 1
 
 
-def batched_richardson_lucy(jobs: List[RLJob], runner: Optional[BatchRunner] = None) -> List[RLResult]:
-    """Runs every job to its own ``num_iter`` in lockstep; returns their RLResults."""
-    runner = runner or BatchRunner(max(len(jobs), 1))
+def batched_richardson_lucy(jobs: List[RLJob], runner: Optional[BatchRunner] = None,
+                            graphs: bool = False) -> List[RLResult]:
+    """Runs every job to its own ``num_iter`` in lockstep; returns their RLResults.
+    ``graphs``: replay the steady-state iteration as a HIP graph (see BatchRunner)."""
+    runner = runner or BatchRunner(max(len(jobs), 1), graphs=graphs)
     while any(not j.done for j in jobs):
         runner.step(jobs)
         for j in jobs:
