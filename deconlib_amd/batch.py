@@ -113,6 +113,7 @@ class BatchRunner:
         self.graphs = bool(graphs)
         self._graph = None
         self._graph_jobs = None
+        self._stream = None
         self.replays = 0
         self.captures = 0
 
@@ -120,18 +121,34 @@ class BatchRunner:
         """Runs the iteration of ``fused`` as a graph replay when possible."""
         if not self.graphs or not fused or len(fused) > self.max_items or not all(j._steady() for j in fused):
             return False
-        key = tuple(id(j) for j in fused)
-        if self._graph_jobs != key:
+        # the graph holds the jobs themselves: their buffers (the graph's kernel
+        # arguments) stay alive while it exists, and identity cannot be mistaken
+        same = (self._graph_jobs is not None and len(self._graph_jobs) == len(fused)
+                and all(a is b for a, b in zip(self._graph_jobs, fused)))
+        if not same:
             arr = (N.RLItem * len(fused))(*[j._rl_item() for j in fused])
             g = torch.cuda.CUDAGraph()
+            # captured on a side stream ordered after the current one; unlike the
+            # torch.cuda.graph context this neither synchronises the device nor
+            # empties the caching allocator (the capture allocates nothing)
+            cur = torch.cuda.current_stream()
+            if self._stream is None:
+                self._stream = torch.cuda.Stream()
+            self._stream.wait_stream(cur)
             try:
-                with torch.cuda.graph(g):
-                    N.call("dh_batch_rl_iteration", len(fused), arr, N.stream_ptr())
+                with torch.cuda.stream(self._stream):
+                    g.capture_begin()
+                    try:
+                        N.call("dh_batch_rl_iteration", len(fused), arr, N.stream_ptr())
+                    finally:
+                        g.capture_end()
             except Exception:  # capture unsupported here: keep stepping eagerly
+                cur.wait_stream(self._stream)
                 self.graphs = False
                 self._graph = self._graph_jobs = None
                 return False
-            self._graph, self._graph_jobs = g, key
+            cur.wait_stream(self._stream)
+            self._graph, self._graph_jobs = g, list(fused)
             self.captures += 1
         self._graph.replay()
         self.replays += 1

@@ -180,7 +180,9 @@ bool ct_row_tile_ok(int m, int R) {
 // workload, canvas 128x1024x1024): columns per workgroup = the widest power of
 // two within 64 KiB LDS and 512 threads (256 for the fused Z pass), one block for
 // the blocked Y layout; rows per workgroup = the widest power of two within 512
-// threads and 96 KiB LDS.
+// threads and 96 KiB LDS, halved (down to one wave) while a volume has fewer
+// than kMinRowTiles tiles: an X pass keeps a tile's load, transforms and stores
+// in sequence, so small volumes need many workgroups in flight to hide latency.
 int ct_col_tile(int n, int64_t ncols, int over, int kind) {
   if (over > 0 && ct_col_tile_ok(n, over, kind)) return over;
   const int T = dh::ct_threads(n);
@@ -191,11 +193,14 @@ int ct_col_tile(int n, int64_t ncols, int over, int kind) {
   while (L > 1 && L / 2 >= ncols) L /= 2;
   return L;
 }
-int ct_row_tile(int m, int over) {
+constexpr int64_t kMinRowTiles = 2048;  // 8 workgroups per CU
+int ct_row_tile(int m, int over, int64_t nrows) {
   if (over > 0 && ct_row_tile_ok(m, over)) return over;
   const int T = dh::ct_threads(m);
   int R = 1;
   while (R < 64 && 2 * R * T <= kCtMaxThreads && dh::xrow_lds_elems(m, 2 * R) * 8 <= kCtRowDefaultLds) R *= 2;
+  const int r_min = T >= 64 ? 1 : 64 / T;
+  while (R > r_min && (nrows + R - 1) / R < kMinRowTiles) R /= 2;
   return R;
 }
 
@@ -248,7 +253,7 @@ CtPasses ct_passes(const dh_conv_plan* p, int adjoint, const float* in, float* o
   const int s0 = p->s[0], s1 = p->s[1], s2 = p->s[2], P1 = p->P[1];
   const int64_t H = p->H, nrows = (int64_t)s0 * s1;
   const float2* tw = p->tw_dev;
-  c.t_xf = ct_row_tile(p->M, tile_override(p, DH_PASS_X_FWD));
+  c.t_xf = ct_row_tile(p->M, tile_override(p, DH_PASS_X_FWD), nrows);
   c.n_xf = (nrows + c.t_xf - 1) / c.t_xf;
   c.t_xf |= persist_flag(p, DH_PASS_X_FWD);
   c.xf = dh::XfArgs{in, B1, tw + p->tw_xm, tw + p->tw_xp, nrows, s2, s2, 0, (int)H, pro, xf_vec2(in, s2),
@@ -273,7 +278,7 @@ CtPasses ct_passes(const dh_conv_plan* p, int adjoint, const float* in, float* o
   c.n_z = (ncols + c.t_z - 1) / c.t_z;
   c.t_z |= persist_flag(p, DH_PASS_Z);
   c.z = dh::ZArgs{c.has_y ? B2 : B1, (const float2*)otf, tw + p->tw_z, ncols, s0, s0, adjoint ? 1 : 0};
-  c.t_xi = ct_row_tile(p->M, tile_override(p, DH_PASS_X_INV));
+  c.t_xi = ct_row_tile(p->M, tile_override(p, DH_PASS_X_INV), nrows);
   c.n_xi = (nrows + c.t_xi - 1) / c.t_xi;
   c.t_xi |= persist_flag(p, DH_PASS_X_INV);
   c.xi = dh::XiArgs{B1, out, nullptr, tw + p->tw_xm, tw + p->tw_xp, nrows, s2, (int)H, epi, xi_vec2(out, s2, epi),
@@ -540,7 +545,7 @@ int dh_conv_plan_get_tile(const dh_conv_plan* p, int pass) {
   const int64_t ncols = (pass == DH_PASS_Z) ? (int64_t)p->P[1] * p->H : p->H;
   if (p->ct)
     return persist_flag(p, pass) |
-           (row ? ct_row_tile(n, tile_override(p, pass))
+           (row ? ct_row_tile(n, tile_override(p, pass), (int64_t)p->s[0] * p->s[1])
                 : ct_col_tile(n, ncols, tile_override(p, pass), pass == DH_PASS_Z ? COL_Z : COL_BLOCKED));
   if (row) return dh::row_launch_for(n, p->tile[pass]).R;
   return dh::col_launch_for(n, ncols, p->tile[pass]).L;
@@ -576,7 +581,7 @@ int dh_otf_build(const dh_conv_plan* p, const float* kernel_dev, int normalize, 
   float2* zsrc = B1;
   if (p->ct) {
     const float2* tw = p->tw_dev;
-    const int R = ct_row_tile(p->M, 0);
+    const int R = ct_row_tile(p->M, 0, krows);
     const int64_t nbx = (krows + R - 1) / R;
     dh::Pack<dh::XfArgs> px;
     pack_one(px, dh::XfArgs{kernel_dev, B1, tw + p->tw_xm, tw + p->tw_xp, krows, p->k[2], p->klo[2], p->khi[2],

@@ -154,8 +154,9 @@ class _ConvPlan:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and h.value:
-            N.fns["dh_conv_plan_destroy"](h)
+        fns = getattr(N, "fns", None)  # None while the interpreter tears modules down
+        if h is not None and h.value and fns is not None:
+            fns["dh_conv_plan_destroy"](h)
             self._h = None
 
 
