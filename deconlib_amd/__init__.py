@@ -3,7 +3,8 @@
 Drop-in for the hot path of ``deconlib.deconvolution``: the operator protocol
 and composition, ``LinearFFTConvolver`` / ``FFTConvolver``, ``Pad`` / ``Crop`` /
 ``FractionalAreaDownsample``, ``Hessian3D``, Richardson–Lucy and PDHG with
-Hessian regularization.  All arithmetic runs in hand-written HIP kernels for
+Hessian regularization, and the tiled path (``make_forward_model``,
+``plan_tiles`` / ``process_tiles``, ``richardson_lucy_solver``).  All arithmetic runs in hand-written HIP kernels for
 gfx950 (``deconlib_amd/_lib/libdeconhip.so``, C ABI in ``include/deconhip.h``);
 PyTorch only owns device buffers and streams.
 """
@@ -13,7 +14,10 @@ from .composition import Compose, LinearOperator, compose
 from .core_operators import Crop, FractionalAreaDownsample, Pad
 from .linops import FFTConvolver, Hessian3D, LinearFFTConvolver, fast_padded_shape
 from .pdhg import HessianRegularizer, solve_pdhg_mlx, solve_pdhg_with_operator
-from .rl import RLResult, poisson_i_divergence, richardson_lucy_with_operator
+from .rl import RLResult, poisson_i_divergence, richardson_lucy_solver, richardson_lucy_with_operator
+from .forward_model import ForwardModel, make_forward_model
+from .shapes import compute_padded_shape, compute_visible_shape, get_valid_slices
+from .tile_processing import TilePlan, TileSpec, optimal_tile_size, plan_tiles, process_tiles
 
 __all__ = [
     "solve_pdhg_mlx",
@@ -24,6 +28,17 @@ __all__ = [
     "DeconvolutionResult",
     "RLResult",
     "richardson_lucy_with_operator",
+    "richardson_lucy_solver",
+    "ForwardModel",
+    "make_forward_model",
+    "TileSpec",
+    "TilePlan",
+    "plan_tiles",
+    "process_tiles",
+    "optimal_tile_size",
+    "compute_padded_shape",
+    "compute_visible_shape",
+    "get_valid_slices",
     "poisson_i_divergence",
     "FFTConvolver",
     "LinearFFTConvolver",

@@ -21,7 +21,7 @@ if not _LIB_PATH.exists():
 
 lib = C.CDLL(str(_LIB_PATH))
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 DH_OK, DH_EINVAL, DH_EHIP, DH_EWORKSPACE, DH_EUNSUPPORTED = 0, -1, -2, -3, -4
 
@@ -85,6 +85,7 @@ class RLItem(C.Structure):
         ("reduce_out", C.c_void_p),
         ("spectrum_ready", C.c_int32),
         ("keep_spectrum", C.c_int32),
+        ("data_box", C.c_int32 * 6),
     ]
 
 

@@ -17,7 +17,7 @@ from typing import List, Optional
 import torch
 
 from . import _native as N
-from ._tensor import device
+from ._tensor import as_f32, device, empty_f32
 from .linops import FFTConvolver, LinearFFTConvolver
 from .rl import RLResult, _RLState, _EPS
 
@@ -67,16 +67,19 @@ class RLJob:
         k = self.k
         red = st.slot(k - 1) if (k > 0 and st.wants_loss(k - 1)) else None
         keep = k + 1 < self.num_iter
-        ready = st._spec_gen is not None and plan.gen == st._spec_gen
-        return N.RLItem(plan=plan._h, otf=plan.otf_buf.data_ptr(), work=plan.work.data_ptr(),
+        ready = st._spec_gen is not None and st.ws.gen == st._spec_gen
+        item = N.RLItem(plan=plan._h, otf=plan.otf_buf.data_ptr(), work=st.ws.work.data_ptr(),
                         work_bytes=int(plan.info.work_bytes), x=st.x.data_ptr(), obs=st.obs.data_ptr(),
                         sens=st.sens.data_ptr(), background=st.bg, eps=_EPS, reduce_out=red,
                         spectrum_ready=int(ready), keep_spectrum=int(keep))
+        if st.box is not None:
+            item.data_box[:] = st.box
+        return item
 
     def _after_fused(self) -> None:
         st = self.state
-        st.plan.gen += 1
-        st._spec_gen = st.plan.gen if self.k + 1 < self.num_iter else None
+        st.ws.gen += 1
+        st._spec_gen = st.ws.gen if self.k + 1 < self.num_iter else None
 
     def _steady(self) -> bool:
         """True when this job's next fused iteration is the steady-state call: its X
@@ -85,7 +88,7 @@ class RLJob:
         st = self.state
         k = self.k
         return (st.fused_iter and k >= 1 and k + 1 < self.num_iter and not st.wants_loss(k - 1)
-                and st._spec_gen is not None and st.plan.gen == st._spec_gen)
+                and st._spec_gen is not None and st.ws.gen == st._spec_gen)
 
     def finish(self) -> RLResult:
         st = self.state
@@ -196,3 +199,135 @@ def batched_richardson_lucy(jobs: List[RLJob], runner: Optional[BatchRunner] = N
             if j.done and j.result is None:
                 j.finish()
     return [j.result for j in jobs]
+
+
+# ---- tiles of one forward model (tile_processing.process_tiles) ----
+
+class _Workspace:
+    """A private plan workspace with its own write counter (tiles share one plan)."""
+
+    def __init__(self, plan):
+        self.work = torch.empty(int(plan.info.work_bytes), dtype=torch.uint8, device=device())
+        self.gen = 0
+
+
+class _TileRLState(_RLState):
+    """RL on ``crop . conv`` (a zoom-1 ForwardModel) run on the padded domain.
+
+    The observed tile is embedded in the padded grid, zero outside the data
+    box, so the ratio ``obs / max(conv x + bg, eps)`` is zero there: exactly the
+    reference's ``crop^T`` of the data-space ratio, and the fused iteration of
+    the convolution alone applies.  ``x0`` and ``sens = A^T 1`` come from the
+    composed operator; the I-divergence sums only the data box.
+    """
+
+    def __init__(self, tile, model, background, init_value, num_iter, eval_interval):
+        parts = model._parts
+        conv = parts.conv
+        plan = conv._plan
+        dev = device()
+        data = as_f32(tile)
+        if tuple(data.shape) != tuple(model.data_shape):
+            raise ValueError(f"tile shape {tuple(data.shape)} != model data_shape {model.data_shape}")
+        self.op = model.op
+        self.bg = float(background)
+        self.num_iter = int(num_iter)
+        self.eval_interval = int(eval_interval)
+        self.plan = plan
+        self.loss_slots = torch.zeros(self.num_iter // max(self.eval_interval, 1) + 3, dtype=torch.float64,
+                                      device=dev)
+        self.slot_of = {}
+        self.n_obs = data.numel()
+        self.fused_iter = True
+        self.ws = _Workspace(plan)
+        self._spec_gen = None
+        start = parts.crop_start
+        box = tuple(slice(s, s + n) for s, n in zip(start, data.shape))
+        self.box = [v for s, n in zip(start, data.shape) for v in (s, s + n)]
+        self.data = data
+        self.obs = torch.zeros(model.padded_shape, dtype=torch.float32, device=dev)
+        self.obs[box] = data
+        st = N.stream_ptr()
+        if init_value is None:
+            dmb = empty_f32(data.shape)
+            N.call("dh_sub_floor", data.data_ptr(), dmb.data_ptr(), dmb.numel(), self.bg, 1e-6, st)
+            a = as_f32(model.op.adjoint(dmb))
+        else:
+            a = torch.full(model.padded_shape, float(init_value), dtype=torch.float32, device=dev)
+        self.x = empty_f32(a.shape)
+        N.call("dh_clamp_min", a.data_ptr(), self.x.data_ptr(), a.numel(), 1e-6, st)
+        ones = torch.ones(data.shape, dtype=torch.float32, device=dev)
+        self.sens = as_f32(model.op.adjoint(ones)).clone()
+        self._support()
+        self.ratio = None
+
+    def finish(self, k_last: int) -> torch.Tensor:
+        """pred = crop(conv x) + bg on the data grid, summing the last pending loss."""
+        red = self.slot(k_last) if self.wants_loss(k_last) else None
+        model_out = as_f32(self.op.forward(self.x))
+        n = model_out.numel()
+        pred = empty_f32(model_out.shape)
+        part = torch.empty(N.fns["dh_reduce_partials"](n), dtype=torch.float64, device=device())
+        N.call("dh_add_scalar", model_out.data_ptr(), self.data.data_ptr(), pred.data_ptr(), n, self.bg,
+               part.data_ptr(), red, N.stream_ptr())
+        return pred
+
+
+class TileRLJob(RLJob):
+    """One tile's RL solve against a shared zoom-1 ForwardModel (padded-domain form)."""
+
+    def __init__(self, tile, model, num_iter: int = 50, background: float = 0.0,
+                 init_value: Optional[float] = None, eval_interval: int = 10):
+        if num_iter < 1:
+            raise ValueError("num_iter must be >= 1")
+        self.state = _TileRLState(tile, model, background, init_value, num_iter, eval_interval)
+        self.num_iter = int(num_iter)
+        self.k = 0
+        self.result: Optional[RLResult] = None
+
+
+def fused_tiles_ok(model) -> bool:
+    """A ForwardModel whose tiles can run the batched fused RL iteration: no
+    downsample stage (zoom 1) and a 3-D canvas with specialised kernels."""
+    parts = getattr(model, "_parts", None)
+    if parts is None or parts.downsample is not None or not isinstance(parts.conv, LinearFFTConvolver):
+        return False
+    return len(model.padded_shape) == 3 and int(parts.conv._plan.info.spectrum_block) > 0
+
+
+def _tile_job_bytes(model) -> int:
+    plan = model._parts.conv._plan
+    pad = 1
+    for n in model.padded_shape:
+        pad *= int(n)
+    data = 1
+    for n in model.data_shape:
+        data *= int(n)
+    # workspace, obs (padded), x, sens, plus the data tile and set-up temporaries
+    return int(plan.info.work_bytes) + 4 * 4 * pad + 3 * 4 * data
+
+
+def tiles_richardson_lucy(tiles, model, num_iter: int = 50, background: float = 0.0,
+                          init_value: Optional[float] = None, solve_one=None, eval_interval: int = 10,
+                          callback=None, verbose: bool = False, max_tiles: Optional[int] = None):
+    """RL on every tile against the one model; returns restored[valid_slices] per tile.
+
+    Zoom-1 models on canvases with specialised kernels run as lockstep groups of
+    TileRLJobs through ``dh_batch_rl_iteration`` (group size bounded by free HBM
+    and ``max_tiles``); anything else (a downsample stage, callbacks, verbose
+    output) runs ``solve_one`` per tile.  Results agree with the per-tile
+    solve to float32 rounding (the fused iteration contracts multiply-adds
+    differently from the composed operator's separate kernels).
+    """
+    if not fused_tiles_ok(model) or callback is not None or verbose:
+        return [solve_one(t, model) for t in tiles]
+    free, _ = torch.cuda.mem_get_info()
+    group = max(1, int(0.8 * free) // max(_tile_job_bytes(model), 1))
+    if max_tiles is not None:
+        group = min(group, max(1, int(max_tiles)))
+    out = []
+    for s in range(0, len(tiles), group):
+        jobs = [TileRLJob(t, model, num_iter, background, init_value, eval_interval) for t in tiles[s:s + group]]
+        batched_richardson_lucy(jobs)
+        out.extend(j.state.x[model.valid_slices] for j in jobs)
+    return out

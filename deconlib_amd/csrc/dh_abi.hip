@@ -950,6 +950,7 @@ int ct_batch_rl(int nitems, const dh_rl_item* items, hipStream_t st) {
     const dh::Pro none{dh::PRO_NONE, 0.f, 0.f, nullptr};
     dh::Epi ratio{dh::EPI_RL_RATIO, it.background, it.eps, it.obs, nullptr, nullptr};
     if (it.reduce_out) ratio.partials = (double*)(ws + p->part_off);
+    for (int d = 0; d < 6; ++d) ratio.box[d] = it.data_box[d];
     const dh::Epi update{dh::EPI_RL_UPDATE, 0.f, it.eps, it.x, it.sens, nullptr};
     CtPasses f = ct_passes(p, 0, it.x, nullptr, it.otf, ws, none, ratio);
     f.xi.spec = B1;
@@ -1003,6 +1004,13 @@ int dh_batch_rl_iteration(int nitems, const dh_rl_item* items, void* stream) {
     if (!it.plan->tw_dev) return fail(DH_EINVAL, "plan is not bound (dh_conv_plan_bind)");
     if (it.work_bytes < it.plan->ws_bytes) return fail(DH_EWORKSPACE, "workspace too small");
     if (!it.plan->ct) return fail(DH_EUNSUPPORTED, "the fused RL iteration needs a canvas with specialised kernels");
+    const int32_t* b = it.data_box;
+    if (b[1] > b[0]) {
+      const int* s = it.plan->s;
+      for (int d = 0; d < 3; ++d)
+        if (b[2 * d] < 0 || b[2 * d + 1] <= b[2 * d] || b[2 * d + 1] > s[d])
+          return fail(DH_EINVAL, "data_box outside the signal grid");
+    }
   }
   return ct_batch_rl(nitems, items, (hipStream_t)stream);
 }

@@ -36,6 +36,9 @@ struct Epi {
   const float* aux0;
   const float* aux1;
   double* partials;  // per-block partial sums (nullptr: no reduction)
+  // Reduction box [z0, z1) x [y0, y1) x [x0, x1) of the output grid: only its
+  // samples enter the partial sums (the X inverse passes); z1 <= z0: all.
+  int box[6];
 };
 
 // Launch helpers (host).

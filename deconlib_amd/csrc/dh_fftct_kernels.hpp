@@ -44,19 +44,20 @@ __device__ __forceinline__ float idiv(float o, float mdl) {
   return o * logf(os / ms) - (o - ms);
 }
 
-__device__ __forceinline__ float epi_apply(const Epi& e, float v, int64_t idx, double& acc) {
+// cnt: the sample lies in the reduction box (always, without one).
+__device__ __forceinline__ float epi_apply(const Epi& e, float v, int64_t idx, double& acc, bool cnt = true) {
   switch (e.kind) {
     case EPI_FLOOR: return fmaxf(v, e.a);
     case EPI_RL_RATIO: {
       const float mdl = v + e.a;
       const float o = e.aux0[idx];
-      if (e.partials) acc += (double)idiv(o, mdl);
+      if (e.partials && cnt) acc += (double)idiv(o, mdl);
       return o / fmaxf(mdl, e.b);
     }
     case EPI_RL_UPDATE: return fmaxf(e.aux0[idx] * (v / e.aux1[idx]), e.b);
     case EPI_ADD: {
       const float mdl = v + e.a;
-      if (e.partials && e.aux0) acc += (double)idiv(e.aux0[idx], mdl);
+      if (e.partials && e.aux0 && cnt) acc += (double)idiv(e.aux0[idx], mdl);
       return mdl;
     }
     default: return v;
@@ -65,15 +66,16 @@ __device__ __forceinline__ float epi_apply(const Epi& e, float v, int64_t idx, d
 
 // epi_apply on the sample pair (idx, idx + 1) with float2 operand loads;
 // the pair is accumulated in the same order as two epi_apply calls.
-__device__ __forceinline__ float2 epi_apply2(const Epi& e, float2 v, int64_t idx, double& acc) {
+__device__ __forceinline__ float2 epi_apply2(const Epi& e, float2 v, int64_t idx, double& acc, bool c0 = true,
+                                             bool c1 = true) {
   switch (e.kind) {
     case EPI_FLOOR: return make_float2(fmaxf(v.x, e.a), fmaxf(v.y, e.a));
     case EPI_RL_RATIO: {
       const float2 o = *reinterpret_cast<const float2*>(e.aux0 + idx);
       const float m0 = v.x + e.a, m1 = v.y + e.a;
       if (e.partials) {
-        acc += (double)idiv(o.x, m0);
-        acc += (double)idiv(o.y, m1);
+        if (c0) acc += (double)idiv(o.x, m0);
+        if (c1) acc += (double)idiv(o.y, m1);
       }
       return make_float2(o.x / fmaxf(m0, e.b), o.y / fmaxf(m1, e.b));
     }
@@ -86,8 +88,8 @@ __device__ __forceinline__ float2 epi_apply2(const Epi& e, float2 v, int64_t idx
       const float m0 = v.x + e.a, m1 = v.y + e.a;
       if (e.partials && e.aux0) {
         const float2 o = *reinterpret_cast<const float2*>(e.aux0 + idx);
-        acc += (double)idiv(o.x, m0);
-        acc += (double)idiv(o.y, m1);
+        if (c0) acc += (double)idiv(o.x, m0);
+        if (c1) acc += (double)idiv(o.y, m1);
       }
       return make_float2(m0, m1);
     }
@@ -656,20 +658,31 @@ __device__ __forceinline__ void xi_body(const XiArgs& a, int64_t lb, int RPB, fl
   float* drow = a.dst ? a.dst + row * s2 : nullptr;
   const int64_t base = row * s2;
   const bool vec2 = a.vec2 != 0;
+  // reduction box: the row's (z, y) once, the sample's x per store
+  const int* bx = a.epi.box;
+  int bx0 = 0, bx1 = s2;
+  if (bx[1] > bx[0]) {
+    const int64_t z = row / a.prow;
+    const int y = (int)(row - z * a.prow);
+    const bool in_row = z >= bx[0] && z < bx[1] && y >= bx[2] && y < bx[3];
+    bx0 = in_row ? bx[4] : 0;
+    bx1 = in_row ? bx[5] : 0;
+  }
   auto gst = [&](int e, float2 v) {
     const int q0 = 2 * e, q1 = 2 * e + 1;
+    const bool c0 = q0 >= bx0 && q0 < bx1, c1 = q1 >= bx0 && q1 < bx1;
     float2 o = make_float2(0.f, 0.f);
     if (ok) {
       if (vec2 && q1 < s2) {
-        o = epi_apply2(a.epi, v, base + q0, acc);
+        o = epi_apply2(a.epi, v, base + q0, acc, c0, c1);
         if (!RT || drow) *reinterpret_cast<float2*>(drow + q0) = o;
       } else {
         if (q0 < s2) {
-          o.x = epi_apply(a.epi, v.x, base + q0, acc);
+          o.x = epi_apply(a.epi, v.x, base + q0, acc, c0);
           if (!RT || drow) drow[q0] = o.x;
         }
         if (q1 < s2) {
-          o.y = epi_apply(a.epi, v.y, base + q1, acc);
+          o.y = epi_apply(a.epi, v.y, base + q1, acc, c1);
           if (!RT || drow) drow[q1] = o.y;
         }
       }

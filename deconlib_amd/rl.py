@@ -22,7 +22,7 @@ library's elementwise kernels.
 from __future__ import annotations
 
 from dataclasses import dataclass
-from typing import Callable, Optional, Tuple
+from typing import Any, Callable, Optional, Tuple
 
 import torch
 
@@ -84,6 +84,10 @@ class _RLState:
         self.n_obs = self.obs.numel()
         # one-call fused iteration (ratio never in HBM) on specialised canvases
         self.fused_iter = self.plan is not None and int(self.plan.info.spectrum_block) > 0
+        # the workspace the fused iterations use and carry the X spectrum in, with
+        # its write counter: the plan's own unless a caller brings a private one
+        self.ws = self.plan
+        self.box = None  # I-divergence box of the signal grid (None: all of it)
         self._spec_gen = None
         if self.plan is not None:
             self._init_fused(init, shape)
@@ -148,14 +152,14 @@ class _RLState:
         red = self.slot(k - 1) if (k > 0 and self.wants_loss(k - 1)) else None
         st = N.stream_ptr()
         if self.fused_iter:
-            plan = self.plan
+            plan, ws = self.plan, self.ws
             keep = k + 1 < self.num_iter
-            ready = self._spec_gen is not None and plan.gen == self._spec_gen
-            N.call("dh_rl_iteration", plan._h, plan.otf_buf.data_ptr(), plan.work.data_ptr(),
+            ready = self._spec_gen is not None and ws.gen == self._spec_gen
+            N.call("dh_rl_iteration", plan._h, plan.otf_buf.data_ptr(), ws.work.data_ptr(),
                    int(plan.info.work_bytes), self.x.data_ptr(), self.obs.data_ptr(), self.sens.data_ptr(),
                    self.bg, _EPS, red, int(ready), int(keep), st)
-            plan.gen += 1
-            self._spec_gen = plan.gen if keep else None
+            ws.gen += 1
+            self._spec_gen = ws.gen if keep else None
         elif self.plan is not None:
             f = N.FuseArgs(epilogue=N.EPI_RL_RATIO, epi_a=self.bg, epi_b=_EPS,
                            aux0=self.obs.data_ptr(), reduce_out=red)
@@ -227,3 +231,33 @@ def richardson_lucy_with_operator(
         full_shape=tuple(st.x.shape),
         valid_slices=None,
     )
+
+
+def richardson_lucy_solver(num_iter: int = 50, background: float = 0.0, init_value: Optional[float] = None,
+                           **rl_kwargs: Any) -> Callable:
+    """RL adapted to the ``solve(data, model) -> visible array`` contract (rl_mlx.py:188-228).
+
+    ``solve(data, model)`` runs ``richardson_lucy_with_operator`` on
+    ``model.op`` (initial estimate ``init_value`` everywhere on the padded
+    domain when given) and returns ``restored[model.valid_slices]`` as a device
+    tensor.  ``solve.solve_batch(tiles, model)`` solves several equally shaped
+    data tiles against the one model and returns their results in order, each
+    identical to ``solve(tile, model)``; ``process_tiles`` uses it.
+    """
+
+    def solve(data, model):
+        init = None
+        if init_value is not None:
+            init = torch.full(model.padded_shape, float(init_value), dtype=torch.float32, device=device())
+        result = richardson_lucy_with_operator(observed=data, blur_op=model.op, num_iter=num_iter,
+                                               background=background, init=init, **rl_kwargs)
+        return result.restored[model.valid_slices]
+
+    def solve_batch(tiles, model):
+        from .batch import tiles_richardson_lucy
+
+        return tiles_richardson_lucy(tiles, model, num_iter=num_iter, background=background,
+                                     init_value=init_value, solve_one=solve, **rl_kwargs)
+
+    solve.solve_batch = solve_batch
+    return solve

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define DH_ABI_VERSION 3
+#define DH_ABI_VERSION 4
 
 #define DH_OK 0
 #define DH_EINVAL (-1)
@@ -168,6 +168,12 @@ typedef struct {
   double* reduce_out;          /* device, may be NULL */
   int32_t spectrum_ready;
   int32_t keep_spectrum;
+  /* I-divergence box {z0, z1, y0, y1, x0, x1} of the signal grid: only samples
+   * inside it enter reduce_out (a data box embedded in a padded domain, obs zero
+   * outside, as a tiled forward model's crop); z1 <= z0: the whole grid.  No
+   * counterpart in the reference's RL call: its tiled path crops instead
+   * (deconvolution/forward_model.py:106-118). */
+  int32_t data_box[6];
 } dh_rl_item;
 int dh_batch_rl_iteration(int nitems, const dh_rl_item* items, void* stream);
 
