@@ -307,22 +307,32 @@ def _tile_job_bytes(model) -> int:
     return int(plan.info.work_bytes) + 4 * 4 * pad + 3 * 4 * data
 
 
+# Padded voxels per lockstep group: one volume of about this size already keeps
+# every pass at its bandwidth, so larger tiles run one at a time (grouping them
+# gains no launches back and spreads the working set over more of the cache).
+_GROUP_VOXELS = 16 << 20
+
+
 def tiles_richardson_lucy(tiles, model, num_iter: int = 50, background: float = 0.0,
                           init_value: Optional[float] = None, solve_one=None, eval_interval: int = 10,
                           callback=None, verbose: bool = False, max_tiles: Optional[int] = None):
     """RL on every tile against the one model; returns restored[valid_slices] per tile.
 
-    Zoom-1 models on canvases with specialised kernels run as lockstep groups of
-    TileRLJobs through ``dh_batch_rl_iteration`` (group size bounded by free HBM
-    and ``max_tiles``); anything else (a downsample stage, callbacks, verbose
-    output) runs ``solve_one`` per tile.  Results agree with the per-tile
+    Zoom-1 models on canvases with specialised kernels run as TileRLJobs through
+    ``dh_batch_rl_iteration``: small tiles in lockstep groups of about
+    16M padded voxels (bounded by free HBM and ``max_tiles``), large ones one
+    at a time; anything else (a downsample stage, callbacks, verbose output)
+    runs ``solve_one`` per tile.  Results agree with the composed operator's
     solve to float32 rounding (the fused iteration contracts multiply-adds
-    differently from the composed operator's separate kernels).
+    differently from the separate kernels).
     """
     if not fused_tiles_ok(model) or callback is not None or verbose:
         return [solve_one(t, model) for t in tiles]
     free, _ = torch.cuda.mem_get_info()
-    group = max(1, int(0.8 * free) // max(_tile_job_bytes(model), 1))
+    pad = 1
+    for n in model.padded_shape:
+        pad *= int(n)
+    group = max(1, min(int(0.8 * free) // max(_tile_job_bytes(model), 1), -(-_GROUP_VOXELS // pad)))
     if max_tiles is not None:
         group = min(group, max(1, int(max_tiles)))
     out = []

@@ -6,6 +6,7 @@
 // one length is a single launch per pass.
 #pragma once
 #include <mutex>
+#include <type_traits>
 
 #include "dh_fftct.hpp"
 
@@ -44,20 +45,35 @@ __device__ __forceinline__ float idiv(float o, float mdl) {
   return o * logf(os / ms) - (o - ms);
 }
 
-// cnt: the sample lies in the reduction box (always, without one).
-__device__ __forceinline__ float epi_apply(const Epi& e, float v, int64_t idx, double& acc, bool cnt = true) {
+// Every sample counts (no reduction box).
+struct AllSamples {
+  __device__ __forceinline__ bool operator()(int) const { return true; }
+};
+
+// Samples q0 + j of a row inside the reduction box: in_row = the row's (z, y)
+// lies in it (a per-lane flag), [x0, x1) its uniform x range.
+struct InBox {
+  bool in_row;
+  int x0, x1, q0;
+  __device__ __forceinline__ bool operator()(int j) const { return in_row && q0 + j >= x0 && q0 + j < x1; }
+};
+
+// cnt(j): sample j (0, or 1 for the second of a pair) enters the partial sums;
+// evaluated only when the epilogue reduces.
+template <class Cnt = AllSamples>
+__device__ __forceinline__ float epi_apply(const Epi& e, float v, int64_t idx, double& acc, Cnt cnt = Cnt()) {
   switch (e.kind) {
     case EPI_FLOOR: return fmaxf(v, e.a);
     case EPI_RL_RATIO: {
       const float mdl = v + e.a;
       const float o = e.aux0[idx];
-      if (e.partials && cnt) acc += (double)idiv(o, mdl);
+      if (e.partials && cnt(0)) acc += (double)idiv(o, mdl);
       return o / fmaxf(mdl, e.b);
     }
     case EPI_RL_UPDATE: return fmaxf(e.aux0[idx] * (v / e.aux1[idx]), e.b);
     case EPI_ADD: {
       const float mdl = v + e.a;
-      if (e.partials && e.aux0 && cnt) acc += (double)idiv(e.aux0[idx], mdl);
+      if (e.partials && e.aux0 && cnt(0)) acc += (double)idiv(e.aux0[idx], mdl);
       return mdl;
     }
     default: return v;
@@ -66,16 +82,16 @@ __device__ __forceinline__ float epi_apply(const Epi& e, float v, int64_t idx, d
 
 // epi_apply on the sample pair (idx, idx + 1) with float2 operand loads;
 // the pair is accumulated in the same order as two epi_apply calls.
-__device__ __forceinline__ float2 epi_apply2(const Epi& e, float2 v, int64_t idx, double& acc, bool c0 = true,
-                                             bool c1 = true) {
+template <class Cnt = AllSamples>
+__device__ __forceinline__ float2 epi_apply2(const Epi& e, float2 v, int64_t idx, double& acc, Cnt cnt = Cnt()) {
   switch (e.kind) {
     case EPI_FLOOR: return make_float2(fmaxf(v.x, e.a), fmaxf(v.y, e.a));
     case EPI_RL_RATIO: {
       const float2 o = *reinterpret_cast<const float2*>(e.aux0 + idx);
       const float m0 = v.x + e.a, m1 = v.y + e.a;
       if (e.partials) {
-        if (c0) acc += (double)idiv(o.x, m0);
-        if (c1) acc += (double)idiv(o.y, m1);
+        if (cnt(0)) acc += (double)idiv(o.x, m0);
+        if (cnt(1)) acc += (double)idiv(o.y, m1);
       }
       return make_float2(o.x / fmaxf(m0, e.b), o.y / fmaxf(m1, e.b));
     }
@@ -88,8 +104,8 @@ __device__ __forceinline__ float2 epi_apply2(const Epi& e, float2 v, int64_t idx
       const float m0 = v.x + e.a, m1 = v.y + e.a;
       if (e.partials && e.aux0) {
         const float2 o = *reinterpret_cast<const float2*>(e.aux0 + idx);
-        if (c0) acc += (double)idiv(o.x, m0);
-        if (c1) acc += (double)idiv(o.y, m1);
+        if (cnt(0)) acc += (double)idiv(o.x, m0);
+        if (cnt(1)) acc += (double)idiv(o.y, m1);
       }
       return make_float2(m0, m1);
     }
@@ -629,7 +645,9 @@ __device__ __forceinline__ void xi_tile(const XiArgs& a, int64_t row0, int RPB,
 // LDS row instead of (or besides) a.dst, transformed by the forward X pass in
 // LDS, and the spectrum stored to a.spec; bit-identical to an X inverse pass
 // writing the rows followed by an X forward pass reading them.
-template <int M, bool RT>
+// BOX: the epilogue's reduction honours a.epi.box (a separate instantiation, so
+// that launches without a box carry none of its registers).
+template <int M, bool RT, bool BOX = false>
 __device__ __forceinline__ void xi_body(const XiArgs& a, int64_t lb, int RPB, float2* lds, const float2* twm,
                                         const float2* twp, int thr) {
   constexpr int T = Geo<M>::T;
@@ -658,23 +676,31 @@ __device__ __forceinline__ void xi_body(const XiArgs& a, int64_t lb, int RPB, fl
   float* drow = a.dst ? a.dst + row * s2 : nullptr;
   const int64_t base = row * s2;
   const bool vec2 = a.vec2 != 0;
-  // reduction box: the row's (z, y) once, the sample's x per store
-  const int* bx = a.epi.box;
+  // reduction box: whether this row's (z, y) lies in it, and its x range
+  bool in_row = true;
   int bx0 = 0, bx1 = s2;
-  if (bx[1] > bx[0]) {
-    const int64_t z = row / a.prow;
-    const int y = (int)(row - z * a.prow);
-    const bool in_row = z >= bx[0] && z < bx[1] && y >= bx[2] && y < bx[3];
-    bx0 = in_row ? bx[4] : 0;
-    bx1 = in_row ? bx[5] : 0;
+  if constexpr (BOX) {
+    const int* bx = a.epi.box;
+    if (bx[1] > bx[0]) {
+      const int64_t z = row / a.prow;
+      const int y = (int)(row - z * a.prow);
+      in_row = z >= bx[0] && z < bx[1] && y >= bx[2] && y < bx[3];
+      bx0 = bx[4];
+      bx1 = bx[5];
+    }
   }
   auto gst = [&](int e, float2 v) {
     const int q0 = 2 * e, q1 = 2 * e + 1;
-    const bool c0 = q0 >= bx0 && q0 < bx1, c1 = q1 >= bx0 && q1 < bx1;
+    using Cnt = std::conditional_t<BOX, InBox, AllSamples>;
+    Cnt c0, c1;
+    if constexpr (BOX) {
+      c0 = InBox{in_row, bx0, bx1, q0};
+      c1 = InBox{in_row, bx0, bx1, q1};
+    }
     float2 o = make_float2(0.f, 0.f);
     if (ok) {
       if (vec2 && q1 < s2) {
-        o = epi_apply2(a.epi, v, base + q0, acc, c0, c1);
+        o = epi_apply2(a.epi, v, base + q0, acc, c0);
         if (!RT || drow) *reinterpret_cast<float2*>(drow + q0) = o;
       } else {
         if (q0 < s2) {
@@ -739,7 +765,7 @@ __device__ __forceinline__ void xi_body(const XiArgs& a, int64_t lb, int RPB, fl
 
 // Persistent: the next tile's spectrum samples are loaded into registers while
 // the current tile is transformed; twiddles in LDS as for k_xf.
-template <int M, bool RT>
+template <int M, bool RT, bool BOX = false>
 __global__ void __launch_bounds__(kMaxThreads) k_xi(Pack<XiArgs> pk, int RPB) {
   extern __shared__ float2 lds[];
   constexpr int RS = xrow_stride(M);
@@ -761,7 +787,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_xi(Pack<XiArgs> pk, int RPB) {
       xi_tile<M>(a, lb * RPB, RPB, q, lds, true, thr);
       xi_tile<M>(a, lb * RPB, RPB, q, lds, false, thr);
       __syncthreads();
-      xi_body<M, RT>(a, lb, RPB, lds, ltw, ltw + M, thr);
+      xi_body<M, RT, BOX>(a, lb, RPB, lds, ltw, ltw + M, thr);
       __syncthreads();
     }
     return;
@@ -876,12 +902,20 @@ struct Launch {
     hipLaunchKernelGGL((k_xi<N, false>), dim3(grid(k, R * T, shm, nblocks, persist)), dim3(R * T), shm, st, pk, R);
     return hipGetLastError();
   }
+  // the boxed instantiation when any volume of the pack has a reduction box
   static hipError_t xr(const Pack<XiArgs>& pk, int R, int64_t nblocks, bool persist, hipStream_t st) {
     size_t shm = (size_t)xrow_lds_elems(N, R) * sizeof(float2);
     if (shm < 16 * sizeof(double)) shm = 16 * sizeof(double);
-    const void* k = (const void*)k_xi<N, true>;
+    bool boxed = false;
+    for (int v = 0; v < pk.nvol; ++v) boxed = boxed || pk.a[v].epi.box[1] > pk.a[v].epi.box[0];
+    const void* k = boxed ? (const void*)k_xi<N, true, true> : (const void*)k_xi<N, true, false>;
     (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    hipLaunchKernelGGL((k_xi<N, true>), dim3(grid(k, R * T, shm, nblocks, persist)), dim3(R * T), shm, st, pk, R);
+    if (boxed)
+      hipLaunchKernelGGL((k_xi<N, true, true>), dim3(grid(k, R * T, shm, nblocks, persist)), dim3(R * T), shm, st,
+                         pk, R);
+    else
+      hipLaunchKernelGGL((k_xi<N, true, false>), dim3(grid(k, R * T, shm, nblocks, persist)), dim3(R * T), shm, st,
+                         pk, R);
     return hipGetLastError();
   }
 };

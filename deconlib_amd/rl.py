@@ -237,15 +237,27 @@ def richardson_lucy_solver(num_iter: int = 50, background: float = 0.0, init_val
                            **rl_kwargs: Any) -> Callable:
     """RL adapted to the ``solve(data, model) -> visible array`` contract (rl_mlx.py:188-228).
 
-    ``solve(data, model)`` runs ``richardson_lucy_with_operator`` on
-    ``model.op`` (initial estimate ``init_value`` everywhere on the padded
-    domain when given) and returns ``restored[model.valid_slices]`` as a device
-    tensor.  ``solve.solve_batch(tiles, model)`` solves several equally shaped
+    ``solve(data, model)`` runs Richardson–Lucy on ``model.op`` (initial
+    estimate ``init_value`` everywhere on the padded domain when given) and
+    returns ``restored[model.valid_slices]`` as a device tensor; a zoom-1 model
+    runs the fused iteration on the padded domain (the crop folded into the
+    data), any other ``richardson_lucy_with_operator`` on the composed operator.
+    ``solve.solve_composed`` always takes the latter.  ``solve.solve_batch(tiles, model)`` solves several equally shaped
     data tiles against the one model and returns their results in order, each
     identical to ``solve(tile, model)``; ``process_tiles`` uses it.
     """
 
     def solve(data, model):
+        from .batch import fused_tiles_ok, tiles_richardson_lucy
+
+        if fused_tiles_ok(model) and not rl_kwargs.get("callback") and not rl_kwargs.get("verbose"):
+            # zoom-1 model: the fused iteration on the padded domain
+            return tiles_richardson_lucy([data], model, num_iter=num_iter, background=background,
+                                         init_value=init_value,
+                                         eval_interval=rl_kwargs.get("eval_interval", 10))[0]
+        return solve_composed(data, model)
+
+    def solve_composed(data, model):
         init = None
         if init_value is not None:
             init = torch.full(model.padded_shape, float(init_value), dtype=torch.float32, device=device())
@@ -257,7 +269,8 @@ def richardson_lucy_solver(num_iter: int = 50, background: float = 0.0, init_val
         from .batch import tiles_richardson_lucy
 
         return tiles_richardson_lucy(tiles, model, num_iter=num_iter, background=background,
-                                     init_value=init_value, solve_one=solve, **rl_kwargs)
+                                     init_value=init_value, solve_one=solve_composed, **rl_kwargs)
 
     solve.solve_batch = solve_batch
+    solve.solve_composed = solve_composed
     return solve

@@ -9,9 +9,11 @@ output, so every visible pixel is written exactly once.
 ``plan_tiles`` / ``optimal_tile_size`` are host arithmetic with the reference's
 rounding.  ``process_tiles`` builds the one forward model and stitches the
 owned cores; a solver that also offers ``solve.solve_batch(tiles, model)``
-(``richardson_lucy_solver`` does) receives the tiles in groups and advances a
-group in lockstep on the GPU, one batched launch per pass for the whole group,
-instead of one tile after another.
+(``richardson_lucy_solver`` does) receives the tiles in groups (small tiles
+advance in lockstep, one batched launch per pass for the group), and when the
+volume and the output fit comfortably in device memory and no per-tile
+callback is asked for, the volume is uploaded once, tiles are cut and cores
+stitched on the device, and the output is downloaded once.
 """
 
 from __future__ import annotations
@@ -125,8 +127,8 @@ def process_tiles(data, psf, zoom: Union[float, Tuple[float, ...]], solve: Solve
                   batch_tiles: Optional[int] = None) -> np.ndarray:
     """Deconvolve tile by tile with any solver; returns the stitched float32 volume (tile_processing.py:230-288).
 
-    ``batch_tiles``: tiles per lockstep group for solvers with ``solve_batch``
-    (default: every tile of the plan in one group; 1 = one tile at a time).
+    ``batch_tiles``: tiles handed to ``solve_batch`` at a time (default: all;
+    the solver sizes its lockstep groups itself; 1 = one tile at a time).
     Results and callback order are those of the one-tile loop: tiles are
     stitched and ``on_tile_done`` called in plan order, and a truthy return
     stops before any later tile is written.
@@ -135,12 +137,14 @@ def process_tiles(data, psf, zoom: Union[float, Tuple[float, ...]], solve: Solve
         guard = max(psf.shape[-2:]) // 2
     plan = plan_tiles(data.shape, zoom, guard, tile_size, min_z_slices)
     model = make_forward_model(psf, plan.tile_shape, zoom)
-    output = np.zeros(plan.visible_shape, dtype=np.float32)
     n = len(plan.tiles)
     batched = getattr(solve, "solve_batch", None)
     group = n if batch_tiles is None else max(1, int(batch_tiles))
     if batched is None:
         group = 1
+    if batched is not None and on_tile_done is None and _fits_device(data, plan):
+        return _process_resident(data, plan, model, batched, group, verbose)
+    output = np.zeros(plan.visible_shape, dtype=np.float32)
 
     i = 0
     while i < n:
@@ -161,3 +165,38 @@ def process_tiles(data, psf, zoom: Union[float, Tuple[float, ...]], solve: Solve
             if on_tile_done is not None and on_tile_done(spec, output):
                 return output
     return output
+
+
+def _fits_device(data, plan) -> bool:
+    """Input and stitched output together take at most a quarter of free HBM."""
+    import torch
+
+    if not torch.cuda.is_available():
+        return False
+    vis = 1
+    for v in plan.visible_shape:
+        vis *= int(v)
+    free, _ = torch.cuda.mem_get_info()
+    return 4 * (int(np.prod(data.shape)) + vis) <= free // 4
+
+
+def _process_resident(data, plan, model, batched, group, verbose) -> np.ndarray:
+    """process_tiles with the volume uploaded once, tiles cut and cores stitched on
+    the device, and the output downloaded once (no per-tile host transfers)."""
+    import torch
+
+    from ._tensor import device
+
+    dev_data = torch.from_numpy(np.ascontiguousarray(data, dtype=np.float32)).to(device())
+    out = torch.zeros(plan.visible_shape, dtype=torch.float32, device=device())
+    n = len(plan.tiles)
+    for s in range(0, n, group):
+        specs = plan.tiles[s:s + group]
+        results = batched([dev_data[sp.read].contiguous() for sp in specs], model)
+        for j, (spec, res) in enumerate(zip(specs, results)):
+            crop = tuple(slice(c.start, min(c.stop, res.shape[k])) for k, c in enumerate(spec.crop))
+            core = res[crop]
+            out[tuple(slice(w.start, w.start + core.shape[k]) for k, w in enumerate(spec.write))] = core
+            if verbose:
+                print(f"  tile {s + j + 1}/{n} {spec.index} done")
+    return out.cpu().numpy()

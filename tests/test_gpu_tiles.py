@@ -135,10 +135,12 @@ def test_batched_tiles_match_per_tile_solve_and_losses(gpu):
     rng = np.random.default_rng(7)
     tiles = [rng.poisson(30 * rng.random((12, 40, 40)) + 2).astype(np.float32) for _ in range(5)]
     solve = richardson_lucy_solver(num_iter=12, eval_interval=4)
-    one = [_host(solve(t, model)) for t in tiles]
+    composed = [_host(solve.solve_composed(t, model)) for t in tiles]
+    fused_one = [_host(solve(t, model)) for t in tiles]
     many = [_host(r) for r in solve.solve_batch(tiles, model)]
-    for a, b in zip(many, one):
-        assert rel_l2(a, b) <= 1e-5
+    for a, b, c in zip(many, fused_one, composed):
+        assert np.array_equal(a, b)  # lockstep == alone, same kernels
+        assert rel_l2(a, c) <= 1e-5
     # loss history of a tile job against the oracle's RL on the composed operator
     jobs = [TileRLJob(t, model, num_iter=12, eval_interval=4) for t in tiles[:2]]
     res = batched_richardson_lucy(jobs)
