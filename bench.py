@@ -27,7 +27,7 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per k_z<128> launch at this workload, from rocprofv3 FETCH_SIZE (x2) and
 # WRITE_SIZE (x1), both scaled by what they read for a device copy of known size
 # (profiles/fft_passes_64x512x512_hbm_counters.md): (394075.8 * 2 + 262656.0) KiB.
-Z_TRAFFIC_BYTES = int((394075.8 * 2 + 262656.0) * 1024)
+Z_TRAFFIC_BYTES = int((394086.0 * 2 + 262656.0) * 1024)
 
 
 def log(msg):
@@ -306,6 +306,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--tune-cache", default=None, help="JSON file the autotuner reads/writes")
     ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the host-to-host, small-volume batching and tiled legs (profiled runs)")
     ap.add_argument("--serving-jobs", type=int, default=40)
     ap.add_argument("--serving-rate", type=float, default=1000.0)
     args = ap.parse_args()
@@ -363,12 +365,14 @@ def main():
     ok = bool(torch.isfinite(st.x).all().item())
     pdhg = bench_pdhg(torch, N, obs, blur, args.steps, args.warmup)
     log(f"pdhg: {json.dumps(pdhg)}")
-    host = bench_host_to_host(torch, psf, obs)
-    log(f"host-to-host solve: {json.dumps(host)}")
-    small = bench_batched_small(torch)
-    log(f"batched small volumes: {json.dumps(small)}")
-    tiled = bench_tiled(torch)
-    log(f"tiled: {json.dumps(tiled)}")
+    host = small = tiled = None
+    if not args.no_extras:
+        host = bench_host_to_host(torch, psf, obs)
+        log(f"host-to-host solve: {json.dumps(host)}")
+        small = bench_batched_small(torch)
+        log(f"batched small volumes: {json.dumps(small)}")
+        tiled = bench_tiled(torch)
+        log(f"tiled: {json.dumps(tiled)}")
     if args.copy_probe:
         # a plain device copy of known size: the reference point for the HBM counters
         src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")  # 256 MiB
