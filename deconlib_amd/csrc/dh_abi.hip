@@ -178,8 +178,8 @@ bool ct_row_tile_ok(int m, int R) {
 }
 // Default launch shapes (what the autotuner settles on for the 64x512x512
 // workload, canvas 128x1024x1024): columns per workgroup = the widest power of
-// two within 64 KiB LDS and 512 threads (256 for the fused Z pass), one block for
-// the blocked Y layout; rows per workgroup = the widest power of two within 512
+// two within 64 KiB LDS and 512 threads (256 for the fused Z pass), one whole
+// block for the blocked Y layout whenever it fits 160 KiB and 512 threads; rows per workgroup = the widest power of two within 512
 // threads and 96 KiB LDS, halved (down to one wave) while a volume has fewer
 // than kMinRowTiles tiles: an X pass keeps a tile's load, transforms and stores
 // in sequence, so small volumes need many workgroups in flight to hide latency.
@@ -188,8 +188,11 @@ int ct_col_tile(int n, int64_t ncols, int over, int kind) {
   const int T = dh::ct_threads(n);
   const int cap = kind == COL_BLOCKED ? dh::kBlk : 64;
   const int max_threads = kind == COL_Z ? 256 : kCtMaxThreads;
+  // a blocked pass takes a whole block when it fits at all: narrower tiles
+  // split each block's 64-byte row runs between workgroups
+  const int64_t lds_cap = kind == COL_BLOCKED ? kCtMaxLds : kCtDefaultLds;
   int L = 1;
-  while (L < cap && 2 * L * T <= max_threads && col_lds_bytes(n, 2 * L, kind) <= kCtDefaultLds) L *= 2;
+  while (L < cap && 2 * L * T <= max_threads && col_lds_bytes(n, 2 * L, kind) <= lds_cap) L *= 2;
   while (L > 1 && L / 2 >= ncols) L /= 2;
   return L;
 }

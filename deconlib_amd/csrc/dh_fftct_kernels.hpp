@@ -127,6 +127,20 @@ __device__ __forceinline__ double bsum(double v, double* scratch) {
   return r;
 }
 
+// ---- streaming global accesses ----
+// Every pass reads each of its inputs once and writes each output once per
+// launch: non-temporal loads and stores keep these streams from displacing
+// reusable lines (twiddles, the other operands of the pass) in the caches.
+typedef float f2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 ld_nt(const float2* p) {
+  const f2v v = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(p));
+  return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void st_nt(float2* p, float2 v) {
+  const f2v w = {v.x, v.y};
+  __builtin_nontemporal_store(w, reinterpret_cast<f2v*>(p));
+}
+
 // ---- column pass (Y forward / inverse, OTF Z) ----
 // LDS slot of line element e, column ln: 8 extra slots every 16 elements spread
 // a wave's (rows x columns) accesses over the banks.
@@ -169,7 +183,7 @@ __device__ __forceinline__ float2 col_ld(const ColArgs& a, const ColLane& c, int
   if (e < a.lo) r = e;
   else if (e >= N - a.hi) r = e - (N - a.lo - a.hi);
   if (!c.ok || r < 0) return make_float2(0.f, 0.f);
-  return c.sb[(int64_t)r * c.srs];
+  return ld_nt(c.sb + (int64_t)r * c.srs);
 }
 
 // Persistent column pass: each workgroup walks tiles b = blockIdx.x,
@@ -219,7 +233,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_col(Pack<ColArgs> pk, int L) {
     const ColArgs& a = pk.a[v];
     const ColLane c = col_lane(a, b - pk.prefix[v], L, ln);
     auto gst = [&](int e, float2 val) {
-      if (c.ok && e < a.n_out) c.db[(int64_t)e * c.drs] = make_float2(val.x * a.scale, val.y * a.scale);
+      if (c.ok && e < a.n_out) st_nt(c.db + (int64_t)e * c.drs, make_float2(val.x * a.scale, val.y * a.scale));
     };
     const int64_t bn = b + gridDim.x;
     if constexpr (S.np == 1) {
@@ -238,6 +252,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_col(Pack<ColArgs> pk, int L) {
 }
 
 // ---- fused Z pass: forward FFT, x OTF, inverse FFT, in place ----
+
 // Entries of the forward-schedule tables; the inverse schedule's rows follow them.
 template <int N>
 constexpr int kTwF = tw_table_size(Geo<N>::F);
@@ -259,7 +274,7 @@ __device__ __forceinline__ void z_mid(int t, const float2* __restrict__ tw, floa
       for (int r = 0; r < R; ++r) {
         const int e = i + r * NB;
         if (from_lds) u[m][r] = lds[e * L + ln];
-        else u[m][r] = (ok && e < lo) ? cb[(int64_t)e * rs] : make_float2(0.f, 0.f);
+        else u[m][r] = (ok && e < lo) ? ld_nt(cb + (int64_t)e * rs) : make_float2(0.f, 0.f);
       }
     }
   }
@@ -275,7 +290,7 @@ __device__ __forceinline__ void z_mid(int t, const float2* __restrict__ tw, floa
       Dft<R, -1>::run(u[m]);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const float2 o = ok ? otf[(int64_t)(i + r * NB) * rs + col] : make_float2(0.f, 0.f);
+        const float2 o = ok ? ld_nt(otf + (int64_t)(i + r * NB) * rs + col) : make_float2(0.f, 0.f);
         u[m][r] = conj ? cmulc(u[m][r], o) : cmul(u[m][r], o);
       }
       Dft<R, 1>::run(u[m]);
@@ -283,7 +298,7 @@ __device__ __forceinline__ void z_mid(int t, const float2* __restrict__ tw, floa
       for (int r = 0; r < R; ++r) {
         const int e = i * R + r;
         if (from_lds) lds[e * L + ln] = u[m][r];
-        else if (ok && e < n_out) gdst[(int64_t)e * rs] = u[m][r];
+        else if (ok && e < n_out) st_nt(gdst + (int64_t)e * rs, u[m][r]);
       }
     }
   }
@@ -319,9 +334,9 @@ __device__ __forceinline__ void z_body(const ZArgs& a, int64_t lb, int L, float2
       cb[0] = a.conj ? cmulc(v, o) : cmul(v, o);
     }
   } else {
-    auto gld = [&](int e) -> float2 { return (ok && e < lo) ? cb[(int64_t)e * rs] : make_float2(0.f, 0.f); };
+    auto gld = [&](int e) -> float2 { return (ok && e < lo) ? ld_nt(cb + (int64_t)e * rs) : make_float2(0.f, 0.f); };
     auto gst = [&](int e, float2 v) {
-      if (ok && e < n_out) cb[(int64_t)e * rs] = v;
+      if (ok && e < n_out) st_nt(cb + (int64_t)e * rs, v);
     };
     auto lld = [&](int e) { return lds[e * L + ln]; };
     auto lst = [&](int e, float2 v) { lds[e * L + ln] = v; };
@@ -369,7 +384,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_z(Pack<ZArgs> pk, int L) {
       const bool ok = col < a.ncols;
       const float2* cb = a.buf + col;
       pass_load<N, T, R0>(u, t, [&](int e) {
-        return (ok && e < a.lo) ? cb[(int64_t)e * a.ncols] : make_float2(0.f, 0.f);
+        return (ok && e < a.lo) ? ld_nt(cb + (int64_t)e * a.ncols) : make_float2(0.f, 0.f);
       });
     };
     fetch(b);
@@ -385,7 +400,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_z(Pack<ZArgs> pk, int L) {
       float2* cb = a.buf + col;
       const int64_t rs = a.ncols;
       auto gst = [&](int e, float2 val) {
-        if (ok && e < a.n_out) cb[(int64_t)e * rs] = val;
+        if (ok && e < a.n_out) st_nt(cb + (int64_t)e * rs, val);
       };
       pass_store<N, T, R0, 1, -1>(u, t, a.tw, lst);
       const int64_t bn = b + gridDim.x;
@@ -466,7 +481,7 @@ __device__ __forceinline__ float2 xf_ld(const XfArgs& a, int64_t row, int e) {
   bool ok0, ok1;
   if (a.vec2) {  // hi == 0, even row length, 8-byte aligned rows
     ok0 = ok1 = ok && 2 * e + 1 < a.lo;
-    if (ok0 && read) raw = *reinterpret_cast<const float2*>(sr + 2 * e);
+    if (ok0 && read) raw = ld_nt(reinterpret_cast<const float2*>(sr + 2 * e));
   } else {
     auto src_of = [&](int q) {
       if (q < a.lo) return q;
@@ -476,8 +491,8 @@ __device__ __forceinline__ float2 xf_ld(const XfArgs& a, int64_t row, int e) {
     const int c0 = src_of(2 * e), c1 = src_of(2 * e + 1);
     ok0 = ok && c0 >= 0;
     ok1 = ok && c1 >= 0;
-    if (ok0 && read) raw.x = sr[c0];
-    if (ok1 && read) raw.y = sr[c1];
+    if (ok0 && read) raw.x = __builtin_nontemporal_load(sr + c0);
+    if (ok1 && read) raw.y = __builtin_nontemporal_load(sr + c1);
   }
   auto pro = [&](float x, bool valid) {
     const float y = kind == PRO_ONES ? 1.0f : (kind == PRO_SUB_FLOOR ? fmaxf(x - a.pro.a, a.pro.b) : x);
@@ -566,11 +581,11 @@ __global__ void __launch_bounds__(kMaxThreads) k_xf(Pack<XfArgs> pk, int RPB) {
     __syncthreads();
     if (a.blk > 0) {
       blocked_rows(lb * RPB, RPB, a.nrows, a.H, a.prow, [&](int r, int k, int64_t off, bool valid) {
-        if (valid) a.dst[off] = xf_post<M>(lds + r * RS, twp, k);
+        if (valid) st_nt(a.dst + off, xf_post<M>(lds + r * RS, twp, k));
       });
     } else if (row < a.nrows) {
       float2* dr = a.dst + row * a.H;
-      for (int k = t; k <= M; k += T) dr[k] = xf_post<M>(lrow, twp, k);
+      for (int k = t; k <= M; k += T) st_nt(dr + k, xf_post<M>(lrow, twp, k));
     }
     if (bn >= total) break;
     b = bn;
@@ -624,7 +639,7 @@ __device__ __forceinline__ void xi_tile(const XiArgs& a, int64_t row0, int RPB,
           y %= a.prow;
         }
         const int w = (bk == nbk - 1) ? H - bk * kBlk : kBlk;
-        val = a.src[z * a.prow * (int64_t)H + (int64_t)bk * kBlk * a.prow + (int64_t)y * w + j];
+        val = ld_nt(a.src + (z * a.prow * (int64_t)H + (int64_t)bk * kBlk * a.prow + (int64_t)y * w + j));
       }
       q[i] = val;
     }
@@ -636,7 +651,7 @@ __device__ __forceinline__ void xi_tile(const XiArgs& a, int64_t row0, int RPB,
       const int k = t + i * T;
       if (k >= H) continue;
       if (!load) lds[rl * RS + xrow_lds(k)] = q[i];
-      else q[i] = row < a.nrows ? a.src[row * H + k] : make_float2(0.f, 0.f);
+      else q[i] = row < a.nrows ? ld_nt(a.src + row * H + k) : make_float2(0.f, 0.f);
     }
   }
 }
@@ -701,15 +716,15 @@ __device__ __forceinline__ void xi_body(const XiArgs& a, int64_t lb, int RPB, fl
     if (ok) {
       if (vec2 && q1 < s2) {
         o = epi_apply2(a.epi, v, base + q0, acc, c0);
-        if (!RT || drow) *reinterpret_cast<float2*>(drow + q0) = o;
+        if (!RT || drow) st_nt(reinterpret_cast<float2*>(drow + q0), o);
       } else {
         if (q0 < s2) {
           o.x = epi_apply(a.epi, v.x, base + q0, acc, c0);
-          if (!RT || drow) drow[q0] = o.x;
+          if (!RT || drow) __builtin_nontemporal_store(o.x, drow + q0);
         }
         if (q1 < s2) {
           o.y = epi_apply(a.epi, v.y, base + q1, acc, c1);
-          if (!RT || drow) drow[q1] = o.y;
+          if (!RT || drow) __builtin_nontemporal_store(o.y, drow + q1);
         }
       }
     }
@@ -749,11 +764,11 @@ __device__ __forceinline__ void xi_body(const XiArgs& a, int64_t lb, int RPB, fl
     __syncthreads();
     if (a.blk > 0) {
       blocked_rows(lb * RPB, RPB, a.nrows, a.H, a.prow, [&](int r, int k, int64_t off, bool valid) {
-        if (valid) a.spec[off] = xf_post<M>(lds + r * RS, twp, k);
+        if (valid) st_nt(a.spec + off, xf_post<M>(lds + r * RS, twp, k));
       });
     } else if (ok) {
       float2* sr = a.spec + row * a.H;
-      for (int k = t2; k <= M; k += T) sr[k] = xf_post<M>(lrow2, twp, k);
+      for (int k = t2; k <= M; k += T) st_nt(sr + k, xf_post<M>(lrow2, twp, k));
     }
   }
   if (a.epi.partials) {
