@@ -27,7 +27,7 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per k_z<128> launch at this workload, from rocprofv3 FETCH_SIZE (x2) and
 # WRITE_SIZE (x1), both scaled by what they read for a device copy of known size
 # (profiles/fft_passes_64x512x512_hbm_counters.md): (394075.8 * 2 + 262656.0) KiB.
-Z_TRAFFIC_BYTES = int((394086.0 * 2 + 262656.0) * 1024)
+Z_TRAFFIC_BYTES = int((394084.0 * 2 + 262656.0) * 1024)
 
 
 def log(msg):
