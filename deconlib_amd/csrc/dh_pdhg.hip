@@ -15,6 +15,12 @@
 
 namespace dh {
 
+// Operands each thread reads once and outputs it writes once per launch
+// (stencil inputs excepted: neighbouring tiles re-read their halos through L2)
+// take the non-temporal hint, so the streams do not displace the halo lines.
+__device__ __forceinline__ float ldnt(const float* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void stnt(float* p, float v) { __builtin_nontemporal_store(v, p); }
+
 #define PD_LOOP(i, n) \
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
 
@@ -91,9 +97,9 @@ __global__ void k_pd_y1(float* __restrict__ y1, const float* __restrict__ ax, co
   const float inv1ps = (float)(1.0 + sigma);
   PD_LOOP(i, n) {
     // A xbar by linearity: (1 + theta) A x_k - theta A x_{k-1}
-    const float axb = one_pth * ax[i] - th * axp[i];
-    const float arg = y1[i] + sf * axb;
-    const float d = data[i];
+    const float axb = one_pth * ldnt(ax + i) - th * ldnt(axp + i);
+    const float arg = ldnt(y1 + i) + sf * axb;
+    const float d = ldnt(data + i);
     float out;
     if (loss_type == 0) {
       const float c = 1.0f - arg - sb;
@@ -107,7 +113,7 @@ __global__ void k_pd_y1(float* __restrict__ y1, const float* __restrict__ ax, co
     } else {
       out = (arg - sf * (d - bg)) / inv1ps;
     }
-    y1[i] = out;
+    stnt(y1 + i, out);
   }
 }
 
@@ -299,7 +305,7 @@ __global__ void __launch_bounds__(kTileThreads) k_pd_dual(float* __restrict__ y2
   if (t.in) {
     const int64_t i0 = ((int64_t)t.z0 * d.ny + t.y) * d.nx + t.x;
 #pragma unroll
-    for (int q = 0; q < 6; ++q) yc[q] = y2[q * n + i0];
+    for (int q = 0; q < 6; ++q) yc[q] = ldnt(y2 + q * n + i0);
   }
   for (int z = t.z0; z < t.z1; ++z) {
     pp.store(pl + slot_of(z + 1, t.z0) * kPlane, xbar(a0, b0), xbar(a1, b1));
@@ -313,7 +319,7 @@ __global__ void __launch_bounds__(kTileThreads) k_pd_dual(float* __restrict__ y2
     float yn[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (t.in && z + 1 < t.z1) {
 #pragma unroll
-      for (int q = 0; q < 6; ++q) yn[q] = y2[q * n + i + pstride];
+      for (int q = 0; q < 6; ++q) yn[q] = ldnt(y2 + q * n + i + pstride);
     }
     if (t.in) {
       auto f = [&](int zz, int yy, int xx) { return lds_at(pl, slot_of(zz, t.z0), t, yy, xx); };
@@ -324,7 +330,7 @@ __global__ void __launch_bounds__(kTileThreads) k_pd_dual(float* __restrict__ y2
       for (int q = 0; q < 6; ++q) av[q] = yc[q] + sf * h[q];
       if (!l12) {
 #pragma unroll
-        for (int q = 0; q < 6; ++q) y2[q * n + i] = fminf(fmaxf(av[q], -alpha), alpha);
+        for (int q = 0; q < 6; ++q) stnt(y2 + q * n + i, fminf(fmaxf(av[q], -alpha), alpha));
       } else {
         float ss = 0.0f;
 #pragma unroll
@@ -332,7 +338,7 @@ __global__ void __launch_bounds__(kTileThreads) k_pd_dual(float* __restrict__ y2
         const float nrm = sqrtf(ss + 1e-12f);
         const float s = fmaxf(nrm / alpha, 1.0f);
 #pragma unroll
-        for (int q = 0; q < 6; ++q) y2[q * n + i] = av[q] / s;
+        for (int q = 0; q < 6; ++q) stnt(y2 + q * n + i, av[q] / s);
       }
     }
 #pragma unroll
@@ -381,8 +387,8 @@ __global__ void __launch_bounds__(kTileThreads, 6) k_pd_primal(float* __restrict
   float gc = 0.f, xc = 0.f;
   if (t.in) {
     const int64_t i0 = ((int64_t)t.z0 * d.ny + t.y) * d.nx + t.x;
-    gc = g[i0];
-    xc = x[i0];
+    gc = ldnt(g + i0);
+    xc = ldnt(x + i0);
   }
   for (int z = t.z0; z < t.z1; ++z) {
 #pragma unroll
@@ -402,8 +408,8 @@ __global__ void __launch_bounds__(kTileThreads, 6) k_pd_primal(float* __restrict
     float gn = 0.f, xnext = 0.f;
     if (t.in && z + 1 < t.z1) {
       const int64_t i1 = ((int64_t)(z + 1) * d.ny + t.y) * d.nx + t.x;
-      gn = g[i1];
-      xnext = x[i1];
+      gn = ldnt(g + i1);
+      xnext = ldnt(x + i1);
     }
     if (t.in) {
       auto hw = [&](int q, int zz, int yy, int xx) {
@@ -422,7 +428,7 @@ __global__ void __launch_bounds__(kTileThreads, 6) k_pd_primal(float* __restrict
       const float grad = gc + lt;
       const float xv = xc;
       const float v = fmaxf(xv - tf * grad, 0.0f);
-      xn[i] = v;
+      stnt(xn + i, v);
       const float dv = v - xv;
       s_dx += (double)(dv * dv);
       s_x += (double)(xv * xv);
@@ -541,12 +547,12 @@ __global__ void k_pd_data_sums(const float* __restrict__ a, const float* __restr
   __shared__ double sh[16];
   double s_sq = 0.0, s_loss = 0.0;
   PD_LOOP(i, n) {
-    const float av = a[i];
-    const float dd = av - b[i];
+    const float av = ldnt(a + i);
+    const float dd = av - ldnt(b + i);
     s_sq += (double)(dd * dd);
     if (want_loss) {
       const float fm = av + bg;
-      const float dv = data[i];
+      const float dv = ldnt(data + i);
       if (loss_type == 0) {
         const float fms = fmaxf(fm, 1e-6f);
         s_loss += (double)(dv * logf((dv + 1e-6f) / fms) - (dv - fm));
