@@ -231,6 +231,28 @@ def bench_tiled(torch, shape=(48, 1024, 1024), psf_shape=(32, 64, 64), tile_size
     }
 
 
+def bench_psf_synthesis(torch, shape=SHAPE):
+    """The RL workload's widefield PSF (pupil_to_psf): GPU synthesis (dh_psf_synth)
+    against the host restatement (numpy complex128, plane by plane)."""
+    from deconlib_amd.psf import widefield_psf
+    from deconlib_amd.synth import CONFIG0_OPTICS
+
+    widefield_psf((2,) + tuple(shape[1:]), LATERAL, AXIAL, CONFIG0_OPTICS, device=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dev = widefield_psf(shape, LATERAL, AXIAL, CONFIG0_OPTICS, device=True)
+    torch.cuda.synchronize()
+    t_dev = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    host = widefield_psf(shape, LATERAL, AXIAL, CONFIG0_OPTICS)
+    t_host = time.perf_counter() - t0
+    d = dev.cpu().numpy().astype(np.float64)
+    err = float(np.linalg.norm(d - host) / np.linalg.norm(host))
+    return {"shape": list(shape), "gpu_seconds": t_dev, "host_seconds": t_host, "speedup": t_host / t_dev,
+            "rel_l2_vs_host": err,
+            "note": "end to end: pupil geometry on the host, uploads, synthesis, normalisation"}
+
+
 def bench_serving(torch, n_jobs, rate, seed=1003):
     """Serving sample: Poisson arrivals of mixed RL (100 it) / PDHG-Hessian (200 it) jobs,
     sizes log-uniform 16x64x64 .. 48x256x256; continuous batching vs the plain one-job loop."""
@@ -365,7 +387,7 @@ def main():
     ok = bool(torch.isfinite(st.x).all().item())
     pdhg = bench_pdhg(torch, N, obs, blur, args.steps, args.warmup)
     log(f"pdhg: {json.dumps(pdhg)}")
-    host = small = tiled = None
+    host = small = tiled = psf_synth = None
     if not args.no_extras:
         host = bench_host_to_host(torch, psf, obs)
         log(f"host-to-host solve: {json.dumps(host)}")
@@ -373,6 +395,8 @@ def main():
         log(f"batched small volumes: {json.dumps(small)}")
         tiled = bench_tiled(torch)
         log(f"tiled: {json.dumps(tiled)}")
+        psf_synth = bench_psf_synthesis(torch)
+        log(f"psf synthesis: {json.dumps(psf_synth)}")
     if args.copy_probe:
         # a plain device copy of known size: the reference point for the HBM counters
         src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")  # 256 MiB
@@ -433,6 +457,7 @@ def main():
         "host_to_host": host,
         "batched_small": small,
         "tiled": tiled,
+        "psf_synthesis": psf_synth,
         "serving": serving,
         "cpu_baseline": cpu,
     }

@@ -554,6 +554,96 @@ int dh_conv_plan_get_tile(const dh_conv_plan* p, int pass) {
   return dh::col_launch_for(n, ncols, p->tile[pass]).L;
 }
 
+// ---- PSF synthesis (psf/widefield.py:18-73) ----
+struct dh_psf_plan {
+  int ny = 0, nx = 0;
+  int L = 1, R = 1;  // Y columns / X rows per workgroup
+  std::vector<float2> tw_host;
+  int64_t tw_y = 0, tw_x = 0;
+  const float2* tw_dev = nullptr;
+};
+
+namespace {
+int64_t psf_row_tiles(const dh_psf_plan* p, int nz) { return ((int64_t)nz * p->ny + p->R - 1) / p->R; }
+int64_t psf_work_elems(const dh_psf_plan* p, int nz) { return (int64_t)nz * p->ny * p->nx; }
+int64_t psf_part_off(const dh_psf_plan* p, int nz) { return (psf_work_elems(p, nz) * 8 + 255) / 256 * 256; }
+}  // namespace
+
+int dh_psf_plan_create(int ny, int nx, dh_psf_plan** out) {
+  if (!out) return fail(DH_EINVAL, "null argument");
+  if (ny < 1 || nx < 1) return fail(DH_EINVAL, "pupil shape must be positive");
+  if (!dh::ct_supported(ny) || !dh::ct_supported(nx))
+    return fail(DH_EUNSUPPORTED, "GPU PSF synthesis needs pupil sides of the form 2^a * {1, 3, 5} <= 4096");
+  auto* p = new dh_psf_plan;
+  p->ny = ny;
+  p->nx = nx;
+  p->tw_y = 0;
+  push_pass_tables(p->tw_host, dh::ct::make_sched(ny));
+  p->tw_x = (int64_t)p->tw_host.size();
+  push_pass_tables(p->tw_host, dh::ct::make_sched(nx));
+  if (p->tw_host.empty()) p->tw_host.push_back(make_float2(1.f, 0.f));
+  // Y: up to 16 consecutive columns (128-byte row runs) within 512 threads and 160 KiB
+  const int Ty = dh::ct_threads(ny);
+  while (p->L < 16 && 2 * p->L * Ty <= kCtMaxThreads && (int64_t)ny * 2 * p->L * 8 <= kCtMaxLds) p->L *= 2;
+  // X: up to 16 rows within 512 threads and 96 KiB
+  const int Tx = dh::ct_threads(nx);
+  while (p->R < 16 && 2 * p->R * Tx <= kCtMaxThreads &&
+         (int64_t)2 * p->R * dh::xrow_stride(nx) * 8 <= kCtRowDefaultLds)
+    p->R *= 2;
+  *out = p;
+  return DH_OK;
+}
+
+int dh_psf_plan_info(const dh_psf_plan* p, int nz, int64_t* twiddle_bytes, int64_t* work_bytes) {
+  if (!p || nz < 1) return fail(DH_EINVAL, "bad PSF plan info arguments");
+  if (twiddle_bytes) *twiddle_bytes = (int64_t)p->tw_host.size() * 8;
+  if (work_bytes) *work_bytes = psf_part_off(p, nz) + (psf_row_tiles(p, nz) + 1) * 8;
+  return DH_OK;
+}
+
+int dh_psf_plan_bind(dh_psf_plan* p, void* twiddles_dev, void* stream) {
+  if (!p || !twiddles_dev) return fail(DH_EINVAL, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  DH_TRY(hip_check(hipMemcpyAsync(twiddles_dev, p->tw_host.data(), p->tw_host.size() * 8, hipMemcpyHostToDevice, st),
+                   "twiddle upload"));
+  DH_TRY(hip_check(hipStreamSynchronize(st), "twiddle upload sync"));
+  p->tw_dev = (const float2*)twiddles_dev;
+  return DH_OK;
+}
+
+int dh_psf_synth(const dh_psf_plan* p, const void* pupil, const double* kz, const double* z, int nz, int normalize,
+                 float* psf, void* work, int64_t work_bytes, double* total_out, void* stream) {
+  if (!p || !pupil || !kz || !z || !psf || !work) return fail(DH_EINVAL, "null argument");
+  if (nz < 1) return fail(DH_EINVAL, "nz must be positive");
+  if (!p->tw_dev) return fail(DH_EINVAL, "PSF plan is not bound (dh_psf_plan_bind)");
+  int64_t need = 0;
+  dh_psf_plan_info(p, nz, nullptr, &need);
+  if (work_bytes < need) return fail(DH_EWORKSPACE, "workspace too small");
+  hipStream_t st = (hipStream_t)stream;
+  char* ws = (char*)work;
+  float2* buf = (float2*)ws;
+  double* part = (double*)(ws + psf_part_off(p, nz));
+  const int64_t nrt = psf_row_tiles(p, nz);
+  double* total = total_out ? total_out : part + nrt;
+  const int64_t ntiles = (p->nx + p->L - 1) / p->L;
+  dh::Pack<dh::PsfYArgs> py;
+  pack_one(py, dh::PsfYArgs{(const float2*)pupil, kz, z, buf, p->tw_dev + p->tw_y, p->nx, ntiles},
+           (int64_t)nz * ntiles);
+  DH_TRY(hip_check(dh::ct_launch(dh::CT_PSF_Y, p->ny, 1, &py, p->L, (int64_t)nz * ntiles, st), "psf y pass"));
+  const double s = 1.0 / ((double)p->ny * (double)p->nx);
+  dh::Pack<dh::PsfXArgs> px;
+  pack_one(px, dh::PsfXArgs{buf, psf, p->tw_dev + p->tw_x, (int64_t)nz * p->ny, (float)(s * s), part}, nrt);
+  DH_TRY(hip_check(dh::ct_launch(dh::CT_PSF_X, p->nx, 1, &px, p->R, nrt, st), "psf x pass"));
+  DH_TRY(hip_check(dh::launch_sum_partials(part, nrt, total, st), "psf sum"));
+  if (normalize) DH_TRY(hip_check(dh::launch_div_total(psf, psf_work_elems(p, nz), total, st), "psf normalisation"));
+  return DH_OK;
+}
+
+int dh_psf_plan_destroy(dh_psf_plan* p) {
+  delete p;
+  return DH_OK;
+}
+
 int dh_conv_plan_destroy(dh_conv_plan* p) {
   delete p;
   return DH_OK;

@@ -152,6 +152,26 @@ struct XiArgs {  // X complex-to-real rows with the epilogue
   int prow;
 };
 
+// PSF synthesis (pupil_to_psf): Y pass over the columns of every z-plane, with the
+// defocused pupil formed on load; X pass over the rows, storing |.|^2.
+struct PsfYArgs {
+  const float2* pupil;  // (ny, nx) complex pupil
+  const double* kz;     // (ny, nx) axial spatial frequency
+  const double* z;      // (nz) axial positions
+  float2* work;         // (nz, ny, nx) out: the y-transformed defocused pupils
+  const float2* tw;     // pass tables of length ny
+  int nx;
+  int64_t ntiles;       // column tiles per plane
+};
+struct PsfXArgs {
+  const float2* work;   // (nz*ny, nx) rows
+  float* psf;           // (nz*ny, nx) out
+  const float2* tw;     // pass tables of length nx
+  int64_t nrows;
+  float scale;          // 1 / (ny*nx)^2: the inverse transform's normalisation, squared
+  double* partials;     // one per workgroup tile
+};
+
 // Up to kPack volumes of one transform length per launch, passed by value.
 constexpr int kPack = 16;
 template <class A>
@@ -163,7 +183,7 @@ struct Pack {
 
 // CT_XR: X round trip = inverse X pass with the epilogue, whose output rows are
 // transformed again by the forward X pass (XiArgs.spec) without leaving the CU.
-enum CtKind { CT_COL = 0, CT_Z = 1, CT_XF = 2, CT_XI = 3, CT_XR = 4 };
+enum CtKind { CT_COL = 0, CT_Z = 1, CT_XF = 2, CT_XI = 3, CT_XR = 4, CT_PSF_Y = 5, CT_PSF_X = 6 };
 // True if length n has specialised kernels.
 bool ct_supported(int n);
 // Threads per line of the specialised kernels for length n.

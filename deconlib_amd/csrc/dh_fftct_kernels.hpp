@@ -113,9 +113,16 @@ __device__ __forceinline__ float2 epi_apply2(const Epi& e, float2 v, int64_t idx
   }
 }
 
+// Workgroup sum of v, returned by thread 0.  A workgroup smaller than a wave
+// (or a partial last wave) adds only its existing lanes; for whole waves lane
+// 0's reduction tree is the plain one.
 __device__ __forceinline__ double bsum(double v, double* scratch) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int valid = min(64, (int)blockDim.x - (w << 6));
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_down(v, off, 64);
+    if (lane + off < valid) v += o;
+  }
   __syncthreads();
   if (lane == 0) scratch[w] = v;
   __syncthreads();
@@ -875,6 +882,89 @@ inline unsigned persistent_grid(const void* kernel, int threads, size_t shm, int
   return (unsigned)(nblocks < cap ? nblocks : cap);
 }
 
+// ---- PSF synthesis: |IFFT2(P exp(2 pi i kz z))|^2 per z-plane ----
+// Y pass: L consecutive columns of one z-plane per workgroup.  The first pass
+// loads the pupil and applies the defocus phase, reduced to [0, 1) turns in
+// float64 before sincospi; the inverse transform along y is stored to work.
+template <int N>
+__global__ void __launch_bounds__(kMaxThreads) k_psf_y(Pack<PsfYArgs> pk, int L) {
+  extern __shared__ float2 lds[];
+  constexpr int T = Geo<N>::T;
+  const int ln = threadIdx.x % L;
+  const int t = threadIdx.x / L;
+  const int64_t total = pk.prefix[pk.nvol];
+  for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
+    const int v = pick(pk, b);
+    const PsfYArgs& a = pk.a[v];
+    const int64_t lb = b - pk.prefix[v];
+    const int64_t zi = lb / a.ntiles;
+    const int64_t c = (lb - zi * a.ntiles) * L + ln;
+    const bool ok = c < a.nx;
+    const double zz = a.z[zi];
+    auto gld = [&](int e) -> float2 {
+      if (!ok) return make_float2(0.f, 0.f);
+      const int64_t i = (int64_t)e * a.nx + c;
+      const double turns = a.kz[i] * zz;
+      float sn, cs;
+      sincospif(2.0f * (float)(turns - floor(turns)), &sn, &cs);
+      return cmul(a.pupil[i], make_float2(cs, sn));
+    };
+    auto gst = [&](int e, float2 val) {
+      if (ok) st_nt(a.work + (zi * N + e) * a.nx + c, val);
+    };
+    auto lld = [&](int e) { return lds[e * L + ln]; };
+    auto lst = [&](int e, float2 val) { lds[e * L + ln] = val; };
+    if constexpr (N == 1) {
+      if (t == 0) gst(0, gld(0));
+    } else {
+      passes<N, false, 1, 0>(t, a.tw, gld, gst, lld, lst, true);
+    }
+    __syncthreads();
+  }
+}
+
+// X pass: R rows per workgroup; the last pass stores |v|^2 * scale and the
+// workgroup's float64 sum of them goes to partials[tile].
+template <int N>
+__global__ void __launch_bounds__(kMaxThreads) k_psf_x(Pack<PsfXArgs> pk, int R) {
+  extern __shared__ float2 lds[];
+  constexpr int T = Geo<N>::T;
+  constexpr int RS = xrow_stride(N);
+  const int rl = threadIdx.x / T;
+  const int t = threadIdx.x % T;
+  const int64_t total = pk.prefix[pk.nvol];
+  for (int64_t b = blockIdx.x; b < total; b += gridDim.x) {
+    const int v = pick(pk, b);
+    const PsfXArgs& a = pk.a[v];
+    const int64_t lb = b - pk.prefix[v];
+    const int64_t row = lb * R + rl;
+    const bool ok = row < a.nrows;
+    const float2* src = a.work + row * N;
+    float* dst = a.psf + row * N;
+    float2* lrow = lds + rl * RS;
+    double acc = 0.0;
+    auto gld = [&](int e) { return ok ? ld_nt(src + e) : make_float2(0.f, 0.f); };
+    auto gst = [&](int e, float2 val) {
+      if (ok) {
+        const float p = (val.x * val.x + val.y * val.y) * a.scale;
+        __builtin_nontemporal_store(p, dst + e);
+        acc += (double)p;
+      }
+    };
+    auto lld = [&](int e) { return lrow[xrow_lds(e)]; };
+    auto lst = [&](int e, float2 val) { lrow[xrow_lds(e)] = val; };
+    if constexpr (N == 1) {
+      if (t == 0) gst(0, gld(0));
+    } else {
+      passes<N, false, 1, 0>(t, a.tw, gld, gst, lld, lst, true);
+    }
+    __syncthreads();
+    const double tot = bsum(acc, reinterpret_cast<double*>(lds));
+    if (threadIdx.x == 0) a.partials[lb] = tot;
+    __syncthreads();
+  }
+}
+
 template <int N>
 struct Launch {
   static constexpr int T = Geo<N>::T;
@@ -915,6 +1005,21 @@ struct Launch {
     const void* k = (const void*)k_xi<N, false>;
     (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     hipLaunchKernelGGL((k_xi<N, false>), dim3(grid(k, R * T, shm, nblocks, persist)), dim3(R * T), shm, st, pk, R);
+    return hipGetLastError();
+  }
+  static hipError_t psfy(const Pack<PsfYArgs>& pk, int L, int64_t nblocks, hipStream_t st) {
+    const size_t shm = (size_t)N * L * sizeof(float2);
+    const void* k = (const void*)k_psf_y<N>;
+    (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    hipLaunchKernelGGL((k_psf_y<N>), dim3((unsigned)nblocks), dim3(L * T), shm, st, pk, L);
+    return hipGetLastError();
+  }
+  static hipError_t psfx(const Pack<PsfXArgs>& pk, int R, int64_t nblocks, hipStream_t st) {
+    size_t shm = (size_t)R * xrow_stride(N) * sizeof(float2);
+    if (shm < 16 * sizeof(double)) shm = 16 * sizeof(double);
+    const void* k = (const void*)k_psf_x<N>;
+    (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    hipLaunchKernelGGL((k_psf_x<N>), dim3((unsigned)nblocks), dim3(R * T), shm, st, pk, R);
     return hipGetLastError();
   }
   // the boxed instantiation when any volume of the pack has a reduction box

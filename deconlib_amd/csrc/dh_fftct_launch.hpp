@@ -14,6 +14,8 @@ hipError_t launch_kind(int kind, int dir, const void* pk, int tile, int64_t nblo
     case CT_Z: return Launch<N>::z(*(const Pack<ZArgs>*)pk, t, nblocks, persist, st);
     case CT_XF: return Launch<N>::xf(*(const Pack<XfArgs>*)pk, t, nblocks, persist, st);
     case CT_XI: return Launch<N>::xi(*(const Pack<XiArgs>*)pk, t, nblocks, persist, st);
+    case CT_PSF_Y: return Launch<N>::psfy(*(const Pack<PsfYArgs>*)pk, t, nblocks, st);
+    case CT_PSF_X: return Launch<N>::psfx(*(const Pack<PsfXArgs>*)pk, t, nblocks, st);
     default: return Launch<N>::xr(*(const Pack<XiArgs>*)pk, t, nblocks, persist, st);
   }
 }

@@ -117,6 +117,14 @@ __global__ void k_rl_support(float* sens, float* x, int64_t n, const double* max
   }
 }
 
+// x /= total (a device float64 sum; unchanged when it is not positive)
+__global__ void k_div_total(float* x, int64_t n, const double* total) {
+  const double tv = *total;
+  if (!(tv > 0.0)) return;
+  const float inv = (float)(1.0 / tv);
+  DH_EW_LOOP(i, n) x[i] = x[i] * inv;
+}
+
 __global__ void k_clamp_min(const float* in, float* out, int64_t n, float lo) {
   DH_EW_LOOP(i, n) out[i] = fmaxf(in[i], lo);
 }
@@ -169,6 +177,11 @@ hipError_t launch_rl_support(float* sens, float* x, int64_t n, const double* max
   hipLaunchKernelGGL(k_rl_support, ew_grid(n), dim3(256), 0, st, sens, x, n, maxval);
   return hipGetLastError();
 }
+hipError_t launch_div_total(float* x, int64_t n, const double* total, hipStream_t st) {
+  hipLaunchKernelGGL(k_div_total, ew_grid(n), dim3(256), 0, st, x, n, total);
+  return hipGetLastError();
+}
+
 hipError_t launch_clamp_min(const float* in, float* out, int64_t n, float lo, hipStream_t st) {
   hipLaunchKernelGGL(k_clamp_min, ew_grid(n), dim3(256), 0, st, in, out, n, lo);
   return hipGetLastError();

@@ -12,6 +12,7 @@ hipError_t launch_reduce(int op, const float* x, int64_t n, double* partials, do
 hipError_t launch_cmax_abs2(const float2* z, int64_t n, double* partials, double* out, hipStream_t st);
 hipError_t launch_rl_support(float* sens, float* x, int64_t n, const double* maxval, hipStream_t st);
 hipError_t launch_clamp_min(const float* in, float* out, int64_t n, float lo, hipStream_t st);
+hipError_t launch_div_total(float* x, int64_t n, const double* total, hipStream_t st);
 hipError_t launch_sub_floor(const float* in, float* out, int64_t n, float a, float b, hipStream_t st);
 hipError_t launch_rl_ratio(const float* model, const float* obs, float* out, int64_t n, float bg, float eps,
                            double* partials, double* reduce_out, hipStream_t st);

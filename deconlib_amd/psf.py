@@ -14,6 +14,9 @@ without the reference installed:
 
 ``pupil_to_psf`` works z-plane by z-plane (the reference forms the whole
 (nz, ny, nx) complex128 stack at once), so large PSFs fit in host memory.
+``pupil_to_psf_device`` is the same computation on the GPU (``dh_psf_synth``:
+defocus formed on load, float32 transforms, float64 normalisation); the pupil,
+geometry and aberration phases are 2-D and stay on the host.
 """
 
 from __future__ import annotations
@@ -128,6 +131,73 @@ def pupil_to_psf(pupil: np.ndarray, geom: Geometry, z: np.ndarray, normalize: bo
     return out
 
 
+class _PsfPlan:
+    """A bound dh_psf_plan for one pupil shape (its twiddle rows on the device)."""
+
+    def __init__(self, ny: int, nx: int):
+        import ctypes as C
+
+        import torch
+
+        from . import _native as N
+        from ._tensor import device
+
+        h = C.c_void_p()
+        N.call("dh_psf_plan_create", int(ny), int(nx), C.byref(h))
+        self._h = h
+        self._destroy = N.fns["dh_psf_plan_destroy"]  # bound now: no imports at interpreter exit
+        tb = C.c_int64()
+        N.call("dh_psf_plan_info", h, 1, C.byref(tb), None)
+        self.twiddles = torch.empty(int(tb.value), dtype=torch.uint8, device=device())
+        N.call("dh_psf_plan_bind", h, self.twiddles.data_ptr(), N.stream_ptr())
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        destroy = getattr(self, "_destroy", None)
+        if h is not None and h.value and destroy is not None:
+            destroy(h)
+            self._h = None
+
+
+_PSF_PLANS: Dict[Tuple[int, int], _PsfPlan] = {}
+
+
+def pupil_to_psf_device(pupil, geom: Geometry, z, normalize: bool = True):
+    """pupil_to_psf (psf/widefield.py:18-73) on the GPU; a (nz, ny, nx) float32 device tensor.
+
+    ``pupil`` (ny, nx) complex, numpy or a device tensor; ``geom.kz`` and ``z`` as
+    for ``pupil_to_psf``.  Pupil sides must be 2^a * {1, 3, 5} <= 4096.
+    """
+    import ctypes as C
+
+    import torch
+
+    from . import _native as N
+    from ._tensor import device
+
+    dev = device()
+    if isinstance(pupil, torch.Tensor):
+        pt = pupil.to(device=dev, dtype=torch.complex64).contiguous()
+    else:
+        pt = torch.from_numpy(np.ascontiguousarray(pupil, dtype=np.complex64)).to(dev)
+    ny, nx = (int(v) for v in pt.shape)
+    zz = np.atleast_1d(np.asarray(z, dtype=np.float64))
+    nz = len(zz)
+    key = (ny, nx)
+    if key not in _PSF_PLANS:
+        _PSF_PLANS[key] = _PsfPlan(ny, nx)
+    plan = _PSF_PLANS[key]
+    kz = torch.from_numpy(np.ascontiguousarray(geom.kz, dtype=np.float64)).to(dev)
+    zt = torch.from_numpy(zz).to(dev)
+    wb = C.c_int64()
+    N.call("dh_psf_plan_info", plan._h, nz, None, C.byref(wb))
+    work = torch.empty(int(wb.value), dtype=torch.uint8, device=dev)
+    out = torch.empty((nz, ny, nx), dtype=torch.float32, device=dev)
+    N.call("dh_psf_synth", plan._h, pt.data_ptr(), kz.data_ptr(), zt.data_ptr(), nz, int(bool(normalize)),
+           out.data_ptr(), work.data_ptr(), int(wb.value), None, N.stream_ptr())
+    return out
+
+
 class IndexMismatch:
     """Gibson–Lanni depth aberration: exp(2 pi i d (n2 cos t2 - n1 cos t1) / lambda)."""
 
@@ -183,13 +253,18 @@ def apply_aberrations(pupil, geom, optics, aberrations) -> np.ndarray:
 
 
 def widefield_psf(shape, lateral: float, axial: float, optics: Optics = None, aberrations=(),
-                  dtype=np.float32) -> np.ndarray:
-    """(nz, ny, nx) corner-origin PSF normalised to sum 1 (the configs' recipe)."""
+                  dtype=np.float32, device: bool = False):
+    """(nz, ny, nx) corner-origin PSF normalised to sum 1 (the configs' recipe).
+
+    ``device=True``: synthesised on the GPU, returned as a float32 device tensor.
+    """
     optics = optics or Optics(wavelength=0.525, na=1.4, ni=1.515, ns=1.334)
     nz, ny, nx = shape
     geom = make_geometry((ny, nx), lateral, optics)
     pupil = make_pupil(geom)
     if aberrations:
         pupil = apply_aberrations(pupil, geom, optics, list(aberrations))
+    if device:
+        return pupil_to_psf_device(pupil, geom, fft_coords(nz, axial), normalize=True)
     psf = pupil_to_psf(pupil, geom, fft_coords(nz, axial), normalize=True)
     return psf.astype(dtype)
