@@ -147,9 +147,9 @@ SYMBOLS = {
     "dh_rl_iteration": (_int, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _f, _f, _vp, _int, _int, _vp),
     "dh_batch_rl_iteration": (_int, _int, C.POINTER(RLItem), _vp),
     "dh_psf_plan_create": (_int, _int, _int, C.POINTER(C.c_void_p)),
-    "dh_psf_plan_info": (_int, _vp, _int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)),
+    "dh_psf_plan_info": (_int, _vp, _int, _int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)),
     "dh_psf_plan_bind": (_int, _vp, _vp, _vp),
-    "dh_psf_synth": (_int, _vp, _vp, _vp, _vp, _int, _int, _vp, _vp, _i64, _vp, _vp),
+    "dh_psf_synth": (_int, _vp, _int, _vp, C.POINTER(_d), _vp, _vp, _int, _int, _vp, _vp, _i64, _vp, _vp),
     "dh_psf_plan_destroy": (_int, _vp),
     "dh_timing_enable": (_int, _int),
     "dh_timing_read": (_int, C.POINTER(_d), C.POINTER(_i64), _int),

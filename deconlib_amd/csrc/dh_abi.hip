@@ -594,10 +594,10 @@ int dh_psf_plan_create(int ny, int nx, dh_psf_plan** out) {
   return DH_OK;
 }
 
-int dh_psf_plan_info(const dh_psf_plan* p, int nz, int64_t* twiddle_bytes, int64_t* work_bytes) {
-  if (!p || nz < 1) return fail(DH_EINVAL, "bad PSF plan info arguments");
+int dh_psf_plan_info(const dh_psf_plan* p, int nz, int nchan, int64_t* twiddle_bytes, int64_t* work_bytes) {
+  if (!p || nz < 1 || nchan < 1) return fail(DH_EINVAL, "bad PSF plan info arguments");
   if (twiddle_bytes) *twiddle_bytes = (int64_t)p->tw_host.size() * 8;
-  if (work_bytes) *work_bytes = psf_part_off(p, nz) + (psf_row_tiles(p, nz) + 1) * 8;
+  if (work_bytes) *work_bytes = psf_part_off(p, nz) + ((int64_t)nchan * psf_row_tiles(p, nz) + 1) * 8;
   return DH_OK;
 }
 
@@ -611,30 +611,38 @@ int dh_psf_plan_bind(dh_psf_plan* p, void* twiddles_dev, void* stream) {
   return DH_OK;
 }
 
-int dh_psf_synth(const dh_psf_plan* p, const void* pupil, const double* kz, const double* z, int nz, int normalize,
-                 float* psf, void* work, int64_t work_bytes, double* total_out, void* stream) {
-  if (!p || !pupil || !kz || !z || !psf || !work) return fail(DH_EINVAL, "null argument");
-  if (nz < 1) return fail(DH_EINVAL, "nz must be positive");
+int dh_psf_synth(const dh_psf_plan* p, int nchan, const void* pupils, const double* weights, const double* kz,
+                 const double* z, int nz, int normalize, float* psf, void* work, int64_t work_bytes, double* total_out,
+                 void* stream) {
+  if (!p || !pupils || !weights || !kz || !z || !psf || !work) return fail(DH_EINVAL, "null argument");
+  if (nz < 1 || nchan < 1) return fail(DH_EINVAL, "nz and nchan must be positive");
   if (!p->tw_dev) return fail(DH_EINVAL, "PSF plan is not bound (dh_psf_plan_bind)");
   int64_t need = 0;
-  dh_psf_plan_info(p, nz, nullptr, &need);
+  dh_psf_plan_info(p, nz, nchan, nullptr, &need);
   if (work_bytes < need) return fail(DH_EWORKSPACE, "workspace too small");
   hipStream_t st = (hipStream_t)stream;
   char* ws = (char*)work;
   float2* buf = (float2*)ws;
   double* part = (double*)(ws + psf_part_off(p, nz));
   const int64_t nrt = psf_row_tiles(p, nz);
-  double* total = total_out ? total_out : part + nrt;
+  double* total = total_out ? total_out : part + nchan * nrt;
   const int64_t ntiles = (p->nx + p->L - 1) / p->L;
-  dh::Pack<dh::PsfYArgs> py;
-  pack_one(py, dh::PsfYArgs{(const float2*)pupil, kz, z, buf, p->tw_dev + p->tw_y, p->nx, ntiles},
-           (int64_t)nz * ntiles);
-  DH_TRY(hip_check(dh::ct_launch(dh::CT_PSF_Y, p->ny, 1, &py, p->L, (int64_t)nz * ntiles, st), "psf y pass"));
   const double s = 1.0 / ((double)p->ny * (double)p->nx);
-  dh::Pack<dh::PsfXArgs> px;
-  pack_one(px, dh::PsfXArgs{buf, psf, p->tw_dev + p->tw_x, (int64_t)nz * p->ny, (float)(s * s), part}, nrt);
-  DH_TRY(hip_check(dh::ct_launch(dh::CT_PSF_X, p->nx, 1, &px, p->R, nrt, st), "psf x pass"));
-  DH_TRY(hip_check(dh::launch_sum_partials(part, nrt, total, st), "psf sum"));
+  const int64_t plane = (int64_t)p->ny * p->nx;
+  // channel by channel through one work buffer: y pass, then x pass storing
+  // (first channel) or adding w_c |.|^2; per-channel partials, summed in order
+  for (int c = 0; c < nchan; ++c) {
+    const float2* pc = (const float2*)pupils + c * plane;
+    dh::Pack<dh::PsfYArgs> py;
+    pack_one(py, dh::PsfYArgs{pc, kz, z, buf, p->tw_dev + p->tw_y, p->nx, ntiles}, (int64_t)nz * ntiles);
+    DH_TRY(hip_check(dh::ct_launch(dh::CT_PSF_Y, p->ny, 1, &py, p->L, (int64_t)nz * ntiles, st), "psf y pass"));
+    dh::Pack<dh::PsfXArgs> px;
+    pack_one(px, dh::PsfXArgs{buf, psf, p->tw_dev + p->tw_x, (int64_t)nz * p->ny, (float)(weights[c] * s * s),
+                              c > 0 ? 1 : 0, part + c * nrt},
+             nrt);
+    DH_TRY(hip_check(dh::ct_launch(dh::CT_PSF_X, p->nx, 1, &px, p->R, nrt, st), "psf x pass"));
+  }
+  DH_TRY(hip_check(dh::launch_sum_partials(part, nchan * nrt, total, st), "psf sum"));
   if (normalize) DH_TRY(hip_check(dh::launch_div_total(psf, psf_work_elems(p, nz), total, st), "psf normalisation"));
   return DH_OK;
 }

@@ -168,7 +168,8 @@ struct PsfXArgs {
   float* psf;           // (nz*ny, nx) out
   const float2* tw;     // pass tables of length nx
   int64_t nrows;
-  float scale;          // 1 / (ny*nx)^2: the inverse transform's normalisation, squared
+  float scale;          // channel weight / (ny*nx)^2 (the inverse transform's normalisation, squared)
+  int accumulate;       // 0: psf = scale |v|^2; 1: psf += scale |v|^2 (later channels)
   double* partials;     // one per workgroup tile
 };
 

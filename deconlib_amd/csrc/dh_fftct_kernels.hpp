@@ -923,8 +923,9 @@ __global__ void __launch_bounds__(kMaxThreads) k_psf_y(Pack<PsfYArgs> pk, int L)
   }
 }
 
-// X pass: R rows per workgroup; the last pass stores |v|^2 * scale and the
-// workgroup's float64 sum of them goes to partials[tile].
+// X pass: R rows per workgroup; the last pass stores (or, for a later channel
+// of a vectorial PSF, adds) |v|^2 * scale and the workgroup's float64 sum of
+// them goes to partials[tile].
 template <int N>
 __global__ void __launch_bounds__(kMaxThreads) k_psf_x(Pack<PsfXArgs> pk, int R) {
   extern __shared__ float2 lds[];
@@ -947,7 +948,8 @@ __global__ void __launch_bounds__(kMaxThreads) k_psf_x(Pack<PsfXArgs> pk, int R)
     auto gst = [&](int e, float2 val) {
       if (ok) {
         const float p = (val.x * val.x + val.y * val.y) * a.scale;
-        __builtin_nontemporal_store(p, dst + e);
+        if (a.accumulate) dst[e] += p;
+        else __builtin_nontemporal_store(p, dst + e);
         acc += (double)p;
       }
     };

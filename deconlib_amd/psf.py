@@ -147,7 +147,7 @@ class _PsfPlan:
         self._h = h
         self._destroy = N.fns["dh_psf_plan_destroy"]  # bound now: no imports at interpreter exit
         tb = C.c_int64()
-        N.call("dh_psf_plan_info", h, 1, C.byref(tb), None)
+        N.call("dh_psf_plan_info", h, 1, 1, C.byref(tb), None)
         self.twiddles = torch.empty(int(tb.value), dtype=torch.uint8, device=device())
         N.call("dh_psf_plan_bind", h, self.twiddles.data_ptr(), N.stream_ptr())
 
@@ -162,12 +162,8 @@ class _PsfPlan:
 _PSF_PLANS: Dict[Tuple[int, int], _PsfPlan] = {}
 
 
-def pupil_to_psf_device(pupil, geom: Geometry, z, normalize: bool = True):
-    """pupil_to_psf (psf/widefield.py:18-73) on the GPU; a (nz, ny, nx) float32 device tensor.
-
-    ``pupil`` (ny, nx) complex, numpy or a device tensor; ``geom.kz`` and ``z`` as
-    for ``pupil_to_psf``.  Pupil sides must be 2^a * {1, 3, 5} <= 4096.
-    """
+def _synth_device(channels, weights, geom: Geometry, z, normalize: bool):
+    """sum_c w_c |ifft2(channel_c * defocus)|^2 on the GPU (dh_psf_synth)."""
     import ctypes as C
 
     import torch
@@ -176,11 +172,11 @@ def pupil_to_psf_device(pupil, geom: Geometry, z, normalize: bool = True):
     from ._tensor import device
 
     dev = device()
-    if isinstance(pupil, torch.Tensor):
-        pt = pupil.to(device=dev, dtype=torch.complex64).contiguous()
+    if isinstance(channels, torch.Tensor):
+        pt = channels.to(device=dev, dtype=torch.complex64).contiguous()
     else:
-        pt = torch.from_numpy(np.ascontiguousarray(pupil, dtype=np.complex64)).to(dev)
-    ny, nx = (int(v) for v in pt.shape)
+        pt = torch.from_numpy(np.ascontiguousarray(channels, dtype=np.complex64)).to(dev)
+    nchan, ny, nx = (int(v) for v in pt.shape)
     zz = np.atleast_1d(np.asarray(z, dtype=np.float64))
     nz = len(zz)
     key = (ny, nx)
@@ -189,13 +185,102 @@ def pupil_to_psf_device(pupil, geom: Geometry, z, normalize: bool = True):
     plan = _PSF_PLANS[key]
     kz = torch.from_numpy(np.ascontiguousarray(geom.kz, dtype=np.float64)).to(dev)
     zt = torch.from_numpy(zz).to(dev)
+    w = (C.c_double * nchan)(*[float(v) for v in weights])
     wb = C.c_int64()
-    N.call("dh_psf_plan_info", plan._h, nz, None, C.byref(wb))
+    N.call("dh_psf_plan_info", plan._h, nz, nchan, None, C.byref(wb))
     work = torch.empty(int(wb.value), dtype=torch.uint8, device=dev)
     out = torch.empty((nz, ny, nx), dtype=torch.float32, device=dev)
-    N.call("dh_psf_synth", plan._h, pt.data_ptr(), kz.data_ptr(), zt.data_ptr(), nz, int(bool(normalize)),
-           out.data_ptr(), work.data_ptr(), int(wb.value), None, N.stream_ptr())
+    N.call("dh_psf_synth", plan._h, nchan, pt.data_ptr(), w, kz.data_ptr(), zt.data_ptr(), nz,
+           int(bool(normalize)), out.data_ptr(), work.data_ptr(), int(wb.value), None, N.stream_ptr())
     return out
+
+
+def pupil_to_psf_device(pupil, geom: Geometry, z, normalize: bool = True):
+    """pupil_to_psf (psf/widefield.py:18-73) on the GPU; a (nz, ny, nx) float32 device tensor.
+
+    ``pupil`` (ny, nx) complex, numpy or a device tensor; ``geom.kz`` and ``z`` as
+    for ``pupil_to_psf``.  Pupil sides must be 2^a * {1, 3, 5} <= 4096.
+    """
+    p = pupil[None] if not hasattr(pupil, "unsqueeze") else pupil.unsqueeze(0)
+    return _synth_device(p, [1.0], geom, z, normalize)
+
+
+# ---- vectorial PSF (psf/pupil.py:48-75,156-262; psf/widefield.py:110-234) ----
+
+def aplanatic_apodization(geom: Geometry, eps: float = None) -> np.ndarray:
+    """1 / sqrt(cos theta) on the binary NA mask (pupil.py:48-75)."""
+    eps = np.finfo(np.float64).eps if eps is None else eps
+    return 1.0 / np.sqrt(np.maximum(geom.cos_theta, eps)) * geom.mask.astype(np.float64)
+
+
+def compute_fresnel_coefficients(geom: Geometry, optics: Optics):
+    """(t_s, t_p) transmission sample -> immersion, on the binary NA mask (pupil.py:156-202)."""
+    c1, s1 = geom.cos_theta, geom.sin_theta
+    s2 = np.clip((optics.ni / optics.ns) * s1, 0.0, 1.0)
+    c2 = np.sqrt(1.0 - s2 ** 2)
+    num = 2.0 * optics.ns * c2
+    eps = np.finfo(np.float64).eps
+    mask = geom.mask.astype(np.float64)
+    t_s = num / np.maximum(optics.ns * c2 + optics.ni * c1, eps) * mask
+    t_p = num / np.maximum(optics.ni * c2 + optics.ns * c1, eps) * mask
+    return t_s, t_p
+
+
+def compute_vectorial_factors(geom: Geometry, optics: Optics) -> np.ndarray:
+    """(3 dipoles x, y, z; 2 fields Ex, Ey; ny, nx) dipole-to-field factors (pupil.py:205-262)."""
+    t_s, t_p = compute_fresnel_coefficients(geom, optics)
+    c1, s1 = geom.cos_theta, geom.sin_theta
+    cp, sp = np.cos(geom.phi), np.sin(geom.phi)
+    px, py = t_p * c1 * cp, t_p * c1 * sp  # p polarisation (plane of incidence)
+    sx, sy = -t_s * sp, t_s * cp           # s polarisation
+    f = np.empty((3, 2) + geom.shape, dtype=np.float64)
+    f[0, 0], f[0, 1] = cp * px - sp * sx, cp * py - sp * sy
+    f[1, 0], f[1, 1] = sp * px + cp * sx, sp * py + cp * sy
+    f[2, 0], f[2, 1] = -s1 * cp * t_p, -s1 * sp * t_p
+    return f
+
+
+def _vectorial_channels(pupil, geom: Geometry, optics: Optics, dipole):
+    """(channels (K, ny, nx), weights) of the incoherent sum for a dipole mode (widefield.py:172-224)."""
+    f = compute_vectorial_factors(geom, optics)
+    pa = np.asarray(pupil) * aplanatic_apodization(geom)
+    if isinstance(dipole, str) and dipole == "isotropic":
+        return np.stack([pa * f[d, c] for d in range(3) for c in range(2)]), [1.0 / 3.0] * 6
+    if isinstance(dipole, str) and dipole in ("x", "y", "z"):
+        d = {"x": 0, "y": 1, "z": 2}[dipole]
+        return np.stack([pa * f[d, 0], pa * f[d, 1]]), [1.0, 1.0]
+    if isinstance(dipole, tuple):
+        th, ph = dipole
+        mu = (np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th))
+        mx = mu[0] * f[0, 0] + mu[1] * f[1, 0] + mu[2] * f[2, 0]
+        my = mu[0] * f[0, 1] + mu[1] * f[1, 1] + mu[2] * f[2, 1]
+        return np.stack([pa * mx, pa * my]), [1.0, 1.0]
+    raise ValueError(f"Unknown dipole type: {dipole}")
+
+
+def pupil_to_vectorial_psf(pupil, geom: Geometry, optics: Optics, z, dipole="isotropic",
+                           normalize: bool = True) -> np.ndarray:
+    """Vectorial PSF on the host (widefield.py:110-234), plane by plane in complex128."""
+    ch, w = _vectorial_channels(pupil, geom, optics, dipole)
+    z = np.atleast_1d(z)
+    out = np.zeros((len(z),) + geom.shape, dtype=np.float64)
+    for i, zi in enumerate(z):
+        d = np.exp(2j * np.pi * geom.kz * zi)
+        for c in range(len(w)):
+            a = np.fft.ifft2(ch[c] * d)
+            out[i] += (a.real ** 2 + a.imag ** 2) * w[c]
+    if normalize:
+        total = out.sum()
+        if total > 0:
+            out /= total
+    return out
+
+
+def pupil_to_vectorial_psf_device(pupil, geom: Geometry, optics: Optics, z, dipole="isotropic",
+                                  normalize: bool = True):
+    """pupil_to_vectorial_psf on the GPU (dh_psf_synth over the field channels)."""
+    ch, w = _vectorial_channels(pupil, geom, optics, dipole)
+    return _synth_device(ch, w, geom, z, normalize)
 
 
 class IndexMismatch:

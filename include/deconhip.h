@@ -177,20 +177,24 @@ typedef struct {
 } dh_rl_item;
 int dh_batch_rl_iteration(int nitems, const dh_rl_item* items, void* stream);
 
-/* ---- PSF synthesis: pupil_to_psf (psf/widefield.py:18-73) ----
- * psf[z, y, x] = |IFFT2(pupil * exp(2 pi i kz z))|^2 per z-plane (numpy's ifft2
- * normalisation, DC at the corner), normalised to sum 1 when normalize != 0.
- * pupil: (ny, nx) complex64 interleaved; kz: (ny, nx) float64; z: (nz) float64;
- * psf: (nz, ny, nx) float32; all device pointers.  The defocus phase is reduced to
- * [0, 1) turns in float64; the transforms run in float32 and the sum in float64.
- * total_out (device, may be NULL) receives the sum before normalisation.  ny and
- * nx must be 2^a * {1, 3, 5} <= 4096 (DH_EUNSUPPORTED otherwise). */
+/* ---- PSF synthesis: pupil_to_psf / pupil_to_vectorial_psf (psf/widefield.py:18-73,110-234) ----
+ * psf[z, y, x] = sum_c w_c |IFFT2(pupil_c * exp(2 pi i kz z))|^2 per z-plane (numpy's
+ * ifft2 normalisation, DC at the corner), normalised to sum 1 when normalize != 0.
+ * One channel of weight 1 is pupil_to_psf; the vectorial PSF's channels are the
+ * apodised pupil times each dipole-to-field factor, incoherently summed.
+ * pupils: (nchan, ny, nx) complex64 interleaved, kz: (ny, nx) float64, z: (nz)
+ * float64, psf: (nz, ny, nx) float32 (device); weights: nchan float64 (host).
+ * The defocus phase is reduced to [0, 1) turns in float64; the transforms run in
+ * float32 and the sums in float64.  total_out (device, may be NULL) receives the
+ * sum before normalisation.  ny and nx must be 2^a * {1, 3, 5} <= 4096
+ * (DH_EUNSUPPORTED otherwise). */
 typedef struct dh_psf_plan dh_psf_plan;
 int dh_psf_plan_create(int ny, int nx, dh_psf_plan** out);
-int dh_psf_plan_info(const dh_psf_plan* plan, int nz, int64_t* twiddle_bytes, int64_t* work_bytes);
+int dh_psf_plan_info(const dh_psf_plan* plan, int nz, int nchan, int64_t* twiddle_bytes, int64_t* work_bytes);
 int dh_psf_plan_bind(dh_psf_plan* plan, void* twiddles_dev, void* stream);
-int dh_psf_synth(const dh_psf_plan* plan, const void* pupil, const double* kz, const double* z, int nz,
-                 int normalize, float* psf, void* work, int64_t work_bytes, double* total_out, void* stream);
+int dh_psf_synth(const dh_psf_plan* plan, int nchan, const void* pupils, const double* weights, const double* kz,
+                 const double* z, int nz, int normalize, float* psf, void* work, int64_t work_bytes,
+                 double* total_out, void* stream);
 int dh_psf_plan_destroy(dh_psf_plan* plan);
 
 /* ---- reductions and elementwise steps (rl_mlx.py:123-150) ---- */

@@ -76,3 +76,19 @@ def test_device_psf_rejects_unsupported_sides(gpu):
 
     with pytest.raises(NotImplementedError):
         pupil_to_psf_device(np.ones((7, 9), complex), G, [0.0])
+
+
+@pytest.mark.parametrize("dipole", ["isotropic", "z", (0.6, 2.0)])
+def test_device_vectorial_psf_matches_oracle(gpu, dipole):
+    from deconlib_amd.psf import (IndexMismatch, Optics, apply_aberrations, fft_coords, make_geometry, make_pupil,
+                                  pupil_to_vectorial_psf_device)
+
+    optics = Optics(wavelength=0.525, na=1.4, ni=1.515, ns=1.334)
+    g = make_geometry((32, 48), 0.085, optics)
+    pupil = apply_aberrations(make_pupil(g), g, optics, [IndexMismatch(10.0)])
+    z = fft_coords(6, 0.1)
+    for norm in (True, False):
+        got = pupil_to_vectorial_psf_device(pupil, g, optics, z, dipole, normalize=norm).cpu().numpy()
+        ref = OP.vectorial_psf(pupil, g.kz, g.cos_theta, g.sin_theta, g.phi, g.mask, optics.ni, optics.ns, z,
+                               dipole, norm)
+        assert rel_l2(got, ref) <= 1e-5, (dipole, norm, rel_l2(got, ref))

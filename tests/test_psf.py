@@ -71,3 +71,50 @@ def test_host_pupil_to_psf_matches_oracle(shape):
     got = pupil_to_psf(pupil, geom, z)
     ref = OP.pupil_to_psf(pupil, geom.kz, z)
     np.testing.assert_allclose(got, ref, rtol=1e-10, atol=1e-16)
+
+
+def _geom(ny, nx):
+    from deconlib_amd.psf import Optics, make_geometry
+
+    optics = Optics(wavelength=0.525, na=1.4, ni=1.515, ns=1.334)
+    return make_geometry((ny, nx), 0.085, optics), optics
+
+
+def _vec(g, o, pupil, z, dipole, normalize=False):
+    return OP.vectorial_psf(pupil, g.kz, g.cos_theta, g.sin_theta, g.phi, g.mask, o.ni, o.ns, z, dipole, normalize)
+
+
+def test_oracle_vectorial_dipole_relations():
+    """isotropic = mean of the axis dipoles; a fixed dipole along an axis is that axis."""
+    g, o = _geom(16, 20)
+    pupil = g.support_weight.astype(complex)
+    z = np.array([-0.3, 0.0, 0.2])
+    px, py, pz = (_vec(g, o, pupil, z, d) for d in "xyz")
+    np.testing.assert_allclose(_vec(g, o, pupil, z, "isotropic"), (px + py + pz) / 3.0, rtol=1e-12, atol=1e-18)
+    np.testing.assert_allclose(_vec(g, o, pupil, z, (0.0, 0.0)), pz, rtol=1e-12, atol=1e-18)
+    np.testing.assert_allclose(_vec(g, o, pupil, z, (np.pi / 2, 0.0)), px, rtol=1e-10, atol=1e-16)
+    np.testing.assert_allclose(_vec(g, o, pupil, z, (np.pi / 2, np.pi / 2)), py, rtol=1e-10, atol=1e-16)
+
+
+def test_oracle_vectorial_parseval():
+    """Each raw plane holds sum_c w_c sum |P_c|^2 / (ny nx), independent of z."""
+    g, o = _geom(12, 16)
+    pupil = g.support_weight.astype(complex)
+    f = OP.vectorial_factors(g.cos_theta, g.sin_theta, g.phi, g.mask, o.ni, o.ns)
+    apod = 1.0 / np.sqrt(np.maximum(g.cos_theta, np.finfo(float).eps)) * g.mask
+    energy = sum(np.sum(np.abs(pupil * apod * f[d, c]) ** 2) for d in range(3) for c in range(2)) / 3.0
+    psf = _vec(g, o, pupil, [-1.0, 0.0, 0.7], "isotropic")
+    for plane in psf:
+        assert abs(plane.sum() - energy / (12 * 16)) <= 1e-12 * energy
+
+
+@pytest.mark.parametrize("dipole", ["isotropic", "y", (0.7, 1.1)])
+def test_host_vectorial_matches_oracle(dipole):
+    from deconlib_amd.psf import IndexMismatch, apply_aberrations, fft_coords, make_pupil, pupil_to_vectorial_psf
+
+    g, o = _geom(16, 24)
+    pupil = apply_aberrations(make_pupil(g), g, o, [IndexMismatch(4.0)])
+    z = fft_coords(5, 0.1)
+    got = pupil_to_vectorial_psf(pupil, g, o, z, dipole)
+    ref = _vec(g, o, pupil, z, dipole, normalize=True)
+    np.testing.assert_allclose(got, ref, rtol=1e-10, atol=1e-16)
