@@ -248,9 +248,28 @@ def bench_psf_synthesis(torch, shape=SHAPE):
     t_host = time.perf_counter() - t0
     d = dev.cpu().numpy().astype(np.float64)
     err = float(np.linalg.norm(d - host) / np.linalg.norm(host))
-    return {"shape": list(shape), "gpu_seconds": t_dev, "host_seconds": t_host, "speedup": t_host / t_dev,
-            "rel_l2_vs_host": err,
-            "note": "end to end: pupil geometry on the host, uploads, synthesis, normalisation"}
+    # the vectorial PSF (isotropic dipole: six field channels), with index mismatch
+    from deconlib_amd.psf import (IndexMismatch, apply_aberrations, fft_coords, make_geometry, make_pupil,
+                                  pupil_to_vectorial_psf, pupil_to_vectorial_psf_device)
+
+    g = make_geometry(tuple(shape[1:]), LATERAL, CONFIG0_OPTICS)
+    pupil = apply_aberrations(make_pupil(g), g, CONFIG0_OPTICS, [IndexMismatch(10.0)])
+    z = fft_coords(shape[0], AXIAL)
+    pupil_to_vectorial_psf_device(pupil, g, CONFIG0_OPTICS, z[:2])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    vdev = pupil_to_vectorial_psf_device(pupil, g, CONFIG0_OPTICS, z)
+    torch.cuda.synchronize()
+    tv_dev = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    vhost = pupil_to_vectorial_psf(pupil, g, CONFIG0_OPTICS, z)
+    tv_host = time.perf_counter() - t0
+    verr = float(np.linalg.norm(vdev.cpu().numpy().astype(np.float64) - vhost) / np.linalg.norm(vhost))
+    return {"shape": list(shape),
+            "scalar": {"gpu_seconds": t_dev, "host_seconds": t_host, "speedup": t_host / t_dev, "rel_l2_vs_host": err},
+            "vectorial_isotropic": {"gpu_seconds": tv_dev, "host_seconds": tv_host, "speedup": tv_host / tv_dev,
+                                    "rel_l2_vs_host": verr},
+            "note": "end to end: pupil geometry and aberrations on the host, uploads, synthesis, normalisation"}
 
 
 def bench_serving(torch, n_jobs, rate, seed=1003):
