@@ -14,6 +14,7 @@ from __future__ import annotations
 import ctypes as C
 import json
 import os
+import sys
 from typing import Dict, Optional
 
 import torch
@@ -87,7 +88,8 @@ def autotune(op, reps: int = 10, candidates=(1, 2, 4, 8, 16, 32, 64), cache_file
             if verbose:
                 print(f"[autotune] pass {N.PASS_NAMES[p]}: " +
                       ", ".join(f"{_name(t)}:{v * 1e3:.1f}us" for t, v in sorted(timings.items())) +
-                      f" default {default * 1e3:.1f}us -> {_name(best[p]) if p in best else 'default'}")
+                      f" default {default * 1e3:.1f}us -> {_name(best[p]) if p in best else 'default'}",
+                      file=sys.stderr)
     _CACHE[key] = best
     if cache_file:
         data = {}
