@@ -2,8 +2,9 @@
 
 Drop-in for the hot path of ``deconlib.deconvolution``: the operator protocol
 and composition, ``LinearFFTConvolver`` / ``FFTConvolver``, ``Pad`` / ``Crop`` /
-``FractionalAreaDownsample``, ``Hessian3D``, Richardson–Lucy and PDHG with
-Hessian regularization, and the tiled path (``make_forward_model``,
+``FractionalAreaDownsample``, ``Hessian3D`` / ``Hessian2D``, Richardson–Lucy,
+PDHG with Hessian regularization, NLCG (``nlcg_with_operator``,
+``nlcg_solver``), and the tiled path (``make_forward_model``,
 ``plan_tiles`` / ``process_tiles``, ``richardson_lucy_solver``).  All arithmetic runs in hand-written HIP kernels for
 gfx950 (``deconlib_amd/_lib/libdeconhip.so``, C ABI in ``include/deconhip.h``);
 PyTorch only owns device buffers and streams.
@@ -12,7 +13,8 @@ PyTorch only owns device buffers and streams.
 from .base import DeconvolutionResult, MLXDeconvolutionResult
 from .composition import Compose, LinearOperator, compose
 from .core_operators import Crop, FractionalAreaDownsample, Pad
-from .linops import FFTConvolver, Hessian3D, LinearFFTConvolver, fast_padded_shape
+from .linops import FFTConvolver, Hessian2D, Hessian3D, LinearFFTConvolver, fast_padded_shape
+from .nlcg import NLCGResult, nlcg_solver, nlcg_with_operator
 from .pdhg import HessianRegularizer, solve_pdhg_mlx, solve_pdhg_with_operator
 from .rl import RLResult, poisson_i_divergence, richardson_lucy_solver, richardson_lucy_with_operator
 from .forward_model import ForwardModel, make_forward_model
@@ -24,6 +26,10 @@ __all__ = [
     "solve_pdhg_with_operator",
     "HessianRegularizer",
     "Hessian3D",
+    "Hessian2D",
+    "NLCGResult",
+    "nlcg_with_operator",
+    "nlcg_solver",
     "MLXDeconvolutionResult",
     "DeconvolutionResult",
     "RLResult",

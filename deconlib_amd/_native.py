@@ -21,7 +21,7 @@ if not _LIB_PATH.exists():
 
 lib = C.CDLL(str(_LIB_PATH))
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 DH_OK, DH_EINVAL, DH_EHIP, DH_EWORKSPACE, DH_EUNSUPPORTED = 0, -1, -2, -3, -4
 
@@ -101,6 +101,11 @@ class PdParams(C.Structure):
 
 PD_NSLOTS = 16
 
+# NLCG float64 scalar block slots (DH_NL_* in deconhip.h)
+NL_LAM, NL_NEWTON_OK, NL_DPHI1, NL_DPHI2, NL_PHI0, NL_PHI_TRIAL, NL_ACCEPT = 0, 1, 2, 3, 4, 5, 6
+NL_TRIAL_DATA, NL_TRIAL_LOSS, NL_MDATA, NL_LOSS, NL_IDIV, NL_SUMF, NL_RR = 7, 8, 9, 10, 11, 12, 13
+NL_GAMMA, NL_LAM_ACCEPTED, NL_A0, NL_NSLOTS = 14, 15, 16, 32
+
 _vp, _i64, _int, _f, _d = C.c_void_p, C.c_int64, C.c_int, C.c_float, C.c_double
 
 
@@ -151,6 +156,18 @@ SYMBOLS = {
     "dh_psf_plan_bind": (_int, _vp, _vp, _vp),
     "dh_psf_synth": (_int, _vp, _int, _vp, C.POINTER(_d), _vp, _vp, _int, _int, _vp, _vp, _i64, _vp, _vp),
     "dh_psf_plan_destroy": (_int, _vp),
+    "dh_nlcg_partials": (_i64,),
+    "dh_nlcg_init": (_int, _vp, _vp, _vp, _i64, _f, _vp),
+    "dh_nlcg_model": (_int, _vp, _vp, _vp, _i64, _f, _vp, _vp, _vp),
+    "dh_nlcg_begin_step": (_int, _vp, _vp, _vp, _vp, _i64, _vp, _int, _vp),
+    "dh_nlcg_reg_dots": (_int, _vp, _vp, _vp, _i64, _vp, _vp, _vp),
+    "dh_nlcg_newton": (_int, _vp, _vp, _vp, _vp, _i64, _f, _d, _d, _vp, _vp, _vp),
+    "dh_nlcg_trial": (_int, _vp, _vp, _vp, _vp, _i64, _f, _d, _d, _vp, _vp, _vp, _vp, _vp),
+    "dh_nlcg_update": (_int, _vp, _vp, _vp, _i64, _d, _vp, _vp, _vp),
+    "dh_nlcg_gradient": (_int, _vp, _vp, _vp, _f, _vp, _i64, _int, _d, _vp, _vp, _vp),
+    "dh_nlcg_direction": (_int, _vp, _vp, _i64, _vp, _vp),
+    "dh_nlcg_quad": (_int, _vp, _vp, _vp, _vp, _vp, _i64, _d, _vp, _vp, _vp),
+    "dh_nlcg_idiv": (_int, _vp, _vp, _i64, _d, _vp, _vp, _vp),
     "dh_timing_enable": (_int, _int),
     "dh_timing_read": (_int, C.POINTER(_d), C.POINTER(_i64), _int),
 }
@@ -165,7 +182,8 @@ if fns["dh_abi_version"]() != ABI_VERSION:
     )
 
 
-_VALUE_RETURNING = {"dh_reduce_partials", "dh_abi_version", "dh_pd_partials", "dh_conv_plan_get_tile"}
+_VALUE_RETURNING = {"dh_reduce_partials", "dh_abi_version", "dh_pd_partials", "dh_conv_plan_get_tile",
+                    "dh_nlcg_partials"}
 
 
 class NativeError(RuntimeError):

@@ -11,6 +11,7 @@
 #include "dh_fft.hpp"
 #include "dh_fftct.hpp"
 #include "dh_ops.hpp"
+#include "dh_nlcg.hpp"
 #include "dh_pdhg.hpp"
 
 namespace {
@@ -877,6 +878,81 @@ int dh_hessian_forward(const float* f, float* out, int nz, int ny, int nx, const
 int dh_hessian_adjoint(const float* y, float* out, int nz, int ny, int nx, const float* w, void* stream) {
   if (!y || !out || !w || !dims_ok(nz, ny, nx)) return fail(DH_EINVAL, "bad hessian_adjoint arguments");
   return hip_check(dh::hess_adjoint(y, out, nz, ny, nx, w, (hipStream_t)stream), "hessian_adjoint");
+}
+
+// ---- NLCG ----
+int64_t dh_nlcg_partials(void) { return dh::nl_partials_needed(); }
+
+int dh_nlcg_init(const float* f0, float* s, float* f, int64_t n, float floor_value, void* stream) {
+  if (!f0 || !s || !f || n < 1) return fail(DH_EINVAL, "bad nlcg_init arguments");
+  return hip_check(dh::nl_init(f0, s, f, n, floor_value, (hipStream_t)stream), "nlcg_init");
+}
+
+int dh_nlcg_model(const float* m, const float* g, float* w, int64_t n, float eps, double* scalars,
+                  double* partials, void* stream) {
+  if (!m || !g || !scalars || !partials || n < 1) return fail(DH_EINVAL, "bad nlcg_model arguments");
+  return hip_check(dh::nl_model(m, g, w, n, eps, scalars, partials, (hipStream_t)stream), "nlcg_model");
+}
+
+int dh_nlcg_begin_step(const float* s, const float* d, float* sd, float* dd, int64_t n, double* scalars,
+                       int newton_iters, void* stream) {
+  if (!s || !d || !sd || !dd || !scalars || n < 1 || newton_iters < 0)
+    return fail(DH_EINVAL, "bad nlcg_begin_step arguments");
+  return hip_check(dh::nl_begin_step(s, d, sd, dd, n, scalars, newton_iters, (hipStream_t)stream),
+                   "nlcg_begin_step");
+}
+
+int dh_nlcg_reg_dots(const float* cf, const float* csd, const float* cdd, int64_t n, double* scalars,
+                     double* partials, void* stream) {
+  if (!cf || !csd || !cdd || !scalars || !partials || n < 1) return fail(DH_EINVAL, "bad nlcg_reg_dots arguments");
+  return hip_check(dh::nl_reg_dots(cf, csd, cdd, n, scalars, partials, (hipStream_t)stream), "nlcg_reg_dots");
+}
+
+int dh_nlcg_newton(const float* m, const float* ksd, const float* kdd, const float* g, int64_t n, float background,
+                   double beta, double eps, double* scalars, double* partials, void* stream) {
+  if (!m || !ksd || !kdd || !g || !scalars || !partials || n < 1) return fail(DH_EINVAL, "bad nlcg_newton arguments");
+  return hip_check(dh::nl_newton(m, ksd, kdd, g, n, background, beta, eps, scalars, partials, (hipStream_t)stream),
+                   "nlcg_newton");
+}
+
+int dh_nlcg_trial(const float* m, const float* ksd, const float* kdd, const float* g, int64_t n, float background,
+                  double beta, double eps, double* scalars, float* m_new, float* w, double* partials, void* stream) {
+  if (!m || !ksd || !kdd || !g || !scalars || !m_new || !w || !partials || n < 1)
+    return fail(DH_EINVAL, "bad nlcg_trial arguments");
+  return hip_check(
+      dh::nl_trial(m, ksd, kdd, g, n, background, beta, eps, scalars, m_new, w, partials, (hipStream_t)stream),
+      "nlcg_trial");
+}
+
+int dh_nlcg_update(float* s, const float* d, float* f, int64_t n, double eps, double* scalars, double* partials,
+                   void* stream) {
+  if (!s || !d || !f || !scalars || !partials || n < 1) return fail(DH_EINVAL, "bad nlcg_update arguments");
+  return hip_check(dh::nl_update(s, d, f, n, eps, scalars, partials, (hipStream_t)stream), "nlcg_update");
+}
+
+int dh_nlcg_gradient(const float* s, const float* ktw, const float* ctcf, float c4b, float* r, int64_t n, int mode,
+                     double eps, double* scalars, double* partials, void* stream) {
+  if (!s || !ktw || !r || !scalars || !partials || n < 1 || mode < 0 || mode > 2)
+    return fail(DH_EINVAL, "bad nlcg_gradient arguments");
+  return hip_check(dh::nl_gradient(s, ktw, ctcf, c4b, r, n, mode, eps, scalars, partials, (hipStream_t)stream),
+                   "nlcg_gradient");
+}
+
+int dh_nlcg_direction(const float* r, float* d, int64_t n, const double* scalars, void* stream) {
+  if (!r || !d || !scalars || n < 1) return fail(DH_EINVAL, "bad nlcg_direction arguments");
+  return hip_check(dh::nl_direction(r, d, n, scalars, (hipStream_t)stream), "nlcg_direction");
+}
+
+int dh_nlcg_quad(const float* g, const float* m, const float* ksd, const float* v, const float* d, int64_t n,
+                 double eps, double* out2, double* partials, void* stream) {
+  if ((g && (!m || !ksd)) || !v || !d || !out2 || !partials || n < 1) return fail(DH_EINVAL, "bad nlcg_quad arguments");
+  return hip_check(dh::nl_quad(g, m, ksd, v, d, n, eps, out2, partials, (hipStream_t)stream), "nlcg_quad");
+}
+
+int dh_nlcg_idiv(const float* fp, const float* fc, int64_t n, double eps, double* out2, double* partials,
+                 void* stream) {
+  if (!fp || !fc || !out2 || !partials || n < 1) return fail(DH_EINVAL, "bad nlcg_idiv arguments");
+  return hip_check(dh::nl_idiv(fp, fc, n, eps, out2, partials, (hipStream_t)stream), "nlcg_idiv");
 }
 
 int dh_timing_enable(int on) {

@@ -100,6 +100,45 @@ class Hessian3D:
         return self.forward(f)
 
 
+class Hessian2D:
+    """[H_yy, H_xx, sqrt2 H_xy] with Neumann boundaries (linops_mlx.py:153-179).
+
+    Runs the Hessian3D stencil kernels on a single plane, where the z terms
+    vanish (replicate padding of one sample), with weights [0, 1, 1, 0, 0, sqrt2].
+    """
+
+    operator_norm_sq = 48.0
+
+    def __init__(self):
+        self._w = np.array([0.0, 1.0, 1.0, 0.0, 0.0, SQRT2], dtype=np.float32)
+
+    def _wp(self):
+        return self._w.ctypes.data_as(C.POINTER(C.c_float))
+
+    def forward(self, f) -> torch.Tensor:
+        f = as_f32(f)
+        if f.dim() != 2:
+            raise ValueError("Hessian2D expects a 2-D image")
+        ny, nx = f.shape
+        h6 = empty_f32((6, 1, ny, nx))
+        N.call("dh_hessian_forward", f.data_ptr(), h6.data_ptr(), 1, ny, nx, self._wp(), N.stream_ptr())
+        return h6[[1, 2, 5], 0].contiguous()
+
+    def adjoint(self, H) -> torch.Tensor:
+        H = as_f32(H)
+        if H.dim() != 3 or H.shape[0] != 3:
+            raise ValueError("Hessian2D.adjoint expects a (3, H, W) array")
+        ny, nx = H.shape[1:]
+        h6 = torch.zeros((6, 1, ny, nx), dtype=torch.float32, device=H.device)
+        h6[[1, 2, 5], 0] = H
+        out = empty_f32((ny, nx))
+        N.call("dh_hessian_adjoint", h6.data_ptr(), out.data_ptr(), 1, ny, nx, self._wp(), N.stream_ptr())
+        return out
+
+    def __call__(self, f):
+        return self.forward(f)
+
+
 class _ConvPlan:
     """Owns one native plan and its device buffers (twiddles, OTF, workspace)."""
 

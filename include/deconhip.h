@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define DH_ABI_VERSION 5
+#define DH_ABI_VERSION 6
 
 #define DH_OK 0
 #define DH_EINVAL (-1)
@@ -277,6 +277,77 @@ int dh_pd_scalars(double* scalars, double* hist, int k, const dh_pd_params* para
 /* Hessian3D.forward f -> out (6, nz, ny, nx) and adjoint y -> out (linops_mlx.py:224-259). */
 int dh_hessian_forward(const float* f, float* out, int nz, int ny, int nx, const float* w, void* stream);
 int dh_hessian_adjoint(const float* y, float* out, int nz, int ny, int nx, const float* w, void* stream);
+
+/* ---- NLCG: Poisson ML by nonlinear conjugate gradients (nlcg_mlx.py:84-529) ----
+ * f = s^2, m = K f + b.  phi = sum m - sum g ln max(m, eps) + beta ||C f||^2.
+ * Vectors are float32 device arrays of n elements (domain: s, f, d, r, sd, dd, KTw,
+ * CtCf; data: m, g, w, Ksd, Kdd); the convolutions are dh_conv_apply calls made
+ * in between.  Scalars live in a caller-owned device block of DH_NL_NSLOTS
+ * doubles, zeroed once; partials is a device array of dh_nlcg_partials()
+ * doubles.  No call synchronises: the host reads the block when it decides. */
+#define DH_NL_LAM 0          /* step length lambda of the current line search   */
+#define DH_NL_NEWTON_OK 1    /* 1 while the Newton-Raphson refinement continues  */
+#define DH_NL_DPHI1 2        /* last dphi/dlambda                                */
+#define DH_NL_DPHI2 3        /* last d2phi/dlambda2                              */
+#define DH_NL_PHI0 4         /* phi at the current iterate                       */
+#define DH_NL_PHI_TRIAL 5    /* phi at s + lambda d                              */
+#define DH_NL_ACCEPT 6       /* 1 if the trial decreases phi (not phi >= phi0)   */
+#define DH_NL_TRIAL_DATA 7   /* data part of phi at the trial model              */
+#define DH_NL_TRIAL_LOSS 8   /* sum of the Poisson I-divergence of the trial     */
+#define DH_NL_MDATA 9        /* data part of phi at the current model            */
+#define DH_NL_LOSS 10        /* sum of the Poisson I-divergence of the current m */
+#define DH_NL_IDIV 11        /* I-divergence between the last two iterates       */
+#define DH_NL_SUMF 12        /* sum f of the current iterate                     */
+#define DH_NL_RR 13          /* sum r^2 of the current residual                  */
+#define DH_NL_GAMMA 14       /* Fletcher-Reeves gamma of the last direction      */
+#define DH_NL_LAM_ACCEPTED 15 /* lambda of the last accepted step                */
+#define DH_NL_A0 16          /* regularizer dots: sum Cf.Cf, Cf.Csd, Cf.Cdd,     */
+#define DH_NL_NSLOTS 32      /*   Csd.Csd, Csd.Cdd, Cdd.Cdd in slots A0..A0+5     */
+
+/* Partials (doubles) every dh_nlcg_* reduction may use. */
+int64_t dh_nlcg_partials(void);
+/* s = sqrt(max(f0, floor)), f = s * s; f0 may alias s  (nlcg_mlx.py:407-412, 88-93). */
+int dh_nlcg_init(const float* f0, float* s, float* f, int64_t n, float floor_value, void* stream);
+/* w = 1 - g / max(m, eps); MDATA = sum m - sum g ln max(m, eps); LOSS = the
+ * I-divergence sum of m against g  (nlcg_mlx.py:95-107, rl_mlx.py:58-80). */
+int dh_nlcg_model(const float* m, const float* g, float* w, int64_t n, float eps, double* scalars,
+                  double* partials, void* stream);
+/* Starts a line search along d: sd = s * d, dd = d * d, LAM = 0, NEWTON_OK =
+ * (newton_iters > 0)  (nlcg_mlx.py:253-262). */
+int dh_nlcg_begin_step(const float* s, const float* d, float* sd, float* dd, int64_t n, double* scalars,
+                       int newton_iters, void* stream);
+/* A0..A5 = the six dot products of (Cf, C(sd), C(dd)), arrays of n elements
+ * each (nlcg_mlx.py:257-260, 273-281, with C f(lambda) expanded in lambda). */
+int dh_nlcg_reg_dots(const float* cf, const float* csd, const float* cdd, int64_t n, double* scalars,
+                     double* partials, void* stream);
+/* One Newton-Raphson refinement of LAM on the exact quadratic-in-lambda
+ * model K f(lambda) = (m - b) + 2 lambda Ksd + lambda^2 Kdd; a no-op once
+ * NEWTON_OK is 0  (nlcg_mlx.py:263-288). */
+int dh_nlcg_newton(const float* m, const float* ksd, const float* kdd, const float* g, int64_t n, float background,
+                   double beta, double eps, double* scalars, double* partials, void* stream);
+/* Trial model m_new = (m - b) + 2 LAM Ksd + LAM^2 Kdd + b, w = 1 - g / max(m_new, eps),
+ * PHI0, PHI_TRIAL, ACCEPT, TRIAL_DATA, TRIAL_LOSS  (nlcg_mlx.py:290, 446-457). */
+int dh_nlcg_trial(const float* m, const float* ksd, const float* kdd, const float* g, int64_t n, float background,
+                  double beta, double eps, double* scalars, float* m_new, float* w, double* partials, void* stream);
+/* Accepts the trial: s += LAM d, f = s * s; IDIV (between f before and after),
+ * SUMF; MDATA, LOSS from the trial  (nlcg_mlx.py:450-451, 470-471, 294-314, 498). */
+int dh_nlcg_update(float* s, const float* d, float* f, int64_t n, double eps, double* scalars, double* partials,
+                   void* stream);
+/* r = -(2 s KTw + c4b s CtCf) (ctcf may be NULL), rr_new = sum r^2.  mode 0: RR =
+ * rr_new (first gradient); 1: GAMMA = rr_new / RR if RR > eps else 0, RR = rr_new;
+ * 2: as 1 with GAMMA = 0 (restart)  (nlcg_mlx.py:104-113, 417-419, 472-479). */
+int dh_nlcg_gradient(const float* s, const float* ktw, const float* ctcf, float c4b, float* r, int64_t n, int mode,
+                     double eps, double* scalars, double* partials, void* stream);
+/* d = r + GAMMA d  (nlcg_mlx.py:480). */
+int dh_nlcg_direction(const float* r, float* d, int64_t n, const double* scalars, void* stream);
+/* out2[0] = sum (g / max(m*m, eps)) Ksd^2 (skipped when g is NULL), out2[1] = sum v d^2
+ * (the Hessian quadratic form's sums, nlcg_mlx.py:155-192). */
+int dh_nlcg_quad(const float* g, const float* m, const float* ksd, const float* v, const float* d, int64_t n,
+                 double eps, double* out2, double* partials, void* stream);
+/* out2[0] = sum fp (ln max(fp, eps) - ln max(fc, eps)) - fp + fc, out2[1] = sum fc
+ * (i_divergence_between, nlcg_mlx.py:294-314). */
+int dh_nlcg_idiv(const float* fp, const float* fc, int64_t n, double eps, double* out2, double* partials,
+                 void* stream);
 
 /* ---- measurement ---- */
 /* Pass ids for dh_timing_read. */
