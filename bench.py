@@ -116,6 +116,65 @@ def bench_pdhg(torch, N, obs, blur, steps, warmup):
     }
 
 
+def nlcg_iteration_bytes(S, P):
+    """Algorithmic HBM bytes of one NLCG iteration: three plain convolutions (K(s d),
+    K(d d), K^T w) plus the elementwise passes, in float32 words per voxel: line-search
+    set-up 4 (s, d in; s d, d d out), three Newton passes 4 each (m, Ksd, Kdd, g), trial
+    6 (those four in; m_new, w out), update 5 (s, d, f in; s, f out), gradient 3 (s, KTw
+    in; r out), direction 3 (r, d in; d out): 33 words."""
+    return 3 * sum(conv_pass_bytes(S, P).values()) + 33 * 4 * int(np.prod(S))
+
+
+def bench_nlcg(torch, obs, blur, steps, warmup):
+    """NLCG iterations (3 Newton refinements, no regulariser) on the same volume and
+    operator: each is the solver's loop body, line search, one host read of the accept
+    flag, accepted step, gradient and direction."""
+    from deconlib_amd import _native as N
+    from deconlib_amd.nlcg import _NLCGState
+
+    st = _NLCGState(obs, blur, 0.0, None, 0.0, None, 3)
+    fallbacks = 0
+
+    def iteration():
+        nonlocal fallbacks
+        st.line_search()
+        if st.S.read()[N.NL_ACCEPT] == 0.0:
+            fallbacks += 1
+            st.fall_back_to_steepest_descent()
+            st.line_search()
+            if st.S.read()[N.NL_ACCEPT] == 0.0:
+                return False
+        st.accept(restart=False)
+        return True
+
+    for _ in range(warmup):
+        iteration()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done = 0
+    for _ in range(steps):
+        if not iteration():
+            break
+        done += 1
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    vox = int(np.prod(obs.shape))
+    it_bytes = nlcg_iteration_bytes(obs.shape, blur.canvas_shape)
+    v = st.S.read()
+    return {
+        "metric": "voxel-iters/s (NLCG, 3 Newton refinements) at 64x512x512 fp32",
+        "value": vox * done / dt,
+        "ms_per_step": dt * 1e3 / max(done, 1),
+        "steps": done,
+        "warmup": warmup,
+        "steepest_descent_fallbacks": fallbacks,
+        "algorithmic_bytes_per_iteration": it_bytes,
+        "effective_gbs": it_bytes * done / dt / 1e9,
+        "mean_idiv_last": v[N.NL_LOSS] / vox,
+        "finite": bool(torch.isfinite(st.f).all().item()),
+    }
+
+
 def bench_host_to_host(torch, psf, obs, iters=50):
     """One whole RL solve from host arrays to a host array: operator construction (OTF
     build), PCIe upload, `iters` iterations, PCIe download.  Reported beside `value`
@@ -406,6 +465,8 @@ def main():
     ok = bool(torch.isfinite(st.x).all().item())
     pdhg = bench_pdhg(torch, N, obs, blur, args.steps, args.warmup)
     log(f"pdhg: {json.dumps(pdhg)}")
+    nlcg = bench_nlcg(torch, obs, blur, args.steps, args.warmup)
+    log(f"nlcg: {json.dumps(nlcg)}")
     host = small = tiled = psf_synth = None
     if not args.no_extras:
         host = bench_host_to_host(torch, psf, obs)
@@ -473,6 +534,7 @@ def main():
         "iteration": {"algorithmic_bytes": it_bytes,
                       "hbm_frac": it_bytes / (dt / args.steps) / (PEAK_HBM_GBS * 1e9)},
         "pdhg": pdhg,
+        "nlcg": nlcg,
         "host_to_host": host,
         "batched_small": small,
         "tiled": tiled,
