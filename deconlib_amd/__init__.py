@@ -4,7 +4,8 @@ Drop-in for the hot path of ``deconlib.deconvolution``: the operator protocol
 and composition, ``LinearFFTConvolver`` / ``FFTConvolver``, ``Pad`` / ``Crop`` /
 ``FractionalAreaDownsample``, ``Hessian3D`` / ``Hessian2D``, Richardson–Lucy,
 PDHG with Hessian regularization, NLCG (``nlcg_with_operator``,
-``nlcg_solver``), and the tiled path (``make_forward_model``,
+``nlcg_solver``), ER-Decon (``erdecon_with_operator``, ``erdecon_solver``), and the
+tiled path (``make_forward_model``,
 ``plan_tiles`` / ``process_tiles``, ``richardson_lucy_solver``).  All arithmetic runs in hand-written HIP kernels for
 gfx950 (``deconlib_amd/_lib/libdeconhip.so``, C ABI in ``include/deconhip.h``);
 PyTorch only owns device buffers and streams.
@@ -15,6 +16,7 @@ from .composition import Compose, LinearOperator, compose
 from .core_operators import Crop, FractionalAreaDownsample, Pad
 from .linops import FFTConvolver, Hessian2D, Hessian3D, LinearFFTConvolver, fast_padded_shape
 from .nlcg import NLCGResult, nlcg_solver, nlcg_with_operator
+from .erdecon import ERDeconResult, erdecon_solver, erdecon_with_operator
 from .pdhg import HessianRegularizer, solve_pdhg_mlx, solve_pdhg_with_operator
 from .rl import RLResult, poisson_i_divergence, richardson_lucy_solver, richardson_lucy_with_operator
 from .forward_model import ForwardModel, make_forward_model
@@ -30,6 +32,9 @@ __all__ = [
     "NLCGResult",
     "nlcg_with_operator",
     "nlcg_solver",
+    "ERDeconResult",
+    "erdecon_with_operator",
+    "erdecon_solver",
     "MLXDeconvolutionResult",
     "DeconvolutionResult",
     "RLResult",

@@ -106,6 +106,11 @@ NL_LAM, NL_NEWTON_OK, NL_DPHI1, NL_DPHI2, NL_PHI0, NL_PHI_TRIAL, NL_ACCEPT = 0, 
 NL_TRIAL_DATA, NL_TRIAL_LOSS, NL_MDATA, NL_LOSS, NL_IDIV, NL_SUMF, NL_RR = 7, 8, 9, 10, 11, 12, 13
 NL_GAMMA, NL_LAM_ACCEPTED, NL_A0, NL_NSLOTS = 14, 15, 16, 32
 
+# ER-Decon float64 scalar block slots and elementwise ops (DH_ER_* in deconhip.h)
+ER_RS, ER_RS_THR, ER_DENOM, ER_BETA, ER_STOP, ER_GG, ER_DOT = 0, 1, 2, 3, 4, 5, 6
+ER_DATA, ER_IDIV, ER_LOGQ, ER_Q, ER_NSLOTS = 7, 8, 9, 10, 16
+ER_EW_DIV, ER_EW_SQUARE_SCALE, ER_EW_AXPY_SQUARE, ER_EW_NEG, ER_EW_MUL, ER_EW_SCALE = 0, 1, 2, 3, 4, 5
+
 _vp, _i64, _int, _f, _d = C.c_void_p, C.c_int64, C.c_int, C.c_float, C.c_double
 
 
@@ -168,6 +173,17 @@ SYMBOLS = {
     "dh_nlcg_direction": (_int, _vp, _vp, _i64, _vp, _vp),
     "dh_nlcg_quad": (_int, _vp, _vp, _vp, _vp, _vp, _i64, _d, _vp, _vp, _vp),
     "dh_nlcg_idiv": (_int, _vp, _vp, _i64, _d, _vp, _vp, _vp),
+    "dh_er_partials": (_i64,),
+    "dh_er_hess_weights": (_int, _vp, _int, _int, _int, C.POINTER(_f), _f, _f, _f, _f, _vp, _vp, _vp, _vp, _vp),
+    "dh_er_weights_stacked": (_int, _vp, _int, _vp, _i64, _f, _f, _f, _f, _vp, _vp, _vp, _vp, _vp),
+    "dh_er_hess_apply_w": (_int, _vp, _vp, _int, _int, _int, C.POINTER(_f), _vp, _vp),
+    "dh_er_scale_stacked": (_int, _vp, _vp, _int, _i64, _vp, _vp),
+    "dh_er_data": (_int, _vp, _vp, _i64, _int, _vp, _vp, _vp, _vp, _vp),
+    "dh_er_gradient": (_int, _vp, _vp, _vp, _vp, _vp, _f, _i64, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp),
+    "dh_er_cg_ap": (_int, _vp, _vp, _vp, _vp, _vp, _f, _vp, _vp, _i64, _vp, _vp, _vp),
+    "dh_er_cg_update": (_int, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp),
+    "dh_er_dot": (_int, _vp, _vp, _i64, _int, _vp, _vp, _vp),
+    "dh_er_ew": (_int, _int, _vp, _vp, _f, _vp, _vp, _i64, _vp),
     "dh_timing_enable": (_int, _int),
     "dh_timing_read": (_int, C.POINTER(_d), C.POINTER(_i64), _int),
 }
@@ -183,7 +199,7 @@ if fns["dh_abi_version"]() != ABI_VERSION:
 
 
 _VALUE_RETURNING = {"dh_reduce_partials", "dh_abi_version", "dh_pd_partials", "dh_conv_plan_get_tile",
-                    "dh_nlcg_partials"}
+                    "dh_nlcg_partials", "dh_er_partials"}
 
 
 class NativeError(RuntimeError):

@@ -11,6 +11,7 @@
 #include "dh_fft.hpp"
 #include "dh_fftct.hpp"
 #include "dh_ops.hpp"
+#include "dh_erdecon.hpp"
 #include "dh_nlcg.hpp"
 #include "dh_pdhg.hpp"
 
@@ -953,6 +954,90 @@ int dh_nlcg_idiv(const float* fp, const float* fc, int64_t n, double eps, double
                  void* stream) {
   if (!fp || !fc || !out2 || !partials || n < 1) return fail(DH_EINVAL, "bad nlcg_idiv arguments");
   return hip_check(dh::nl_idiv(fp, fc, n, eps, out2, partials, (hipStream_t)stream), "nlcg_idiv");
+}
+
+// ---- ER-Decon ----
+int64_t dh_er_partials(void) { return dh::er_partials_needed(); }
+
+int dh_er_hess_weights(const float* g, int nz, int ny, int nx, const float* w6, float lam, float eps_reg,
+                       float w_floor, float intensity_weight, float* w_out, float* u_out, double* scalars,
+                       double* partials, void* stream) {
+  if (!g || !w6 || !scalars || !partials || !dims_ok(nz, ny, nx) || (w_out && !u_out) || !(eps_reg > 0.0f))
+    return fail(DH_EINVAL, "bad er_hess_weights arguments");
+  return hip_check(dh::er_hess_weights(g, nz, ny, nx, w6, lam, eps_reg, w_floor, intensity_weight, w_out, u_out,
+                                       scalars, partials, (hipStream_t)stream),
+                   "er_hess_weights");
+}
+
+int dh_er_weights_stacked(const float* hg, int nc, const float* g, int64_t n, float lam, float eps_reg,
+                          float w_floor, float intensity_weight, float* w_out, float* u_out, double* scalars,
+                          double* partials, void* stream) {
+  if (!hg || !g || nc < 1 || n < 1 || !scalars || !partials || (w_out && !u_out) || !(eps_reg > 0.0f))
+    return fail(DH_EINVAL, "bad er_weights_stacked arguments");
+  return hip_check(dh::er_weights_stacked(hg, nc, g, n, lam, eps_reg, w_floor, intensity_weight, w_out, u_out,
+                                          scalars, partials, (hipStream_t)stream),
+                   "er_weights_stacked");
+}
+
+int dh_er_hess_apply_w(const float* v, const float* w, int nz, int ny, int nx, const float* w6, float* u_out,
+                       void* stream) {
+  if (!v || !w || !w6 || !u_out || !dims_ok(nz, ny, nx)) return fail(DH_EINVAL, "bad er_hess_apply_w arguments");
+  return hip_check(dh::er_hess_apply_w(v, w, nz, ny, nx, w6, u_out, (hipStream_t)stream), "er_hess_apply_w");
+}
+
+int dh_er_scale_stacked(const float* x, const float* w, int nc, int64_t n, float* out, void* stream) {
+  if (!x || !w || !out || nc < 1 || n < 1) return fail(DH_EINVAL, "bad er_scale_stacked arguments");
+  return hip_check(dh::er_scale_stacked(x, w, nc, n, out, (hipStream_t)stream), "er_scale_stacked");
+}
+
+int dh_er_data(const float* m, const float* f, int64_t n, int poisson, float* score, float* hessw,
+               double* scalars, double* partials, void* stream) {
+  if (!m || !f || n < 1 || !scalars || !partials || (score && poisson && !hessw))
+    return fail(DH_EINVAL, "bad er_data arguments");
+  return hip_check(dh::er_data(m, f, n, poisson ? 1 : 0, score, hessw, scalars, partials, (hipStream_t)stream),
+                   "er_data");
+}
+
+int dh_er_gradient(const float* s, const float* dg, const float* rh, const float* w, const float* g,
+                   float intensity_weight, int64_t n, double cg_tol, float* grad, float* x, float* r, float* p,
+                   float* sv, double* scalars, double* partials, void* stream) {
+  if (!s || !dg || !rh || !grad || !x || !r || !p || !sv || !scalars || !partials || n < 1 ||
+      (intensity_weight > 0.0f && (!w || !g)))
+    return fail(DH_EINVAL, "bad er_gradient arguments");
+  return hip_check(dh::er_gradient(s, dg, rh, w, g, intensity_weight, n, cg_tol, grad, x, r, p, sv, scalars, partials,
+                                   (hipStream_t)stream),
+                   "er_gradient");
+}
+
+int dh_er_cg_ap(const float* s, const float* kt, const float* htu, const float* w, const float* sv,
+                float intensity_weight, const float* p, float* ap, int64_t n, double* scalars, double* partials,
+                void* stream) {
+  if (!s || !kt || !htu || !p || !ap || !scalars || !partials || n < 1 || (intensity_weight > 0.0f && (!w || !sv)))
+    return fail(DH_EINVAL, "bad er_cg_ap arguments");
+  return hip_check(
+      dh::er_cg_ap(s, kt, htu, w, sv, intensity_weight, p, ap, n, scalars, partials, (hipStream_t)stream),
+      "er_cg_ap");
+}
+
+int dh_er_cg_update(float* x, float* r, float* p, const float* ap, const float* s, float* sv, int64_t n,
+                    double* scalars, double* partials, void* stream) {
+  if (!x || !r || !p || !ap || !s || !sv || !scalars || !partials || n < 1)
+    return fail(DH_EINVAL, "bad er_cg_update arguments");
+  return hip_check(dh::er_cg_update(x, r, p, ap, s, sv, n, scalars, partials, (hipStream_t)stream), "er_cg_update");
+}
+
+int dh_er_dot(const float* a, const float* b, int64_t n, int slot, double* scalars, double* partials, void* stream) {
+  if (!a || !b || n < 1 || slot < 0 || slot >= DH_ER_NSLOTS || !scalars || !partials)
+    return fail(DH_EINVAL, "bad er_dot arguments");
+  return hip_check(dh::er_dot(a, b, n, slot, scalars, partials, (hipStream_t)stream), "er_dot");
+}
+
+int dh_er_ew(int op, const float* a, const float* b, float alpha, float* out, float* out2, int64_t n, void* stream) {
+  const bool needs_b = op == DH_ER_EW_AXPY_SQUARE || op == DH_ER_EW_MUL;
+  if (op < DH_ER_EW_DIV || op > DH_ER_EW_SCALE || !a || !out || n < 1 || (needs_b && !b) ||
+      (op == DH_ER_EW_AXPY_SQUARE && !out2))
+    return fail(DH_EINVAL, "bad er_ew arguments");
+  return hip_check(dh::er_ew(op, a, b, alpha, out, out2, n, (hipStream_t)stream), "er_ew");
 }
 
 int dh_timing_enable(int on) {

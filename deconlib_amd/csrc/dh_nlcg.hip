@@ -12,11 +12,12 @@
 #include <math.h>
 #include <stdint.h>
 
+#include "dh_blocksum.hpp"
 #include "dh_nlcg.hpp"
 
 namespace dh {
 
-constexpr int kNlThreads = 256;  // 4 waves; the block sums assume exactly this
+constexpr int kNlThreads = kSumThreads;  // 4 waves; the block sums assume exactly this
 constexpr int kNlBlocks = 1024;
 constexpr int kNlMaxSums = 6;
 
@@ -31,52 +32,6 @@ static int nl_blocks(int64_t n) {
   int64_t b = (n + kNlThreads - 1) / kNlThreads;
   if (b > kNlBlocks) b = kNlBlocks;
   return (int)(b < 1 ? 1 : b);
-}
-
-// Per-workgroup sums of NV values into part[blockIdx.x * NV + j].
-template <int NV>
-__device__ __forceinline__ void block_partials(double (&v)[NV], double* __restrict__ part) {
-  __shared__ double sh[NV][kNlThreads / 64];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-  for (int j = 0; j < NV; ++j) {
-    double x = v[j];
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
-    if (lane == 0) sh[j][w] = x;
-  }
-  __syncthreads();
-  if (threadIdx.x < NV) {
-    double r = 0.0;
-#pragma unroll
-    for (int q = 0; q < kNlThreads / 64; ++q) r += sh[threadIdx.x][q];
-    part[(int64_t)blockIdx.x * NV + threadIdx.x] = r;
-  }
-}
-
-// Sums the NV partial columns [block][NV] in a fixed order; the result is
-// valid in every thread of the (single, kNlThreads-wide) finishing workgroup.
-template <int NV>
-__device__ __forceinline__ void sum_partials(const double* __restrict__ part, int nb, double (&out)[NV]) {
-  __shared__ double sh[NV][kNlThreads / 64];
-  __shared__ double res[NV];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-  for (int j = 0; j < NV; ++j) {
-    double x = 0.0;
-    for (int b = threadIdx.x; b < nb; b += kNlThreads) x += part[(int64_t)b * NV + j];
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
-    if (lane == 0) sh[j][w] = x;
-  }
-  __syncthreads();
-  if (threadIdx.x < NV) {
-    double r = 0.0;
-#pragma unroll
-    for (int q = 0; q < kNlThreads / 64; ++q) r += sh[threadIdx.x][q];
-    res[threadIdx.x] = r;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < NV; ++j) out[j] = res[j];
 }
 
 // Poisson I-divergence term (rl_mlx.py:58-80), eps 1e-6.

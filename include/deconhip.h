@@ -349,6 +349,75 @@ int dh_nlcg_quad(const float* g, const float* m, const float* ksd, const float* 
 int dh_nlcg_idiv(const float* fp, const float* fc, int64_t n, double eps, double* out2, double* partials,
                  void* stream);
 
+/* ---- ER-Decon: Hessian-log regularised Gauss-Newton-CG (erdecon_mlx.py:144-692) ----
+ * g = s^2, m = K g + b, q = sum_i (H_i g)^2 [+ iw g^2], w = lam / (eps + q) [+ w_floor].
+ * Domain vectors have n elements, stacked regulariser responses (nc, n); the
+ * convolutions and H^T (dh_hessian_adjoint or the operator's own) are made in between.
+ * Scalars live in a caller-owned device block of DH_ER_NSLOTS doubles; partials is a
+ * device array of dh_er_partials() doubles.  No call synchronises. */
+#define DH_ER_RS 0        /* CG: sum r^2                                       */
+#define DH_ER_RS_THR 1    /* CG: tol^2 * sum r0^2                              */
+#define DH_ER_DENOM 2     /* CG: sum p (H_GN p)                                */
+#define DH_ER_BETA 3      /* CG: rs_new / rs                                   */
+#define DH_ER_STOP 4      /* CG: 1 once the solve has ended                    */
+#define DH_ER_GG 5        /* sum grad^2                                        */
+#define DH_ER_DOT 6       /* a dh_er_dot result slot                           */
+#define DH_ER_DATA 7      /* data misfit sum                                   */
+#define DH_ER_IDIV 8      /* Poisson I-divergence sum of m against f           */
+#define DH_ER_LOGQ 9      /* sum ln(eps + q)                                   */
+#define DH_ER_Q 10        /* sum q                                             */
+#define DH_ER_NSLOTS 16
+/* dh_er_ew operations (out2 only for AXPY_SQUARE, b only where named) */
+#define DH_ER_EW_DIV 0           /* out = a / alpha                    */
+#define DH_ER_EW_SQUARE_SCALE 1  /* out = (a * a) * alpha              */
+#define DH_ER_EW_AXPY_SQUARE 2   /* out = a + alpha b, out2 = out^2    */
+#define DH_ER_EW_NEG 3           /* out = -a                           */
+#define DH_ER_EW_MUL 4           /* out = a * b                        */
+#define DH_ER_EW_SCALE 5         /* out = alpha * a                    */
+
+int64_t dh_er_partials(void);
+/* Hessian3D (weights w6 = [r^2, 1, 1, r sqrt2, r sqrt2, sqrt2]; Hessian2D = one plane with
+ * [0, 1, 1, 0, 0, sqrt2]) of g, fused with _weights (erdecon_mlx.py:144-214): LOGQ = sum
+ * ln(eps + q), Q = sum q; with w_out (n) also w and u_out (6, n) = w * H g, the operand of
+ * H^T in the gradient (erdecon_mlx.py:301-302). */
+int dh_er_hess_weights(const float* g, int nz, int ny, int nx, const float* w6, float lam, float eps_reg,
+                       float w_floor, float intensity_weight, float* w_out, float* u_out, double* scalars,
+                       double* partials, void* stream);
+/* The same from any operator's stacked responses hg (nc, n); u_out may alias hg. */
+int dh_er_weights_stacked(const float* hg, int nc, const float* g, int64_t n, float lam, float eps_reg,
+                          float w_floor, float intensity_weight, float* w_out, float* u_out, double* scalars,
+                          double* partials, void* stream);
+/* u_out (6, n) = w * H(v) (the regulariser half of erdecon_gn_hvp, erdecon_mlx.py:339-341). */
+int dh_er_hess_apply_w(const float* v, const float* w, int nz, int ny, int nx, const float* w6, float* u_out,
+                       void* stream);
+/* out (nc, n) = w * x for stacked responses x (nc, n). */
+int dh_er_scale_stacked(const float* x, const float* w, int nc, int64_t n, float* out, void* stream);
+/* DATA = Gaussian sum (m - f)^2 or Poisson sum(mm - f ln mm), mm = max(m, 1e-8); IDIV = the
+ * I-divergence sum of m against f; score / hessw (Poisson only) if score is non-NULL
+ * (_data_misfit, _data_deriv: erdecon_mlx.py:220-250; rl_mlx.py:58-80). */
+int dh_er_data(const float* m, const float* f, int64_t n, int poisson, float* score, float* hessw,
+               double* scalars, double* partials, void* stream);
+/* grad = 2 s dg + 2 s rh [+ 2 s (iw w g)] (erdecon_mlx.py:300-308); starts CG at x = 0,
+ * r = p = -grad, sv = s p: GG = RS = sum grad^2, RS_THR = cg_tol^2 RS, STOP = (RS <= 1e-20)
+ * (erdecon_mlx.py:366-372). */
+int dh_er_gradient(const float* s, const float* dg, const float* rh, const float* w, const float* g,
+                   float intensity_weight, int64_t n, double cg_tol, float* grad, float* x, float* r, float* p,
+                   float* sv, double* scalars, double* partials, void* stream);
+/* ap = 4 s kt + 4 s htu [+ 4 iw s (w sv)] (erdecon_gn_hvp, erdecon_mlx.py:336-345), DENOM =
+ * sum p ap; STOP if DENOM is not finite or <= 1e-20 (erdecon_mlx.py:374-380). */
+int dh_er_cg_ap(const float* s, const float* kt, const float* htu, const float* w, const float* sv,
+                float intensity_weight, const float* p, float* ap, int64_t n, double* scalars, double* partials,
+                void* stream);
+/* Unless STOP: x += a p, r -= a ap (a = RS / DENOM); STOP if sum r^2 is not finite, negative
+ * or <= RS_THR; else p = r + (rs_new / RS) p, sv = s p, RS = rs_new (erdecon_mlx.py:381-392). */
+int dh_er_cg_update(float* x, float* r, float* p, const float* ap, const float* s, float* sv, int64_t n,
+                    double* scalars, double* partials, void* stream);
+/* scalars[slot] = sum a b (grad^T p, erdecon_mlx.py:611). */
+int dh_er_dot(const float* a, const float* b, int64_t n, int slot, double* scalars, double* partials, void* stream);
+/* Elementwise steps of the outer loop (DH_ER_EW_*): normalisation by the data scale, g = s^2,
+ * the trial point s + alpha p and its square, -grad, d K(sv) (erdecon_mlx.py:544-560, 620-624, 687). */
+int dh_er_ew(int op, const float* a, const float* b, float alpha, float* out, float* out2, int64_t n, void* stream);
+
 /* ---- measurement ---- */
 /* Pass ids for dh_timing_read. */
 #define DH_PASS_X_FWD 0
