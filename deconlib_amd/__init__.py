@@ -4,8 +4,8 @@ Drop-in for the hot path of ``deconlib.deconvolution``: the operator protocol
 and composition, ``LinearFFTConvolver`` / ``FFTConvolver``, ``Pad`` / ``Crop`` /
 ``FractionalAreaDownsample``, ``Hessian3D`` / ``Hessian2D``, Richardson–Lucy,
 PDHG with Hessian regularization, NLCG (``nlcg_with_operator``,
-``nlcg_solver``), ER-Decon (``erdecon_with_operator``, ``erdecon_solver``), and the
-tiled path (``make_forward_model``,
+``nlcg_solver``), ER-Decon (``erdecon_with_operator``, ``erdecon_solver``), JetNewton
+(``jetnewton_with_operator``), and the tiled path (``make_forward_model``,
 ``plan_tiles`` / ``process_tiles``, ``richardson_lucy_solver``).  All arithmetic runs in hand-written HIP kernels for
 gfx950 (``deconlib_amd/_lib/libdeconhip.so``, C ABI in ``include/deconhip.h``);
 PyTorch only owns device buffers and streams.
@@ -14,9 +14,11 @@ PyTorch only owns device buffers and streams.
 from .base import DeconvolutionResult, MLXDeconvolutionResult
 from .composition import Compose, LinearOperator, compose
 from .core_operators import Crop, FractionalAreaDownsample, Pad
-from .linops import FFTConvolver, Hessian2D, Hessian3D, LinearFFTConvolver, fast_padded_shape
+from .linops import (AnisotropicHessian2D, AnisotropicHessian3D, FFTConvolver, Hessian2D, Hessian3D,
+                     LinearFFTConvolver, fast_padded_shape)
 from .nlcg import NLCGResult, nlcg_solver, nlcg_with_operator
 from .erdecon import ERDeconResult, erdecon_solver, erdecon_with_operator
+from .jetnewton import JetNewtonResult, jetnewton_with_operator
 from .pdhg import HessianRegularizer, solve_pdhg_mlx, solve_pdhg_with_operator
 from .rl import RLResult, poisson_i_divergence, richardson_lucy_solver, richardson_lucy_with_operator
 from .forward_model import ForwardModel, make_forward_model
@@ -35,6 +37,10 @@ __all__ = [
     "ERDeconResult",
     "erdecon_with_operator",
     "erdecon_solver",
+    "JetNewtonResult",
+    "jetnewton_with_operator",
+    "AnisotropicHessian2D",
+    "AnisotropicHessian3D",
     "MLXDeconvolutionResult",
     "DeconvolutionResult",
     "RLResult",

@@ -110,6 +110,11 @@ NL_GAMMA, NL_LAM_ACCEPTED, NL_A0, NL_NSLOTS = 14, 15, 16, 32
 ER_RS, ER_RS_THR, ER_DENOM, ER_BETA, ER_STOP, ER_GG, ER_DOT = 0, 1, 2, 3, 4, 5, 6
 ER_DATA, ER_IDIV, ER_LOGQ, ER_Q, ER_NSLOTS = 7, 8, 9, 10, 16
 ER_EW_DIV, ER_EW_SQUARE_SCALE, ER_EW_AXPY_SQUARE, ER_EW_NEG, ER_EW_MUL, ER_EW_SCALE = 0, 1, 2, 3, 4, 5
+ER_EW_AFFINE, ER_EW_MAX_DIV, ER_EW_AXPY = 6, 7, 8
+
+# JetNewton float64 scalar block slots (DH_JN_* in deconhip.h)
+JN_EPS, JN_PG, JN_KKT2, JN_ACTIVE, JN_RZ, JN_TOL_K, JN_STOP, JN_DENOM, JN_BETA = 0, 1, 2, 3, 4, 5, 6, 7, 8
+JN_NCG, JN_HITNEG, JN_USE_RHS, JN_ND, JN_DIRECTIONAL, JN_LOGU, JN_USUM, JN_R0, JN_NSLOTS = 9, 10, 11, 12, 13, 14, 15, 16, 24
 
 _vp, _i64, _int, _f, _d = C.c_void_p, C.c_int64, C.c_int, C.c_float, C.c_double
 
@@ -183,7 +188,18 @@ SYMBOLS = {
     "dh_er_cg_ap": (_int, _vp, _vp, _vp, _vp, _vp, _f, _vp, _vp, _i64, _vp, _vp, _vp),
     "dh_er_cg_update": (_int, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp),
     "dh_er_dot": (_int, _vp, _vp, _i64, _int, _vp, _vp, _vp),
-    "dh_er_ew": (_int, _int, _vp, _vp, _f, _vp, _vp, _i64, _vp),
+    "dh_er_ew": (_int, _int, _vp, _vp, _f, _f, _vp, _vp, _i64, _vp),
+    "dh_jn_partials": (_i64,),
+    "dh_jn_penalty": (_int, _vp, _int, _int, _int, C.POINTER(_f), _f, _f, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp),
+    "dh_jn_penalty_stacked": (_int, _vp, _int, _i64, _f, _f, _vp, _vp, _vp, _vp, _vp, _vp, _vp),
+    "dh_jn_hvp_operand": (_int, _vp, _vp, _vp, _vp, _int, _int, _int, C.POINTER(_f), _vp, _vp),
+    "dh_jn_hvp_operand_stacked": (_int, _vp, _vp, _vp, _vp, _int, _i64, _vp, _vp),
+    "dh_jn_kkt": (_int, _vp, _vp, _i64, _d, _vp, _vp, _vp),
+    "dh_jn_active": (_int, _vp, _vp, _i64, _f, _f, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp),
+    "dh_jn_cg_ap": (_int, _vp, _f, _vp, _vp, _vp, _vp, _i64, _d, _vp, _vp, _vp),
+    "dh_jn_cg_update": (_int, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp),
+    "dh_jn_direction": (_int, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp),
+    "dh_jn_trial": (_int, _vp, _vp, _f, _vp, _vp, _i64, _vp, _vp, _vp),
     "dh_timing_enable": (_int, _int),
     "dh_timing_read": (_int, C.POINTER(_d), C.POINTER(_i64), _int),
 }
@@ -199,7 +215,7 @@ if fns["dh_abi_version"]() != ABI_VERSION:
 
 
 _VALUE_RETURNING = {"dh_reduce_partials", "dh_abi_version", "dh_pd_partials", "dh_conv_plan_get_tile",
-                    "dh_nlcg_partials", "dh_er_partials"}
+                    "dh_nlcg_partials", "dh_er_partials", "dh_jn_partials"}
 
 
 class NativeError(RuntimeError):

@@ -12,6 +12,7 @@
 #include "dh_fftct.hpp"
 #include "dh_ops.hpp"
 #include "dh_erdecon.hpp"
+#include "dh_jetnewton.hpp"
 #include "dh_nlcg.hpp"
 #include "dh_pdhg.hpp"
 
@@ -1032,12 +1033,89 @@ int dh_er_dot(const float* a, const float* b, int64_t n, int slot, double* scala
   return hip_check(dh::er_dot(a, b, n, slot, scalars, partials, (hipStream_t)stream), "er_dot");
 }
 
-int dh_er_ew(int op, const float* a, const float* b, float alpha, float* out, float* out2, int64_t n, void* stream) {
-  const bool needs_b = op == DH_ER_EW_AXPY_SQUARE || op == DH_ER_EW_MUL;
-  if (op < DH_ER_EW_DIV || op > DH_ER_EW_SCALE || !a || !out || n < 1 || (needs_b && !b) ||
+int dh_er_ew(int op, const float* a, const float* b, float alpha, float beta, float* out, float* out2, int64_t n,
+             void* stream) {
+  const bool needs_b = op == DH_ER_EW_AXPY_SQUARE || op == DH_ER_EW_MUL || op == DH_ER_EW_AXPY;
+  if (op < DH_ER_EW_DIV || op > DH_ER_EW_AXPY || !a || !out || n < 1 || (needs_b && !b) ||
       (op == DH_ER_EW_AXPY_SQUARE && !out2))
     return fail(DH_EINVAL, "bad er_ew arguments");
-  return hip_check(dh::er_ew(op, a, b, alpha, out, out2, n, (hipStream_t)stream), "er_ew");
+  return hip_check(dh::er_ew(op, a, b, alpha, beta, out, out2, n, (hipStream_t)stream), "er_ew");
+}
+
+// ---- JetNewton ----
+int64_t dh_jn_partials(void) { return dh::jn_partials_needed(); }
+
+int dh_jn_penalty(const float* x, int nz, int ny, int nx, const float* w6, float beta, float eta, float* r_out,
+                  float* w_out, float* c_out, float* wr_out, float* u_out, double* scalars, double* partials,
+                  void* stream) {
+  if (!x || !w6 || !scalars || !partials || !dims_ok(nz, ny, nx) || (r_out && (!w_out || !c_out || !wr_out)) ||
+      !(beta > 0.0f))
+    return fail(DH_EINVAL, "bad jn_penalty arguments");
+  return hip_check(dh::jn_penalty(x, nz, ny, nx, w6, beta, eta, r_out, w_out, c_out, wr_out, u_out, scalars, partials,
+                                  (hipStream_t)stream),
+                   "jn_penalty");
+}
+
+int dh_jn_penalty_stacked(const float* r, int nc, int64_t n, float beta, float eta, float* w_out, float* c_out,
+                          float* wr_out, float* u_out, double* scalars, double* partials, void* stream) {
+  if (!r || nc < 1 || n < 1 || !scalars || !partials || (w_out && (!c_out || !wr_out)) || !(beta > 0.0f))
+    return fail(DH_EINVAL, "bad jn_penalty_stacked arguments");
+  return hip_check(dh::jn_penalty_stacked(r, nc, n, beta, eta, w_out, c_out, wr_out, u_out, scalars, partials,
+                                          (hipStream_t)stream),
+                   "jn_penalty_stacked");
+}
+
+int dh_jn_hvp_operand(const float* v, const float* r, const float* w, const float* c, int nz, int ny, int nx,
+                      const float* w6, float* out, void* stream) {
+  if (!v || !r || !w || !c || !w6 || !out || !dims_ok(nz, ny, nx)) return fail(DH_EINVAL, "bad jn_hvp_operand arguments");
+  return hip_check(dh::jn_hvp_operand(v, r, w, c, nz, ny, nx, w6, out, (hipStream_t)stream), "jn_hvp_operand");
+}
+
+int dh_jn_hvp_operand_stacked(const float* q, const float* r, const float* w, const float* c, int nc, int64_t n,
+                              float* out, void* stream) {
+  if (!q || !r || !w || !c || !out || nc < 1 || n < 1) return fail(DH_EINVAL, "bad jn_hvp_operand_stacked arguments");
+  return hip_check(dh::jn_hvp_operand_stacked(q, r, w, c, nc, n, out, (hipStream_t)stream), "jn_hvp_operand_stacked");
+}
+
+int dh_jn_kkt(const float* x, const float* g, int64_t n, double eps_bar, double* scalars, double* partials,
+              void* stream) {
+  if (!x || !g || n < 1 || !scalars || !partials) return fail(DH_EINVAL, "bad jn_kkt arguments");
+  return hip_check(dh::jn_kkt(x, g, n, eps_bar, scalars, partials, (hipStream_t)stream), "jn_kkt");
+}
+
+int dh_jn_active(const float* x, const float* g, int64_t n, float freeze_tau, float freeze_delta, uint8_t* frozen,
+                 uint8_t* active, float* r, float* p, float* xcg, double eps, double* scalars, double* partials,
+                 void* stream) {
+  if (!g || !active || n < 1 || !scalars || !partials || (r && (!p || !xcg)) || (!x && (!r || frozen)))
+    return fail(DH_EINVAL, "bad jn_active arguments");
+  return hip_check(dh::jn_active(x, g, n, freeze_tau, freeze_delta, frozen, active, r, p, xcg, eps, scalars, partials,
+                                 (hipStream_t)stream),
+                   "jn_active");
+}
+
+int dh_jn_cg_ap(const float* kt, float scale, const float* htu, const uint8_t* active, const float* p, float* ap,
+                int64_t n, double eps, double* scalars, double* partials, void* stream) {
+  if (!kt || !active || !p || !ap || n < 1 || !scalars || !partials) return fail(DH_EINVAL, "bad jn_cg_ap arguments");
+  return hip_check(dh::jn_cg_ap(kt, scale, htu, active, p, ap, n, eps, scalars, partials, (hipStream_t)stream),
+                   "jn_cg_ap");
+}
+
+int dh_jn_cg_update(float* x, float* r, float* p, const float* ap, int64_t n, double* scalars, double* partials,
+                    void* stream) {
+  if (!x || !r || !p || !ap || n < 1 || !scalars || !partials) return fail(DH_EINVAL, "bad jn_cg_update arguments");
+  return hip_check(dh::jn_cg_update(x, r, p, ap, n, scalars, partials, (hipStream_t)stream), "jn_cg_update");
+}
+
+int dh_jn_direction(const float* xcg, const float* g, const uint8_t* active, float* d, int64_t n, double* scalars,
+                    double* partials, void* stream) {
+  if (!xcg || !g || !active || !d || n < 1 || !scalars || !partials) return fail(DH_EINVAL, "bad jn_direction arguments");
+  return hip_check(dh::jn_direction(xcg, g, active, d, n, scalars, partials, (hipStream_t)stream), "jn_direction");
+}
+
+int dh_jn_trial(const float* x, const float* d, float t, const float* g, float* x_t, int64_t n, double* scalars,
+                double* partials, void* stream) {
+  if (!x || !d || !g || !x_t || n < 1 || !scalars || !partials) return fail(DH_EINVAL, "bad jn_trial arguments");
+  return hip_check(dh::jn_trial(x, d, t, g, x_t, n, scalars, partials, (hipStream_t)stream), "jn_trial");
 }
 
 int dh_timing_enable(int on) {

@@ -310,7 +310,8 @@ __global__ void __launch_bounds__(kErThreads) k_er_dot_finish(const double* __re
 }
 
 // out may alias a (in-place steps); each element is read before it is written by one thread.
-__global__ void k_er_ew(int op, const float* a, const float* b, float alpha, float* out, float* out2, int64_t n) {
+__global__ void k_er_ew(int op, const float* a, const float* b, float alpha, float beta, float* out, float* out2,
+                        int64_t n) {
   ER_LOOP(i, n) {
     const float av = a[i];
     switch (op) {
@@ -324,6 +325,9 @@ __global__ void k_er_ew(int op, const float* a, const float* b, float alpha, flo
       }
       case DH_ER_EW_NEG: out[i] = -av; break;
       case DH_ER_EW_MUL: out[i] = av * b[i]; break;
+      case DH_ER_EW_AFFINE: out[i] = alpha * av + beta; break;
+      case DH_ER_EW_MAX_DIV: out[i] = fmaxf(av, alpha) / alpha; break;
+      case DH_ER_EW_AXPY: out[i] = alpha * av + b[i]; break;
       default: out[i] = alpha * av; break;  // DH_ER_EW_SCALE
     }
   }
@@ -417,9 +421,9 @@ hipError_t er_dot(const float* a, const float* b, int64_t n, int slot, double* s
   return hipGetLastError();
 }
 
-hipError_t er_ew(int op, const float* a, const float* b, float alpha, float* out, float* out2, int64_t n,
+hipError_t er_ew(int op, const float* a, const float* b, float alpha, float beta, float* out, float* out2, int64_t n,
                  hipStream_t st) {
-  ER_LAUNCH(k_er_ew, er_blocks(n), op, a, b, alpha, out, out2, n);
+  ER_LAUNCH(k_er_ew, er_blocks(n), op, a, b, alpha, beta, out, out2, n);
   return hipGetLastError();
 }
 

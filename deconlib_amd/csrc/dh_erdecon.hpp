@@ -28,7 +28,7 @@ hipError_t er_cg_ap(const float* s, const float* kt, const float* htu, const flo
 hipError_t er_cg_update(float* x, float* r, float* p, const float* ap, const float* s, float* sv, int64_t n,
                         double* sc, double* part, hipStream_t st);
 hipError_t er_dot(const float* a, const float* b, int64_t n, int slot, double* sc, double* part, hipStream_t st);
-hipError_t er_ew(int op, const float* a, const float* b, float alpha, float* out, float* out2, int64_t n,
+hipError_t er_ew(int op, const float* a, const float* b, float alpha, float beta, float* out, float* out2, int64_t n,
                  hipStream_t st);
 
 }  // namespace dh

@@ -84,8 +84,8 @@ class _Scalars:
         return self.host.tolist()
 
 
-def _ew(op, a, out, alpha=0.0, b=None, out2=None):
-    N.call("dh_er_ew", op, a.data_ptr(), N.ptr(b), float(alpha), out.data_ptr(), N.ptr(out2), a.numel(),
+def _ew(op, a, out, alpha=0.0, b=None, out2=None, beta=0.0):
+    N.call("dh_er_ew", op, a.data_ptr(), N.ptr(b), float(alpha), float(beta), out.data_ptr(), N.ptr(out2), a.numel(),
            N.stream_ptr())
     return out
 

@@ -139,6 +139,51 @@ class Hessian2D:
         return self.forward(f)
 
 
+def _f32(v) -> np.float32:
+    return np.float32(v)
+
+
+class AnisotropicHessian3D(Hessian3D):
+    """Hessian with per-axis coefficients kappa = (kz, ky, kx) (linops_mlx.py:314-381).
+
+    Components [zz, yy, xx, yz, xz, xy] weighted [kz^2, ky^2, kx^2, sqrt2 ky kz,
+    sqrt2 kx kz, sqrt2 kx ky] on the Hessian3D stencil kernels;
+    ``Hessian3D(r)`` is ``AnisotropicHessian3D((r, 1, 1))``.
+    """
+
+    def __init__(self, kappa):
+        self.kappa = tuple(float(k) for k in kappa)
+        kz, ky, kx = self.kappa
+        self.r = kz
+        self.operator_norm_sq = 16.0 * (kz ** 4 + ky ** 4 + kx ** 4) + 2.0 * (
+            (ky ** 2) * (kz ** 2) + (kx ** 2) * (kz ** 2) + (kx ** 2) * (ky ** 2))
+        s2 = _f32(SQRT2)
+        self._w = np.array([_f32(kz ** 2), _f32(ky ** 2), _f32(kx ** 2), _f32(ky * kz) * s2, _f32(kx * kz) * s2,
+                            _f32(kx * ky) * s2], dtype=np.float32)
+
+    @classmethod
+    def from_lengths(cls, ell, spacing):
+        """kappa_a = ell_a / h_a (z, y, x order)."""
+        return cls(kappa=tuple(e / h for e, h in zip(ell, spacing)))
+
+
+class AnisotropicHessian2D(Hessian2D):
+    """[ky^2 H_yy, kx^2 H_xx, sqrt2 ky kx H_xy] (linops_mlx.py:265-311), on one plane of
+    the Hessian3D stencil kernels; ``Hessian2D()`` is ``AnisotropicHessian2D((1, 1))``."""
+
+    def __init__(self, kappa):
+        self.kappa = tuple(float(k) for k in kappa)
+        ky, kx = self.kappa
+        self.operator_norm_sq = 16.0 * (ky ** 4 + kx ** 4) + 2.0 * (ky ** 2) * (kx ** 2)
+        self._w = np.array([0.0, _f32(ky ** 2), _f32(kx ** 2), 0.0, 0.0, _f32(ky * kx) * _f32(SQRT2)],
+                           dtype=np.float32)
+
+    @classmethod
+    def from_lengths(cls, ell, spacing):
+        """kappa_a = ell_a / h_a (y, x order)."""
+        return cls(kappa=tuple(e / h for e, h in zip(ell, spacing)))
+
+
 class _ConvPlan:
     """Owns one native plan and its device buffers (twiddles, OTF, workspace)."""
 

@@ -374,6 +374,9 @@ int dh_nlcg_idiv(const float* fp, const float* fc, int64_t n, double eps, double
 #define DH_ER_EW_NEG 3           /* out = -a                           */
 #define DH_ER_EW_MUL 4           /* out = a * b                        */
 #define DH_ER_EW_SCALE 5         /* out = alpha * a                    */
+#define DH_ER_EW_AFFINE 6        /* out = alpha * a + beta             */
+#define DH_ER_EW_MAX_DIV 7       /* out = max(a, alpha) / alpha        */
+#define DH_ER_EW_AXPY 8          /* out = alpha * a + b                */
 
 int64_t dh_er_partials(void);
 /* Hessian3D (weights w6 = [r^2, 1, 1, r sqrt2, r sqrt2, sqrt2]; Hessian2D = one plane with
@@ -414,9 +417,77 @@ int dh_er_cg_update(float* x, float* r, float* p, const float* ap, const float* 
                     double* scalars, double* partials, void* stream);
 /* scalars[slot] = sum a b (grad^T p, erdecon_mlx.py:611). */
 int dh_er_dot(const float* a, const float* b, int64_t n, int slot, double* scalars, double* partials, void* stream);
-/* Elementwise steps of the outer loop (DH_ER_EW_*): normalisation by the data scale, g = s^2,
- * the trial point s + alpha p and its square, -grad, d K(sv) (erdecon_mlx.py:544-560, 620-624, 687). */
-int dh_er_ew(int op, const float* a, const float* b, float alpha, float* out, float* out2, int64_t n, void* stream);
+/* Elementwise steps of the outer loops (DH_ER_EW_*): normalisation by the data scale, g = s^2,
+ * the trial point s + alpha p and its square, -grad, d K(sv) (erdecon_mlx.py:544-560, 620-624, 687);
+ * s0 K x + b, max(., s0) / s0 and the scaled gradient sum of JetNewton (jetnewton_mlx.py:197-225, 619-624). */
+int dh_er_ew(int op, const float* a, const float* b, float alpha, float beta, float* out, float* out2, int64_t n,
+             void* stream);
+
+/* ---- JetNewton: active-set projected Newton, exact Hessian (jetnewton_mlx.py:111-785) ----
+ * x is the dimensionless iterate (n voxels); r = H x is cached as (6, n) (or the operator's
+ * stacked (nc, n)); masks are uint8 (1 = active / frozen).  Scalars live in a caller-owned
+ * device block of DH_JN_NSLOTS doubles; partials holds dh_jn_partials() doubles. */
+#define DH_JN_EPS 0          /* active-set eps = min(eps_bar, ||x - P+(x - g)||)          */
+#define DH_JN_PG 1           /* ||x - P+(x - g)||_inf                                     */
+#define DH_JN_KKT2 2         /* ||x - P+(x - g)||^2                                       */
+#define DH_JN_ACTIVE 3       /* active-set size                                           */
+#define DH_JN_RZ 4           /* CG: r . z (z = r)                                         */
+#define DH_JN_TOL_K 5        /* CG: min(0.5, sqrt(r0)) r0                                 */
+#define DH_JN_STOP 6         /* CG: 1 once the solve has ended                            */
+#define DH_JN_DENOM 7        /* CG: p . A p                                               */
+#define DH_JN_BETA 8         /* CG: rz_new / rz                                           */
+#define DH_JN_NCG 9          /* CG: steps started                                         */
+#define DH_JN_HITNEG 10      /* CG: 1 after non-positive curvature                        */
+#define DH_JN_USE_RHS 11     /* CG: 1 if that happened on the first step (d_free = rhs)   */
+#define DH_JN_ND 12          /* grad . d                                                  */
+#define DH_JN_DIRECTIONAL 13 /* grad . (x_t - x)                                          */
+#define DH_JN_LOGU 14        /* sum ln(eta + u)                                           */
+#define DH_JN_USUM 15        /* sum u                                                     */
+#define DH_JN_R0 16          /* CG: ||r0||                                                */
+#define DH_JN_NSLOTS 24
+
+int64_t dh_jn_partials(void);
+/* Penalty weights of x (_penalty_weights, jetnewton_mlx.py:111-133) through the Hessian3D
+ * stencil with weights w6 (one plane for the 2-D operators): LOGU, USUM; optional outputs
+ * r (6, n), w, c = -w^2 / beta, w r (6, n) (the operand of H^T in the gradient), u. */
+int dh_jn_penalty(const float* x, int nz, int ny, int nx, const float* w6, float beta, float eta, float* r_out,
+                  float* w_out, float* c_out, float* wr_out, float* u_out, double* scalars, double* partials,
+                  void* stream);
+/* The same from any operator's stacked responses r (nc, n). */
+int dh_jn_penalty_stacked(const float* r, int nc, int64_t n, float beta, float eta, float* w_out, float* c_out,
+                          float* wr_out, float* u_out, double* scalars, double* partials, void* stream);
+/* out (6, n) = w q + 2 (c <r, q>) r with q = H v (jetnewton_hvp, jetnewton_mlx.py:284-293). */
+int dh_jn_hvp_operand(const float* v, const float* r, const float* w, const float* c, int nz, int ny, int nx,
+                      const float* w6, float* out, void* stream);
+/* The same from the operator's stacked q = H v (nc, n); out may alias q. */
+int dh_jn_hvp_operand_stacked(const float* q, const float* r, const float* w, const float* c, int nc, int64_t n,
+                              float* out, void* stream);
+/* KKT2, EPS, PG of (x, g) (identify_active_set, jetnewton_mlx.py:302-315; proj-grad, :654-656). */
+int dh_jn_kkt(const float* x, const float* g, int64_t n, double eps_bar, double* scalars, double* partials,
+              void* stream);
+/* frozen |= (x < tau) & (g > delta) (if frozen is non-NULL); active = ((x <= EPS) & (g > 0)) |
+ * frozen; ACTIVE = |active|; with r non-NULL also starts the reduced CG at xcg = 0,
+ * r = p = where(free, -g, 0): RZ, R0, TOL_K, STOP = (R0 <= max(TOL_K, eps)), NCG = HITNEG =
+ * USE_RHS = 0 (ActiveSetState.update, solve_reduced_newton; jetnewton_mlx.py:334-352, 393-407).
+ * x NULL: active is a given mask (read, not written; frozen must be NULL, r non-NULL) and
+ * only the CG start is made. */
+int dh_jn_active(const float* x, const float* g, int64_t n, float freeze_tau, float freeze_delta, uint8_t* frozen,
+                 uint8_t* active, float* r, float* p, float* xcg, double eps, double* scalars, double* partials,
+                 void* stream);
+/* ap = where(free, scale kt + htu, 0) (htu may be NULL), DENOM = p . ap, NCG += 1; on DENOM not
+ * finite or <= eps: STOP, HITNEG and, on the first step, USE_RHS (jetnewton_mlx.py:411-418). */
+int dh_jn_cg_ap(const float* kt, float scale, const float* htu, const uint8_t* active, const float* p, float* ap,
+                int64_t n, double eps, double* scalars, double* partials, void* stream);
+/* Unless STOP: x += a p, r -= a ap (a = RZ / DENOM); STOP if ||r|| is not finite or <= TOL_K;
+ * else p = r + (rz_new / RZ) p (jetnewton_mlx.py:419-430). */
+int dh_jn_cg_update(float* x, float* r, float* p, const float* ap, int64_t n, double* scalars, double* partials,
+                    void* stream);
+/* d = d_free + where(active, -g, 0), ND = g . d (jetnewton_mlx.py:627-637). */
+int dh_jn_direction(const float* xcg, const float* g, const uint8_t* active, float* d, int64_t n, double* scalars,
+                    double* partials, void* stream);
+/* x_t = max(x + t d, 0), DIRECTIONAL = g . (x_t - x) (projected_armijo, jetnewton_mlx.py:460-466). */
+int dh_jn_trial(const float* x, const float* d, float t, const float* g, float* x_t, int64_t n, double* scalars,
+                double* partials, void* stream);
 
 /* ---- measurement ---- */
 /* Pass ids for dh_timing_read. */
