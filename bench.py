@@ -175,6 +175,97 @@ def bench_nlcg(torch, obs, blur, steps, warmup):
     }
 
 
+def _time_cg_steps(torch, step, read, steps, warmup):
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+        read()  # the solvers read the CG flags once per step
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def bench_erdecon(torch, obs, blur, steps=10, warmup=2):
+    """ER-Decon (Hessian3D, Gaussian data term) on the same volume and operator: the
+    inner Gauss-Newton CG step (two convolutions, the weighted Hessian stencil and its
+    adjoint, the CG updates; the relative-residual stop disabled so every step runs in
+    full), and one whole outer iteration through erdecon_with_operator."""
+    from deconlib_amd import _native as N
+    from deconlib_amd import erdecon_with_operator
+    from deconlib_amd.erdecon import _ERState
+    from deconlib_amd.linops import Hessian3D
+
+    f = torch.from_numpy(obs).cuda() / float(obs.max())
+    st = _ERState(f, blur, Hessian3D(), 0.0, 0.05, 1e-2, "gaussian", 0.0, 0.0, None)
+    st.gradient(0.1)
+    st.S.sc[N.ER_RS_THR] = 0.0
+
+    def step():
+        kt, htu = st.hvp_parts()
+        N.call("dh_er_cg_ap", st.s.data_ptr(), kt.data_ptr(), htu.data_ptr(), st.w.data_ptr(), st.sv.data_ptr(),
+               0.0, st.p.data_ptr(), st.ap.data_ptr(), st.n, st.S.sc.data_ptr(), st.S.part.data_ptr(),
+               N.stream_ptr())
+        N.call("dh_er_cg_update", st.x.data_ptr(), st.r.data_ptr(), st.p.data_ptr(), st.ap.data_ptr(),
+               st.s.data_ptr(), st.sv.data_ptr(), st.n, st.S.sc.data_ptr(), st.S.part.data_ptr(), N.stream_ptr())
+
+    per_step = _time_cg_steps(torch, step, st.S.read, steps, warmup)
+    del st
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = erdecon_with_operator(obs, blur, num_iter=1, newton_tol=0.0)
+    torch.cuda.synchronize()
+    outer = time.perf_counter() - t0
+    vox = int(np.prod(obs.shape))
+    return {
+        "cg_step_ms": per_step * 1e3,
+        "voxel_cg_steps_per_s": vox / per_step,
+        "one_outer_iteration_s": outer,
+        "note": "one outer iteration = set-up, gradient, up to 25 CG steps, Armijo search, result",
+        "finite": bool(torch.isfinite(res.restored).all().item()),
+    }
+
+
+def bench_jetnewton(torch, obs, blur, steps=10, warmup=2):
+    """JetNewton (AnisotropicHessian3D, Poisson) on the same volume and operator: the
+    inner reduced CG step with the exact Hessian product (forcing stop disabled), and one
+    whole outer iteration through jetnewton_with_operator."""
+    from deconlib_amd import AnisotropicHessian3D
+    from deconlib_amd import _native as N
+    from deconlib_amd import jetnewton_with_operator
+    from deconlib_amd.jetnewton import _JNState
+
+    s0 = float(np.sqrt(max(float(obs.mean()), 1.0)))
+    H = AnisotropicHessian3D((0.85, 1.0, 1.0))
+    f = torch.from_numpy(obs).cuda()
+    st = _JNState(f, blur, H, s0, 10.0, 1.0, 1.0, "poisson", None)
+    st.gradient()
+    st.active_set(1e-2, 1e-3, 1e-6)
+    st.B.jn[N.JN_TOL_K] = 0.0
+
+    def step():
+        st.hvp_apply()
+        N.call("dh_jn_cg_update", st.xcg.data_ptr(), st.rcg.data_ptr(), st.pcg.data_ptr(), st.ap.data_ptr(), st.n,
+               st.B.jn.data_ptr(), st.B.part.data_ptr(), N.stream_ptr())
+
+    per_step = _time_cg_steps(torch, step, st.B.read, steps, warmup)
+    del st
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = jetnewton_with_operator(obs, blur, H, s0, background=10.0, num_iter=1, newton_tol=0.0)
+    torch.cuda.synchronize()
+    outer = time.perf_counter() - t0
+    vox = int(np.prod(obs.shape))
+    return {
+        "cg_step_ms": per_step * 1e3,
+        "voxel_cg_steps_per_s": vox / per_step,
+        "one_outer_iteration_s": outer,
+        "cg_steps_in_outer_iteration": res.cg_iterations_history,
+        "finite": bool(torch.isfinite(res.restored).all().item()),
+    }
+
+
 def bench_host_to_host(torch, psf, obs, iters=50):
     """One whole RL solve from host arrays to a host array: operator construction (OTF
     build), PCIe upload, `iters` iterations, PCIe download.  Reported beside `value`
@@ -467,6 +558,10 @@ def main():
     log(f"pdhg: {json.dumps(pdhg)}")
     nlcg = bench_nlcg(torch, obs, blur, args.steps, args.warmup)
     log(f"nlcg: {json.dumps(nlcg)}")
+    erdecon = bench_erdecon(torch, obs, blur)
+    log(f"erdecon: {json.dumps(erdecon)}")
+    jetnewton = bench_jetnewton(torch, obs, blur)
+    log(f"jetnewton: {json.dumps(jetnewton)}")
     host = small = tiled = psf_synth = None
     if not args.no_extras:
         host = bench_host_to_host(torch, psf, obs)
@@ -535,6 +630,8 @@ def main():
                       "hbm_frac": it_bytes / (dt / args.steps) / (PEAK_HBM_GBS * 1e9)},
         "pdhg": pdhg,
         "nlcg": nlcg,
+        "erdecon": erdecon,
+        "jetnewton": jetnewton,
         "host_to_host": host,
         "batched_small": small,
         "tiled": tiled,
