@@ -383,6 +383,71 @@ __global__ void __launch_bounds__(kTileThreads, 6) k_pd_primal(float* __restrict
 // sum |H(a - b)|^2 (dL of the step-size rule; H is linear, so this is the
 // reference's |L x_new - L x|^2 up to rounding) and, if want_abs, the
 // regularizer loss of a: sum |H a| (1, "L1") or sum sqrt(sum_q (H a)_q^2 + 1e-12) (2, "L1_2")
+// Hessian3D.adjoint, out = H^T y2, through the same sliding LDS window as k_pd_primal:
+// each workgroup stages the weighted components (0, 3, 4 over three planes; 1, 2, 5 at
+// the current plane) with their halo once, instead of every voxel gathering ~19
+// neighbours across six component planes (the same arithmetic as hess_adj on
+// global memory, so the results are bit-identical).
+__global__ void __launch_bounds__(kTileThreads, 6) k_hess_adj_tiled(const float* __restrict__ y2,
+                                                                 float* __restrict__ out, Dims d, W6 w) {
+  __shared__ float win[3][3 * kPlane];  // components 0, 3, 4
+  __shared__ float cur[3][kPlane];      // components 1, 2, 5 at the current plane
+  const Tile t = tile_of(d);
+  const PlanePos pp(d, t);
+  const int64_t n = (int64_t)d.nz * d.ny * d.nx;
+  const float* wsrc[3] = {y2, y2 + 3 * n, y2 + 4 * n};
+  const float* csrc[3] = {y2 + 1 * n, y2 + 2 * n, y2 + 5 * n};
+  const float ww[3] = {w.w[0], w.w[3], w.w[4]}, cw[3] = {w.w[1], w.w[2], w.w[5]};
+  float wv[3][2], cv[3][2];
+  for (int zz = t.z0 - 1; zz <= t.z0; ++zz) {
+    const int64_t bz = pp.base(d, zz);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      pp.load(wsrc[j], bz, wv[j][0], wv[j][1]);
+      pp.store(&win[j][slot_of(zz, t.z0) * kPlane], wv[j][0] * ww[j], wv[j][1] * ww[j]);
+    }
+  }
+  {
+    const int64_t bn = pp.base(d, t.z0 + 1), bc = pp.base(d, t.z0);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      pp.load(wsrc[j], bn, wv[j][0], wv[j][1]);
+      pp.load(csrc[j], bc, cv[j][0], cv[j][1]);
+    }
+  }
+  for (int z = t.z0; z < t.z1; ++z) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      pp.store(&win[j][slot_of(z + 1, t.z0) * kPlane], wv[j][0] * ww[j], wv[j][1] * ww[j]);
+      pp.store(cur[j], cv[j][0] * cw[j], cv[j][1] * cw[j]);
+    }
+    __syncthreads();
+    if (z + 1 < t.z1) {
+      const int64_t bn = pp.base(d, z + 2), bc = pp.base(d, z + 1);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        pp.load(wsrc[j], bn, wv[j][0], wv[j][1]);
+        pp.load(csrc[j], bc, cv[j][0], cv[j][1]);
+      }
+    }
+    if (t.in) {
+      auto hw = [&](int q, int zz, int yy, int xx) {
+        const int o = (yy - t.y0 + 1) * kPW + (xx - t.x0 + 1);
+        switch (q) {
+          case 0: return win[0][slot_of(zz, t.z0) * kPlane + o];
+          case 3: return win[1][slot_of(zz, t.z0) * kPlane + o];
+          case 4: return win[2][slot_of(zz, t.z0) * kPlane + o];
+          case 1: return cur[0][o];
+          case 2: return cur[1][o];
+          default: return cur[2][o];
+        }
+      };
+      stnt(out + ((int64_t)z * d.ny + t.y) * d.nx + t.x, hess_adj(hw, d, z, t.y, t.x));
+    }
+    __syncthreads();
+  }
+}
+
 __global__ void __launch_bounds__(kTileThreads) k_pd_hess_sums(const float* __restrict__ a,
                                                                const float* __restrict__ b, Dims d, W6 w,
                                                                int want_abs, double* __restrict__ part) {
@@ -442,7 +507,7 @@ __global__ void __launch_bounds__(kTileThreads) k_pd_hess_sums(const float* __re
   }
 }
 
-// Hessian3D.forward into out (6, nz, ny, nx) and Hessian3D.adjoint of y (6, ...)
+// Hessian3D.forward into out (6, nz, ny, nx)
 __global__ void k_hess_fwd(const float* __restrict__ f, float* __restrict__ out, Dims d, W6 w) {
   const int64_t n = (int64_t)d.nz * d.ny * d.nx;
   auto fv = [&](int z, int y, int xx) { return f[((int64_t)z * d.ny + y) * d.nx + xx]; };
@@ -454,19 +519,6 @@ __global__ void k_hess_fwd(const float* __restrict__ f, float* __restrict__ out,
     hess6(fv, d, z, y, xx, w.w, h);
 #pragma unroll
     for (int q = 0; q < 6; ++q) out[q * n + i] = h[q];
-  }
-}
-
-__global__ void k_hess_adj(const float* __restrict__ y2, float* __restrict__ out, Dims d, W6 w) {
-  const int64_t n = (int64_t)d.nz * d.ny * d.nx;
-  auto hw = [&](int q, int z, int y, int xx) {
-    return y2[q * n + ((int64_t)z * d.ny + y) * d.nx + xx] * w.w[q];
-  };
-  PD_LOOP(i, n) {
-    const int xx = (int)(i % d.nx);
-    const int y = (int)((i / d.nx) % d.ny);
-    const int z = (int)(i / ((int64_t)d.nx * d.ny));
-    out[i] = hess_adj(hw, d, z, y, xx);
   }
 }
 
@@ -634,7 +686,7 @@ hipError_t hess_adjoint(const float* y2, float* out, int nz, int ny, int nx, con
   Dims d{nz, ny, nx};
   W6 ww;
   for (int q = 0; q < 6; ++q) ww.w[q] = w[q];
-  hipLaunchKernelGGL(k_hess_adj, ew_blocks((int64_t)nz * ny * nx), dim3(kPdThreads), 0, st, y2, out, d, ww);
+  hipLaunchKernelGGL(k_hess_adj_tiled, tile_grid(nz, ny, nx), dim3(kTileThreads), 0, st, y2, out, d, ww);
   return hipGetLastError();
 }
 
