@@ -119,20 +119,35 @@ __global__ void __launch_bounds__(kErThreads) k_er_reg_finish(const double* __re
   }
 }
 
-// u = w * H(v) (six weighted components).
-__global__ void k_er_hess_apply_w(const float* __restrict__ v, const float* __restrict__ w, Dims d, W6 w6,
-                                  float* __restrict__ u) {
+// u = w * H(v) (six weighted components) through the sliding LDS plane window of v
+// (each input voxel read from HBM about once, the six outputs streamed).
+__global__ void __launch_bounds__(kTileThreads) k_er_hess_apply_w_tiled(const float* __restrict__ v,
+                                                                        const float* __restrict__ w, Dims d, W6 w6,
+                                                                        float* __restrict__ u) {
+  __shared__ float pl[3 * kPlane];
+  const Tile t = tile_of(d);
+  const PlanePos pp(d, t);
   const int64_t n = (int64_t)d.nz * d.ny * d.nx;
-  auto vv = [&](int z, int y, int x) { return v[((int64_t)z * d.ny + y) * d.nx + x]; };
-  ER_LOOP(i, n) {
-    const int x = (int)(i % d.nx);
-    const int y = (int)((i / d.nx) % d.ny);
-    const int z = (int)(i / ((int64_t)d.nx * d.ny));
-    float h[6];
-    hess6(vv, d, z, y, x, w6.w, h);
-    const float wv = w[i];
+  float a0, a1;
+  for (int zz = t.z0 - 1; zz <= t.z0; ++zz) {
+    pp.load(v, pp.base(d, zz), a0, a1);
+    pp.store(pl + slot_of(zz, t.z0) * kPlane, a0, a1);
+  }
+  pp.load(v, pp.base(d, t.z0 + 1), a0, a1);
+  for (int z = t.z0; z < t.z1; ++z) {
+    pp.store(pl + slot_of(z + 1, t.z0) * kPlane, a0, a1);
+    __syncthreads();
+    if (z + 1 < t.z1) pp.load(v, pp.base(d, z + 2), a0, a1);
+    if (t.in) {
+      auto f = [&](int zz, int yy, int xx) { return lds_at(pl, slot_of(zz, t.z0), t, yy, xx); };
+      float h[6];
+      hess6(f, d, z, t.y, t.x, w6.w, h);
+      const int64_t i = ((int64_t)z * d.ny + t.y) * d.nx + t.x;
+      const float wv = __builtin_nontemporal_load(w + i);
 #pragma unroll
-    for (int c = 0; c < 6; ++c) u[c * n + i] = wv * h[c];
+      for (int c = 0; c < 6; ++c) __builtin_nontemporal_store(wv * h[c], u + c * n + i);
+    }
+    __syncthreads();
   }
 }
 
@@ -371,7 +386,9 @@ hipError_t er_weights_stacked(const float* hg, int nc, const float* g, int64_t n
 hipError_t er_hess_apply_w(const float* v, const float* w, int nz, int ny, int nx, const float* w6, float* u_out,
                            hipStream_t st) {
   const int64_t n = (int64_t)nz * ny * nx;
-  ER_LAUNCH(k_er_hess_apply_w, er_blocks(n), v, w, Dims{nz, ny, nx}, w6_of(w6), u_out);
+  hipLaunchKernelGGL(k_er_hess_apply_w_tiled, tile_grid(nz, ny, nx), dim3(kTileThreads), 0, st, v, w,
+                     Dims{nz, ny, nx}, w6_of(w6), u_out);
+  (void)n;
   return hipGetLastError();
 }
 

@@ -136,26 +136,42 @@ __global__ void __launch_bounds__(kJnThreads) k_jn_penalty_finish(const double* 
   }
 }
 
-// q = H v; out = w q + 2 (c <r, q>) r (the operand of H^T in the exact product).
-__global__ void k_jn_hvp_operand(const float* __restrict__ v, const float* __restrict__ r,
-                                 const float* __restrict__ w, const float* __restrict__ c, Dims d, W6 w6,
-                                 float* __restrict__ out) {
+// q = H v; out = w q + 2 (c <r, q>) r (the operand of H^T in the exact product), through
+// the sliding LDS plane window of v.
+__global__ void __launch_bounds__(kTileThreads) k_jn_hvp_operand_tiled(const float* __restrict__ v,
+                                                                       const float* __restrict__ r,
+                                                                       const float* __restrict__ w,
+                                                                       const float* __restrict__ c, Dims d, W6 w6,
+                                                                       float* __restrict__ out) {
+  __shared__ float pl[3 * kPlane];
+  const Tile t = tile_of(d);
+  const PlanePos pp(d, t);
   const int64_t n = (int64_t)d.nz * d.ny * d.nx;
-  auto vv = [&](int z, int y, int xx) { return v[((int64_t)z * d.ny + y) * d.nx + xx]; };
-  JN_LOOP(i, n) {
-    const int xx = (int)(i % d.nx);
-    const int y = (int)((i / d.nx) % d.ny);
-    const int z = (int)(i / ((int64_t)d.nx * d.ny));
-    float q[6], rr[6];
-    hess6(vv, d, z, y, xx, w6.w, q);
+  float a0, a1;
+  for (int zz = t.z0 - 1; zz <= t.z0; ++zz) {
+    pp.load(v, pp.base(d, zz), a0, a1);
+    pp.store(pl + slot_of(zz, t.z0) * kPlane, a0, a1);
+  }
+  pp.load(v, pp.base(d, t.z0 + 1), a0, a1);
+  for (int z = t.z0; z < t.z1; ++z) {
+    pp.store(pl + slot_of(z + 1, t.z0) * kPlane, a0, a1);
+    __syncthreads();
+    if (z + 1 < t.z1) pp.load(v, pp.base(d, z + 2), a0, a1);
+    if (t.in) {
+      auto f = [&](int zz, int yy, int xx) { return lds_at(pl, slot_of(zz, t.z0), t, yy, xx); };
+      float q[6], rr[6];
+      hess6(f, d, z, t.y, t.x, w6.w, q);
+      const int64_t i = ((int64_t)z * d.ny + t.y) * d.nx + t.x;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) rr[k] = r[k * n + i];
-    float g = rr[0] * q[0];
+      for (int k = 0; k < 6; ++k) rr[k] = __builtin_nontemporal_load(r + k * n + i);
+      float g = rr[0] * q[0];
 #pragma unroll
-    for (int k = 1; k < 6; ++k) g += rr[k] * q[k];
-    const float m2 = 2.0f * (c[i] * g), wv = w[i];
+      for (int k = 1; k < 6; ++k) g += rr[k] * q[k];
+      const float m2 = 2.0f * (__builtin_nontemporal_load(c + i) * g), wv = __builtin_nontemporal_load(w + i);
 #pragma unroll
-    for (int k = 0; k < 6; ++k) out[k * n + i] = wv * q[k] + m2 * rr[k];
+      for (int k = 0; k < 6; ++k) __builtin_nontemporal_store(wv * q[k] + m2 * rr[k], out + k * n + i);
+    }
+    __syncthreads();
   }
 }
 
@@ -426,7 +442,9 @@ hipError_t jn_penalty_stacked(const float* r, int nc, int64_t n, float beta, flo
 hipError_t jn_hvp_operand(const float* v, const float* r, const float* w, const float* c, int nz, int ny, int nx,
                           const float* w6, float* out, hipStream_t st) {
   const int64_t n = (int64_t)nz * ny * nx;
-  JN_LAUNCH(k_jn_hvp_operand, jn_blocks(n), v, r, w, c, Dims{nz, ny, nx}, jn_w6(w6), out);
+  hipLaunchKernelGGL(k_jn_hvp_operand_tiled, tile_grid(nz, ny, nx), dim3(kTileThreads), 0, st, v, r, w, c,
+                     Dims{nz, ny, nx}, jn_w6(w6), out);
+  (void)n;
   return hipGetLastError();
 }
 

@@ -118,95 +118,6 @@ __global__ void k_pd_y1(float* __restrict__ y1, const float* __restrict__ ax, co
   }
 }
 
-// ---------------------------------------------------------------------------
-// LDS-tiled stencils: a workgroup owns a TY x TX column of the volume over ZC
-// consecutive planes and slides a three-plane window (with a one-voxel halo,
-// filled from clamped coordinates = the Neumann boundary) down z, so each
-// voxel of x, x_prev and y2 is read from HBM about once.
-// ---------------------------------------------------------------------------
-
-constexpr int kTX = 64, kTY = 8, kZC = 16;
-constexpr int kPW = kTX + 2, kPH = kTY + 2, kPlane = kPW * kPH;
-constexpr int kTileThreads = kTX * kTY;
-
-struct Tile {
-  int x0, y0, z0, z1;  // z1 = one past the last plane of this workgroup
-  int tx, ty, x, y;
-  bool in;             // (y, x) inside the volume
-};
-
-// Workgroups are dispatched round-robin over the 8 XCDs; remap so that each XCD
-// (and its L2) runs a contiguous x-then-y range of tiles, whose halos overlap.
-__device__ __forceinline__ void tile_coords(int& bx, int& by, int& bz) {
-  const int64_t nb = (int64_t)gridDim.x * gridDim.y * gridDim.z;
-  const int64_t b = ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-  const int64_t q = nb / 8, r = nb % 8, x = b % 8, i = b / 8;
-  const int64_t l = x * q + (x < r ? x : r) + i;
-  bx = (int)(l % gridDim.x);
-  by = (int)((l / gridDim.x) % gridDim.y);
-  bz = (int)(l / ((int64_t)gridDim.x * gridDim.y));
-}
-
-__device__ __forceinline__ Tile tile_of(const Dims& d) {
-  Tile t;
-  int bx, by, bz;
-  tile_coords(bx, by, bz);
-  t.x0 = bx * kTX;
-  t.y0 = by * kTY;
-  t.z0 = bz * kZC;
-  t.z1 = min(t.z0 + kZC, d.nz);
-  t.tx = threadIdx.x % kTX;
-  t.ty = threadIdx.x / kTX;
-  t.x = t.x0 + t.tx;
-  t.y = t.y0 + t.ty;
-  t.in = t.x < d.nx && t.y < d.ny;
-  return t;
-}
-
-// Fills one halo'd plane: dst[ly * kPW + lx] = f(global index at clamped (z, y, x)).
-template <class F>
-__device__ __forceinline__ void fill_plane(float* dst, const Dims& d, const Tile& t, int z, F f) {
-  const int zc = clampi(z, d.nz);
-  for (int i = threadIdx.x; i < kPlane; i += kTileThreads) {
-    const int ly = i / kPW, lx = i - ly * kPW;
-    const int gy = clampi(t.y0 + ly - 1, d.ny), gx = clampi(t.x0 + lx - 1, d.nx);
-    dst[i] = f(((int64_t)zc * d.ny + gy) * d.nx + gx);
-  }
-}
-
-__device__ __forceinline__ int slot_of(int z, int z0) { return (z - z0 + 1) % 3; }
-
-// Each thread stages at most two samples of every halo'd plane (kPlane <=
-// 2 * kTileThreads).  Their in-plane offsets are fixed, so the next plane's
-// loads are issued into registers before the current plane is computed and
-// land in LDS after it (software pipelining of the z sweep).
-struct PlanePos {
-  int64_t off0, off1;
-  bool has1;
-  __device__ PlanePos(const Dims& d, const Tile& t) {
-    const int i0 = threadIdx.x, i1 = threadIdx.x + kTileThreads;
-    const int ly0 = i0 / kPW, lx0 = i0 - ly0 * kPW;
-    off0 = (int64_t)clampi(t.y0 + ly0 - 1, d.ny) * d.nx + clampi(t.x0 + lx0 - 1, d.nx);
-    has1 = i1 < kPlane;
-    const int ly1 = has1 ? i1 / kPW : 0, lx1 = has1 ? i1 - ly1 * kPW : 0;
-    off1 = (int64_t)clampi(t.y0 + ly1 - 1, d.ny) * d.nx + clampi(t.x0 + lx1 - 1, d.nx);
-  }
-  __device__ __forceinline__ int64_t base(const Dims& d, int z) const {
-    return (int64_t)clampi(z, d.nz) * d.ny * d.nx;
-  }
-  __device__ __forceinline__ void load(const float* __restrict__ src, int64_t b, float& v0, float& v1) const {
-    v0 = src[b + off0];
-    v1 = has1 ? src[b + off1] : 0.0f;
-  }
-  __device__ __forceinline__ void store(float* dst, float v0, float v1) const {
-    dst[threadIdx.x] = v0;
-    if (has1) dst[threadIdx.x + kTileThreads] = v1;
-  }
-};
-
-__device__ __forceinline__ float lds_at(const float* planes, int slot, const Tile& t, int yy, int xx) {
-  return planes[slot * kPlane + (yy - t.y0 + 1) * kPW + (xx - t.x0 + 1)];
-}
 
 // y2 <- prox(y2 + sigma * H(xbar)), xbar = x + theta (x - xp)  (pdhg_mlx.py:729-734)
 __global__ void __launch_bounds__(kTileThreads) k_pd_dual(float* __restrict__ y2, const float* __restrict__ x,
@@ -596,14 +507,6 @@ hipError_t pd_y1(float* y1, const float* ax, const float* axp, const float* data
   return hipGetLastError();
 }
 
-static dim3 tile_grid(int nz, int ny, int nx) {
-  return dim3((unsigned)((nx + kTX - 1) / kTX), (unsigned)((ny + kTY - 1) / kTY), (unsigned)((nz + kZC - 1) / kZC));
-}
-
-static int tile_blocks(int nz, int ny, int nx) {
-  const dim3 g = tile_grid(nz, ny, nx);
-  return (int)(g.x * g.y * g.z);
-}
 
 hipError_t pd_dual(float* y2, const float* x, const float* xp, int nz, int ny, int nx, const double* sc,
                    const float* w, float alpha, int l12, hipStream_t st) {
