@@ -106,5 +106,6 @@ def host_linear_blur(psf: np.ndarray, obj: np.ndarray) -> np.ndarray:
         d[npos:] = pp - (mm - npos) + np.arange(mm - npos)
         idx.append(d)
     k[np.ix_(*idx)] = psf / psf.sum()
-    out = np.fft.irfftn(np.fft.rfftn(obj, P) * np.fft.rfftn(k), P)
+    ax = tuple(range(len(P)))
+    out = np.fft.irfftn(np.fft.rfftn(obj, P, axes=ax) * np.fft.rfftn(k, axes=ax), P, axes=ax)
     return out[: s[0], : s[1], : s[2]]
