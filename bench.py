@@ -266,6 +266,28 @@ def bench_jetnewton(torch, obs, blur, steps=10, warmup=2):
     }
 
 
+def stream_ceiling(torch, mib=1024, reps=10):
+    """Device streaming rates this GPU reaches with library kernels (no FFT structure):
+    a device copy (1 read + 1 write) and an elementwise add (2 reads + 1 write), both
+    timed with HIP events over `reps` launches; context for the roofline fraction."""
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b, c = torch.empty_like(a), torch.empty_like(a)
+    out = {}
+    for name, fn, words in (("copy", lambda: b.copy_(a), 2), ("add", lambda: torch.add(a, b, out=c), 3)):
+        for _ in range(3):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        out[name + "_gbs"] = words * n * 4 / (e0.elapsed_time(e1) / reps * 1e-3) / 1e9
+    del a, b, c
+    return out
+
+
 def bench_host_to_host(torch, psf, obs, iters=50):
     """One whole RL solve from host arrays to a host array: operator construction (OTF
     build), PCIe upload, `iters` iterations, PCIe download.  Reported beside `value`
@@ -580,6 +602,8 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         log("copy probe: 3 x 268435456-byte device-to-device copies")
+    ceiling = stream_ceiling(torch)
+    log(f"stream ceiling (1 GiB device copy / add): {json.dumps(ceiling)}")
     serving = None
     if args.serving_jobs > 0:
         serving = bench_serving(torch, args.serving_jobs, args.serving_rate)
@@ -624,6 +648,7 @@ def main():
                               "(profiles/fft_passes_64x512x512_hbm_counters.md)",
             "algorithmic_bytes": pb["z_conv"],
             "avg_ms": z["avg_ms"],
+            "stream_ceiling": ceiling,
         },
         "passes": per_pass,
         "iteration": {"algorithmic_bytes": it_bytes,

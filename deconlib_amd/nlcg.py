@@ -367,7 +367,9 @@ def nlcg_gradient(s, blur_op, observed, background: float = 0.0, regularizer=Non
     r = empty_f32(s.shape)
     N.call("dh_nlcg_gradient", s.data_ptr(), ktw.data_ptr(), N.ptr(ctcf), 4.0 * float(reg_weight), r.data_ptr(),
            s.numel(), 0, eps, S.sc.data_ptr(), S.part.data_ptr(), N.stream_ptr())
-    return torch.neg(r), (m, ktw, ctcf)
+    grad = empty_f32(s.shape)
+    N.call("dh_er_ew", N.ER_EW_NEG, r.data_ptr(), None, 0.0, 0.0, grad.data_ptr(), None, r.numel(), N.stream_ptr())
+    return grad, (m, ktw, ctcf)
 
 
 def nlcg_hessian_quadform(s, d, blur_op, observed, background: float = 0.0, regularizer=None,
