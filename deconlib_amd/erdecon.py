@@ -414,6 +414,7 @@ def erdecon_solver(reg_weight: float = 0.05, eps_reg: float = 1e-2, num_iter: in
                                        newton_tol=newton_tol, **erdecon_kwargs)
         return result.restored[model.valid_slices]
 
+    solve.accepts_device_tiles = True
     return solve
 
 

@@ -290,6 +290,7 @@ def nlcg_solver(num_iter: int = 50, background: float = 0.0, regularizer=None, r
                                     regularizer=regularizer, reg_weight=reg_weight, init=init, **nlcg_kwargs)
         return result.restored[model.valid_slices]
 
+    solve.accepts_device_tiles = True
     return solve
 
 

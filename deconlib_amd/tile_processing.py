@@ -10,10 +10,13 @@ output, so every visible pixel is written exactly once.
 rounding.  ``process_tiles`` builds the one forward model and stitches the
 owned cores; a solver that also offers ``solve.solve_batch(tiles, model)``
 (``richardson_lucy_solver`` does) receives the tiles in groups (small tiles
-advance in lockstep, one batched launch per pass for the group), and when the
+advance in lockstep, one batched launch per pass for the group).  When the
 volume and the output fit comfortably in device memory and no per-tile
 callback is asked for, the volume is uploaded once, tiles are cut and cores
-stitched on the device, and the output is downloaded once.
+stitched on the device, and the output is downloaded once — for a batching
+solver, and for any solver marked ``solve.accepts_device_tiles`` (the
+package's NLCG and ER-Decon adapters), which then receives device tiles one at
+a time.
 """
 
 from __future__ import annotations
@@ -142,8 +145,12 @@ def process_tiles(data, psf, zoom: Union[float, Tuple[float, ...]], solve: Solve
     group = n if batch_tiles is None else max(1, int(batch_tiles))
     if batched is None:
         group = 1
-    if batched is not None and on_tile_done is None and _fits_device(data, plan):
-        return _process_resident(data, plan, model, batched, group, verbose)
+    if on_tile_done is None and _fits_device(data, plan):
+        if batched is not None:
+            return _process_resident(data, plan, model, batched, group, verbose)
+        if getattr(solve, "accepts_device_tiles", False):
+            return _process_resident(data, plan, model, lambda tiles, m: [solve(tl, m) for tl in tiles], 1,
+                                     verbose)
     output = np.zeros(plan.visible_shape, dtype=np.float32)
 
     i = 0
