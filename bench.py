@@ -175,6 +175,44 @@ def bench_nlcg(torch, obs, blur, steps, warmup):
     }
 
 
+def nlcg_vs_rl(torch, obs, blur, rl_iters=30, max_nlcg=30):
+    """Wall time for NLCG to reach the mean I-divergence RL reaches after `rl_iters`
+    iterations on the same volume and operator (both from their default starts;
+    RL's loss summed in every iteration's fused forward pass, NLCG's read with its
+    accept flag)."""
+    from deconlib_amd import _native as N
+    from deconlib_amd.nlcg import _NLCGState
+    from deconlib_amd.rl import _RLState
+
+    vox = int(np.prod(obs.shape))
+    st = _RLState(obs, blur, background=0.0, num_iter=rl_iters + 1, eval_interval=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(rl_iters + 1):
+        st.step(k)  # step k sums the loss of iterate k - 1
+    torch.cuda.synchronize()
+    rl_s = (time.perf_counter() - t0) * rl_iters / (rl_iters + 1)
+    target = float(st.loss_slots[st.slot_of[rl_iters - 1]].item()) / vox
+    del st
+    ns = _NLCGState(obs, blur, 0.0, None, 0.0, None, 3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reached = None
+    for k in range(max_nlcg):
+        ns.line_search()
+        if ns.S.read()[N.NL_ACCEPT] == 0.0:
+            break
+        ns.accept(restart=False)
+        if ns.S.read()[N.NL_LOSS] / vox <= target:
+            reached = k + 1
+            break
+    torch.cuda.synchronize()
+    nl_s = time.perf_counter() - t0
+    return {"target_mean_idiv": target, "rl_iterations": rl_iters, "rl_seconds": rl_s,
+            "nlcg_iterations": reached, "nlcg_seconds": nl_s if reached else None,
+            "speedup": (rl_s / nl_s) if reached else None}
+
+
 def _time_cg_steps(torch, step, read, steps, warmup):
     for _ in range(warmup):
         step()
@@ -579,6 +617,7 @@ def main():
     pdhg = bench_pdhg(torch, N, obs, blur, args.steps, args.warmup)
     log(f"pdhg: {json.dumps(pdhg)}")
     nlcg = bench_nlcg(torch, obs, blur, args.steps, args.warmup)
+    nlcg["vs_rl_to_equal_fit"] = nlcg_vs_rl(torch, obs, blur)
     log(f"nlcg: {json.dumps(nlcg)}")
     erdecon = bench_erdecon(torch, obs, blur)
     log(f"erdecon: {json.dumps(erdecon)}")

@@ -16,7 +16,12 @@ on the size:
   is bit-identical to its solo result;
 * the large single volume (256x2048x2048, canvas 512x4096x4096): adjoint
   identity of the convolution and one PDHG-Hessian iteration at full size
-  (skipped when the device has less than 200 GB free).
+  (skipped when the device has less than 200 GB free);
+* NLCG, ER-Decon and JetNewton on the RL workload (64x256x256): NLCG's first
+  iteration against the float32 oracle (relative L2 <= 1e-5), and for all three
+  over several iterations: finite non-negative iterates, a decreasing objective
+  (NLCG and ER-Decon accept only decreasing steps, JetNewton only Armijo steps)
+  and pred equal to the operator applied to the result.
 """
 
 import numpy as np
@@ -157,3 +162,42 @@ def test_large_volume_full_size(gpu):
     torch.cuda.synchronize()
     assert bool(torch.isfinite(st.x).all()) and float(st.x.min()) >= 0.0
     assert np.isfinite(st.loss_value(0))
+
+
+def test_other_solvers_config_full_size(gpu):
+    from deconlib_amd import (AnisotropicHessian3D, LinearFFTConvolver, erdecon_with_operator,
+                              jetnewton_with_operator, nlcg_with_operator)
+    from deconlib_amd.psf import widefield_psf
+    from deconlib_amd.synth import CONFIG0_OPTICS, blob_volume
+    from oracle import nlcg as NL
+
+    shape = (64, 256, 256)
+    print(f"NLCG / ER-Decon / JetNewton at {shape}", flush=True)
+    psf = widefield_psf(shape, 0.085, 0.1, CONFIG0_OPTICS).astype(np.float32)
+    obj = blob_volume(shape, 200, np.random.default_rng(1001))
+    obs = _gpu_observation(psf, obj, 1001)
+    op = LinearFFTConvolver(psf, shape)
+
+    one = nlcg_with_operator(obs, op, num_iter=1, background=10.0, tol=0.0, slack=0.0)
+    ref = NL.nlcg_with_operator(obs, O.LinearFFTConvolver(psf, shape), num_iter=1, background=10.0, tol=0.0,
+                                slack=0.0, dtype=np.float32)
+    assert rel_l2(one.restored.cpu().numpy(), ref.restored) <= 1e-5
+
+    def check(res, bg, scale=1.0):
+        x = res.restored
+        assert bool(torch.isfinite(x).all()) and float(x.min()) >= 0.0
+        pred = op.forward(x).cpu().numpy() + bg
+        assert rel_l2(res.pred.cpu().numpy(), pred) <= 1e-6
+        losses = np.asarray(res.loss_history)
+        assert np.all(np.isfinite(losses)) and np.all(np.diff(losses) < 0), losses
+
+    nl = nlcg_with_operator(obs, op, num_iter=12, background=10.0, eval_interval=1, tol=0.0, slack=0.0)
+    assert nl.iterations == 12
+    check(nl, 10.0)
+    er = erdecon_with_operator(obs, op, num_iter=3, background=10.0, data_term="poisson", eval_interval=1,
+                               newton_tol=0.0)
+    check(er, 10.0)
+    jn = jetnewton_with_operator(obs, op, AnisotropicHessian3D((0.85, 1.0, 1.0)), s0=3.0, background=10.0,
+                                 num_iter=3, eval_interval=1, newton_tol=0.0)
+    check(jn, 10.0)
+
