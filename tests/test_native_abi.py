@@ -67,3 +67,34 @@ def test_plan_errors_map_to_reference_exceptions():
     assert rc == N.DH_EUNSUPPORTED
     with pytest.raises(NotImplementedError):
         N.check(rc)
+
+
+def _null_args(sig):
+    """Null pointers, zero sizes and zero scalars for a ctypes argument list."""
+    out = []
+    for a in sig:
+        if a in (C.c_float, C.c_double):
+            out.append(0.0)
+        elif a in (C.c_int, C.c_int32, C.c_int64):
+            out.append(0)
+        else:
+            out.append(None)
+    return out
+
+
+@pytest.mark.parametrize("prefix", ["dh_nlcg_", "dh_er_", "dh_jn_"])
+def test_solver_entry_points_reject_bad_arguments(prefix):
+    """Every solver entry point validates its arguments before any device work: null
+    pointers and empty sizes return the invalid-argument status with a message (the
+    Python layer raises ValueError), and the partials sizes are positive."""
+    names = [n for n in N.SYMBOLS if n.startswith(prefix)]
+    assert len(names) >= 9
+    for name in names:
+        if name.endswith("_partials"):
+            assert N.fns[name]() > 0
+            continue
+        rc = N.fns[name](*_null_args(N.SYMBOLS[name][1:]))
+        assert rc == N.DH_EINVAL, name
+        with pytest.raises(ValueError, match="arguments"):
+            N.check(rc, name)
+
