@@ -81,10 +81,11 @@ __device__ __forceinline__ float epi_apply(const Epi& e, float v, int64_t idx, d
 }
 
 // epi_apply on the sample pair (idx, idx + 1) with float2 operand loads;
-// the pair is accumulated in the same order as two epi_apply calls.
-template <class Cnt = AllSamples>
+// the pair is accumulated in the same order as two epi_apply calls.  K >= 0
+// fixes the kind at compile time (e.kind is then not read).
+template <int K = -1, class Cnt = AllSamples>
 __device__ __forceinline__ float2 epi_apply2(const Epi& e, float2 v, int64_t idx, double& acc, Cnt cnt = Cnt()) {
-  switch (e.kind) {
+  switch (K >= 0 ? K : e.kind) {
     case EPI_FLOOR: return make_float2(fmaxf(v.x, e.a), fmaxf(v.y, e.a));
     case EPI_RL_RATIO: {
       const float2 o = *reinterpret_cast<const float2*>(e.aux0 + idx);
@@ -711,50 +712,80 @@ __device__ __forceinline__ void xi_body(const XiArgs& a, int64_t lb, int RPB, fl
       bx1 = bx[5];
     }
   }
-  auto gst = [&](int e, float2 v) {
-    const int q0 = 2 * e, q1 = 2 * e + 1;
-    using Cnt = std::conditional_t<BOX, InBox, AllSamples>;
-    Cnt c0, c1;
-    if constexpr (BOX) {
-      c0 = InBox{in_row, bx0, bx1, q0};
-      c1 = InBox{in_row, bx0, bx1, q1};
-    }
-    float2 o = make_float2(0.f, 0.f);
-    if (ok) {
-      if (vec2 && q1 < s2) {
-        o = epi_apply2(a.epi, v, base + q0, acc, c0);
-        if (!RT || drow) st_nt(reinterpret_cast<float2*>(drow + q0), o);
-      } else {
-        if (q0 < s2) {
-          o.x = epi_apply(a.epi, v.x, base + q0, acc, c0);
-          if (!RT || drow) __builtin_nontemporal_store(o.x, drow + q0);
-        }
-        if (q1 < s2) {
-          o.y = epi_apply(a.epi, v.y, base + q1, acc, c1);
-          if (!RT || drow) __builtin_nontemporal_store(o.y, drow + q1);
-        }
-      }
-    }
-    if constexpr (RT) lrow[xrow_lds(e)] = o;
-  };
   auto lld = [&](int e) { return lrow[xrow_lds(e)]; };
   auto lst = [&](int e, float2 v) { lrow[xrow_lds(e)] = v; };
+  // the inverse transform, its output row left in LDS in natural order
   if constexpr (M == 1) {
     float2 v = make_float2(0.f, 0.f);
     if (t == 0) v = pre(0);
     __syncthreads();
-    if (t == 0) gst(0, v);
+    if (t == 0) lst(0, v);
   } else {
-    // the first pass reads the LDS row and overwrites it: full barrier between
+    // the first pass reads the LDS row and overwrites it: full barrier between;
+    // so does the last one (LAST_LDS)
     constexpr Sched S = Geo<M>::F;
     constexpr int R0 = S.r[0];
-    if constexpr (S.np == 1) {
-      pass<M, T, R0, 1, 1>(t, twm, pre, gst, true);
-    } else {
-      pass<M, T, R0, 1, 1>(t, twm, pre, lst, true);
+    pass<M, T, R0, 1, 1>(t, twm, pre, lst, true);
+    if constexpr (S.np > 1) {
       __syncthreads();
-      // a round trip's stores go to the LDS row the last pass reads: LAST_LDS
-      passes<M, false, 1, 1, RT>(t, twm, pre, gst, lld, lst, false);
+      passes<M, false, 1, 1, true>(t, twm, pre, lst, lld, lst, false);
+    }
+  }
+  __syncthreads();
+  // The epilogue over this thread's sample pairs e = t, t + T, ... of its row:
+  // one loop per epilogue kind, so that each kind's code appears once (not once
+  // per pair of the unrolled transform) and four pairs' operand loads are in
+  // flight together.  Output pairs beyond s2 (and rows past the volume) are
+  // zero; a round trip writes every pair back to the LDS row.
+  constexpr int XE = (M + T - 1) / T;
+  auto cnt_of = [&](int q) {
+    using Cnt = std::conditional_t<BOX, InBox, AllSamples>;
+    Cnt c;
+    if constexpr (BOX) c = InBox{in_row, bx0, bx1, q};
+    return c;
+  };
+  auto sweep2 = [&](auto kind) {
+    constexpr int K = decltype(kind)::value;
+#pragma unroll 4
+    for (int i = 0; i < XE; ++i) {
+      const int e = t + i * T;
+      if (XE * T != M && e >= M) break;
+      const int q0 = 2 * e;
+      float2 o = make_float2(0.f, 0.f);
+      if (ok && q0 < s2) {
+        o = epi_apply2<K>(a.epi, lrow[xrow_lds(e)], base + q0, acc, cnt_of(q0));
+        if (!RT || drow) st_nt(reinterpret_cast<float2*>(drow + q0), o);
+      }
+      if constexpr (RT) lrow[xrow_lds(e)] = o;
+    }
+  };
+  if (vec2) {
+    // (s2 even: a pair lies wholly inside the row or wholly beyond it)
+    switch (a.epi.kind) {
+      case EPI_FLOOR: sweep2(std::integral_constant<int, EPI_FLOOR>()); break;
+      case EPI_RL_RATIO: sweep2(std::integral_constant<int, EPI_RL_RATIO>()); break;
+      case EPI_RL_UPDATE: sweep2(std::integral_constant<int, EPI_RL_UPDATE>()); break;
+      case EPI_ADD: sweep2(std::integral_constant<int, EPI_ADD>()); break;
+      default: sweep2(std::integral_constant<int, EPI_NONE>()); break;
+    }
+  } else {
+    // odd row lengths or unaligned operands: one sample at a time
+#pragma unroll 1
+    for (int e = t; e < M; e += T) {
+      const float2 v = lrow[xrow_lds(e)];
+      const int q0 = 2 * e, q1 = 2 * e + 1;
+      float2 o = make_float2(0.f, 0.f);
+      if (ok) {
+        if (q0 < s2) {
+          o.x = epi_apply(a.epi, v.x, base + q0, acc, cnt_of(q0));
+          if (!RT || drow) __builtin_nontemporal_store(o.x, drow + q0);
+        }
+        if (q1 < s2) {
+          o.y = epi_apply(a.epi, v.y, base + q1, acc, cnt_of(q1));
+          if (!RT || drow) __builtin_nontemporal_store(o.y, drow + q1);
+        }
+      }
+      if constexpr (RT) lrow[xrow_lds(e)] = o;
     }
   }
   if constexpr (RT) {

@@ -51,13 +51,14 @@ def rl_workload(shape=(64, 256, 256), n_blobs=200, seed=1000, lateral=0.085, axi
     return psf.astype(np.float32), obj.astype(np.float32), obs
 
 
-def aberrated_psf(shape, rng: np.random.Generator, lateral=0.085, axial=0.1):
-    """Config-0 optics with IndexMismatch depth ~ U[0, 20] um and Zernike spherical ~ U[-0.5, 0.5] rad."""
+def aberrated_psf(shape, rng: np.random.Generator, lateral=0.085, axial=0.1, device: bool = False):
+    """Config-0 optics with IndexMismatch depth ~ U[0, 20] um and Zernike spherical ~ U[-0.5, 0.5] rad
+    (``device=True``: synthesised on the GPU, a float32 device tensor)."""
     from .psf import IndexMismatch, ZernikeAberration
 
     ab = [IndexMismatch(depth=float(rng.uniform(0.0, 20.0))),
           ZernikeAberration({ZernikeAberration.SPHERICAL: float(rng.uniform(-0.5, 0.5))})]
-    return widefield_psf(shape, lateral, axial, CONFIG0_OPTICS, aberrations=ab)
+    return widefield_psf(shape, lateral, axial, CONFIG0_OPTICS, aberrations=ab, device=device)
 
 
 def serving_requests(n_jobs: int, rate_per_s: float, seed: int, z_range=(16, 48), xy_range=(64, 256),
