@@ -631,7 +631,7 @@ def main():
     ap.add_argument("--serving-rate", type=float, default=1000.0)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is a design note, not a measured path)", file=sys.stderr)
         sys.exit(2)
 

@@ -41,7 +41,7 @@ _TAU, _SIGMA, _THETA, _STEP, _COUNT, _DONE, _FINAL = 0, 1, 2, 3, 4, 5, 6
 
 
 class HessianRegularizer:
-    """Hessian regularizer wrapper (pdhg_mlx.py:380-451); 3-D only in this build."""
+    """Hessian regularizer wrapper (pdhg_mlx.py:380-451); 3-D volumes only."""
 
     def __init__(self, ndim: int, r: float = 1.0, norm: str = "L1"):
         if ndim not in (1, 2, 3):
@@ -268,7 +268,7 @@ def solve_pdhg_mlx(
     if regularization not in ("identity", "gradient", "hessian"):
         raise ValueError(f"Unknown regularization: {regularization}")
     if regularization != "hessian":
-        raise NotImplementedError(f"regularization={regularization!r} is not implemented in this build")
+        raise NotImplementedError(f"regularization={regularization!r} is not implemented by this package")
     psf_shape = tuple(int(s) for s in (psf.shape if hasattr(psf, "shape") else np.shape(psf)))
     if sampling_factors is not None:
         f = _normalize_sampling_factors(sampling_factors, ndim)
@@ -355,7 +355,7 @@ def solve_pdhg_with_operator(
     """
     obs = as_f32(observed)
     if regularization != "hessian":
-        raise NotImplementedError(f"regularization={regularization!r} is not implemented in this build")
+        raise NotImplementedError(f"regularization={regularization!r} is not implemented by this package")
     shape = tuple(getattr(blur_op, "in_shape", None) or obs.shape)
     x0 = as_f32(init) if init is not None else torch.full(shape, 0.01, dtype=torch.float32, device=device())
     x0 = _nonneg(x0)
