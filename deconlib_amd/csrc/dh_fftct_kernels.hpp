@@ -460,6 +460,41 @@ __device__ __forceinline__ void blocked_rows(int64_t row0, int RPB, int64_t nrow
   }
 }
 
+// blocked_rows for a workgroup of RPB x T threads (T threads per row, a
+// multiple of kBlk): thread thr then always has column j = thr % kBlk of row
+// rl = (thr / kBlk) % RPB, and its blocks advance by T / kBlk, so the row split
+// and the 64-bit offset are formed once and stepped (the same visits, in the
+// same order, as blocked_rows).
+// thr = threadIdx.x, passed through opaque() by the caller so that the set-up is
+// formed here rather than hoisted (and held in registers) across earlier passes.
+template <int T, class Fn>
+__device__ __forceinline__ void blocked_rows_t(int thr, int64_t row0, int RPB, int64_t nrows, int H, int prow,
+                                               Fn fn) {
+  if constexpr (T % kBlk != 0) {
+    blocked_rows(row0, RPB, nrows, H, prow, fn);
+  } else {
+    const int nbk = (H + kBlk - 1) / kBlk;
+    const int j = thr & (kBlk - 1);
+    const int rl = (thr / kBlk) & (RPB - 1);
+    int64_t z = row0 / prow;
+    int y = (int)(row0 - z * prow) + rl;
+    if (y >= prow) {
+      z += y / prow;
+      y %= prow;
+    }
+    const bool valid = row0 + rl < nrows;
+    const int64_t zb = z * prow * (int64_t)H;
+    const int64_t step = (int64_t)(T / kBlk) * kBlk * prow;
+    int b = thr >> (__builtin_ctz(RPB) + 3);
+    int64_t off = zb + (int64_t)b * kBlk * prow + (int64_t)y * kBlk + j;  // a full-width block
+    for (; b < nbk - 1; b += T / kBlk, off += step) fn(rl, b * kBlk + j, off, valid);
+    if (b == nbk - 1) {
+      const int w = H - b * kBlk;
+      if (j < w) fn(rl, b * kBlk + j, zb + (int64_t)b * kBlk * prow + (int64_t)y * w + j, valid);
+    }
+  }
+}
+
 // Both twiddle tables of an X workgroup in LDS after its rows: the pass tables
 // of the length-M transform at [0, tw_table_size), the M+1 split twiddles at
 // [M, 2M].  One transform length per launch, so one copy serves every volume.
@@ -588,7 +623,9 @@ __global__ void __launch_bounds__(kMaxThreads) k_xf(Pack<XfArgs> pk, int RPB) {
     passes<M, false, -1, 1, true>(t, twm, lld, lst, lld, lst, false);
     __syncthreads();
     if (a.blk > 0) {
-      blocked_rows(lb * RPB, RPB, a.nrows, a.H, a.prow, [&](int r, int k, int64_t off, bool valid) {
+      int thr = threadIdx.x;
+      opaque(thr);
+      blocked_rows_t<T>(thr, lb * RPB, RPB, a.nrows, a.H, a.prow, [&](int r, int k, int64_t off, bool valid) {
         if (valid) st_nt(a.dst + off, xf_post<M>(lds + r * RS, twp, k));
       });
     } else if (row < a.nrows) {
@@ -801,7 +838,9 @@ __device__ __forceinline__ void xi_body(const XiArgs& a, int64_t lb, int RPB, fl
     passes<M, false, -1, 0, true>(t2, twm, lld2, lst2, lld2, lst2, false);
     __syncthreads();
     if (a.blk > 0) {
-      blocked_rows(lb * RPB, RPB, a.nrows, a.H, a.prow, [&](int r, int k, int64_t off, bool valid) {
+      int thr3 = thr;
+      opaque(thr3);
+      blocked_rows_t<T>(thr3, lb * RPB, RPB, a.nrows, a.H, a.prow, [&](int r, int k, int64_t off, bool valid) {
         if (valid) st_nt(a.spec + off, xf_post<M>(lds + r * RS, twp, k));
       });
     } else if (ok) {
