@@ -116,6 +116,48 @@ def bench_pdhg(torch, N, obs, blur, steps, warmup):
     }
 
 
+def bench_pdhg_binned(torch, steps=10, warmup=2, seed=1001):
+    """The binned PDHG configuration at full size: object 128x512x512 (2000 blobs + 10),
+    widefield PSF of the same optics at 0.0425/0.05 um (synthesised on the GPU), data
+    64x256x256 = FractionalAreaDownsample(2) o LinearFFTConvolver (canvas 256x1024x1024),
+    Poisson at peak ~500 counts; PDHG-Hessian (alpha 1e-3, L1) iterations from the start
+    solve_pdhg_mlx(bin_factors=2) uses.  Voxel-iterations count the 128x512x512 object."""
+    from deconlib_amd import FractionalAreaDownsample, LinearFFTConvolver, compose
+    from deconlib_amd.pdhg import _PDState, _nonneg
+    from deconlib_amd.psf import widefield_psf
+    from deconlib_amd.synth import CONFIG0_OPTICS, blob_volume
+
+    hi = (128, 512, 512)
+    psf = widefield_psf(hi, 0.0425, 0.05, CONFIG0_OPTICS, device=True)
+    obj = blob_volume(hi, 2000, np.random.default_rng(seed)).astype(np.float32)
+    conv = LinearFFTConvolver(psf, hi)
+    A = compose(FractionalAreaDownsample(scale=(2.0, 2.0, 2.0)), conv)
+    clean = A.forward(torch.from_numpy(obj).cuda()).clamp_min(0.0)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    obs = torch.poisson(clean * (500.0 / float(clean.max())), generator=gen)
+    st = _PDState(obs, A, _nonneg(A.adjoint(obs)), 1e-3, warmup + steps, background=0.0, r=1.0, tol=0.0)
+    for k in range(warmup):
+        st.step(k, False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(warmup, warmup + steps):
+        st.step(k, False)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    vox = int(np.prod(hi))
+    return {
+        "metric": "voxel-iters/s (binned PDHG-Hessian) on the 128x512x512 object grid, data 64x256x256",
+        "value": vox * steps / dt,
+        "ms_per_step": dt * 1e3 / steps,
+        "steps": steps,
+        "warmup": warmup,
+        "canvas": list(conv.canvas_shape),
+        "data_shape": list(obs.shape),
+        "finite": bool(torch.isfinite(st.x).all().item()),
+    }
+
+
 def nlcg_iteration_bytes(S, P):
     """Algorithmic HBM bytes of one NLCG iteration: three plain convolutions (K(s d),
     K(d d), K^T w) plus the elementwise passes, in float32 words per voxel: line-search
@@ -684,6 +726,8 @@ def main():
     ok = bool(torch.isfinite(st.x).all().item())
     pdhg = bench_pdhg(torch, N, obs, blur, args.steps, args.warmup)
     log(f"pdhg: {json.dumps(pdhg)}")
+    pdhg["binned_configuration"] = bench_pdhg_binned(torch)
+    log(f"pdhg binned configuration: {json.dumps(pdhg['binned_configuration'])}")
     nlcg = bench_nlcg(torch, obs, blur, args.steps, args.warmup)
     nlcg["vs_rl_to_equal_fit"] = nlcg_vs_rl(torch, obs, blur)
     log(f"nlcg: {json.dumps(nlcg)}")
