@@ -158,6 +158,76 @@ def bench_pdhg_binned(torch, steps=10, warmup=2, seed=1001):
     }
 
 
+def bench_large_volume(torch, steps=3, warmup=1, seed=1004):
+    """The large single-volume configuration at full size: 256x2048x2048 float32 on a
+    512x4096x4096 canvas, PDHG-Hessian (alpha 5e-4, L1) iterations.  Object: 1e5 blobs
+    + 10 and 64 thin filaments (intensity 300); widefield PSF of the volume's size with
+    IndexMismatch depth 10 um, synthesised on the GPU; Poisson data.  Skipped below
+    200 GB of free device memory (about 160 GB stay resident)."""
+    import ctypes as C
+
+    from deconlib_amd import LinearFFTConvolver
+    from deconlib_amd import _native as N
+    from deconlib_amd.pdhg import _PDState, _nonneg
+    from deconlib_amd.psf import IndexMismatch, widefield_psf
+    from deconlib_amd.synth import CONFIG0_OPTICS, blob_volume, filaments
+
+    free, _ = torch.cuda.mem_get_info()
+    if free < 200e9:
+        return {"skipped": f"needs 200 GB free device memory, {free / 1e9:.0f} GB free"}
+    shape = (256, 2048, 2048)
+    t0 = time.perf_counter()
+    rng = np.random.default_rng(seed)
+    psf = widefield_psf(shape, LATERAL, AXIAL, CONFIG0_OPTICS, aberrations=[IndexMismatch(depth=10.0)],
+                        device=True)
+    op = LinearFFTConvolver(psf, shape)
+    del psf
+    obj = blob_volume(shape, 100000, rng)
+    obj = torch.from_numpy(np.maximum(obj, filaments(shape, 64, rng)).astype(np.float32)).cuda()
+    clean = op.forward(obj).clamp_min_(0.0)
+    del obj
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    obs = torch.poisson(clean, generator=gen)
+    del clean
+    st = _PDState(obs, op, _nonneg(obs), 5e-4, warmup + steps, background=0.0, r=1.0, tol=0.0)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    for k in range(warmup):
+        st.step(k, False)
+    torch.cuda.synchronize()
+    N.call("dh_timing_read", None, None, 1)
+    N.call("dh_timing_enable", 1)
+    t0 = time.perf_counter()
+    for k in range(warmup, warmup + steps):
+        st.step(k, False)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    N.call("dh_timing_enable", 0)
+    ms = (C.c_double * 5)()
+    cnt = (C.c_int64 * 5)()
+    N.call("dh_timing_read", ms, cnt, 1)
+    pb = conv_pass_bytes(shape, op.canvas_shape)
+    passes = {name: {"avg_ms": ms[i] / cnt[i], "gbs": pb[name] / (ms[i] / cnt[i] * 1e-3) / 1e9,
+                     "launches": int(cnt[i])} for i, name in enumerate(N.PASS_NAMES) if cnt[i]}
+    vox = int(np.prod(shape))
+    out = {
+        "metric": "voxel-iters/s (PDHG-Hessian) at 256x2048x2048 fp32, canvas 512x4096x4096",
+        "value": vox * steps / dt,
+        "ms_per_step": dt * 1e3 / steps,
+        "steps": steps,
+        "warmup": warmup,
+        "setup_s": setup_s,
+        "canvas": list(op.canvas_shape),
+        "device_gb_resident": torch.cuda.memory_allocated() / 1e9,
+        "passes": passes,
+        "finite": bool(torch.isfinite(st.x).all().item()),
+    }
+    del st, op, obs
+    torch.cuda.empty_cache()
+    return out
+
+
 def nlcg_iteration_bytes(S, P):
     """Algorithmic HBM bytes of one NLCG iteration: three plain convolutions (K(s d),
     K(d d), K^T w) plus the elementwise passes, in float32 words per voxel: line-search
@@ -728,6 +798,8 @@ def main():
     log(f"pdhg: {json.dumps(pdhg)}")
     pdhg["binned_configuration"] = bench_pdhg_binned(torch)
     log(f"pdhg binned configuration: {json.dumps(pdhg['binned_configuration'])}")
+    pdhg["large_volume"] = bench_large_volume(torch)
+    log(f"pdhg large volume: {json.dumps(pdhg['large_volume'])}")
     nlcg = bench_nlcg(torch, obs, blur, args.steps, args.warmup)
     nlcg["vs_rl_to_equal_fit"] = nlcg_vs_rl(torch, obs, blur)
     log(f"nlcg: {json.dumps(nlcg)}")

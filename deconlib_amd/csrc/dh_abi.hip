@@ -181,7 +181,8 @@ bool ct_row_tile_ok(int m, int R) {
 }
 // Default launch shapes (what the autotuner settles on for the 64x512x512
 // workload, canvas 128x1024x1024): columns per workgroup = the widest power of
-// two within 64 KiB LDS and 512 threads (256 for the fused Z pass), one whole
+// two within 64 KiB LDS and 512 threads (256 for the fused Z pass unless that
+// leaves it fewer than 16 columns, as for 512-point columns), one whole
 // block for the blocked Y layout whenever it fits 160 KiB and 512 threads; rows per workgroup = the widest power of two within 512
 // threads and 96 KiB LDS, halved (down to one wave) while a volume has fewer
 // than kMinRowTiles tiles: an X pass keeps a tile's load, transforms and stores
@@ -196,6 +197,10 @@ int ct_col_tile(int n, int64_t ncols, int over, int kind) {
   const int64_t lds_cap = kind == COL_BLOCKED ? kCtMaxLds : kCtDefaultLds;
   int L = 1;
   while (L < cap && 2 * L * T <= max_threads && col_lds_bytes(n, 2 * L, kind) <= lds_cap) L *= 2;
+  // the fused Z pass reads each plane's L columns as one run: below 16 columns
+  // (128-byte runs, long columns) it takes up to the full thread limit
+  if (kind == COL_Z)
+    while (L < 16 && 2 * L * T <= kCtMaxThreads && col_lds_bytes(n, 2 * L, kind) <= lds_cap) L *= 2;
   while (L > 1 && L / 2 >= ncols) L /= 2;
   return L;
 }

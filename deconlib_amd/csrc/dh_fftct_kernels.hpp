@@ -148,6 +148,19 @@ __device__ __forceinline__ void st_nt(float2* p, float2 v) {
   const f2v w = {v.x, v.y};
   __builtin_nontemporal_store(w, reinterpret_cast<f2v*>(p));
 }
+// Non-temporal (NT) or ordinary access: a column pass whose workgroups each take
+// part of a block's 64-byte row runs uses ordinary accesses, so that the cache
+// can join the pieces the workgroups load and store.
+template <bool NT>
+__device__ __forceinline__ float2 ld_s(const float2* p) {
+  if constexpr (NT) return ld_nt(p);
+  else return *p;
+}
+template <bool NT>
+__device__ __forceinline__ void st_s(float2* p, float2 v) {
+  if constexpr (NT) st_nt(p, v);
+  else *p = v;
+}
 
 // ---- column pass (Y forward / inverse, OTF Z) ----
 // LDS slot of line element e, column ln: 8 extra slots every 16 elements spread
@@ -185,19 +198,29 @@ __device__ __forceinline__ ColLane col_lane(const ColArgs& a, int64_t lb, int L,
 
 // Line element e of the zero-padded canvas: rows [0, lo) and [N-hi, N) hold
 // the source rows 0..lo+hi-1, the rest is zero.
-template <int N>
+template <int N, bool NT = true>
 __device__ __forceinline__ float2 col_ld(const ColArgs& a, const ColLane& c, int e) {
   int r = -1;
   if (e < a.lo) r = e;
   else if (e >= N - a.hi) r = e - (N - a.lo - a.hi);
   if (!c.ok || r < 0) return make_float2(0.f, 0.f);
-  return ld_nt(c.sb + (int64_t)r * c.srs);
+  return ld_s<NT>(c.sb + (int64_t)r * c.srs);
 }
 
 // Persistent column pass: each workgroup walks tiles b = blockIdx.x,
 // blockIdx.x + gridDim.x, ... and loads the next tile's first-pass operands
 // into registers while the current tile's remaining passes run.
-template <int N, int DIR>
+// Workgroups are dealt to the 8 XCDs in turn (workgroup b to XCD b % 8).  Numbering
+// the tiles XCD by XCD instead gives each XCD a contiguous run of them, so tiles that
+// share a block's row runs also share an L2.  A bijection of [0, G): for every XCD x
+// the b = x + 8 i < G map to consecutive numbers from x q + min(x, r).
+__device__ __forceinline__ int64_t xcd_major(int64_t b, int64_t G) {
+  constexpr int64_t kXcd = 8;
+  const int64_t q = G / kXcd, r = G % kXcd, x = b % kXcd;
+  return x * q + (x < r ? x : r) + b / kXcd;
+}
+
+template <int N, int DIR, bool NT = true>
 __global__ void __launch_bounds__(kMaxThreads) k_col(Pack<ColArgs> pk, int L) {
   extern __shared__ float2 lds[];
   constexpr int T = Geo<N>::T;
@@ -210,14 +233,14 @@ __global__ void __launch_bounds__(kMaxThreads) k_col(Pack<ColArgs> pk, int L) {
   const int ln0 = thr % G + G * (thr / (G * T));
   const int t0 = (thr / G) % T;
   const int64_t total = pk.prefix[pk.nvol];
-  int64_t b = blockIdx.x;
+  int64_t b = NT ? (int64_t)blockIdx.x : xcd_major(blockIdx.x, gridDim.x);
   if (b >= total) return;
   if constexpr (N == 1) {  // length-1 transform: scaled copy
     for (; b < total; b += gridDim.x) {
       const int v = pick(pk, b);
       const ColArgs& a = pk.a[v];
       const ColLane c = col_lane(a, b - pk.prefix[v], L, ln0);
-      const float2 x = col_ld<N>(a, c, 0);
+      const float2 x = col_ld<N, NT>(a, c, 0);
       if (c.ok && a.n_out > 0) c.db[0] = make_float2(x.x * a.scale, x.y * a.scale);
     }
     return;
@@ -229,7 +252,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_col(Pack<ColArgs> pk, int L) {
     const int v = pick(pk, bb);
     const ColArgs& a = pk.a[v];
     const ColLane c = col_lane(a, bb - pk.prefix[v], L, ln);
-    pass_load<N, T, R0>(u, t, [&](int e) { return col_ld<N>(a, c, e); });
+    pass_load<N, T, R0>(u, t, [&](int e) { return col_ld<N, NT>(a, c, e); });
   };
   fetch(b);
   for (;;) {
@@ -241,7 +264,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_col(Pack<ColArgs> pk, int L) {
     const ColArgs& a = pk.a[v];
     const ColLane c = col_lane(a, b - pk.prefix[v], L, ln);
     auto gst = [&](int e, float2 val) {
-      if (c.ok && e < a.n_out) st_nt(c.db + (int64_t)e * c.drs, make_float2(val.x * a.scale, val.y * a.scale));
+      if (c.ok && e < a.n_out) st_s<NT>(c.db + (int64_t)e * c.drs, make_float2(val.x * a.scale, val.y * a.scale));
     };
     const int64_t bn = b + gridDim.x;
     if constexpr (S.np == 1) {
@@ -1044,16 +1067,22 @@ struct Launch {
   static unsigned grid(const void* k, int threads, size_t shm, int64_t nblocks, bool persist) {
     return persist ? persistent_grid(k, threads, shm, nblocks) : (unsigned)nblocks;
   }
+  template <int DIR, bool NT>
+  static void col_as(const Pack<ColArgs>& pk, int L, int64_t nblocks, bool persist, size_t shm, hipStream_t st) {
+    const void* k = (const void*)k_col<N, DIR, NT>;
+    (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    hipLaunchKernelGGL((k_col<N, DIR, NT>), dim3(grid(k, L * T, shm, nblocks, persist)), dim3(L * T), shm, st, pk, L);
+  }
+  // blocked columns in tiles narrower than a block: ordinary accesses (ld_s/st_s)
   static hipError_t col(int dir, const Pack<ColArgs>& pk, int L, int64_t nblocks, bool persist, hipStream_t st) {
     const size_t shm = (size_t)col_lds_elems(N, L) * sizeof(float2);
+    const bool split = pk.a[0].blk > 0 && L < kBlk;
     if (dir < 0) {
-      const void* k = (const void*)k_col<N, -1>;
-      (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-      hipLaunchKernelGGL((k_col<N, -1>), dim3(grid(k, L * T, shm, nblocks, persist)), dim3(L * T), shm, st, pk, L);
+      if (split) col_as<-1, false>(pk, L, nblocks, persist, shm, st);
+      else col_as<-1, true>(pk, L, nblocks, persist, shm, st);
     } else {
-      const void* k = (const void*)k_col<N, 1>;
-      (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-      hipLaunchKernelGGL((k_col<N, 1>), dim3(grid(k, L * T, shm, nblocks, persist)), dim3(L * T), shm, st, pk, L);
+      if (split) col_as<1, false>(pk, L, nblocks, persist, shm, st);
+      else col_as<1, true>(pk, L, nblocks, persist, shm, st);
     }
     return hipGetLastError();
   }

@@ -61,6 +61,22 @@ def aberrated_psf(shape, rng: np.random.Generator, lateral=0.085, axial=0.1, dev
     return widefield_psf(shape, lateral, axial, CONFIG0_OPTICS, aberrations=ab, device=device)
 
 
+def filaments(shape, n: int, rng: np.random.Generator, intensity: float = 300.0, vertices: int = 6) -> np.ndarray:
+    """Thin filaments: n random 3D polylines one voxel wide (each segment sampled every
+    0.25 voxel and rounded), value `intensity` on the line, 0 elsewhere."""
+    vol = np.zeros(tuple(int(s) for s in shape), np.float32)
+    hi = np.array(vol.shape, np.float64) - 1
+    for _ in range(n):
+        pts = rng.uniform(0, 1, size=(vertices, 3)) * hi
+        idx = []
+        for a, b in zip(pts[:-1], pts[1:]):
+            k = int(np.ceil(np.abs(b - a).max() * 4)) + 1
+            idx.append(np.rint(a + np.linspace(0, 1, k)[:, None] * (b - a)).astype(np.int64))
+        idx = np.unique(np.concatenate(idx), axis=0)
+        vol[idx[:, 0], idx[:, 1], idx[:, 2]] = intensity
+    return vol
+
+
 def serving_requests(n_jobs: int, rate_per_s: float, seed: int, z_range=(16, 48), xy_range=(64, 256),
                      rl_fraction=0.7, rl_iters=100, pdhg_iters=200, density=200 / (64 * 256 * 256),
                      blur=None):

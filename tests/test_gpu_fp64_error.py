@@ -108,25 +108,10 @@ def test_batched_config_fp64_error(gpu):
         _report(f"batched RL {tuple(res.restored.shape)}, {it} it", res.restored.cpu().numpy(), r32, r64)
 
 
-def _filaments(shape, n, rng, intensity=300.0, vertices=6):
-    """Random 3D polylines one voxel wide (samples every 0.25 voxel along each segment)."""
-    vol = np.zeros(shape, np.float64)
-    hi = np.array(shape, np.float64) - 1
-    for _ in range(n):
-        pts = rng.uniform(0, 1, size=(vertices, 3)) * hi
-        idx = []
-        for a, b in zip(pts[:-1], pts[1:]):
-            k = int(np.ceil(np.abs(b - a).max() * 4)) + 1
-            idx.append(np.rint(a + np.linspace(0, 1, k)[:, None] * (b - a)).astype(np.int64))
-        idx = np.unique(np.concatenate(idx), axis=0)
-        vol[idx[:, 0], idx[:, 1], idx[:, 2]] = intensity
-    return vol
-
-
 def test_large_volume_crop_fp64_error(gpu):
     from deconlib_amd import solve_pdhg_mlx
     from deconlib_amd.psf import IndexMismatch, widefield_psf
-    from deconlib_amd.synth import CONFIG0_OPTICS, blob_volume
+    from deconlib_amd.synth import CONFIG0_OPTICS, blob_volume, filaments
 
     shape, it = (128, 512, 512), 2
     print(f"large-volume solver on a {shape} crop, {it} iterations", flush=True)
@@ -134,7 +119,7 @@ def test_large_volume_crop_fp64_error(gpu):
     psf = widefield_psf(shape, 0.085, 0.1, CONFIG0_OPTICS, aberrations=[IndexMismatch(depth=10.0)])
     # 1e5 blobs over 256x2048x2048 voxels: the same density on the crop
     obj = blob_volume(shape, round(1e5 * np.prod(shape) / (256 * 2048 * 2048)), rng)
-    obj = np.maximum(obj, _filaments(shape, 24, rng))
+    obj = np.maximum(obj, filaments(shape, 24, rng))
     obs = _poisson(O.LinearFFTConvolver(psf, shape, dtype=np.float64).forward(obj), 1004)
     kw = dict(alpha=5e-4, num_iter=it, tol=0.0)
     gpu = solve_pdhg_mlx(torch.from_numpy(obs).cuda(), psf, **kw)
