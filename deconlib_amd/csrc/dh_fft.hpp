@@ -105,8 +105,15 @@ constexpr int kBlk = 8;
 // transform (one spare slot every 32 elements, indices 0..m used).
 constexpr int64_t col_lds_elems(int n, int L) { return (int64_t)n * L + 8 * (n >> 4) + 8; }
 constexpr int xrow_stride(int m) { return m + (m >> 5) + 1; }
-// X workgroup of R rows: the rows, then the twiddle tables (m + m + 1 entries).
-constexpr int64_t xrow_lds_elems(int m, int R) { return (int64_t)R * xrow_stride(m) + 2 * m + 1; }
+// Lengths whose X workgroups read the twiddle tables from global memory (through
+// the caches) instead of staging them in LDS: there the tables would take as much
+// LDS as the rows themselves and halve the rows per workgroup.
+constexpr bool xrow_tw_global(int m) { return m >= 2048; }
+// X workgroup of R rows: the rows, then the twiddle tables (m + m + 1 entries) unless
+// they are read from global memory.
+constexpr int64_t xrow_lds_elems(int m, int R) {
+  return (int64_t)R * xrow_stride(m) + (xrow_tw_global(m) ? 0 : 2 * m + 1);
+}
 
 struct ColArgs {  // Y forward/inverse, OTF Z: columns of (batch, n, ncols) planes
   const float2* src;

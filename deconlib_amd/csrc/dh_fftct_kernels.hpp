@@ -605,10 +605,14 @@ __global__ void __launch_bounds__(kMaxThreads) k_xf(Pack<XfArgs> pk, int RPB) {
   const int64_t total = pk.prefix[pk.nvol];
   int64_t b = blockIdx.x;
   if (b >= total) return;
-  float2* ltw = lds + RPB * RS;
-  stage_x_tables<M>(ltw, pk.a[0].twm, pk.a[0].twp);
-  const float2* twm = ltw;
-  const float2* twp = ltw + M;
+  const float2* twm = pk.a[0].twm;
+  const float2* twp = pk.a[0].twp;
+  if constexpr (!xrow_tw_global(M)) {
+    float2* ltw = lds + RPB * RS;
+    stage_x_tables<M>(ltw, pk.a[0].twm, pk.a[0].twp);
+    twm = ltw;
+    twp = ltw + M;
+  }
   float2 u[Bfly<M, T, R0>::M][R0];
   auto fetch = [&](int64_t bb) {
     int rl = rl0, t = t0;
@@ -887,8 +891,14 @@ __global__ void __launch_bounds__(kMaxThreads) k_xi(Pack<XiArgs> pk, int RPB) {
   const int64_t total = pk.prefix[pk.nvol];
   int64_t b = blockIdx.x;
   if (b >= total) return;
-  float2* ltw = lds + RPB * RS;
-  stage_x_tables<M>(ltw, pk.a[0].twm, pk.a[0].twp);
+  const float2* twm = pk.a[0].twm;
+  const float2* twp = pk.a[0].twp;
+  if constexpr (!xrow_tw_global(M)) {
+    float2* ltw = lds + RPB * RS;
+    stage_x_tables<M>(ltw, pk.a[0].twm, pk.a[0].twp);
+    twm = ltw;
+    twp = ltw + M;
+  }
   float2 q[XiEpt<M>::value];
   if constexpr (RT) {
     // the round trip holds two transforms' registers: load each tile at its start
@@ -902,7 +912,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_xi(Pack<XiArgs> pk, int RPB) {
       xi_tile<M>(a, lb * RPB, RPB, q, lds, true, thr);
       xi_tile<M>(a, lb * RPB, RPB, q, lds, false, thr);
       __syncthreads();
-      xi_body<M, RT, BOX>(a, lb, RPB, lds, ltw, ltw + M, thr);
+      xi_body<M, RT, BOX>(a, lb, RPB, lds, twm, twp, thr);
       __syncthreads();
     }
     return;
@@ -928,7 +938,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_xi(Pack<XiArgs> pk, int RPB) {
       xi_tile<M>(pk.a[vn], (bn - pk.prefix[vn]) * RPB, RPB, q, lds, true, thr2);
     }
     __syncthreads();
-    xi_body<M, RT>(a, lb, RPB, lds, ltw, ltw + M, thr);
+    xi_body<M, RT>(a, lb, RPB, lds, twm, twp, thr);
     if (bn >= total) break;
     b = bn;
     __syncthreads();

@@ -34,6 +34,9 @@ SHAPES = [
     ((40, 96, 96), (40, 96, 96)),      # canvas 80x192x192 (radix 3 and 5 on every axis)
     ((7000,), (3001,)),                # canvas 10000: no specialised length, runtime-length kernels
     ((5000, 6), (1201, 3)),            # y canvas 6250: runtime-length column pass
+    ((2, 3, 2048), (2, 3, 2049)),      # x canvas 4096: 2048-point rows, twiddles read from global memory
+    ((1, 2, 4096), (1, 1, 4097)),      # x canvas 8192: 4096-point rows
+    ((2, 2048, 4), (1, 2049, 3)),      # y canvas 4096: blocks split over 2-column tiles (cached, XCD-major)
 ]
 
 
