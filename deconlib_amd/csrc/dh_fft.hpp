@@ -101,9 +101,11 @@ int threads_per_line(int n);
 constexpr int kBlk = 8;
 
 // LDS elements of the specialised kernels: one column workgroup of L lines of
-// length n (8 spare slots every 16 line elements), and one X row of a length-m
-// transform (one spare slot every 32 elements, indices 0..m used).
-constexpr int64_t col_lds_elems(int n, int L) { return (int64_t)n * L + 8 * (n >> 4) + 8; }
+// length n (min(L, 8) spare slots every 16 line elements, so that narrow tiles of
+// long lines, as 2 columns of 4096, keep their padding in proportion and fit two
+// workgroups per CU), and one X row of a length-m transform (one spare slot every
+// 32 elements, indices 0..m used).
+constexpr int64_t col_lds_elems(int n, int L) { return (int64_t)n * L + (L < 8 ? L : 8) * (n >> 4) + 8; }
 constexpr int xrow_stride(int m) { return m + (m >> 5) + 1; }
 // Lengths whose X workgroups read the twiddle tables from global memory (through
 // the caches) instead of staging them in LDS: there the tables would take as much

@@ -165,7 +165,7 @@ __device__ __forceinline__ void st_s(float2* p, float2 v) {
 // ---- column pass (Y forward / inverse, OTF Z) ----
 // LDS slot of line element e, column ln: 8 extra slots every 16 elements spread
 // a wave's (rows x columns) accesses over the banks.
-__device__ __forceinline__ int col_lds(int e, int ln, int L) { return e * L + ln + 8 * (e >> 4); }
+__device__ __forceinline__ int col_lds(int e, int ln, int L) { return e * L + ln + (L < 8 ? L : 8) * (e >> 4); }
 
 // Hides the lane coordinates from loop-invariant code motion in the persistent
 // loops below: otherwise every pass's LDS and twiddle addresses are hoisted out
