@@ -12,9 +12,12 @@ It restates, in NumPy/SciPy, the reference's algorithm for:
 * ``FractionalAreaDownsample`` and banded weights   (deconlib/deconvolution/core_operators.py:151-352)
 * ``Compose`` / ``compose``                         (deconlib/deconvolution/composition.py:52-122)
 * ``FFTConvolver`` / ``LinearFFTConvolver``         (deconlib/deconvolution/linops_mlx.py:384-533)
+* ``d1_fwd`` / ``d1_fwd_adj``                      (deconlib/deconvolution/linops_core_mlx.py:25-39)
 * ``d1_cen`` / ``d1_cen_adj`` / ``d2``              (deconlib/deconvolution/linops_core_mlx.py:42-80)
-* ``Hessian3D``                                     (deconlib/deconvolution/linops_mlx.py:182-262)
+* ``Gradient1D/2D/3D``                              (deconlib/deconvolution/linops_mlx.py:46-127)
+* ``Hessian1D/2D/3D``                               (deconlib/deconvolution/linops_mlx.py:130-262)
 * dual proxes                                       (deconlib/deconvolution/pdhg_mlx.py:115-233)
+* Identity/Gradient/Hessian regularizers            (deconlib/deconvolution/pdhg_mlx.py:241-456, 485-499)
 * ``richardson_lucy_with_operator``                 (deconlib/deconvolution/rl_mlx.py:58-185)
 * ``solve_pdhg_mlx`` / ``solve_pdhg_with_operator`` (deconlib/deconvolution/pdhg_mlx.py:459-1242)
 
@@ -31,11 +34,19 @@ Parity pins (see tests/test_oracle_*.py):
                             preservation; adjoint identity incl. scale 1.325.
 * Hessian3D              -- quadratic-phantom curvature, adjoint identity,
                             closed-form norm bound via power iteration.
+* Gradient / Hessian 1-D/2-D, identity -- linear-ramp and quadratic known
+                            answers, brute-force transpose (dense matrix),
+                            closed-form norm bounds by power iteration.
 * proxes                 -- known values and the Poisson optimality residual.
 * RL                     -- fixed point, EM monotone I-divergence decrease,
                             hand-computed first iteration on a 1-D toy.
-* PDHG                   -- first iteration in closed form from zero duals,
-                            step-size invariants.
+* PDHG                   -- first iteration in closed form from zero duals;
+                            the Malitsky-Pock step rule through its invariants
+                            with dx^2 and dK^2 recomputed by four separate
+                            operator applications; the patience/min_iter stop
+                            on hand-built rel-change sequences; the objective and
+                            the I-divergence on hand-computed values; KKT
+                            optimality of a converged tiny problem.
 """
 
 from __future__ import annotations
@@ -393,6 +404,85 @@ def d2(f, axis):
     return np.swapaxes(fp[:-2] - f.dtype.type(2.0) * fp[1:-1] + fp[2:], axis, 0)
 
 
+def d1_fwd(f, axis):
+    """Forward difference, Neumann BC: f[i+1] - f[i], last sample 0 (linops_core_mlx.py:25-32)."""
+    f = np.swapaxes(f, axis, 0)
+    return np.swapaxes(np.concatenate([f[1:], f[-1:]], axis=0) - f, axis, 0)
+
+
+def d1_fwd_adj(g, axis):
+    """Transpose of d1_fwd: [-g0, g[i-1] - g[i] (1 <= i <= n-2), g[n-2]] (linops_core_mlx.py:35-39).
+
+    For n == 1 this is [-g0], as the reference's concatenation gives.
+    """
+    g = np.swapaxes(g, axis, 0)
+    return np.swapaxes(np.concatenate([-g[:1], g[:-2] - g[1:-1], g[-2:-1]], axis=0), axis, 0)
+
+
+class Gradient1D:
+    """d1_fwd along the only axis; ||D||^2 = 4 (linops_mlx.py:46-66)."""
+
+    operator_norm_sq = 4.0
+
+    def forward(self, f):
+        return d1_fwd(f, 0)
+
+    def adjoint(self, g):
+        return d1_fwd_adj(g, 0)
+
+
+class Gradient2D:
+    """(d/dy, d/dx) forward differences; ||grad||^2 = 8 (linops_mlx.py:69-89)."""
+
+    operator_norm_sq = 8.0
+
+    def forward(self, f):
+        return np.stack([d1_fwd(f, 0), d1_fwd(f, 1)])
+
+    def adjoint(self, g):
+        return d1_fwd_adj(g[0], 0) + d1_fwd_adj(g[1], 1)
+
+
+class Gradient3D:
+    """(r d/dz, d/dy, d/dx); ||grad||^2 = 4 (r^2 + 2) (linops_mlx.py:92-127)."""
+
+    def __init__(self, r: float = 1.0):
+        self.r = r
+        self.operator_norm_sq = 4.0 * (r * r + 2.0)
+
+    def forward(self, f):
+        return np.stack([self.r * d1_fwd(f, 0), d1_fwd(f, 1), d1_fwd(f, 2)])
+
+    def adjoint(self, g):
+        return d1_fwd_adj(self.r * g[0], 0) + d1_fwd_adj(g[1], 1) + d1_fwd_adj(g[2], 2)
+
+
+class Hessian1D:
+    """d2 along the only axis, self-adjoint; ||D2||^2 = 16 (linops_mlx.py:130-150)."""
+
+    operator_norm_sq = 16.0
+
+    def forward(self, f):
+        return d2(f, 0)
+
+    def adjoint(self, g):
+        return d2(g, 0)
+
+
+class Hessian2D:
+    """[H_yy, H_xx, sqrt2 H_xy]; ||H||^2 bound 48 (linops_mlx.py:153-179)."""
+
+    operator_norm_sq = 48.0
+
+    def forward(self, f):
+        s2 = f.dtype.type(SQRT2)
+        return np.stack([d2(f, 0), d2(f, 1), s2 * d1_cen(d1_cen(f, 0), 1)])
+
+    def adjoint(self, H):
+        s2 = H.dtype.type(SQRT2)
+        return d2(H[0], 0) + d2(H[1], 1) + s2 * d1_cen_adj(d1_cen_adj(H[2], 0), 1)
+
+
 class Hessian3D:
     """Six weighted second differences, Neumann BC (linops_mlx.py:182-262)."""
 
@@ -453,6 +543,69 @@ def prox_l1_dual(y, bound):
 def prox_l1_2_dual(y, bound):
     n = np.sqrt(np.sum(y * y, axis=0, keepdims=True) + 1e-12)
     return (y / np.maximum(n / bound, 1.0)).astype(y.dtype, copy=False)
+
+
+class IdentityRegularizer:
+    """L = I with a leading unit component axis; ||L||^2 = 1 (pdhg_mlx.py:241-306)."""
+
+    output_components = 1
+    operator_norm_sq = 1.0
+
+    def __init__(self, norm="L1"):
+        self.norm = norm
+
+    def forward(self, x):
+        return x[None]
+
+    def adjoint(self, y):
+        return y[0]
+
+
+class GradientRegularizer:
+    """Gradient1D/2D/3D with a component axis (1-D wrapped to (1, N)) (pdhg_mlx.py:309-377)."""
+
+    def __init__(self, ndim, r=1.0, norm="L1"):
+        if ndim not in (1, 2, 3):
+            raise ValueError(f"ndim must be 1, 2, or 3, got {ndim}")
+        self.ndim, self.r, self.norm = ndim, r, norm
+        self._op = Gradient1D() if ndim == 1 else (Gradient2D() if ndim == 2 else Gradient3D(r))
+        self.output_components = ndim
+        self.operator_norm_sq = self._op.operator_norm_sq
+
+    def forward(self, x):
+        return self._op.forward(x)[None] if self.ndim == 1 else self._op.forward(x)
+
+    def adjoint(self, y):
+        return self._op.adjoint(y[0]) if self.ndim == 1 else self._op.adjoint(y)
+
+
+class HessianRegularizer:
+    """Hessian1D/2D/3D with a component axis (1, 3 or 6 components) (pdhg_mlx.py:380-456)."""
+
+    def __init__(self, ndim, r=1.0, norm="L1"):
+        if ndim not in (1, 2, 3):
+            raise ValueError(f"ndim must be 1, 2, or 3, got {ndim}")
+        self.ndim, self.r, self.norm = ndim, r, norm
+        self._op = Hessian1D() if ndim == 1 else (Hessian2D() if ndim == 2 else Hessian3D(r))
+        self.output_components = 1 if ndim == 1 else (3 if ndim == 2 else 6)
+        self.operator_norm_sq = self._op.operator_norm_sq
+
+    def forward(self, x):
+        return self._op.forward(x)[None] if self.ndim == 1 else self._op.forward(x)
+
+    def adjoint(self, y):
+        return self._op.adjoint(y[0]) if self.ndim == 1 else self._op.adjoint(y)
+
+
+def create_regularizer(regularization, ndim, r, norm):
+    """(pdhg_mlx.py:485-499)"""
+    if regularization == "identity":
+        return IdentityRegularizer(norm)
+    if regularization == "gradient":
+        return GradientRegularizer(ndim, r, norm)
+    if regularization == "hessian":
+        return HessianRegularizer(ndim, r, norm)
+    raise ValueError(f"Unknown regularization: {regularization}")
 
 
 # ---------------------------------------------------------------------------
@@ -541,19 +694,75 @@ def compute_spacing_ratio(spacing, ndim):
     return lat / dz if dz > 0 else 1.0
 
 
+def pdhg_objective(data, fm, Lx, alpha, norm="L1", loss_type="poisson"):
+    """Loss recorded by both PDHG entry points (pdhg_mlx.py:860-881, 1171-1190).
+
+    ``fm`` is the forward model A x + b.  Poisson:
+    sum D log((D + 1e-6) / max(fm, 1e-6)) - (D - fm); Gaussian: 0.5 sum (D - fm)^2;
+    plus alpha sum |Lx| ("L1") or alpha sum_i sqrt(sum_q (Lx)_q,i^2 + 1e-12) ("L1_2").
+    Sums in float64.
+    """
+    t = data.dtype.type
+    if loss_type == "poisson":
+        fms = np.maximum(fm, t(1e-6))
+        dl = np.sum((data * np.log((data + t(1e-6)) / fms) - (data - fm)).astype(np.float64))
+    else:
+        dl = 0.5 * np.sum(((data - fm) ** 2).astype(np.float64))
+    if norm == "L1":
+        rl = alpha * np.sum(np.abs(Lx).astype(np.float64))
+    else:
+        rl = alpha * np.sum(np.sqrt(np.sum(Lx * Lx, axis=0) + t(1e-12)).astype(np.float64))
+    return float(dl + rl)
+
+
+def malitsky_pock_step(tau, sigma, theta, dx2, dK2, step, delta=0.99, eta=0.5):
+    """Adaptive step sizes of one iteration (pdhg_mlx.py:761-797), in float64.
+
+    Returns (tau_new, sigma_new, theta_new, backtracked).
+    """
+    ratio = dx2 / (2.0 * sigma * max(dK2, 1e-12))
+    tc = min(math.sqrt(1.0 + theta) * tau, delta * math.sqrt(ratio))
+    back = tc * sigma * dK2 > delta * dx2
+    tn = eta * tc if back else tc
+    tn = max(tn, step * 1e-4)
+    thn = tn / tau
+    sn = sigma / max(thn, 0.1)
+    sn = min(max(sn, step * 1e-4), step * 1e4)
+    return tn, sn, thn, back
+
+
+def convergence_step(k, rel, count, min_iter, tol, patience):
+    """Patience/min_iter stopping test (pdhg_mlx.py:805-835).
+
+    Returns (count, stop): from iteration ``min_iter`` on, ``count`` counts
+    consecutive iterations with rel < tol and resets otherwise; stop once it
+    reaches ``patience``.  Before ``min_iter`` the count is left unchanged.
+    """
+    if k >= min_iter:
+        if rel < tol:
+            count += 1
+            if count >= patience:
+                return count, True
+        else:
+            count = 0
+    return count, False
+
+
 def pdhg_loop(observed, blur_op, reg_op, x, alpha, num_iter, background, norm="L1",
               loss_type="poisson", delta=0.99, eta=0.5, eval_interval=10, tol=1e-5,
               min_iter=20, patience=5, dtype=np.float32, loss_every_iter=False,
-              callback=None):
+              loss_on_interval=True, callback=None, callback_with_loss=False, trace=None):
     """The Malitsky-Pock loop shared by solve_pdhg_mlx / solve_pdhg_with_operator.
 
     Scalars (tau, sigma, theta) are Python floats as in the reference
-    (pdhg_mlx.py:851-853); array arithmetic is in ``dtype``.
+    (pdhg_mlx.py:851-853); array arithmetic is in ``dtype``.  ``trace`` (a list)
+    receives one dict per iteration: x, x_new, y1_new, y2_new, tau, sigma, theta
+    (before the update), tau_new, sigma_new, theta_new, backtracked, dx2, dK2, rel.
     """
     t = np.dtype(dtype).type
     data = np.asarray(observed).astype(dtype)
     y1 = np.zeros(data.shape, dtype)
-    y2 = np.zeros((6,) + x.shape, dtype)
+    y2 = np.zeros((reg_op.output_components,) + x.shape, dtype)
     xbar = x
     step = 0.99 / (blur_op.operator_norm_sq + reg_op.operator_norm_sq) ** 0.5
     tau = sigma = step
@@ -576,64 +785,60 @@ def pdhg_loop(observed, blur_op, reg_op, x, alpha, num_iter, background, norm="L
         dA = blur_op.forward(xn) - blur_op.forward(x)
         dL = reg_op.forward(xn) - reg_op.forward(x)
         dK2 = float(np.sum(dA.astype(np.float64) ** 2) + np.sum(dL.astype(np.float64) ** 2))
-        ratio = dx2 / (2.0 * sigma * max(dK2, 1e-12))
-        tc = min(math.sqrt(1.0 + theta) * tau, delta * math.sqrt(ratio))
-        tn = eta * tc if tc * sigma * dK2 > delta * dx2 else tc
-        tn = max(tn, step * 1e-4)
-        thn = tn / tau
-        sn = sigma / max(thn, 0.1)
-        sn = min(max(sn, step * 1e-4), step * 1e4)
+        tn, sn, thn, back = malitsky_pock_step(tau, sigma, theta, dx2, dK2, step, delta, eta)
         xx = float(np.sum(x.astype(np.float64) ** 2))
         xnxn = float(np.sum(xn.astype(np.float64) ** 2))
         rel = math.sqrt(dx2 / (max(xx, xnxn) + 1e-12))
-        if k >= min_iter:
-            if rel < tol:
-                count += 1
-                if count >= patience:
-                    converged = True
-                    final_iter = k + 1
-                    x, y1, y2 = xn, y1n, y2n
-                    taus.append(tn)
-                    sigmas.append(sn)
-                    break
-            else:
-                count = 0
+        if trace is not None:
+            trace.append(dict(x=x, x_new=xn, y1_new=y1n, y2_new=y2n, tau=tau, sigma=sigma, theta=theta,
+                              tau_new=tn, sigma_new=sn, theta_new=thn, backtracked=back, dx2=dx2, dK2=dK2,
+                              rel=rel))
+        count, stop = convergence_step(k, rel, count, min_iter, tol, patience)
+        if stop:
+            converged = True
+            final_iter = k + 1
+            x, y1, y2 = xn, y1n, y2n
+            taus.append(tn)
+            sigmas.append(sn)
+            break
         xbar = xn + t(thn) * (xn - x)
         x, y1, y2 = xn, y1n, y2n
         tau, sigma, theta = tn, sn, thn
         taus.append(tau)
         sigmas.append(sigma)
-        if loss_every_iter or (k + 1) % eval_interval == 0:
-            fm = blur_op.forward(x) + t(bg)
-            fms = np.maximum(fm, t(1e-6))
-            if loss_type == "poisson":
-                dl = np.sum((data * np.log((data + t(1e-6)) / fms) - (data - fm)).astype(np.float64))
-            else:
-                dl = 0.5 * np.sum(((data - fm) ** 2).astype(np.float64))
-            Lx = reg_op.forward(x)
-            if norm == "L1":
-                rl = alpha * np.sum(np.abs(Lx).astype(np.float64))
-            else:
-                rl = alpha * np.sum(np.sqrt(np.sum(Lx * Lx, axis=0) + 1e-12).astype(np.float64))
-            losses.append(float(dl + rl))
-            if callback is not None:
-                callback(k, x, losses[-1])
-        if callback is not None and not loss_every_iter and callback(k, x):
-            final_iter = k + 1
-            break
+        loss = None
+        if loss_every_iter or (loss_on_interval and (k + 1) % eval_interval == 0):
+            loss = pdhg_objective(data, blur_op.forward(x) + t(bg), reg_op.forward(x), alpha, norm, loss_type)
+            losses.append(loss)
+        if callback is not None:
+            if callback_with_loss:
+                callback(k, x, loss)
+            elif callback(k, x):
+                final_iter = k + 1
+                break
     return PDHGResult(restored=x, iterations=final_iter, loss_history=losses,
-                      converged=converged, tau_history=taus, sigma_history=sigmas)
+                      converged=converged, tau_history=taus, sigma_history=sigmas,
+                      metadata=dict(y1=y1, y2=y2))
 
 
 def solve_pdhg(observed, psf, alpha=0.01, num_iter=200, background=0.0, spacing=None,
-               bin_factors=None, init=None, norm="L1", loss_type="poisson", delta=0.99,
-               eta=0.5, eval_interval=10, tol=1e-5, min_iter=20, patience=5,
-               dtype=np.float32):
-    """``solve_pdhg_mlx`` with regularization="hessian" (pdhg_mlx.py:507-920)."""
+               bin_factors=None, sampling_factors=None, init=None, norm="L1", loss_type="poisson",
+               regularization="hessian", delta=0.99, eta=0.5, eval_interval=10, tol=1e-5, min_iter=20,
+               patience=5, callback=None, dtype=np.float32, trace=None):
+    """``solve_pdhg_mlx`` (pdhg_mlx.py:507-920)."""
     obs = np.asarray(observed).astype(dtype)
     psf = np.asarray(psf)
     ndim = obs.ndim
-    if bin_factors is not None:
+    if bin_factors is not None and sampling_factors is not None:
+        raise ValueError("bin_factors and sampling_factors are mutually exclusive")
+    if sampling_factors is not None:
+        f = ((float(sampling_factors),) * ndim if isinstance(sampling_factors, (int, float))
+             else tuple(float(v) for v in sampling_factors))
+        hi = tuple(max(1, int(round(n * v))) for n, v in zip(obs.shape, f))
+        if tuple(psf.shape) != hi:
+            raise ValueError("psf.shape does not match the sampling_factors high-resolution shape")
+        A = compose(FractionalAreaDownsample(scale=f), LinearFFTConvolver(psf, hi, normalize=True, dtype=dtype))
+    elif bin_factors is not None:
         f = (bin_factors,) * ndim if isinstance(bin_factors, int) else tuple(int(b) for b in bin_factors)
         hi = tuple(psf.shape)
         for i, (n, b) in enumerate(zip(hi, f)):
@@ -647,12 +852,46 @@ def solve_pdhg(observed, psf, alpha=0.01, num_iter=200, background=0.0, spacing=
     else:
         A = LinearFFTConvolver(psf, obs.shape, normalize=True, dtype=dtype)
     r = compute_spacing_ratio(spacing, ndim)
-    L = Hessian3D(r)
+    L = create_regularizer(regularization, ndim, r, norm)
     if init is not None:
         x = np.asarray(init).astype(dtype)
     else:
-        x = A.adjoint(obs) if bin_factors is not None else obs.copy()
+        x = A.adjoint(obs) if (bin_factors is not None or sampling_factors is not None) else obs.copy()
         x = np.maximum(x, np.dtype(dtype).type(0.0))
     return pdhg_loop(obs, A, L, x, alpha, num_iter, background, norm=norm,
                      loss_type=loss_type, delta=delta, eta=eta, eval_interval=eval_interval,
-                     tol=tol, min_iter=min_iter, patience=patience, dtype=dtype)
+                     tol=tol, min_iter=min_iter, patience=patience, dtype=dtype, callback=callback,
+                     trace=trace)
+
+
+def solve_pdhg_with_operator(observed, blur_op, alpha=0.01, regularization="identity", norm="L1",
+                             loss_type="poisson", num_iter=200, background=0.0, init=None, verbose=False,
+                             callback=None, delta=0.99, eta=0.5, eval_interval=10, tol=1e-5, min_iter=20,
+                             patience=5, dtype=np.float32, trace=None):
+    """``solve_pdhg_with_operator`` (pdhg_mlx.py:928-1242).
+
+    Solution shape (pdhg_mlx.py:1011-1018): (n,) for an operator with a 2-D
+    ``shape`` (m, n); else the operator's ``in_shape`` when it declares one (the
+    reference assumes ``observed.shape``, equal for the operators it documents);
+    else ``observed.shape``.  r = 1; x0 = max(init or 0.01, 0); the loss is
+    recorded (and ``callback(k, x, loss)`` called, its return value ignored) every
+    iteration when ``verbose`` or ``callback`` is given, never otherwise.
+    """
+    obs = np.asarray(observed).astype(dtype)
+    shp = getattr(blur_op, "shape", None)
+    if shp is not None and len(shp) == 2:
+        shape, ndim = (int(shp[1]),), 1
+    else:
+        shape = tuple(getattr(blur_op, "in_shape", None) or obs.shape)
+        ndim = len(shape)
+    L = create_regularizer(regularization, ndim, 1.0, norm)
+    if init is not None:
+        x = np.asarray(init).astype(dtype)
+    else:
+        x = np.full(shape, 0.01, dtype)
+    x = np.maximum(x, np.dtype(dtype).type(0.0))
+    every = bool(verbose or callback is not None)
+    return pdhg_loop(obs, blur_op, L, x, alpha, num_iter, background, norm=norm, loss_type=loss_type,
+                     delta=delta, eta=eta, eval_interval=eval_interval, tol=tol, min_iter=min_iter,
+                     patience=patience, dtype=dtype, loss_every_iter=every, loss_on_interval=False,
+                     callback=callback, callback_with_loss=True, trace=trace)
