@@ -2,8 +2,8 @@
 
 Drop-in for the hot path of ``deconlib.deconvolution``: the operator protocol
 and composition, ``LinearFFTConvolver`` / ``FFTConvolver``, ``Pad`` / ``Crop`` /
-``FractionalAreaDownsample``, ``Hessian3D`` / ``Hessian2D``, Richardson–Lucy,
-PDHG with Hessian regularization, NLCG (``nlcg_with_operator``,
+``FractionalAreaDownsample``, ``Gradient1D/2D/3D``, ``Hessian1D/2D/3D``, Richardson–Lucy,
+PDHG with identity, gradient and Hessian regularization, NLCG (``nlcg_with_operator``,
 ``nlcg_solver``), ER-Decon (``erdecon_with_operator``, ``erdecon_solver``), JetNewton
 (``jetnewton_with_operator``), and the tiled path (``make_forward_model``,
 ``plan_tiles`` / ``process_tiles``, ``richardson_lucy_solver``).  All arithmetic runs in hand-written HIP kernels for
@@ -14,12 +14,13 @@ PyTorch only owns device buffers and streams.
 from .base import DeconvolutionResult, MLXDeconvolutionResult
 from .composition import Compose, LinearOperator, compose
 from .core_operators import Crop, FractionalAreaDownsample, Pad
-from .linops import (AnisotropicHessian2D, AnisotropicHessian3D, FFTConvolver, Hessian2D, Hessian3D,
-                     LinearFFTConvolver, fast_padded_shape)
+from .linops import (AnisotropicHessian2D, AnisotropicHessian3D, FFTConvolver, Gradient1D, Gradient2D, Gradient3D,
+                     Hessian1D, Hessian2D, Hessian3D, LinearFFTConvolver, fast_padded_shape)
 from .nlcg import NLCGResult, nlcg_solver, nlcg_with_operator
 from .erdecon import ERDeconResult, erdecon_solver, erdecon_with_operator
 from .jetnewton import JetNewtonResult, jetnewton_with_operator
-from .pdhg import HessianRegularizer, solve_pdhg_mlx, solve_pdhg_with_operator
+from .pdhg import (GradientRegularizer, HessianRegularizer, IdentityRegularizer, solve_pdhg_mlx,
+                   solve_pdhg_with_operator)
 from .rl import RLResult, poisson_i_divergence, richardson_lucy_solver, richardson_lucy_with_operator
 from .forward_model import ForwardModel, make_forward_model
 from .shapes import compute_padded_shape, compute_visible_shape, get_valid_slices
@@ -28,7 +29,13 @@ from .tile_processing import TilePlan, TileSpec, optimal_tile_size, plan_tiles, 
 __all__ = [
     "solve_pdhg_mlx",
     "solve_pdhg_with_operator",
+    "IdentityRegularizer",
+    "GradientRegularizer",
     "HessianRegularizer",
+    "Gradient1D",
+    "Gradient2D",
+    "Gradient3D",
+    "Hessian1D",
     "Hessian3D",
     "Hessian2D",
     "NLCGResult",

@@ -21,13 +21,14 @@ if not _LIB_PATH.exists():
 
 lib = C.CDLL(str(_LIB_PATH))
 
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 DH_OK, DH_EINVAL, DH_EHIP, DH_EWORKSPACE, DH_EUNSUPPORTED = 0, -1, -2, -3, -4
 
 PRO_NONE, PRO_ONES, PRO_SUB_FLOOR = 0, 1, 2
 EPI_NONE, EPI_FLOOR, EPI_RL_RATIO, EPI_RL_UPDATE, EPI_ADD = 0, 1, 2, 3, 4
 RED_SUM, RED_MAX, RED_SUMSQ = 0, 1, 2
+REG_HESSIAN, REG_GRADIENT, REG_IDENTITY = 0, 1, 2
 TILE_PERSISTENT = 0x10000  # dh_conv_plan_set_tile flag: persistent grid with next-tile prefetch
 
 
@@ -158,6 +159,13 @@ SYMBOLS = {
     "dh_pd_scalars": (_int, _vp, _vp, _int, C.POINTER(PdParams), _vp),
     "dh_hessian_forward": (_int, _vp, _vp, _int, _int, _int, C.POINTER(_f), _vp),
     "dh_hessian_adjoint": (_int, _vp, _vp, _int, _int, _int, C.POINTER(_f), _vp),
+    "dh_reg_components": (_int, _int),
+    "dh_pd_reg_dual": (_int, _int, _vp, _vp, _vp, _int, _int, _int, _vp, C.POINTER(_f), _f, _int, _vp),
+    "dh_pd_reg_primal": (_int, _int, _vp, _vp, _vp, _vp, _int, _int, _int, _vp, C.POINTER(_f), _vp, _vp),
+    "dh_pd_reg_sums": (_int, _int, _vp, _vp, _int, _int, _int, C.POINTER(_f), _vp, _vp, _int, _vp, _vp),
+    "dh_reg_forward": (_int, _int, _vp, _vp, _int, _int, _int, C.POINTER(_f), _vp),
+    "dh_reg_adjoint": (_int, _int, _vp, _vp, _int, _int, _int, C.POINTER(_f), _vp),
+    "dh_poisson_idiv": (_int, _vp, _vp, _i64, _f, _vp, _vp, _vp),
     "dh_batch_conv_apply": (_int, _int, C.POINTER(BatchItem), _int, _vp),
     "dh_rl_iteration": (_int, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _f, _f, _vp, _int, _int, _vp),
     "dh_batch_rl_iteration": (_int, _int, C.POINTER(RLItem), _vp),
@@ -215,7 +223,7 @@ if fns["dh_abi_version"]() != ABI_VERSION:
 
 
 _VALUE_RETURNING = {"dh_reduce_partials", "dh_abi_version", "dh_pd_partials", "dh_conv_plan_get_tile",
-                    "dh_nlcg_partials", "dh_er_partials", "dh_jn_partials"}
+                    "dh_nlcg_partials", "dh_er_partials", "dh_jn_partials", "dh_reg_components"}
 
 
 class NativeError(RuntimeError):

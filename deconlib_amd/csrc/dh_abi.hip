@@ -139,6 +139,10 @@ int timing_end(hipStream_t st) {
 
 }  // namespace
 
+namespace dh {
+int set_last_error(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace dh
+
 struct dh_conv_plan {
   int ndim = 3;
   int circular = 0;
@@ -887,6 +891,67 @@ int dh_hessian_adjoint(const float* y, float* out, int nz, int ny, int nx, const
   return hip_check(dh::hess_adjoint(y, out, nz, ny, nx, w, (hipStream_t)stream), "hessian_adjoint");
 }
 
+int dh_reg_components(int reg) {
+  return reg == DH_REG_HESSIAN ? 6 : (reg == DH_REG_GRADIENT ? 3 : (reg == DH_REG_IDENTITY ? 1 : 0));
+}
+
+static int reg_check(int reg, const float* w, int nz, int ny, int nx) {
+  if (reg != DH_REG_HESSIAN && reg != DH_REG_GRADIENT && reg != DH_REG_IDENTITY)
+    return fail(DH_EINVAL, "unknown regularizer");
+  if (reg != DH_REG_IDENTITY && !w) return fail(DH_EINVAL, "null weights");
+  if (!dims_ok(nz, ny, nx)) return fail(DH_EINVAL, "bad volume shape");
+  return DH_OK;
+}
+
+int dh_pd_reg_dual(int reg, float* y2, const float* x, const float* xp, int nz, int ny, int nx,
+                   const double* scalars, const float* w, float alpha, int l12, void* stream) {
+  if (reg == DH_REG_HESSIAN) return dh_pd_dual(y2, x, xp, nz, ny, nx, scalars, w, alpha, l12, stream);
+  if (!y2 || !x || !xp || !scalars) return fail(DH_EINVAL, "null argument");
+  DH_TRY(reg_check(reg, w, nz, ny, nx));
+  return hip_check(dh::pr_dual(reg, y2, x, xp, nz, ny, nx, scalars, w, alpha, l12, (hipStream_t)stream), "pd_reg_dual");
+}
+
+int dh_pd_reg_primal(int reg, float* x_new, const float* x, const float* g, const float* y2, int nz, int ny,
+                     int nx, double* scalars, const float* w, double* partials, void* stream) {
+  if (reg == DH_REG_HESSIAN) return dh_pd_primal(x_new, x, g, y2, nz, ny, nx, scalars, w, partials, stream);
+  if (!x_new || !x || !g || !y2 || !scalars || !partials) return fail(DH_EINVAL, "null argument");
+  DH_TRY(reg_check(reg, w, nz, ny, nx));
+  return hip_check(dh::pr_primal(reg, x_new, x, g, y2, nz, ny, nx, scalars, w, partials, (hipStream_t)stream),
+                   "pd_reg_primal");
+}
+
+int dh_pd_reg_sums(int reg, const float* a, const float* b, int nz, int ny, int nx, const float* w,
+                   double* scalars, double* reg_loss_out, int reg_norm, double* partials, void* stream) {
+  if (reg == DH_REG_HESSIAN)
+    return dh_pd_hess_sums(a, b, nz, ny, nx, w, scalars, reg_loss_out, reg_norm, partials, stream);
+  if (!a || !b || !scalars || !partials) return fail(DH_EINVAL, "null argument");
+  DH_TRY(reg_check(reg, w, nz, ny, nx));
+  if (reg_loss_out && reg_norm != 1 && reg_norm != 2) return fail(DH_EINVAL, "reg_norm must be 1 or 2");
+  return hip_check(dh::pr_sums(reg, a, b, nz, ny, nx, w, reg_loss_out ? reg_norm : 0, scalars, reg_loss_out,
+                               partials, (hipStream_t)stream),
+                   "pd_reg_sums");
+}
+
+int dh_reg_forward(int reg, const float* f, float* out, int nz, int ny, int nx, const float* w, void* stream) {
+  if (reg == DH_REG_HESSIAN) return dh_hessian_forward(f, out, nz, ny, nx, w, stream);
+  if (!f || !out) return fail(DH_EINVAL, "null argument");
+  DH_TRY(reg_check(reg, w, nz, ny, nx));
+  hipStream_t st = (hipStream_t)stream;
+  if (reg == DH_REG_IDENTITY)
+    return hip_check(hipMemcpyAsync(out, f, (size_t)nz * ny * nx * 4, hipMemcpyDeviceToDevice, st), "identity copy");
+  return hip_check(dh::grad_forward(f, out, nz, ny, nx, w, st), "gradient forward");
+}
+
+int dh_reg_adjoint(int reg, const float* y, float* out, int nz, int ny, int nx, const float* w, void* stream) {
+  if (reg == DH_REG_HESSIAN) return dh_hessian_adjoint(y, out, nz, ny, nx, w, stream);
+  if (!y || !out) return fail(DH_EINVAL, "null argument");
+  DH_TRY(reg_check(reg, w, nz, ny, nx));
+  hipStream_t st = (hipStream_t)stream;
+  if (reg == DH_REG_IDENTITY)
+    return hip_check(hipMemcpyAsync(out, y, (size_t)nz * ny * nx * 4, hipMemcpyDeviceToDevice, st), "identity copy");
+  return hip_check(dh::grad_adjoint(y, out, nz, ny, nx, w, st), "gradient adjoint");
+}
+
 // ---- NLCG ----
 int64_t dh_nlcg_partials(void) { return dh::nl_partials_needed(); }
 
@@ -1413,6 +1478,14 @@ int dh_rl_ratio(const float* model, const float* obs, float* out, int64_t n, flo
 int dh_rl_update(float* x, const float* corr, const float* sens, int64_t n, float eps, void* stream) {
   if (!x || !corr || !sens) return fail(DH_EINVAL, "null argument");
   return hip_check(dh::launch_rl_update(x, corr, sens, n, eps, (hipStream_t)stream), "rl_update");
+}
+
+int dh_poisson_idiv(const float* obs, const float* model, int64_t n, float eps, double* partials_dev,
+                    double* result_dev, void* stream) {
+  if (!obs || !model || !partials_dev || !result_dev || n < 1) return fail(DH_EINVAL, "bad poisson_idiv arguments");
+  if (!(eps >= 0.0f)) return fail(DH_EINVAL, "eps must be >= 0");
+  return hip_check(dh::launch_poisson_idiv(obs, model, n, eps, partials_dev, result_dev, (hipStream_t)stream),
+                   "poisson_idiv");
 }
 
 int dh_add_scalar(const float* in, const float* obs, float* out, int64_t n, float a, double* partials_dev,

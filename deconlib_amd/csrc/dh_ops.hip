@@ -169,6 +169,19 @@ __global__ void k_add_scalar(const float* in, const float* obs, float* out, int6
   }
 }
 
+// sum of obs log(max(obs, eps) / max(model, eps)) - (obs - max(model, eps))  (rl_mlx.py:75-79)
+__global__ void k_poisson_idiv(const float* obs, const float* model, int64_t n, float eps, double* partials) {
+  __shared__ double sh[16];
+  double acc = 0.0;
+  DH_EW_LOOP(i, n) {
+    const float o = obs[i];
+    const float os = fmaxf(o, eps), ms = fmaxf(model[i], eps);
+    acc += (double)(o * logf(os / ms) - (o - ms));
+  }
+  double r = block_reduce(acc, RED_SUM, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = r;
+}
+
 __global__ void k_rl_update(float* x, const float* corr, const float* sens, int64_t n, float eps) {
   DH_EW_LOOP(i, n) x[i] = fmaxf(x[i] * (corr[i] / sens[i]), eps);
 }
@@ -188,6 +201,13 @@ hipError_t launch_clamp_min(const float* in, float* out, int64_t n, float lo, hi
 }
 hipError_t launch_sub_floor(const float* in, float* out, int64_t n, float a, float b, hipStream_t st) {
   hipLaunchKernelGGL(k_sub_floor, ew_grid(n), dim3(256), 0, st, in, out, n, a, b);
+  return hipGetLastError();
+}
+hipError_t launch_poisson_idiv(const float* obs, const float* model, int64_t n, float eps, double* partials,
+                               double* out, hipStream_t st) {
+  const int64_t nb = reduce_nblocks(n);
+  hipLaunchKernelGGL(k_poisson_idiv, dim3((unsigned)nb), dim3(kRedThreads), 0, st, obs, model, n, eps, partials);
+  hipLaunchKernelGGL(k_reduce_stage2, dim3(1), dim3(1024), 0, st, (int)RED_SUM, (const double*)partials, nb, out);
   return hipGetLastError();
 }
 hipError_t launch_rl_ratio(const float* model, const float* obs, float* out, int64_t n, float bg, float eps,

@@ -18,6 +18,8 @@ hipError_t launch_rl_ratio(const float* model, const float* obs, float* out, int
                            double* partials, double* reduce_out, hipStream_t st);
 hipError_t launch_add_scalar(const float* in, const float* obs, float* out, int64_t n, float a,
                              double* partials, double* reduce_out, hipStream_t st);
+hipError_t launch_poisson_idiv(const float* obs, const float* model, int64_t n, float eps, double* partials,
+                               double* out, hipStream_t st);
 hipError_t launch_rl_update(float* x, const float* corr, const float* sens, int64_t n, float eps,
                             hipStream_t st);
 

@@ -498,6 +498,181 @@ __global__ void k_pd_scalars(double* sc, double* hist, int k, PdParams pp) {
 }
 
 // ---------------------------------------------------------------------------
+// First-order (Gradient1D/2D/3D, linops_mlx.py:46-127) and identity
+// (pdhg_mlx.py:241-306) regularizers.  Components [z, y, x] of the forward
+// difference d1_fwd (linops_core_mlx.py:25-32) with weights w3; the identity
+// has one component, x itself.  Volumes are small next to the FFT passes'
+// traffic, so the neighbours come through the caches (no LDS tiling).
+// ---------------------------------------------------------------------------
+
+struct W3 {
+  float w[3];
+};
+
+struct Vox {
+  int z, y, x;
+};
+
+__device__ __forceinline__ Vox vox_of(int64_t i, const Dims& d) {
+  Vox v;
+  v.x = (int)(i % d.nx);
+  v.y = (int)((i / d.nx) % d.ny);
+  v.z = (int)(i / ((int64_t)d.nx * d.ny));
+  return v;
+}
+
+// weighted forward differences of f at voxel i: w_a (f[next along a] - f[i]), 0 on the last plane
+template <class F>
+__device__ __forceinline__ void grad3(const F& f, int64_t i, const Vox& v, const Dims& d, const W3& w, float* g) {
+  const float c = f(i);
+  const int64_t sz = (int64_t)d.ny * d.nx;
+  g[0] = w.w[0] * ((v.z + 1 < d.nz ? f(i + sz) : c) - c);
+  g[1] = w.w[1] * ((v.y + 1 < d.ny ? f(i + d.nx) : c) - c);
+  g[2] = w.w[2] * ((v.x + 1 < d.nx ? f(i + 1) : c) - c);
+}
+
+// d1_fwd_adj of the weighted component q along one axis of n samples at index k
+// (stride s): [-g0, g[k-1] - g[k], g[n-2]] (linops_core_mlx.py:35-39; n == 1 gives -g0)
+__device__ __forceinline__ float fwd_adj_at(const float* __restrict__ g, int64_t i, int k, int n, int64_t s, float w) {
+  if (k == 0) return -(w * g[i]);
+  if (k == n - 1) return w * g[i - s];
+  return w * g[i - s] - w * g[i];
+}
+
+// adjoint of grad3 at voxel i: adj_z + adj_y + adj_x (Gradient3D.adjoint order)
+__device__ __forceinline__ float grad3_adj(const float* __restrict__ y, int64_t n, int64_t i, const Vox& v,
+                                           const Dims& d, const W3& w) {
+  const int64_t sz = (int64_t)d.ny * d.nx;
+  const float az = fwd_adj_at(y, i, v.z, d.nz, sz, w.w[0]);
+  const float ay = fwd_adj_at(y + n, i, v.y, d.ny, d.nx, w.w[1]);
+  const float ax = fwd_adj_at(y + 2 * n, i, v.x, d.nx, 1, w.w[2]);
+  return (az + ay) + ax;
+}
+
+template <int NC>
+__device__ __forceinline__ void reg_prox_store(float* __restrict__ y2, int64_t n, int64_t i, const float* av,
+                                               float alpha, int l12) {
+  if (!l12) {
+#pragma unroll
+    for (int q = 0; q < NC; ++q) stnt(y2 + q * n + i, fminf(fmaxf(av[q], -alpha), alpha));
+  } else {
+    float ss = 0.0f;
+#pragma unroll
+    for (int q = 0; q < NC; ++q) ss += av[q] * av[q];
+    const float s = fmaxf(sqrtf(ss + 1e-12f) / alpha, 1.0f);
+#pragma unroll
+    for (int q = 0; q < NC; ++q) stnt(y2 + q * n + i, av[q] / s);
+  }
+}
+
+// y2 <- prox(y2 + sigma L(xbar)), xbar = x + theta (x - xp)  (pdhg_mlx.py:729-734)
+template <int KIND>
+__global__ void k_pr_dual(float* __restrict__ y2, const float* __restrict__ x, const float* __restrict__ xp, Dims d,
+                          const double* __restrict__ sc, W3 w, float alpha, int l12) {
+  const int64_t n = (int64_t)d.nz * d.ny * d.nx;
+  const float sf = (float)sc[PD_SIGMA], th = (float)sc[PD_THETA];
+  auto xbar = [&](int64_t j) {
+    const float v = x[j];
+    return v + th * (v - xp[j]);
+  };
+  PD_LOOP(i, n) {
+    if (KIND == DH_REG_GRADIENT) {
+      float g[3];
+      grad3(xbar, i, vox_of(i, d), d, w, g);
+      float av[3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) av[q] = ldnt(y2 + q * n + i) + sf * g[q];
+      reg_prox_store<3>(y2, n, i, av, alpha, l12);
+    } else {
+      float av[1] = {ldnt(y2 + i) + sf * xbar(i)};
+      reg_prox_store<1>(y2, n, i, av, alpha, l12);
+    }
+  }
+}
+
+// xn = max(0, x - tau (g + L^T y2)); partial sums of (xn - x)^2, x^2, xn^2
+template <int KIND>
+__global__ void k_pr_primal(float* __restrict__ xn, const float* __restrict__ x, const float* __restrict__ g,
+                            const float* __restrict__ y2, Dims d, const double* __restrict__ sc, W3 w,
+                            double* __restrict__ part) {
+  __shared__ double sh[16];
+  const int64_t n = (int64_t)d.nz * d.ny * d.nx;
+  const float tf = (float)sc[PD_TAU];
+  double s_dx = 0.0, s_x = 0.0, s_xn = 0.0;
+  PD_LOOP(i, n) {
+    const float lt = KIND == DH_REG_GRADIENT ? grad3_adj(y2, n, i, vox_of(i, d), d, w) : y2[i];
+    const float xv = ldnt(x + i);
+    const float v = fmaxf(xv - tf * (ldnt(g + i) + lt), 0.0f);
+    stnt(xn + i, v);
+    const float dv = v - xv;
+    s_dx += (double)(dv * dv);
+    s_x += (double)(xv * xv);
+    s_xn += (double)(v * v);
+  }
+  const double r0 = pd_block_sum(s_dx, sh);
+  const double r1 = pd_block_sum(s_x, sh);
+  const double r2 = pd_block_sum(s_xn, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x * 3 + 0] = r0;
+    part[blockIdx.x * 3 + 1] = r1;
+    part[blockIdx.x * 3 + 2] = r2;
+  }
+}
+
+// sum |L(a - b)|^2 and, if want_abs, sum |L a| (1) or sum sqrt(sum_q (L a)_q^2 + 1e-12) (2)
+template <int KIND>
+__global__ void k_pr_sums(const float* __restrict__ a, const float* __restrict__ b, Dims d, W3 w, int want_abs,
+                          double* __restrict__ part) {
+  __shared__ double sh[16];
+  const int64_t n = (int64_t)d.nz * d.ny * d.nx;
+  double s_sq = 0.0, s_abs = 0.0;
+  auto fd = [&](int64_t j) { return a[j] - b[j]; };
+  auto fa = [&](int64_t j) { return a[j]; };
+  PD_LOOP(i, n) {
+    constexpr int NC = KIND == DH_REG_GRADIENT ? 3 : 1;
+    float h[NC];
+    const Vox v = vox_of(i, d);
+    if (KIND == DH_REG_GRADIENT) grad3(fd, i, v, d, w, h);
+    else h[0] = fd(i);
+#pragma unroll
+    for (int q = 0; q < NC; ++q) s_sq += (double)(h[q] * h[q]);
+    if (want_abs) {
+      if (KIND == DH_REG_GRADIENT) grad3(fa, i, v, d, w, h);
+      else h[0] = fa(i);
+      float ss = 0.0f;
+#pragma unroll
+      for (int q = 0; q < NC; ++q) {
+        if (want_abs == 1) s_abs += (double)fabsf(h[q]);
+        ss += h[q] * h[q];
+      }
+      if (want_abs == 2) s_abs += (double)sqrtf(ss + 1e-12f);
+    }
+  }
+  const double r0 = pd_block_sum(s_sq, sh);
+  const double r1 = pd_block_sum(s_abs, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x * 2 + 0] = r0;
+    part[blockIdx.x * 2 + 1] = r1;
+  }
+}
+
+__global__ void k_grad_fwd(const float* __restrict__ f, float* __restrict__ out, Dims d, W3 w) {
+  const int64_t n = (int64_t)d.nz * d.ny * d.nx;
+  auto fv = [&](int64_t j) { return f[j]; };
+  PD_LOOP(i, n) {
+    float g[3];
+    grad3(fv, i, vox_of(i, d), d, w, g);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) out[q * n + i] = g[q];
+  }
+}
+
+__global__ void k_grad_adj(const float* __restrict__ y, float* __restrict__ out, Dims d, W3 w) {
+  const int64_t n = (int64_t)d.nz * d.ny * d.nx;
+  PD_LOOP(i, n) out[i] = grad3_adj(y, n, i, vox_of(i, d), d, w);
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 
@@ -563,6 +738,69 @@ hipError_t pd_move(double* sc, int from0, int to0, int from1, double* out1, hipS
 
 hipError_t pd_scalars(double* sc, double* hist, int k, const PdParams& pp, hipStream_t st) {
   hipLaunchKernelGGL(k_pd_scalars, dim3(1), dim3(64), 0, st, sc, hist, k, pp);
+  return hipGetLastError();
+}
+
+static W3 w3_of(const float* w) {
+  W3 r;
+  for (int q = 0; q < 3; ++q) r.w[q] = w ? w[q] : 1.0f;
+  return r;
+}
+
+hipError_t pr_dual(int kind, float* y2, const float* x, const float* xp, int nz, int ny, int nx, const double* sc,
+                   const float* w, float alpha, int l12, hipStream_t st) {
+  const Dims d{nz, ny, nx};
+  const int64_t n = (int64_t)nz * ny * nx;
+  if (kind == DH_REG_GRADIENT)
+    hipLaunchKernelGGL(k_pr_dual<DH_REG_GRADIENT>, dim3(pd_blocks(n)), dim3(kPdThreads), 0, st, y2, x, xp, d, sc,
+                       w3_of(w), alpha, l12);
+  else
+    hipLaunchKernelGGL(k_pr_dual<DH_REG_IDENTITY>, dim3(pd_blocks(n)), dim3(kPdThreads), 0, st, y2, x, xp, d, sc,
+                       w3_of(w), alpha, l12);
+  return hipGetLastError();
+}
+
+hipError_t pr_primal(int kind, float* xn, const float* x, const float* g, const float* y2, int nz, int ny, int nx,
+                     double* sc, const float* w, double* part, hipStream_t st) {
+  const Dims d{nz, ny, nx};
+  const int64_t n = (int64_t)nz * ny * nx;
+  const int nb = pd_blocks(n);
+  if (kind == DH_REG_GRADIENT)
+    hipLaunchKernelGGL(k_pr_primal<DH_REG_GRADIENT>, dim3(nb), dim3(kPdThreads), 0, st, xn, x, g, y2, d, sc, w3_of(w),
+                       part);
+  else
+    hipLaunchKernelGGL(k_pr_primal<DH_REG_IDENTITY>, dim3(nb), dim3(kPdThreads), 0, st, xn, x, g, y2, d, sc, w3_of(w),
+                       part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return finish(part, nb, 3, sc + PD_DX2, st);
+}
+
+hipError_t pr_sums(int kind, const float* a, const float* b, int nz, int ny, int nx, const float* w, int want_abs,
+                   double* sc, double* loss_out, double* part, hipStream_t st) {
+  const Dims d{nz, ny, nx};
+  const int64_t n = (int64_t)nz * ny * nx;
+  const int nb = pd_blocks(n);
+  if (kind == DH_REG_GRADIENT)
+    hipLaunchKernelGGL(k_pr_sums<DH_REG_GRADIENT>, dim3(nb), dim3(kPdThreads), 0, st, a, b, d, w3_of(w), want_abs,
+                       part);
+  else
+    hipLaunchKernelGGL(k_pr_sums<DH_REG_IDENTITY>, dim3(nb), dim3(kPdThreads), 0, st, a, b, d, w3_of(w), want_abs,
+                       part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return finish2(part, nb, sc + PD_DL2, loss_out, st);
+}
+
+hipError_t grad_forward(const float* f, float* out, int nz, int ny, int nx, const float* w, hipStream_t st) {
+  hipLaunchKernelGGL(k_grad_fwd, dim3(pd_blocks((int64_t)nz * ny * nx)), dim3(kPdThreads), 0, st, f, out,
+                     Dims{nz, ny, nx}, w3_of(w));
+  return hipGetLastError();
+}
+
+hipError_t grad_adjoint(const float* y, float* out, int nz, int ny, int nx, const float* w, hipStream_t st) {
+  hipLaunchKernelGGL(k_grad_adj, dim3(pd_blocks((int64_t)nz * ny * nx)), dim3(kPdThreads), 0, st, y, out,
+                     Dims{nz, ny, nx}, w3_of(w));
   return hipGetLastError();
 }
 

@@ -40,6 +40,15 @@ hipError_t pd_data_sums(const float* a, const float* b, const float* data, int64
 hipError_t pd_move(double* sc, int from0, int to0, int from1, double* out1, hipStream_t st);
 hipError_t pd_scalars(double* sc, double* hist, int k, const PdParams& pp, hipStream_t st);
 int64_t pd_partials_needed(int nz, int ny, int nx);
+// gradient (DH_REG_GRADIENT, weights w[3]) and identity (DH_REG_IDENTITY) regularizers
+hipError_t pr_dual(int kind, float* y2, const float* x, const float* xp, int nz, int ny, int nx, const double* sc,
+                   const float* w, float alpha, int l12, hipStream_t st);
+hipError_t pr_primal(int kind, float* xn, const float* x, const float* g, const float* y2, int nz, int ny, int nx,
+                     double* sc, const float* w, double* part, hipStream_t st);
+hipError_t pr_sums(int kind, const float* a, const float* b, int nz, int ny, int nx, const float* w, int want_abs,
+                   double* sc, double* loss_out, double* part, hipStream_t st);
+hipError_t grad_forward(const float* f, float* out, int nz, int ny, int nx, const float* w, hipStream_t st);
+hipError_t grad_adjoint(const float* y, float* out, int nz, int ny, int nx, const float* w, hipStream_t st);
 hipError_t hess_forward(const float* f, float* out, int nz, int ny, int nx, const float* w, hipStream_t st);
 hipError_t hess_adjoint(const float* y2, float* out, int nz, int ny, int nx, const float* w, hipStream_t st);
 

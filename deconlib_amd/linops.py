@@ -61,6 +61,8 @@ class Hessian3D:
     libdeconhip stencil kernels.
     """
 
+    _kind = N.REG_HESSIAN
+
     def __init__(self, r: float = 1.0):
         self.r = r
         self.operator_norm_sq = 16.0 * (r ** 4) + 4.0 * (r ** 2) + 34.0
@@ -108,6 +110,7 @@ class Hessian2D:
     """
 
     operator_norm_sq = 48.0
+    _kind = N.REG_HESSIAN
 
     def __init__(self):
         self._w = np.array([0.0, 1.0, 1.0, 0.0, 0.0, SQRT2], dtype=np.float32)
@@ -137,6 +140,103 @@ class Hessian2D:
 
     def __call__(self, f):
         return self.forward(f)
+
+
+class _StencilOp:
+    """A regularizer operator run by the library's 3-D stencil kernels on the input
+    promoted to (nz, ny, nx) (``dh_reg_forward`` / ``dh_reg_adjoint``).
+
+    ``_kind`` is the DH_REG_* operator family, ``_w`` its weights, ``_ndim`` the
+    input rank and ``_comps`` the internal components (of 6 Hessian or 3 gradient
+    planes) that form the reference's output, in order; unset components are zero.
+    """
+
+    _kind = N.REG_HESSIAN
+    _ndim = 3
+    _comps = (0, 1, 2, 3, 4, 5)
+    _squeeze = False  # a single-component output without the component axis (Gradient1D, Hessian1D)
+
+    def _wp(self):
+        return self._w.ctypes.data_as(C.POINTER(C.c_float))
+
+    def _dims(self, shape):
+        return (1,) * (3 - len(shape)) + tuple(int(v) for v in shape)
+
+    def forward(self, f) -> torch.Tensor:
+        f = as_f32(f)
+        if f.dim() != self._ndim:
+            raise ValueError(f"{type(self).__name__} expects a {self._ndim}-D array")
+        nc = N.fns["dh_reg_components"](self._kind)
+        d = self._dims(f.shape)
+        full = empty_f32((nc,) + d)
+        N.call("dh_reg_forward", self._kind, f.data_ptr(), full.data_ptr(), *d, self._wp(), N.stream_ptr())
+        if len(self._comps) == nc:
+            out = full.reshape((nc,) + tuple(f.shape))
+        else:
+            out = full[list(self._comps)].reshape((len(self._comps),) + tuple(f.shape)).contiguous()
+        return out[0] if self._squeeze else out
+
+    def adjoint(self, y) -> torch.Tensor:
+        y = as_f32(y)
+        if self._squeeze:
+            y = y.unsqueeze(0)
+        if y.dim() != self._ndim + 1 or y.shape[0] != len(self._comps):
+            raise ValueError(f"{type(self).__name__}.adjoint expects a ({len(self._comps)}, *spatial) array")
+        shape = tuple(y.shape[1:])
+        nc = N.fns["dh_reg_components"](self._kind)
+        d = self._dims(shape)
+        if len(self._comps) == nc:
+            full = y.reshape((nc,) + d)
+        else:
+            full = torch.zeros((nc,) + d, dtype=torch.float32, device=y.device)
+            full[list(self._comps)] = y.reshape((len(self._comps),) + d)
+        out = empty_f32(shape)
+        N.call("dh_reg_adjoint", self._kind, full.data_ptr(), out.data_ptr(), *d, self._wp(), N.stream_ptr())
+        return out
+
+    def __call__(self, f):
+        return self.forward(f)
+
+
+class Gradient1D(_StencilOp):
+    """Forward difference with Neumann boundary, ``||D||^2 = 4`` (linops_mlx.py:46-66)."""
+
+    operator_norm_sq = 4.0
+    _kind, _ndim, _comps, _squeeze = N.REG_GRADIENT, 1, (2,), True
+
+    def __init__(self):
+        self._w = np.array([0.0, 0.0, 1.0], dtype=np.float32)
+
+
+class Gradient2D(_StencilOp):
+    """(d/dy, d/dx) forward differences, ``||grad||^2 = 8`` (linops_mlx.py:69-89)."""
+
+    operator_norm_sq = 8.0
+    _kind, _ndim, _comps = N.REG_GRADIENT, 2, (1, 2)
+
+    def __init__(self):
+        self._w = np.array([0.0, 1.0, 1.0], dtype=np.float32)
+
+
+class Gradient3D(_StencilOp):
+    """(r d/dz, d/dy, d/dx), ``||grad||^2 = 4 (r^2 + 2)`` (linops_mlx.py:92-127)."""
+
+    _kind, _ndim, _comps = N.REG_GRADIENT, 3, (0, 1, 2)
+
+    def __init__(self, r: float = 1.0):
+        self.r = r
+        self.operator_norm_sq = 4.0 * (r * r + 2.0)
+        self._w = np.array([r, 1.0, 1.0], dtype=np.float32)
+
+
+class Hessian1D(_StencilOp):
+    """Second difference with Neumann boundary, ``||D2||^2 = 16`` (linops_mlx.py:130-150)."""
+
+    operator_norm_sq = 16.0
+    _kind, _ndim, _comps, _squeeze = N.REG_HESSIAN, 1, (2,), True
+
+    def __init__(self):
+        self._w = np.array([0.0, 0.0, 1.0, 0.0, 0.0, 0.0], dtype=np.float32)
 
 
 def _f32(v) -> np.float32:

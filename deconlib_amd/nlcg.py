@@ -220,7 +220,8 @@ def nlcg_with_operator(
         if len(window) > 5:
             window.pop(0)
         rel_avg = sum(window) / len(window)
-        stop = bool(callback(kp, st.f)) if callback is not None else False
+        # a copy: the next iteration updates st.f in place
+        stop = bool(callback(kp, st.f.clone())) if callback is not None else False
         reason = None
         ready = not stop and kp + 1 >= min_iter
         if ready and not use_reg and slack > 0.0 and loss <= 0.5 * slack:

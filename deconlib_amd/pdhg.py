@@ -29,44 +29,99 @@ from ._tensor import as_f32, device, empty_f32
 from .base import MLXDeconvolutionResult
 from .composition import compose
 from .core_operators import FractionalAreaDownsample
-from .linops import FFTConvolver, Hessian3D, LinearFFTConvolver, SQRT2
+from .linops import (FFTConvolver, Gradient1D, Gradient2D, Gradient3D, Hessian1D, Hessian2D, Hessian3D,
+                     LinearFFTConvolver)
 
 __all__ = [
     "solve_pdhg_mlx",
     "solve_pdhg_with_operator",
+    "IdentityRegularizer",
+    "GradientRegularizer",
     "HessianRegularizer",
 ]
 
 _TAU, _SIGMA, _THETA, _STEP, _COUNT, _DONE, _FINAL = 0, 1, 2, 3, 4, 5, 6
 
 
-class HessianRegularizer:
-    """Hessian regularizer wrapper (pdhg_mlx.py:380-451); 3-D volumes only."""
+class IdentityRegularizer:
+    """L = I for sparsity on x itself (pdhg_mlx.py:241-306); one component, ``||L||^2 = 1``."""
 
-    def __init__(self, ndim: int, r: float = 1.0, norm: str = "L1"):
+    output_components = 1
+    operator_norm_sq = 1.0
+    _kind = N.REG_IDENTITY
+
+    def __init__(self, norm: str = "L1"):
+        self.norm = norm
+        self._w = np.ones(3, dtype=np.float32)
+
+    def forward(self, x):
+        return as_f32(x).unsqueeze(0).clone()
+
+    def adjoint(self, y):
+        return as_f32(y)[0].clone()
+
+    def __call__(self, x):
+        return self.forward(x)
+
+
+class _StencilRegularizer:
+    """Gradient/Hessian regularizer around a stencil operator; 1-D outputs get a leading
+    unit component axis (pdhg_mlx.py:309-456)."""
+
+    def __init__(self, ndim: int, r: float, norm: str, ops):
         if ndim not in (1, 2, 3):
             raise ValueError(f"ndim must be 1, 2, or 3, got {ndim}")
-        if ndim != 3:
-            raise NotImplementedError("HessianRegularizer is implemented for 3-D volumes")
         self.ndim, self.r, self.norm = ndim, r, norm
-        self._op = Hessian3D(r=r)
-
-    @property
-    def output_components(self) -> int:
-        return 6
+        self._op = ops[ndim - 1]() if ndim < 3 else ops[2](r=r)
+        self._kind = self._op._kind
+        self._w = self._op._w
 
     @property
     def operator_norm_sq(self) -> float:
         return self._op.operator_norm_sq
 
     def forward(self, x):
-        return self._op.forward(x)
+        out = self._op.forward(x)
+        return out.unsqueeze(0) if self.ndim == 1 else out
 
     def adjoint(self, y):
-        return self._op.adjoint(y)
+        return self._op.adjoint(as_f32(y)[0] if self.ndim == 1 else y)
 
     def __call__(self, x):
         return self.forward(x)
+
+
+class GradientRegularizer(_StencilRegularizer):
+    """Gradient1D/2D/3D (total variation) regularizer (pdhg_mlx.py:309-377)."""
+
+    def __init__(self, ndim: int, r: float = 1.0, norm: str = "L1"):
+        super().__init__(ndim, r, norm, (Gradient1D, Gradient2D, Gradient3D))
+
+    @property
+    def output_components(self) -> int:
+        return self.ndim
+
+
+class HessianRegularizer(_StencilRegularizer):
+    """Hessian1D/2D/3D second-order regularizer (pdhg_mlx.py:380-456)."""
+
+    def __init__(self, ndim: int, r: float = 1.0, norm: str = "L1"):
+        super().__init__(ndim, r, norm, (Hessian1D, Hessian2D, Hessian3D))
+
+    @property
+    def output_components(self) -> int:
+        return 1 if self.ndim == 1 else (3 if self.ndim == 2 else 6)
+
+
+def _create_regularizer(regularization: str, ndim: int, r: float, norm: str):
+    """(pdhg_mlx.py:485-499)"""
+    if regularization == "identity":
+        return IdentityRegularizer(norm=norm)
+    if regularization == "gradient":
+        return GradientRegularizer(ndim=ndim, r=r, norm=norm)
+    if regularization == "hessian":
+        return HessianRegularizer(ndim=ndim, r=r, norm=norm)
+    raise ValueError(f"Unknown regularization: {regularization}")
 
 
 def _compute_spacing_ratio(spacing, ndim: int) -> float:
@@ -112,10 +167,15 @@ def _nonneg(x: torch.Tensor) -> torch.Tensor:
 
 
 class _PDState:
-    """Device-resident PDHG state for one solve (also driven by the batch/serving layers)."""
+    """Device-resident PDHG state for one solve (also driven by the batch/serving layers).
 
-    def __init__(self, observed, blur_op, x0, alpha, num_iter, background=0.0, r=1.0, norm="L1",
-                 loss_type="poisson", delta=0.99, eta=0.5, tol=1e-5, min_iter=20, patience=5):
+    The regularizer runs on the volume promoted to 3-D (``dh_pd_reg_*``); its dual
+    holds the kernel family's internal components (6 Hessian, 3 gradient, 1
+    identity planes), of which the 1-D and 2-D operators leave the unused ones zero.
+    """
+
+    def __init__(self, observed, blur_op, x0, alpha, num_iter, background=0.0, reg=None, norm="L1",
+                 loss_type="poisson", delta=0.99, eta=0.5, tol=1e-5, min_iter=20, patience=5, r=1.0):
         if norm not in ("L1", "L1_2"):
             raise ValueError(f"Unknown norm: {norm}")
         if loss_type not in ("poisson", "gaussian"):
@@ -123,18 +183,22 @@ class _PDState:
         self.op = blur_op
         self.data = as_f32(observed)
         self.x = as_f32(x0).clone()
-        if self.x.dim() != 3:
-            raise NotImplementedError("PDHG-Hessian is implemented for 3-D volumes")
+        if self.x.dim() not in (1, 2, 3):
+            raise ValueError("PDHG solves 1-D, 2-D and 3-D arrays")
         self.S = tuple(self.x.shape)
+        self.D3 = (1,) * (3 - len(self.S)) + self.S
+        if reg is None:
+            reg = HessianRegularizer(len(self.S), r, norm)
+        self.reg = reg
+        self.kind = int(reg._kind)
         self.alpha = float(alpha)
         self.bg = float(background)
         self.l12 = 1 if norm == "L1_2" else 0
         self.reg_norm = 2 if norm == "L1_2" else 1
         self.lt = 0 if loss_type == "poisson" else 1
         self.num_iter = int(num_iter)
-        self.w = torch.from_numpy(np.array([r * r, 1.0, 1.0, r * SQRT2, r * SQRT2, SQRT2],
-                                           dtype=np.float32))
-        self.wp = self.w.numpy().ctypes.data_as(N.C.POINTER(N.C.c_float))
+        self.w = np.ascontiguousarray(reg._w, dtype=np.float32)
+        self.wp = self.w.ctypes.data_as(N.C.POINTER(N.C.c_float))
         self.params = N.PdParams(delta=float(delta), eta=float(eta), tol=float(tol),
                                  min_iter=int(min_iter), patience=int(patience))
         self._plan = blur_op._plan if isinstance(blur_op, (LinearFFTConvolver, FFTConvolver)) else None
@@ -143,10 +207,10 @@ class _PDState:
         self.Ax = self._fwd(self.x, empty_f32(self.data.shape))
         self.Axp = self.Ax.clone()
         self.y1 = torch.zeros(self.data.shape, dtype=torch.float32, device=dev)
-        self.y2 = torch.zeros((6,) + self.S, dtype=torch.float32, device=dev)
+        nc = int(N.fns["dh_reg_components"](self.kind))
+        self.y2 = torch.zeros((nc,) + self.D3, dtype=torch.float32, device=dev)
         self.g = empty_f32(self.S)
-        L = Hessian3D(r)
-        self.step_scale = 0.99 / math.sqrt(blur_op.operator_norm_sq + L.operator_norm_sq)
+        self.step_scale = 0.99 / math.sqrt(blur_op.operator_norm_sq + reg.operator_norm_sq)
         sc = np.zeros(N.PD_NSLOTS, np.float64)
         sc[_TAU] = sc[_SIGMA] = sc[_STEP] = self.step_scale
         sc[_THETA] = 1.0
@@ -155,7 +219,7 @@ class _PDState:
         self.loss = torch.zeros(2 * (self.num_iter + 1), dtype=torch.float64, device=dev)
         if self.data.numel() > int(np.prod(self.S)):
             raise ValueError("the data domain must not be larger than the reconstruction domain")
-        self.part = torch.empty(int(N.fns["dh_pd_partials"](*self.S)), dtype=torch.float64, device=dev)
+        self.part = torch.empty(int(N.fns["dh_pd_partials"](*self.D3)), dtype=torch.float64, device=dev)
 
     def _fwd(self, x, out):
         if self._plan is not None:
@@ -171,21 +235,21 @@ class _PDState:
 
     def step(self, k: int, want_loss: bool) -> None:
         st = N.stream_ptr()
-        nz, ny, nx = self.S
+        nz, ny, nx = self.D3
         sc = self.sc.data_ptr()
         N.call("dh_pd_y1", self.y1.data_ptr(), self.Ax.data_ptr(), self.Axp.data_ptr(), self.data.data_ptr(),
                self.data.numel(), sc, self.bg, self.lt, st)
-        N.call("dh_pd_dual", self.y2.data_ptr(), self.x.data_ptr(), self.xp.data_ptr(), nz, ny, nx, sc,
-               self.wp, self.alpha, self.l12, st)
+        N.call("dh_pd_reg_dual", self.kind, self.y2.data_ptr(), self.x.data_ptr(), self.xp.data_ptr(), nz, ny, nx,
+               sc, self.wp, self.alpha, self.l12, st)
         self._adj(self.y1, self.g)
-        N.call("dh_pd_primal", self.xp.data_ptr(), self.x.data_ptr(), self.g.data_ptr(), self.y2.data_ptr(),
-               nz, ny, nx, sc, self.wp, self.part.data_ptr(), st)
+        N.call("dh_pd_reg_primal", self.kind, self.xp.data_ptr(), self.x.data_ptr(), self.g.data_ptr(),
+               self.y2.data_ptr(), nz, ny, nx, sc, self.wp, self.part.data_ptr(), st)
         self._fwd(self.xp, self.Axp)
         dl = self.loss[2 * k:].data_ptr() if want_loss else None
         rl = self.loss[2 * k + 1:].data_ptr() if want_loss else None
         N.call("dh_pd_data_sums", self.Axp.data_ptr(), self.Ax.data_ptr(), self.data.data_ptr(),
                self.data.numel(), self.bg, self.lt, sc, dl, self.part.data_ptr(), st)
-        N.call("dh_pd_hess_sums", self.xp.data_ptr(), self.x.data_ptr(), nz, ny, nx, self.wp, sc, rl,
+        N.call("dh_pd_reg_sums", self.kind, self.xp.data_ptr(), self.x.data_ptr(), nz, ny, nx, self.wp, sc, rl,
                self.reg_norm, self.part.data_ptr(), st)
         N.call("dh_pd_scalars", sc, self.hist.data_ptr(), k, N.C.byref(self.params), st)
         self.x, self.xp = self.xp, self.x
@@ -219,9 +283,10 @@ def _run(state: _PDState, num_iter, min_iter, patience, tol, eval_interval, call
             losses_at.append(k)
         if callback is not None:
             torch.cuda.current_stream().synchronize()
+            # a copy: the solver reuses its buffers (the reference hands over an immutable array)
             if callback_with_loss:
-                callback(k, state.x, state.loss_value(k) if want else None)
-            elif callback(k, state.x):
+                callback(k, state.x.clone(), state.loss_value(k) if want else None)
+            elif callback(k, state.x.clone()):
                 final_iter = k + 1
                 break
         if verbose and want and not loss_every_iter:
@@ -267,8 +332,6 @@ def solve_pdhg_mlx(
         )
     if regularization not in ("identity", "gradient", "hessian"):
         raise ValueError(f"Unknown regularization: {regularization}")
-    if regularization != "hessian":
-        raise NotImplementedError(f"regularization={regularization!r} is not implemented by this package")
     psf_shape = tuple(int(s) for s in (psf.shape if hasattr(psf, "shape") else np.shape(psf)))
     if sampling_factors is not None:
         f = _normalize_sampling_factors(sampling_factors, ndim)
@@ -303,7 +366,8 @@ def solve_pdhg_mlx(
         else:
             x0 = obs
         x0 = _nonneg(x0)
-    state = _PDState(obs, A, x0, alpha, num_iter, background, r, norm, loss_type, delta, eta, tol,
+    reg = _create_regularizer(regularization, ndim, r, norm)
+    state = _PDState(obs, A, x0, alpha, num_iter, background, reg, norm, loss_type, delta, eta, tol,
                      min_iter, patience)
     iters, conv, losses, hist = _run(state, num_iter, min_iter, patience, tol, eval_interval, callback,
                                      verbose, False, False)
@@ -354,12 +418,16 @@ def solve_pdhg_with_operator(
     operators whose domain and range differ).
     """
     obs = as_f32(observed)
-    if regularization != "hessian":
-        raise NotImplementedError(f"regularization={regularization!r} is not implemented by this package")
-    shape = tuple(getattr(blur_op, "in_shape", None) or obs.shape)
+    shp = getattr(blur_op, "shape", None)
+    if shp is not None and len(shp) == 2:
+        # a matrix operator (m, n): the solution is a length-n vector (pdhg_mlx.py:1011-1014)
+        shape = (int(shp[1]),)
+    else:
+        shape = tuple(getattr(blur_op, "in_shape", None) or obs.shape)
+    reg = _create_regularizer(regularization, len(shape), 1.0, norm)
     x0 = as_f32(init) if init is not None else torch.full(shape, 0.01, dtype=torch.float32, device=device())
     x0 = _nonneg(x0)
-    state = _PDState(obs, blur_op, x0, alpha, num_iter, background, 1.0, norm, loss_type, delta, eta, tol,
+    state = _PDState(obs, blur_op, x0, alpha, num_iter, background, reg, norm, loss_type, delta, eta, tol,
                      min_iter, patience)
     every = bool(verbose or callback is not None)
     iters, conv, losses, hist = _run(state, num_iter, min_iter, patience, tol, eval_interval, callback,

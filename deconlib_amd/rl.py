@@ -50,14 +50,13 @@ def poisson_i_divergence(observed, model, eps: float = 1e-6) -> float:
     """Mean Poisson I-divergence (rl_mlx.py:58-80), summed in float64 on the GPU."""
     obs = as_f32(observed)
     mdl = as_f32(model)
+    if tuple(obs.shape) != tuple(mdl.shape):
+        raise ValueError(f"observed {tuple(obs.shape)} and model {tuple(mdl.shape)} differ in shape")
     n = obs.numel()
-    tmp = empty_f32(obs.shape)
     part = torch.empty(N.fns["dh_reduce_partials"](n), dtype=torch.float64, device=device())
     out = torch.empty(1, dtype=torch.float64, device=device())
-    if eps != 1e-6:
-        raise NotImplementedError("only the reference default eps=1e-6 is implemented")
-    N.call("dh_add_scalar", mdl.data_ptr(), obs.data_ptr(), tmp.data_ptr(), n, 0.0,
-           part.data_ptr(), out.data_ptr(), N.stream_ptr())
+    N.call("dh_poisson_idiv", obs.data_ptr(), mdl.data_ptr(), n, float(eps), part.data_ptr(), out.data_ptr(),
+           N.stream_ptr())
     return float(out.item()) / n
 
 
@@ -215,7 +214,8 @@ def richardson_lucy_with_operator(
             print(f"  Iter {k:4d}: (loss reported at the end)")
         if callback is not None:
             torch.cuda.current_stream().synchronize()
-            if callback(k, st.x):
+            # a copy: the next iteration updates st.x in place (the reference hands over an immutable array)
+            if callback(k, st.x.clone()):
                 break
     if k is None:
         # the reference reads the loop variable after a zero-length loop

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define DH_ABI_VERSION 6
+#define DH_ABI_VERSION 7
 
 #define DH_OK 0
 #define DH_EINVAL (-1)
@@ -126,8 +126,10 @@ int dh_conv_apply(const dh_conv_plan* plan, int adjoint, const float* in, float*
  * the input x (rl_mlx.py:58-80, unnormalised).  spectrum_ready != 0: the workspace still
  * holds the forward X spectrum of x, left by the previous call on this workspace with
  * keep_spectrum != 0 and nothing run on the workspace since.  keep_spectrum != 0: the
- * update pass also leaves the forward X spectrum of the new x there.  Results are
- * bit-identical to two dh_conv_apply calls with DH_EPI_RL_RATIO / DH_EPI_RL_UPDATE.
+ * update pass also leaves the forward X spectrum of the new x there.  Results agree
+ * with two dh_conv_apply calls with DH_EPI_RL_RATIO / DH_EPI_RL_UPDATE to float32
+ * rounding (the fused kernel may contract multiply-adds differently: relative
+ * difference <= 1e-6), not bit for bit.
  * DH_EUNSUPPORTED for canvases without specialised kernels. */
 int dh_rl_iteration(const dh_conv_plan* plan, const void* otf_dev, void* work_dev, int64_t work_bytes,
                     float* x, const float* obs, const float* sens, float background, float eps,
@@ -222,6 +224,10 @@ int dh_rl_ratio(const float* model, const float* obs, float* out, int64_t n, flo
 /* RL update (rl_mlx.py:149-150): x = max(x * (corr / sens), eps). */
 int dh_rl_update(float* x, const float* corr, const float* sens, int64_t n, float eps,
                  void* stream);
+/* result_dev[0] = sum over i of obs*log(max(obs,eps)/max(model,eps)) - (obs - max(model,eps)),
+ * the unnormalised poisson_i_divergence (rl_mlx.py:58-80) for any eps, float64 partials. */
+int dh_poisson_idiv(const float* obs, const float* model, int64_t n, float eps, double* partials_dev,
+                    double* result_dev, void* stream);
 /* out = in + a (forward model plus background); reduce as dh_rl_ratio. */
 int dh_add_scalar(const float* in, const float* obs, float* out, int64_t n, float a,
                   double* partials_dev, double* reduce_out, void* stream);
@@ -277,6 +283,29 @@ int dh_pd_scalars(double* scalars, double* hist, int k, const dh_pd_params* para
 /* Hessian3D.forward f -> out (6, nz, ny, nx) and adjoint y -> out (linops_mlx.py:224-259). */
 int dh_hessian_forward(const float* f, float* out, int nz, int ny, int nx, const float* w, void* stream);
 int dh_hessian_adjoint(const float* y, float* out, int nz, int ny, int nx, const float* w, void* stream);
+
+/* ---- PDHG with any of the reference's regularizers (pdhg_mlx.py:241-499) ----
+ * reg selects L: DH_REG_HESSIAN (HessianRegularizer, linops_mlx.py:130-262: 6 components
+ * [zz, yy, xx, yz, xz, xy], w = 6 weights; Hessian2D = one plane with [0, 1, 1, 0, 0, sqrt2],
+ * Hessian1D = one row with [0, 0, 1, 0, 0, 0]), DH_REG_GRADIENT (GradientRegularizer,
+ * linops_mlx.py:46-127: forward differences d1_fwd (linops_core_mlx.py:25-39) along [z, y, x]
+ * with w = 3 weights [r, 1, 1]; Gradient2D = [0, 1, 1] on one plane, Gradient1D = [0, 0, 1] on
+ * one row) or DH_REG_IDENTITY (IdentityRegularizer, pdhg_mlx.py:241-306: one component, w
+ * unused).  y2 holds dh_reg_components(reg) planes of nz*ny*nx.  Same semantics as the
+ * Hessian-only calls above, which these extend. */
+#define DH_REG_HESSIAN 0
+#define DH_REG_GRADIENT 1
+#define DH_REG_IDENTITY 2
+int dh_reg_components(int reg);
+int dh_pd_reg_dual(int reg, float* y2, const float* x, const float* xp, int nz, int ny, int nx,
+                   const double* scalars, const float* w, float alpha, int l12, void* stream);
+int dh_pd_reg_primal(int reg, float* x_new, const float* x, const float* g, const float* y2, int nz, int ny,
+                     int nx, double* scalars, const float* w, double* partials, void* stream);
+int dh_pd_reg_sums(int reg, const float* a, const float* b, int nz, int ny, int nx, const float* w,
+                   double* scalars, double* reg_loss_out, int reg_norm, double* partials, void* stream);
+/* L.forward f -> out (components, nz, ny, nx) and L.adjoint y -> out. */
+int dh_reg_forward(int reg, const float* f, float* out, int nz, int ny, int nx, const float* w, void* stream);
+int dh_reg_adjoint(int reg, const float* y, float* out, int nz, int ny, int nx, const float* w, void* stream);
 
 /* ---- NLCG: Poisson ML by nonlinear conjugate gradients (nlcg_mlx.py:84-529) ----
  * f = s^2, m = K f + b.  phi = sum m - sum g ln max(m, eps) + beta ||C f||^2.
