@@ -18,7 +18,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIBDIR = PKG / "_lib"
 LIB = LIBDIR / "libdeconhip.so"
-SOURCES = ["dh_abi.hip", "dh_fft.hip", "dh_ops.hip", "dh_pdhg.hip", "dh_nlcg.hip", "dh_erdecon.hip", "dh_jetnewton.hip",
+SOURCES = ["dh_abi.hip", "dh_noalloc.hip", "dh_fft.hip", "dh_ops.hip", "dh_pdhg.hip", "dh_nlcg.hip", "dh_erdecon.hip", "dh_jetnewton.hip",
            "dh_ct_dispatch.hip",
            "dh_ct_p2a.hip", "dh_ct_p2b.hip", "dh_ct_p2c.hip", "dh_ct_p2d.hip", "dh_ct_p3a.hip",
            "dh_ct_p3b.hip", "dh_ct_p3c.hip", "dh_ct_p5a.hip", "dh_ct_p5b.hip"]

@@ -29,6 +29,7 @@ PRO_NONE, PRO_ONES, PRO_SUB_FLOOR = 0, 1, 2
 EPI_NONE, EPI_FLOOR, EPI_RL_RATIO, EPI_RL_UPDATE, EPI_ADD = 0, 1, 2, 3, 4
 RED_SUM, RED_MAX, RED_SUMSQ = 0, 1, 2
 REG_HESSIAN, REG_GRADIENT, REG_IDENTITY = 0, 1, 2
+STATIC_SCRATCH_BYTES = 128 << 20  # DH_STATIC_SCRATCH_BYTES
 TILE_PERSISTENT = 0x10000  # dh_conv_plan_set_tile flag: persistent grid with next-tile prefetch
 
 
@@ -166,6 +167,7 @@ SYMBOLS = {
     "dh_reg_forward": (_int, _int, _vp, _vp, _int, _int, _int, C.POINTER(_f), _vp),
     "dh_reg_adjoint": (_int, _int, _vp, _vp, _int, _int, _int, C.POINTER(_f), _vp),
     "dh_poisson_idiv": (_int, _vp, _vp, _i64, _f, _vp, _vp, _vp),
+    "dh_conv_apply_static": (_int, _vp, _int, _vp, _vp, _vp, C.POINTER(FuseArgs), _vp),
     "dh_batch_conv_apply": (_int, _int, C.POINTER(BatchItem), _int, _vp),
     "dh_rl_iteration": (_int, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _f, _f, _vp, _int, _int, _vp),
     "dh_batch_rl_iteration": (_int, _int, C.POINTER(RLItem), _vp),

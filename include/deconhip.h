@@ -135,6 +135,19 @@ int dh_rl_iteration(const dh_conv_plan* plan, const void* otf_dev, void* work_de
                     float* x, const float* obs, const float* sens, float background, float eps,
                     double* reduce_out, int spectrum_ready, int keep_spectrum, void* stream);
 
+/* dh_conv_apply without a workspace argument, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The workspace is a file-scope device
+ * array of DH_STATIC_SCRATCH_BYTES per GPU, held until the process ends; the call
+ * allocates nothing.  Plans whose work_bytes exceed it fail with DH_EWORKSPACE (e.g. a
+ * 64x256x256 volume on its 128x512x512 canvas needs 101 MB and fits; 64x512x512 does
+ * not).  Every call on one GPU shares that GPU's array: calls that overlap on one GPU
+ * (concurrent streams or threads) race and must be serialised by the caller.  Nothing
+ * in the array needs clearing between calls.  Results equal dh_conv_apply's bit for bit. */
+#define DH_STATIC_SCRATCH_BYTES (128LL << 20)
+int dh_conv_apply_static(const dh_conv_plan* plan, int adjoint, const float* in, float* out,
+                         const void* otf_dev, const dh_fuse_args* fuse, void* stream);
+
 /* ---- batched application: many plans (different shapes), few launches ---- */
 typedef struct {
   const dh_conv_plan* plan;
