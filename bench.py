@@ -161,8 +161,9 @@ def bench_pdhg_binned(torch, steps=10, warmup=2, seed=1001):
 def bench_large_volume(torch, steps=3, warmup=1, seed=1004):
     """The large single-volume configuration at full size: 256x2048x2048 float32 on a
     512x4096x4096 canvas, PDHG-Hessian (alpha 5e-4, L1) iterations.  Object: 1e5 blobs
-    + 10 and 64 thin filaments (intensity 300); widefield PSF of the volume's size with
-    IndexMismatch depth 10 um, synthesised on the GPU; Poisson data.  Skipped below
+    + 10 and 64 thin filaments (intensity 300); vectorial PSF of an isotropic emitter
+    (pupil_to_vectorial_psf) of the volume's size with IndexMismatch depth 10 um,
+    synthesised on the GPU; Poisson data.  Skipped below
     200 GB of free device memory (about 160 GB stay resident)."""
     import ctypes as C
 
@@ -179,7 +180,7 @@ def bench_large_volume(torch, steps=3, warmup=1, seed=1004):
     t0 = time.perf_counter()
     rng = np.random.default_rng(seed)
     psf = widefield_psf(shape, LATERAL, AXIAL, CONFIG0_OPTICS, aberrations=[IndexMismatch(depth=10.0)],
-                        device=True)
+                        device=True, dipole="isotropic")
     op = LinearFFTConvolver(psf, shape)
     del psf
     obj = blob_volume(shape, 100000, rng)
@@ -213,6 +214,7 @@ def bench_large_volume(torch, steps=3, warmup=1, seed=1004):
     vox = int(np.prod(shape))
     out = {
         "metric": "voxel-iters/s (PDHG-Hessian) at 256x2048x2048 fp32, canvas 512x4096x4096",
+        "psf": "vectorial, isotropic dipole, NA 1.4, IndexMismatch depth 10 um (GPU synthesis)",
         "value": vox * steps / dt,
         "ms_per_step": dt * 1e3 / steps,
         "steps": steps,
@@ -727,6 +729,107 @@ def cpu_baseline(psf, obs, iters):
     }
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _timed_oracle_iterations(run, vox, n_iter):
+    """Runs an oracle solve with a per-iteration callback and times iterations 2..n
+    (the first is the warm-up); returns (voxel-iterations/s, seconds timed)."""
+    stamps = []
+    run(lambda k, x: stamps.append(time.perf_counter()))
+    dt = stamps[-1] - stamps[0]
+    return vox * (len(stamps) - 1) / dt, dt
+
+
+def cpu_baseline_configs(seed=2000):
+    """The oracle (fp32 NumPy/SciPy restatement) timed on the host cores for each workload
+    configuration, on bounded samples (about 10-30 s of CPU work each, inputs made on the
+    host with the configurations' recipes):
+
+    * RL 64x256x256 (canvas 128x512x512): 5 iterations after 1 warm-up;
+    * binned PDHG-Hessian, object 128x512x512 -> data 64x256x256 (canvas 256x1024x1024):
+      1 iteration after 1 warm-up, voxel-iterations on the object grid;
+    * batched RL: 6 volumes drawn from the batched configuration's shapes with aberrated
+      PSFs, 3 iterations each after 1 warm-up, one after another (the reference's loop);
+    * the large-volume solver (PDHG-Hessian alpha 5e-4) on its 128x512x512 crop with the
+      vectorial PSF: 1 iteration after 1 warm-up."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import deconv as O
+
+    from deconlib_amd.psf import IndexMismatch, widefield_psf
+    from deconlib_amd.synth import CONFIG0_OPTICS, aberrated_psf, blob_volume
+
+    rng = np.random.default_rng(seed)
+    out = {}
+
+    def poisson(clean):
+        return rng.poisson(np.maximum(clean, 0.0)).astype(np.float32)
+
+    # RL, 64x256x256
+    shape = (64, 256, 256)
+    psf = widefield_psf(shape, LATERAL, AXIAL, CONFIG0_OPTICS)
+    A = O.LinearFFTConvolver(psf, shape)
+    obs = poisson(A.forward(blob_volume(shape, 200, rng).astype(np.float32)))
+    rate, dt = _timed_oracle_iterations(
+        lambda cb: O.richardson_lucy(obs, A, num_iter=6, callback=cb), int(np.prod(shape)), 6)
+    out["rl_64x256x256"] = {"value": rate, "unit": "voxel-iters/s", "sample": f"5 RL iterations, {dt:.1f}s"}
+    log(f"cpu baseline RL 64x256x256: {rate:.3e} voxel-iters/s")
+
+    # binned PDHG-Hessian
+    hi = (128, 512, 512)
+    psf = widefield_psf(hi, 0.0425, 0.05, CONFIG0_OPTICS)
+    Ab = O.compose(O.FractionalAreaDownsample(scale=(2.0, 2.0, 2.0)), O.LinearFFTConvolver(psf, hi))
+    clean = Ab.forward(blob_volume(hi, 2000, rng).astype(np.float32))
+    obs = poisson(clean * (500.0 / float(clean.max())))
+    rate, dt = _timed_oracle_iterations(
+        lambda cb: O.solve_pdhg(obs, psf, alpha=1e-3, num_iter=2, bin_factors=2, tol=0.0, callback=cb),
+        int(np.prod(hi)), 2)
+    out["pdhg_binned_128x512x512"] = {"value": rate, "unit": "voxel-iters/s",
+                                      "sample": f"1 PDHG-Hessian iteration on the object grid, {dt:.1f}s"}
+    log(f"cpu baseline binned PDHG: {rate:.3e} voxel-iters/s")
+
+    # batched RL, the reference's one-volume-at-a-time loop
+    total_vox_it, total_dt = 0, 0.0
+    shapes = []
+    for _ in range(6):
+        z = int(rng.choice([16, 32, 48, 64]))
+        xy = int(rng.choice([128, 192, 256, 384, 512]))
+        sh = (z, xy, xy)
+        shapes.append(sh)
+        p = aberrated_psf(sh, rng)
+        Av = O.LinearFFTConvolver(p, sh)
+        ob = poisson(Av.forward(blob_volume(sh, max(1, round(200 * np.prod(sh) / 4194304)), rng).astype(np.float32)))
+        r, d = _timed_oracle_iterations(lambda cb: O.richardson_lucy(ob, Av, num_iter=4, callback=cb),
+                                        int(np.prod(sh)), 4)
+        total_vox_it += r * d
+        total_dt += d
+    rate = total_vox_it / total_dt
+    out["batched_rl"] = {"value": rate, "unit": "voxel-iters/s",
+                         "sample": f"3 RL iterations on each of 6 volumes {shapes}, {total_dt:.1f}s"}
+    log(f"cpu baseline batched RL: {rate:.3e} voxel-iters/s")
+
+    # large-volume solver on its 128x512x512 crop
+    crop = (128, 512, 512)
+    psf = widefield_psf(crop, LATERAL, AXIAL, CONFIG0_OPTICS, aberrations=[IndexMismatch(depth=10.0)],
+                        dipole="isotropic")
+    Ac = O.LinearFFTConvolver(psf, crop)
+    obs = poisson(Ac.forward(blob_volume(crop, 3125, rng).astype(np.float32)))
+    rate, dt = _timed_oracle_iterations(
+        lambda cb: O.solve_pdhg(obs, psf, alpha=5e-4, num_iter=2, tol=0.0, callback=cb), int(np.prod(crop)), 2)
+    out["pdhg_large_volume_crop_128x512x512"] = {"value": rate, "unit": "voxel-iters/s",
+                                                 "sample": f"1 PDHG-Hessian iteration, vectorial PSF, {dt:.1f}s"}
+    log(f"cpu baseline large-volume crop: {rate:.3e} voxel-iters/s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -734,6 +837,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-iters", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu-configs", action="store_true", help="CPU baseline at the metric shape only")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--tune-cache", default=None, help="JSON file the autotuner reads/writes")
     ap.add_argument("--copy-probe", action="store_true")
@@ -837,6 +941,9 @@ def main():
     if not args.no_cpu:
         log("cpu baseline ...")
         cpu = cpu_baseline(psf, obs, args.cpu_iters)
+        cpu["cpu_model"] = _cpu_model()
+        if not args.no_cpu_configs:
+            cpu["per_config"] = cpu_baseline_configs()
     out = {
         "metric": METRIC,
         "value": value,

@@ -17,6 +17,7 @@ from .core_operators import Crop, FractionalAreaDownsample, Pad
 from .linops import (AnisotropicHessian2D, AnisotropicHessian3D, FFTConvolver, Gradient1D, Gradient2D, Gradient3D,
                      Hessian1D, Hessian2D, Hessian3D, LinearFFTConvolver, fast_padded_shape)
 from .nlcg import NLCGResult, nlcg_solver, nlcg_with_operator
+from .norms import power_iteration_norm_sq
 from .erdecon import ERDeconResult, erdecon_solver, erdecon_with_operator
 from .jetnewton import JetNewtonResult, jetnewton_with_operator
 from .pdhg import (GradientRegularizer, HessianRegularizer, IdentityRegularizer, solve_pdhg_mlx,
@@ -67,6 +68,7 @@ __all__ = [
     "FFTConvolver",
     "LinearFFTConvolver",
     "fast_padded_shape",
+    "power_iteration_norm_sq",
     "Compose",
     "LinearOperator",
     "compose",

@@ -338,9 +338,12 @@ def apply_aberrations(pupil, geom, optics, aberrations) -> np.ndarray:
 
 
 def widefield_psf(shape, lateral: float, axial: float, optics: Optics = None, aberrations=(),
-                  dtype=np.float32, device: bool = False):
+                  dtype=np.float32, device: bool = False, dipole=None):
     """(nz, ny, nx) corner-origin PSF normalised to sum 1 (the configs' recipe).
 
+    ``dipole=None``: the scalar widefield PSF (pupil_to_psf, widefield.py:18-73);
+    ``dipole="isotropic"`` (or "x"/"y"/"z"/(theta, phi)): the vectorial PSF
+    (pupil_to_vectorial_psf, widefield.py:110-234).
     ``device=True``: synthesised on the GPU, returned as a float32 device tensor.
     """
     optics = optics or Optics(wavelength=0.525, na=1.4, ni=1.515, ns=1.334)
@@ -349,7 +352,12 @@ def widefield_psf(shape, lateral: float, axial: float, optics: Optics = None, ab
     pupil = make_pupil(geom)
     if aberrations:
         pupil = apply_aberrations(pupil, geom, optics, list(aberrations))
+    z = fft_coords(nz, axial)
+    if dipole is not None:
+        if device:
+            return pupil_to_vectorial_psf_device(pupil, geom, optics, z, dipole=dipole, normalize=True)
+        return pupil_to_vectorial_psf(pupil, geom, optics, z, dipole=dipole, normalize=True).astype(dtype)
     if device:
-        return pupil_to_psf_device(pupil, geom, fft_coords(nz, axial), normalize=True)
-    psf = pupil_to_psf(pupil, geom, fft_coords(nz, axial), normalize=True)
+        return pupil_to_psf_device(pupil, geom, z, normalize=True)
+    psf = pupil_to_psf(pupil, geom, z, normalize=True)
     return psf.astype(dtype)

@@ -770,6 +770,9 @@ def pdhg_loop(observed, blur_op, reg_op, x, alpha, num_iter, background, norm="L
     losses, taus, sigmas = [], [], []
     converged, count, final_iter = False, 0, num_iter
     bg = float(background)
+    # A x and L x of the current iterate: the previous iteration's A x' and L x' (the same
+    # arrays the reference recomputes, so the values are identical; this only saves time)
+    Ax_cur = Lx_cur = None
     for k in range(num_iter):
         Axb = blur_op.forward(xbar)
         if loss_type == "poisson":
@@ -782,8 +785,9 @@ def pdhg_loop(observed, blur_op, reg_op, x, alpha, num_iter, background, norm="L
         xn = prox_nonneg(x - t(tau) * grad)
         dx = xn - x
         dx2 = float(np.sum(dx.astype(np.float64) ** 2))
-        dA = blur_op.forward(xn) - blur_op.forward(x)
-        dL = reg_op.forward(xn) - reg_op.forward(x)
+        Axn, Lxn = blur_op.forward(xn), reg_op.forward(xn)
+        dA = Axn - (blur_op.forward(x) if Ax_cur is None else Ax_cur)
+        dL = Lxn - (reg_op.forward(x) if Lx_cur is None else Lx_cur)
         dK2 = float(np.sum(dA.astype(np.float64) ** 2) + np.sum(dL.astype(np.float64) ** 2))
         tn, sn, thn, back = malitsky_pock_step(tau, sigma, theta, dx2, dK2, step, delta, eta)
         xx = float(np.sum(x.astype(np.float64) ** 2))
@@ -803,12 +807,13 @@ def pdhg_loop(observed, blur_op, reg_op, x, alpha, num_iter, background, norm="L
             break
         xbar = xn + t(thn) * (xn - x)
         x, y1, y2 = xn, y1n, y2n
+        Ax_cur, Lx_cur = Axn, Lxn
         tau, sigma, theta = tn, sn, thn
         taus.append(tau)
         sigmas.append(sigma)
         loss = None
         if loss_every_iter or (loss_on_interval and (k + 1) % eval_interval == 0):
-            loss = pdhg_objective(data, blur_op.forward(x) + t(bg), reg_op.forward(x), alpha, norm, loss_type)
+            loss = pdhg_objective(data, Ax_cur + t(bg), Lx_cur, alpha, norm, loss_type)
             losses.append(loss)
         if callback is not None:
             if callback_with_loss:

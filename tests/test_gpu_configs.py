@@ -14,8 +14,9 @@ on the size:
 * batched RL over the batched configuration's uneven shapes (z in {16..64},
   y = x in {128..512}, per-volume aberrated PSFs): every volume's batched result
   is bit-identical to its solo result;
-* the large single volume (256x2048x2048, canvas 512x4096x4096): adjoint
-  identity of the convolution and one PDHG-Hessian iteration at full size
+* the large single volume (256x2048x2048, canvas 512x4096x4096) with its vectorial
+  PSF (isotropic emitter, IndexMismatch 10 um, synthesised on the GPU, sum 1):
+  adjoint identity of the convolution and one PDHG-Hessian iteration at full size
   (skipped when the device has less than 200 GB free);
 * NLCG, ER-Decon and JetNewton on the RL workload (64x256x256): NLCG against the
   float32 oracle on data without the modelled background (both stop after the
@@ -135,19 +136,18 @@ def test_batched_config_shapes(gpu):
 def test_large_volume_full_size(gpu):
     from deconlib_amd import LinearFFTConvolver
     from deconlib_amd.pdhg import _PDState, _nonneg
+    from deconlib_amd.psf import IndexMismatch, widefield_psf
+    from deconlib_amd.synth import CONFIG0_OPTICS
 
     free, _ = torch.cuda.mem_get_info()
     if free < 200e9:
         pytest.skip(f"needs 200 GB free device memory, {free / 1e9:.0f} GB free")
     shape = (256, 2048, 2048)
     print(f"large volume {shape}", flush=True)
-    # separable Gaussian PSF (sigma 3 planes, 1.5 px), corner origin, built on the device
-    axes = []
-    for n, s in zip(shape, (3.0, 1.5, 1.5)):
-        i = torch.arange(n, device="cuda", dtype=torch.float32)
-        d = torch.minimum(i, n - i)
-        axes.append(torch.exp(-0.5 * (d / s) ** 2))
-    psf = axes[0][:, None, None] * axes[1][None, :, None] * axes[2][None, None, :]
+    # the configuration's PSF: vectorial (isotropic emitter), NA 1.4, IndexMismatch 10 um, on the GPU
+    psf = widefield_psf(shape, 0.085, 0.1, CONFIG0_OPTICS, aberrations=[IndexMismatch(depth=10.0)], device=True,
+                        dipole="isotropic")
+    assert abs(float(psf.double().sum()) - 1.0) < 1e-5
     op = LinearFFTConvolver(psf, shape)
     del psf
     assert op.canvas_shape == (512, 4096, 4096)

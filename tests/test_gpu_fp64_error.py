@@ -12,12 +12,16 @@ requires the GPU's to stay within the parity bar (relative L2 <= 1e-4) and withi
   0.0425/0.05 um, peak ~500 counts, 20 iterations;
 * batched RL: two volume shapes of the batched configuration (16x384x384 and
   48x192x192) with their own aberrated PSFs, run together in one batch, 50 iterations;
-* the large-volume configuration's solver (PDHG-Hessian alpha 5e-4) on a 128x512x512
-  crop (canvas 256x1024x1024): blobs at the configuration's density plus thin
-  filaments, scalar PSF with IndexMismatch depth 10 um, 2 iterations.
+* the large-volume configuration's solver (PDHG-Hessian alpha 5e-4), 20 iterations, on
+  a 64x256x256 crop and (with DECONLIB_AMD_LONG_TESTS=1: the float64 oracle takes
+  about 10 minutes of host time there) on the configuration's 128x512x512 crop:
+  blobs at the configuration's density plus thin filaments, the configuration's
+  vectorial PSF (isotropic emitter, NA 1.4) with IndexMismatch depth 10 um.
 
 The serving configuration runs the same RL and PDHG kernels as these.
 """
+
+import os
 
 import numpy as np
 import pytest
@@ -108,22 +112,36 @@ def test_batched_config_fp64_error(gpu):
         _report(f"batched RL {tuple(res.restored.shape)}, {it} it", res.restored.cpu().numpy(), r32, r64)
 
 
-def test_large_volume_crop_fp64_error(gpu):
+LONG = os.environ.get("DECONLIB_AMD_LONG_TESTS") == "1"
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("shape,it", [
+    ((64, 256, 256), 20),
+    pytest.param((128, 512, 512), 20, marks=pytest.mark.skipif(
+        not LONG, reason="the float64 oracle needs ~10 minutes of host time at this size; "
+                         "run with DECONLIB_AMD_LONG_TESTS=1")),
+])
+def test_large_volume_crop_fp64_error(gpu, shape, it):
     from deconlib_amd import solve_pdhg_mlx
     from deconlib_amd.psf import IndexMismatch, widefield_psf
     from deconlib_amd.synth import CONFIG0_OPTICS, blob_volume, filaments
 
-    shape, it = (128, 512, 512), 2
     print(f"large-volume solver on a {shape} crop, {it} iterations", flush=True)
     rng = np.random.default_rng(1004)
-    psf = widefield_psf(shape, 0.085, 0.1, CONFIG0_OPTICS, aberrations=[IndexMismatch(depth=10.0)])
+    psf = widefield_psf(shape, 0.085, 0.1, CONFIG0_OPTICS, aberrations=[IndexMismatch(depth=10.0)],
+                        dipole="isotropic")
     # 1e5 blobs over 256x2048x2048 voxels: the same density on the crop
     obj = blob_volume(shape, round(1e5 * np.prod(shape) / (256 * 2048 * 2048)), rng)
-    obj = np.maximum(obj, filaments(shape, 24, rng))
+    obj = np.maximum(obj, filaments(shape, max(3, round(24 * np.prod(shape) / (128 * 512 * 512))), rng))
     obs = _poisson(O.LinearFFTConvolver(psf, shape, dtype=np.float64).forward(obj), 1004)
     kw = dict(alpha=5e-4, num_iter=it, tol=0.0)
     gpu = solve_pdhg_mlx(torch.from_numpy(obs).cuda(), psf, **kw)
-    r32 = O.solve_pdhg(obs, psf, **kw)
-    r64 = O.solve_pdhg(obs, psf, dtype=np.float64, **kw)
+    def progress(name):
+        return lambda k, x: print(f"  {name} oracle iteration {k + 1}/{it}", flush=True)
+
+    r32 = O.solve_pdhg(obs, psf, callback=progress("float32"), **kw)
+    r64 = O.solve_pdhg(obs, psf, dtype=np.float64, callback=progress("float64"), **kw)
     assert gpu.iterations == r32.iterations == r64.iterations == it
-    _report(f"PDHG-Hessian 128x512x512 crop, {it} it", gpu.restored.cpu().numpy(), r32.restored, r64.restored)
+    _report(f"PDHG-Hessian {'x'.join(map(str, shape))} crop, {it} it", gpu.restored.cpu().numpy(), r32.restored,
+            r64.restored)

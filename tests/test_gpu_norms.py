@@ -1,0 +1,64 @@
+"""FP64 power-iteration operator-norm estimate on the GPU (deconlib_amd.norms).
+
+Checks, per operator, the reference's own norm tests (tests/test_operators_mlx_norms.py:56-130:
+estimate <= closed form * 1.05 and >= 0.8 * closed form) plus what the mathematics fixes:
+the block sum of FractionalAreaDownsample(2) has ||A||^2 = 8 exactly (constant vectors);
+a normalised convolution stays below max|OTF|^2 = 1; a composition stays below the
+product of norms; and every estimate equals a float64 NumPy power iteration of the
+oracle operator from the same start vector to 1e-5.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import deconv as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _cpu_power(op, shape, num_iter, seed=0):
+    v = np.random.default_rng(seed).standard_normal(shape).astype(np.float32).astype(np.float64)
+    v /= np.linalg.norm(v)
+    est = 0.0
+    for _ in range(num_iter):
+        w = op.adjoint(op.forward(v))
+        est = float(np.sum(v * w))
+        v = w / np.linalg.norm(w)
+    return est
+
+
+def _pair(name):
+    import deconlib_amd as D
+
+    rng = np.random.default_rng(60)
+    if name == "downsample":
+        return D.FractionalAreaDownsample(2), O.FractionalAreaDownsample(2), (8, 12, 16), 8.0
+    if name == "hessian":
+        return D.Hessian3D(1.0), O.Hessian3D(1.0), (24, 24, 24), 54.0
+    if name == "gradient":
+        return D.Gradient3D(3.0), O.Gradient3D(3.0), (16, 24, 24), 44.0
+    psf = (rng.random((5, 9, 9)) + 0.05).astype(np.float32)
+    if name == "conv":
+        return D.LinearFFTConvolver(psf, (12, 20, 24)), O.LinearFFTConvolver(psf, (12, 20, 24), dtype=np.float64), \
+            (12, 20, 24), 1.0
+    op = D.compose(D.Crop((6, 10, 12), (6, 9, 10)), D.FractionalAreaDownsample(2), D.LinearFFTConvolver(psf, (12, 20, 24)))
+    ref = O.compose(O.Crop((6, 10, 12), (6, 9, 10)), O.FractionalAreaDownsample(2),
+                    O.LinearFFTConvolver(psf, (12, 20, 24), dtype=np.float64))
+    return op, ref, (12, 20, 24), float(op.operator_norm_sq)
+
+
+@pytest.mark.parametrize("name", ["downsample", "hessian", "gradient", "conv", "composed"])
+def test_power_iteration_norm(gpu, name):
+    from deconlib_amd import power_iteration_norm_sq
+
+    op, ref, shape, bound = _pair(name)
+    est, hist = power_iteration_norm_sq(op, shape, num_iter=60, tol=0.0, return_history=True)
+    assert len(hist) == 60
+    cpu = _cpu_power(ref, shape, 60)
+    print(f"{name}: GPU {est:.8f}, float64 CPU {cpu:.8f}, closed form / bound {bound}")
+    assert est == pytest.approx(cpu, rel=1e-5)
+    assert est <= bound * 1.05 and est >= 0.8 * bound
+    if name == "downsample":
+        assert est == pytest.approx(8.0, rel=1e-6)
+    if name in ("conv", "composed"):
+        assert est <= bound * (1 + 1e-6)
