@@ -664,35 +664,66 @@ def bench_psf_synthesis(torch, shape=SHAPE):
             "note": "end to end: pupil geometry and aberrations on the host, uploads, synthesis, normalisation"}
 
 
+def pdhg_iteration_bytes(S, P):
+    """Algorithmic HBM bytes of one PDHG-Hessian iteration on a plain convolution: two
+    convolutions plus the dual/primal kernels in float32 words per voxel: y1 prox 5
+    (A x, A x_prev, data, y1 in; y1 out), Hessian dual step 14 (x, x_prev, y2[6] in;
+    y2[6] out), primal step 8 (x, g, y2[6] in; x_new out), data sums 2, Hessian sums 2."""
+    return 2 * sum(conv_pass_bytes(S, P).values()) + 31 * 4 * int(np.prod(S))
+
+
 def bench_serving(torch, n_jobs, rate, seed=1003):
-    """Serving sample: Poisson arrivals of mixed RL (100 it) / PDHG-Hessian (200 it) jobs,
-    sizes log-uniform 16x64x64 .. 48x256x256; continuous batching vs the plain one-job loop."""
-    from deconlib_amd.serving import DeconvServer, latency_stats, run_plain
-    from deconlib_amd.synth import serving_requests
-
-    from deconlib_amd import LinearFFTConvolver
-
-    def gpu_blur(psf, obj):
-        return LinearFFTConvolver(psf, obj.shape).forward(obj.astype(np.float32)).cpu().numpy()
+    """The serving configuration: Poisson arrivals at ``rate`` jobs/s of 70 % RL (100 it) /
+    30 % PDHG-Hessian (200 it, alpha log-uniform 1e-4..1e-2) jobs, z log-uniform 32..128 and
+    y = x log-uniform 128..1024 (independent, any integer), per-job aberrated PSFs, blob
+    objects and Poisson observations made on the GPU before the run.  The server admits
+    by HBM budget (free device memory at the start less 16 GiB) and batches iterations
+    across live jobs; the plain loop runs the same jobs one at a time, synchronously.
+    Reports p50/p99 latency, aggregate voxel-iterations/s, the plain loop's rate and the
+    fraction of the HBM roofline (the jobs' algorithmic bytes / makespan / 8 TB/s)."""
+    from deconlib_amd.serving import DeconvServer, latency_stats, plan_info, run_plain
+    from deconlib_amd.synth import serving_requests_device
 
     t0 = time.time()
-    reqs = serving_requests(n_jobs, rate, seed, blur=gpu_blur)
-    log(f"serving: {n_jobs} requests synthesised in {time.time() - t0:.1f}s")
-    warm = serving_requests(2, 1000.0, seed + 1, blur=gpu_blur)
+    reqs = serving_requests_device(n_jobs, rate, seed)
+    torch.cuda.synchronize()
+    log(f"serving: {n_jobs} requests synthesised on the GPU in {time.time() - t0:.1f}s")
+    warm = serving_requests_device(3, 1000.0, seed + 1, z_range=(32, 40), xy_range=(128, 160), rl_iters=3,
+                                   pdhg_iters=3, kinds=["rl", "pdhg", "rl"])
     DeconvServer().run(warm)
-    srv = DeconvServer()
+    run_plain(warm)
+    free, _ = torch.cuda.mem_get_info()
+    srv = DeconvServer(hbm_budget_bytes=int(free - (16 << 30)))
     t0 = time.perf_counter()
     done = srv.run(reqs)
     makespan = time.perf_counter() - t0
     stats = latency_stats(done, makespan)
+    log(f"serving: {json.dumps(stats)}")
     plain_s = run_plain(reqs)
     vox_it = sum(int(np.prod(r.observed.shape)) * r.num_iter for r in reqs)
+    nbytes = 0
+    for r in reqs:
+        S = tuple(int(v) for v in r.observed.shape)
+        P = tuple(int(v) for v in plan_info(S).canvas_shape)
+        nbytes += r.num_iter * (rl_iteration_bytes(S, P, fused=True) if r.kind == "rl" else pdhg_iteration_bytes(S, P))
+    sizes = [int(np.prod(r.observed.shape)) for r in reqs]
     stats.update({
-        "workload": f"{n_jobs} jobs, Poisson arrivals {rate}/s, 70% RL 100 it / 30% PDHG-Hessian 200 it, "
-                    "volumes log-uniform 16x64x64..48x256x256, per-job aberrated PSFs",
+        "workload": f"{n_jobs} jobs, Poisson arrivals {rate}/s, 70% RL 100 it / 30% PDHG-Hessian 200 it "
+                    "(alpha log-uniform 1e-4..1e-2), z log-uniform 32..128, y=x log-uniform 128..1024, "
+                    "per-job aberrated PSFs, inputs on the GPU",
+        "jobs_rl": sum(r.kind == "rl" for r in reqs),
+        "voxels_min_median_max": [min(sizes), int(np.median(sizes)), max(sizes)],
+        "arrival_span_s": max(r.arrival for r in reqs),
+        "ticks": srv.ticks,
+        "plain_loop_s": plain_s,
         "plain_loop_voxel_iters_per_s": vox_it / plain_s,
-        "speedup_vs_plain_loop": (stats["voxel_iters_per_s"] * plain_s / vox_it) if vox_it else None,
+        "speedup_vs_plain_loop": plain_s / makespan,
+        "algorithmic_bytes": nbytes,
+        "hbm_frac": nbytes / makespan / (PEAK_HBM_GBS * 1e9),
+        "plain_loop_hbm_frac": nbytes / plain_s / (PEAK_HBM_GBS * 1e9),
     })
+    del reqs
+    torch.cuda.empty_cache()
     return stats
 
 
@@ -754,13 +785,13 @@ def cpu_baseline_configs(seed=2000):
     configuration, on bounded samples (about 10-30 s of CPU work each, inputs made on the
     host with the configurations' recipes):
 
-    * RL 64x256x256 (canvas 128x512x512): 5 iterations after 1 warm-up;
+    * RL 64x256x256 (canvas 128x512x512): 20 iterations after 1 warm-up;
     * binned PDHG-Hessian, object 128x512x512 -> data 64x256x256 (canvas 256x1024x1024):
-      1 iteration after 1 warm-up, voxel-iterations on the object grid;
+      3 iterations after 1 warm-up, voxel-iterations on the object grid;
     * batched RL: 6 volumes drawn from the batched configuration's shapes with aberrated
-      PSFs, 3 iterations each after 1 warm-up, one after another (the reference's loop);
+      PSFs, 6 iterations each after 1 warm-up, one after another (the reference's loop);
     * the large-volume solver (PDHG-Hessian alpha 5e-4) on its 128x512x512 crop with the
-      vectorial PSF: 1 iteration after 1 warm-up."""
+      vectorial PSF: 3 iterations after 1 warm-up."""
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from oracle import deconv as O
 
@@ -779,8 +810,8 @@ def cpu_baseline_configs(seed=2000):
     A = O.LinearFFTConvolver(psf, shape)
     obs = poisson(A.forward(blob_volume(shape, 200, rng).astype(np.float32)))
     rate, dt = _timed_oracle_iterations(
-        lambda cb: O.richardson_lucy(obs, A, num_iter=6, callback=cb), int(np.prod(shape)), 6)
-    out["rl_64x256x256"] = {"value": rate, "unit": "voxel-iters/s", "sample": f"5 RL iterations, {dt:.1f}s"}
+        lambda cb: O.richardson_lucy(obs, A, num_iter=21, callback=cb), int(np.prod(shape)), 21)
+    out["rl_64x256x256"] = {"value": rate, "unit": "voxel-iters/s", "sample": f"20 RL iterations, {dt:.1f}s"}
     log(f"cpu baseline RL 64x256x256: {rate:.3e} voxel-iters/s")
 
     # binned PDHG-Hessian
@@ -790,10 +821,10 @@ def cpu_baseline_configs(seed=2000):
     clean = Ab.forward(blob_volume(hi, 2000, rng).astype(np.float32))
     obs = poisson(clean * (500.0 / float(clean.max())))
     rate, dt = _timed_oracle_iterations(
-        lambda cb: O.solve_pdhg(obs, psf, alpha=1e-3, num_iter=2, bin_factors=2, tol=0.0, callback=cb),
-        int(np.prod(hi)), 2)
+        lambda cb: O.solve_pdhg(obs, psf, alpha=1e-3, num_iter=4, bin_factors=2, tol=0.0, callback=cb),
+        int(np.prod(hi)), 4)
     out["pdhg_binned_128x512x512"] = {"value": rate, "unit": "voxel-iters/s",
-                                      "sample": f"1 PDHG-Hessian iteration on the object grid, {dt:.1f}s"}
+                                      "sample": f"3 PDHG-Hessian iterations on the object grid, {dt:.1f}s"}
     log(f"cpu baseline binned PDHG: {rate:.3e} voxel-iters/s")
 
     # batched RL, the reference's one-volume-at-a-time loop
@@ -807,13 +838,13 @@ def cpu_baseline_configs(seed=2000):
         p = aberrated_psf(sh, rng)
         Av = O.LinearFFTConvolver(p, sh)
         ob = poisson(Av.forward(blob_volume(sh, max(1, round(200 * np.prod(sh) / 4194304)), rng).astype(np.float32)))
-        r, d = _timed_oracle_iterations(lambda cb: O.richardson_lucy(ob, Av, num_iter=4, callback=cb),
-                                        int(np.prod(sh)), 4)
+        r, d = _timed_oracle_iterations(lambda cb: O.richardson_lucy(ob, Av, num_iter=7, callback=cb),
+                                        int(np.prod(sh)), 7)
         total_vox_it += r * d
         total_dt += d
     rate = total_vox_it / total_dt
     out["batched_rl"] = {"value": rate, "unit": "voxel-iters/s",
-                         "sample": f"3 RL iterations on each of 6 volumes {shapes}, {total_dt:.1f}s"}
+                         "sample": f"6 RL iterations on each of 6 volumes {shapes}, {total_dt:.1f}s"}
     log(f"cpu baseline batched RL: {rate:.3e} voxel-iters/s")
 
     # large-volume solver on its 128x512x512 crop
@@ -823,9 +854,9 @@ def cpu_baseline_configs(seed=2000):
     Ac = O.LinearFFTConvolver(psf, crop)
     obs = poisson(Ac.forward(blob_volume(crop, 3125, rng).astype(np.float32)))
     rate, dt = _timed_oracle_iterations(
-        lambda cb: O.solve_pdhg(obs, psf, alpha=5e-4, num_iter=2, tol=0.0, callback=cb), int(np.prod(crop)), 2)
+        lambda cb: O.solve_pdhg(obs, psf, alpha=5e-4, num_iter=4, tol=0.0, callback=cb), int(np.prod(crop)), 4)
     out["pdhg_large_volume_crop_128x512x512"] = {"value": rate, "unit": "voxel-iters/s",
-                                                 "sample": f"1 PDHG-Hessian iteration, vectorial PSF, {dt:.1f}s"}
+                                                 "sample": f"3 PDHG-Hessian iterations, vectorial PSF, {dt:.1f}s"}
     log(f"cpu baseline large-volume crop: {rate:.3e} voxel-iters/s")
     return out
 
@@ -843,8 +874,8 @@ def main():
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the host-to-host, batching, tiled and PSF synthesis legs (profiled runs)")
-    ap.add_argument("--serving-jobs", type=int, default=40)
-    ap.add_argument("--serving-rate", type=float, default=1000.0)
+    ap.add_argument("--serving-jobs", type=int, default=60)
+    ap.add_argument("--serving-rate", type=float, default=20.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "

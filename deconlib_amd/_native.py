@@ -21,7 +21,7 @@ if not _LIB_PATH.exists():
 
 lib = C.CDLL(str(_LIB_PATH))
 
-ABI_VERSION = 7
+ABI_VERSION = 8
 
 DH_OK, DH_EINVAL, DH_EHIP, DH_EWORKSPACE, DH_EUNSUPPORTED = 0, -1, -2, -3, -4
 
@@ -139,6 +139,8 @@ SYMBOLS = {
     "dh_conv_plan_set_tile": (_int, _vp, _int, _int),
     "dh_conv_plan_get_tile": (_int, _vp, _int),
     "dh_otf_build": (_int, _vp, _vp, _int, _vp, _vp, _i64, C.POINTER(_d), _vp),
+    "dh_otf_build_dev": (_int, _vp, _vp, _int, _vp, _vp, _i64, _vp, _vp),
+    "dh_pd_init_scalars": (_int, _vp, _vp, _d, _vp),
     "dh_conv_apply": (_int, _vp, _int, _vp, _vp, _vp, _vp, _i64, C.POINTER(FuseArgs), _vp),
     "dh_reduce_partials": (_i64, _i64),
     "dh_reduce": (_int, _int, _vp, _i64, _vp, _vp, _vp),
@@ -150,6 +152,8 @@ SYMBOLS = {
     "dh_add_scalar": (_int, _vp, _vp, _vp, _i64, _f, _vp, _vp, _vp),
     "dh_box_copy": (_int, _vp, C.POINTER(_i64), _vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64),
                     C.POINTER(_i64), _vp),
+    "dh_block_sum": (_int, _vp, _vp, _int, _int, _int, _int, _int, _int, _vp),
+    "dh_block_spread": (_int, _vp, _vp, _int, _int, _int, _int, _int, _int, _vp),
     "dh_banded_apply": (_int, _vp, _vp, _i64, _i64, _i64, _i64, _vp, _vp, _int, _vp),
     "dh_pd_partials": (_i64, _int, _int, _int),
     "dh_pd_y1": (_int, _vp, _vp, _vp, _vp, _i64, _vp, _f, _int, _vp),

@@ -150,6 +150,7 @@ class FractionalAreaDownsample:
             )
         self.axes = axes
         self._band_cache: dict = {}
+        self._block_cache: dict = {}
         self._axis_map: dict = {}
         self.in_shape = None
         self.out_shape = None
@@ -203,16 +204,59 @@ class FractionalAreaDownsample:
             cur[a] = ns if forward else nl
         return pairs
 
+    def _block_factors(self, pairs, ndim):
+        """Per-axis integer factors (z, y, x of the volume promoted to 3-D) when every
+        band table of ``pairs`` is a unit block sum -- then one dh_block_sum /
+        dh_block_spread pass equals the banded passes bit for bit -- else None."""
+        key = (ndim, tuple(pairs))
+        if key in self._block_cache:
+            return self._block_cache[key]
+        f = [1, 1, 1]
+        ok = ndim <= 3
+        for a, nl, ns in pairs:
+            if not ok:
+                break
+            if nl % ns:
+                ok = False
+                break
+            fac = nl // ns
+            i_f, w_f, i_a, w_a = _banded_overlap_weights_1d(nl, ns)
+            o = np.arange(ns)[:, None]
+            in_block = (i_f >= fac * o) & (i_f < fac * (o + 1))
+            ok = bool(np.all(np.where(in_block, w_f == 1.0, w_f == 0.0)) and np.all(in_block.sum(1) == fac))
+            j = np.arange(nl)[:, None]
+            parent = i_a == j // fac
+            ok = ok and bool(np.all(np.where(parent, w_a == 1.0, w_a == 0.0)) and np.all(parent.sum(1) == 1))
+            f[(a % ndim) + 3 - ndim] = fac
+        res = tuple(f) if ok else None
+        self._block_cache[key] = res
+        return res
+
     def forward(self, x) -> torch.Tensor:
         r = as_f32(x)
-        for a, nl, ns in self.axis_pairs(r.dim(), r.shape, True):
+        pairs = self.axis_pairs(r.dim(), r.shape, True)
+        f = self._block_factors(pairs, r.dim()) if pairs else None
+        if f is not None:
+            n = (1,) * (3 - r.dim()) + tuple(int(v) for v in r.shape)
+            out = empty_f32(tuple(int(v) // int(q) for v, q in zip(r.shape, f[3 - r.dim():])))
+            N.call("dh_block_sum", r.data_ptr(), out.data_ptr(), *n, *f, N.stream_ptr())
+            return out
+        for a, nl, ns in pairs:
             i_f, w_f, _, _ = self._get_band(nl, ns)
             r = _apply_banded(r, i_f, w_f, a)
         return r
 
     def adjoint(self, y) -> torch.Tensor:
         r = as_f32(y)
-        for a, nl, ns in self.axis_pairs(r.dim(), r.shape, False):
+        pairs = self.axis_pairs(r.dim(), r.shape, False)
+        f = self._block_factors(pairs, r.dim()) if pairs else None
+        if f is not None:
+            fine = tuple(int(v) * int(q) for v, q in zip(r.shape, f[3 - r.dim():]))
+            out = empty_f32(fine)
+            N.call("dh_block_spread", r.data_ptr(), out.data_ptr(), *((1,) * (3 - r.dim()) + fine), *f,
+                   N.stream_ptr())
+            return out
+        for a, nl, ns in pairs:
             _, _, i_a, w_a = self._get_band(nl, ns)
             r = _apply_banded(r, i_a, w_a, a)
         return r

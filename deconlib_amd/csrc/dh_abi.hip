@@ -157,6 +157,7 @@ struct dh_conv_plan {
   int64_t n_partials = 0;
   int tile[DH_PASS_COUNT] = {0, 0, 0, 0, 0};
   bool ct = false;  // every axis length has compile-time specialised kernels
+  hipEvent_t bound = nullptr;  // completion of the twiddle upload (tw_host must outlive it)
 };
 
 namespace {
@@ -530,7 +531,10 @@ int dh_conv_plan_bind(dh_conv_plan* p, void* twiddles_dev, void* stream) {
   DH_TRY(hip_check(hipMemcpyAsync(twiddles_dev, p->tw_host.data(), p->tw_host.size() * 8,
                                   hipMemcpyHostToDevice, st),
                    "twiddle upload"));
-  DH_TRY(hip_check(hipStreamSynchronize(st), "twiddle upload sync"));
+  // no stream synchronisation: the host table lives in the plan, and destroying the
+  // plan first waits for this event
+  if (!p->bound) DH_TRY(hip_check(hipEventCreateWithFlags(&p->bound, hipEventDisableTiming), "event create"));
+  DH_TRY(hip_check(hipEventRecord(p->bound, st), "event record"));
   p->tw_dev = (const float2*)twiddles_dev;
   return DH_OK;
 }
@@ -568,6 +572,7 @@ int dh_conv_plan_get_tile(const dh_conv_plan* p, int pass) {
 
 // ---- PSF synthesis (psf/widefield.py:18-73) ----
 struct dh_psf_plan {
+  hipEvent_t bound = nullptr;  // completion of the twiddle upload
   int ny = 0, nx = 0;
   int L = 1, R = 1;  // Y columns / X rows per workgroup
   std::vector<float2> tw_host;
@@ -618,7 +623,8 @@ int dh_psf_plan_bind(dh_psf_plan* p, void* twiddles_dev, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   DH_TRY(hip_check(hipMemcpyAsync(twiddles_dev, p->tw_host.data(), p->tw_host.size() * 8, hipMemcpyHostToDevice, st),
                    "twiddle upload"));
-  DH_TRY(hip_check(hipStreamSynchronize(st), "twiddle upload sync"));
+  if (!p->bound) DH_TRY(hip_check(hipEventCreateWithFlags(&p->bound, hipEventDisableTiming), "event create"));
+  DH_TRY(hip_check(hipEventRecord(p->bound, st), "event record"));
   p->tw_dev = (const float2*)twiddles_dev;
   return DH_OK;
 }
@@ -660,17 +666,25 @@ int dh_psf_synth(const dh_psf_plan* p, int nchan, const void* pupils, const doub
 }
 
 int dh_psf_plan_destroy(dh_psf_plan* p) {
+  if (p && p->bound) {
+    hipEventSynchronize(p->bound);
+    hipEventDestroy(p->bound);
+  }
   delete p;
   return DH_OK;
 }
 
 int dh_conv_plan_destroy(dh_conv_plan* p) {
+  if (p && p->bound) {
+    hipEventSynchronize(p->bound);
+    hipEventDestroy(p->bound);
+  }
   delete p;
   return DH_OK;
 }
 
-int dh_otf_build(const dh_conv_plan* p, const float* kernel_dev, int normalize, void* otf_dev, void* work_dev,
-                 int64_t work_bytes, double* norm_sq_host, void* stream) {
+static int otf_build(const dh_conv_plan* p, const float* kernel_dev, int normalize, void* otf_dev, void* work_dev,
+                     int64_t work_bytes, double* norm_sq_host, double* norm_sq_dev, void* stream) {
   if (!p || !kernel_dev || !otf_dev || !work_dev) return fail(DH_EINVAL, "null argument");
   if (!p->tw_dev) return fail(DH_EINVAL, "plan is not bound (dh_conv_plan_bind)");
   if (work_bytes < p->ws_bytes) return fail(DH_EWORKSPACE, "workspace too small");
@@ -733,15 +747,29 @@ int dh_otf_build(const dh_conv_plan* p, const float* kernel_dev, int normalize, 
                                         ncols, ncols, p->klo[0], p->khi[0], p->P[0], (float)(1.0 / total), 0, st),
                      "otf z pass"));
   }
-  if (norm_sq_host) {
+  if (norm_sq_host || norm_sq_dev) {
     DH_TRY(hip_check(dh::launch_cmax_abs2((const float2*)otf_dev, (int64_t)p->P[0] * ncols, part, scal + 1, st),
                      "otf norm"));
+  }
+  if (norm_sq_dev)
+    DH_TRY(hip_check(dh::launch_scale_double(scal + 1, total * total, norm_sq_dev, st), "otf norm scale"));
+  if (norm_sq_host) {
     double v = 0.0;
     DH_TRY(hip_check(hipMemcpyAsync(&v, scal + 1, sizeof(double), hipMemcpyDeviceToHost, st), "otf norm copy"));
     DH_TRY(hip_check(hipStreamSynchronize(st), "otf norm sync"));
     *norm_sq_host = v * total * total;
   }
   return DH_OK;
+}
+
+int dh_otf_build(const dh_conv_plan* p, const float* kernel_dev, int normalize, void* otf_dev, void* work_dev,
+                 int64_t work_bytes, double* norm_sq_host, void* stream) {
+  return otf_build(p, kernel_dev, normalize, otf_dev, work_dev, work_bytes, norm_sq_host, nullptr, stream);
+}
+
+int dh_otf_build_dev(const dh_conv_plan* p, const float* kernel_dev, int normalize, void* otf_dev, void* work_dev,
+                     int64_t work_bytes, double* norm_sq_dev, void* stream) {
+  return otf_build(p, kernel_dev, normalize, otf_dev, work_dev, work_bytes, nullptr, norm_sq_dev, stream);
 }
 
 int dh_conv_apply(const dh_conv_plan* p, int adjoint, const float* in, float* out, const void* otf_dev,
@@ -874,6 +902,11 @@ int dh_pd_data_sums(const float* a, const float* b, const float* data, int64_t n
   return hip_check(dh::pd_data_sums(a, b, data, n, background, data_loss_out ? 1 : 0, loss_type, scalars,
                                     data_loss_out, partials, (hipStream_t)stream),
                    "pd_data_sums");
+}
+
+int dh_pd_init_scalars(double* scalars, const double* blur_norm_sq_dev, double reg_norm_sq, void* stream) {
+  if (!scalars || !blur_norm_sq_dev) return fail(DH_EINVAL, "null argument");
+  return hip_check(dh::pd_init(scalars, blur_norm_sq_dev, reg_norm_sq, (hipStream_t)stream), "pd_init_scalars");
 }
 
 int dh_pd_scalars(double* scalars, double* hist, int k, const dh_pd_params* params, void* stream) {
@@ -1508,6 +1541,21 @@ int dh_box_copy(const float* src, const int64_t* src_shape, float* dst, const in
   return hip_check(dh::launch_box_copy(src, src_shape, dst, dst_shape, src_start, dst_start, extent,
                                        (hipStream_t)stream),
                    "box_copy");
+}
+
+int dh_block_sum(const float* fine, float* coarse, int n0, int n1, int n2, int f0, int f1, int f2, void* stream) {
+  if (!fine || !coarse) return fail(DH_EINVAL, "null argument");
+  if (n0 < 1 || n1 < 1 || n2 < 1 || f0 < 1 || f1 < 1 || f2 < 1 || n0 % f0 || n1 % f1 || n2 % f2)
+    return fail(DH_EINVAL, "block factors must divide the fine extents");
+  return hip_check(dh::launch_block_sum(fine, coarse, n0, n1, n2, f0, f1, f2, (hipStream_t)stream), "block sum");
+}
+
+int dh_block_spread(const float* coarse, float* fine, int n0, int n1, int n2, int f0, int f1, int f2, void* stream) {
+  if (!fine || !coarse) return fail(DH_EINVAL, "null argument");
+  if (n0 < 1 || n1 < 1 || n2 < 1 || f0 < 1 || f1 < 1 || f2 < 1 || n0 % f0 || n1 % f1 || n2 % f2)
+    return fail(DH_EINVAL, "block factors must divide the fine extents");
+  return hip_check(dh::launch_block_spread(coarse, fine, n0, n1, n2, f0, f1, f2, (hipStream_t)stream),
+                   "block spread");
 }
 
 int dh_banded_apply(const float* src, float* dst, int64_t outer, int64_t n_in, int64_t inner, int64_t n_out,

@@ -3,6 +3,8 @@
 // Reductions are two-stage: a fixed grid writes float64 partials, one
 // workgroup sums them in a fixed order, so a result never depends on timing.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include <math.h>
 #include <stdint.h>
 
@@ -169,6 +171,15 @@ __global__ void k_add_scalar(const float* in, const float* obs, float* out, int6
   }
 }
 
+__global__ void k_scale_double(const double* in, double s, double* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = in[0] * s;
+}
+
+hipError_t launch_scale_double(const double* in, double s, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_scale_double, dim3(1), dim3(64), 0, st, in, s, out);
+  return hipGetLastError();
+}
+
 // sum of obs log(max(obs, eps) / max(model, eps)) - (obs - max(model, eps))  (rl_mlx.py:75-79)
 __global__ void k_poisson_idiv(const float* obs, const float* model, int64_t n, float eps, double* partials) {
   __shared__ double sh[16];
@@ -289,6 +300,68 @@ __global__ void k_banded(const float* __restrict__ src, float* __restrict__ dst,
     }
     dst[i] = acc;
   }
+}
+
+// Integer-factor block sum (FractionalAreaDownsample with scales that divide the
+// extents, core_operators.py:205-219 applied along z, then y, then x): out[o0, o1, o2]
+// = sum_c sum_b sum_a in[f0 o0 + a, f1 o1 + b, f2 o2 + c], each sum sequential from 0 in
+// the order the three banded passes accumulate, so the value is the passes' bit for bit
+// (their zero-weight taps add +0).  One thread per output sample, rows of the output
+// along the grid's y dimension.
+__global__ void k_block_sum(const float* __restrict__ in, float* __restrict__ out, int64_t rows, int n1, int n2,
+                            int m1, int m2, int f0, int f1, int f2) {
+  const int o2 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o2 >= m2) return;
+  for (int64_t r = blockIdx.y; r < rows; r += gridDim.y) {
+    const int64_t o0 = r / m1;
+    const int o1 = (int)(r - o0 * m1);
+    float r3 = 0.0f;
+    for (int c = 0; c < f2; ++c) {
+      float r2 = 0.0f;
+      for (int b = 0; b < f1; ++b) {
+        const float* p = in + ((o0 * f0) * n1 + (int64_t)o1 * f1 + b) * n2 + (int64_t)o2 * f2 + c;
+        float r1 = 0.0f;
+        for (int a = 0; a < f0; ++a) r1 += p[(int64_t)a * n1 * n2];
+        r2 += r1;
+      }
+      r3 += r2;
+    }
+    __builtin_nontemporal_store(r3, out + r * m2 + o2);
+  }
+}
+
+// Its adjoint (the banded adjoint passes with one unit tap each): in[z, y, x] of the fine
+// grid = coarse[z / f0, y / f1, x / f2].
+__global__ void k_block_spread(const float* __restrict__ coarse, float* __restrict__ fine, int64_t rows, int n1,
+                               int n2, int m1, int m2, int f0, int f1, int f2) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n2) return;
+  for (int64_t r = blockIdx.y; r < rows; r += gridDim.y) {  // fine rows (z, y)
+    const int64_t z = r / n1;
+    const int y = (int)(r - z * n1);
+    const float v = coarse[((z / f0) * m1 + y / f1) * (int64_t)m2 + x / f2];
+    __builtin_nontemporal_store(v, fine + r * n2 + x);
+  }
+}
+
+hipError_t launch_block_sum(const float* in, float* out, int n0, int n1, int n2, int f0, int f1, int f2,
+                            hipStream_t st) {
+  const int m1 = n1 / f1, m2 = n2 / f2;
+  const int64_t rows = (int64_t)(n0 / f0) * m1;
+  const unsigned gy = (unsigned)std::min<int64_t>(rows, 65535);
+  hipLaunchKernelGGL(k_block_sum, dim3((unsigned)((m2 + 127) / 128), gy), dim3(128), 0, st, in, out, rows, n1, n2,
+                     m1, m2, f0, f1, f2);
+  return hipGetLastError();
+}
+
+hipError_t launch_block_spread(const float* coarse, float* fine, int n0, int n1, int n2, int f0, int f1, int f2,
+                               hipStream_t st) {
+  const int m1 = n1 / f1, m2 = n2 / f2;
+  const int64_t rows = (int64_t)n0 * n1;
+  const unsigned gy = (unsigned)std::min<int64_t>(rows, 65535);
+  hipLaunchKernelGGL(k_block_spread, dim3((unsigned)((n2 + 255) / 256), gy), dim3(256), 0, st, coarse, fine, rows,
+                     n1, n2, m1, m2, f0, f1, f2);
+  return hipGetLastError();
 }
 
 hipError_t launch_banded(const float* src, float* dst, int64_t outer, int64_t n_in, int64_t inner, int64_t n_out,

@@ -736,6 +736,20 @@ hipError_t pd_move(double* sc, int from0, int to0, int from1, double* out1, hipS
   return hipGetLastError();
 }
 
+// tau = sigma = step = 0.99 / sqrt(|A|^2 + |L|^2), theta = 1, every other slot 0 (pdhg_mlx.py:693-701)
+__global__ void k_pd_init(double* sc, const double* blur_norm_sq, double reg_norm_sq) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int i = 0; i < DH_PD_NSLOTS; ++i) sc[i] = 0.0;
+  const double s = 0.99 / sqrt(blur_norm_sq[0] + reg_norm_sq);
+  sc[PD_TAU] = sc[PD_SIGMA] = sc[PD_STEP] = s;
+  sc[PD_THETA] = 1.0;
+}
+
+hipError_t pd_init(double* sc, const double* blur_norm_sq, double reg_norm_sq, hipStream_t st) {
+  hipLaunchKernelGGL(k_pd_init, dim3(1), dim3(64), 0, st, sc, blur_norm_sq, reg_norm_sq);
+  return hipGetLastError();
+}
+
 hipError_t pd_scalars(double* sc, double* hist, int k, const PdParams& pp, hipStream_t st) {
   hipLaunchKernelGGL(k_pd_scalars, dim3(1), dim3(64), 0, st, sc, hist, k, pp);
   return hipGetLastError();

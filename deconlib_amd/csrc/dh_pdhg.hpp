@@ -38,6 +38,7 @@ hipError_t pd_hess_sums(const float* a, const float* b, int nz, int ny, int nx, 
 hipError_t pd_data_sums(const float* a, const float* b, const float* data, int64_t n, float bg, int want_loss,
                         int loss_type, double* sc, double* loss_out, double* part, hipStream_t st);
 hipError_t pd_move(double* sc, int from0, int to0, int from1, double* out1, hipStream_t st);
+hipError_t pd_init(double* sc, const double* blur_norm_sq, double reg_norm_sq, hipStream_t st);
 hipError_t pd_scalars(double* sc, double* hist, int k, const PdParams& pp, hipStream_t st);
 int64_t pd_partials_needed(int nz, int ny, int nx);
 // gradient (DH_REG_GRADIENT, weights w[3]) and identity (DH_REG_IDENTITY) regularizers

@@ -303,14 +303,22 @@ class _ConvPlan:
         self.otf_buf = torch.empty(int(info.otf_bytes), dtype=torch.uint8, device=dev)
         self.work = torch.empty(int(info.work_bytes), dtype=torch.uint8, device=dev)
         N.call("dh_conv_plan_bind", h, self.twiddles.data_ptr(), N.stream_ptr())
-        nsq = C.c_double()
-        N.call("dh_otf_build", h, kernel.data_ptr(), int(normalize), self.otf_buf.data_ptr(),
-               self.work.data_ptr(), int(info.work_bytes), C.byref(nsq), N.stream_ptr())
-        self.norm_sq = float(nsq.value)
+        # max|OTF|^2 stays on the device until someone asks for it (no synchronisation here)
+        self.norm_dev = torch.empty(1, dtype=torch.float64, device=dev)
+        N.call("dh_otf_build_dev", h, kernel.data_ptr(), int(normalize), self.otf_buf.data_ptr(),
+               self.work.data_ptr(), int(info.work_bytes), self.norm_dev.data_ptr(), N.stream_ptr())
+        self._norm_sq = None
         self.canvas_shape = tuple(int(v) for v in info.canvas_shape)[3 - nd:]
         # bumped by everything that writes the workspace, so a caller that left a
         # spectrum there (dh_rl_iteration keep_spectrum) can tell whether it survives
         self.gen = 0
+
+    @property
+    def norm_sq(self) -> float:
+        """max|OTF|^2 (read from the device on first use)."""
+        if self._norm_sq is None:
+            self._norm_sq = float(self.norm_dev.item())
+        return self._norm_sq
 
     def apply(self, x: Optional[torch.Tensor], adjoint: bool, out: torch.Tensor,
               fuse: Optional[N.FuseArgs] = None) -> torch.Tensor:
@@ -354,7 +362,11 @@ class FFTConvolver:
         self.out_shape = self.in_shape
         self.axes = tuple(range(-len(self.shape), 0))
         self._plan = _ConvPlan(k, self.shape, self.shape, True, normalize)
-        self.operator_norm_sq = self._plan.norm_sq
+
+    @property
+    def operator_norm_sq(self) -> float:
+        """max|OTF|^2 (linops_mlx.py:415), read from the device on first use."""
+        return self._plan.norm_sq
 
     @property
     def otf(self) -> torch.Tensor:
@@ -410,7 +422,11 @@ class LinearFFTConvolver:
         k = as_f32(kernel)
         self._plan = _ConvPlan(k, self.signal_shape, self.fft_shape, False, normalize)
         self.canvas_shape = self._plan.canvas_shape
-        self.operator_norm_sq = self._plan.norm_sq
+
+    @property
+    def operator_norm_sq(self) -> float:
+        """max|OTF|^2 (linops_mlx.py:415-518), read from the device on first use."""
+        return self._plan.norm_sq
 
     @property
     def otf(self) -> torch.Tensor:

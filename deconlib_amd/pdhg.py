@@ -210,16 +210,27 @@ class _PDState:
         nc = int(N.fns["dh_reg_components"](self.kind))
         self.y2 = torch.zeros((nc,) + self.D3, dtype=torch.float32, device=dev)
         self.g = empty_f32(self.S)
-        self.step_scale = 0.99 / math.sqrt(blur_op.operator_norm_sq + reg.operator_norm_sq)
-        sc = np.zeros(N.PD_NSLOTS, np.float64)
-        sc[_TAU] = sc[_SIGMA] = sc[_STEP] = self.step_scale
-        sc[_THETA] = 1.0
-        self.sc = torch.from_numpy(sc).to(dev)
+        if self._plan is not None:
+            # the step sizes start from the OTF norm on the device: no host round trip
+            self.sc = torch.empty(N.PD_NSLOTS, dtype=torch.float64, device=dev)
+            N.call("dh_pd_init_scalars", self.sc.data_ptr(), self._plan.norm_dev.data_ptr(),
+                   float(reg.operator_norm_sq), N.stream_ptr())
+        else:
+            step = 0.99 / math.sqrt(blur_op.operator_norm_sq + reg.operator_norm_sq)
+            sc = np.zeros(N.PD_NSLOTS, np.float64)
+            sc[_TAU] = sc[_SIGMA] = sc[_STEP] = step
+            sc[_THETA] = 1.0
+            self.sc = torch.from_numpy(sc).to(dev)
         self.hist = torch.zeros(3 * max(self.num_iter, 1), dtype=torch.float64, device=dev)
         self.loss = torch.zeros(2 * (self.num_iter + 1), dtype=torch.float64, device=dev)
         if self.data.numel() > int(np.prod(self.S)):
             raise ValueError("the data domain must not be larger than the reconstruction domain")
         self.part = torch.empty(int(N.fns["dh_pd_partials"](*self.D3)), dtype=torch.float64, device=dev)
+
+    @property
+    def step_scale(self) -> float:
+        """0.99 / sqrt(|A|^2 + |L|^2) (pdhg_mlx.py:693-701)."""
+        return float(self.sc[_STEP].item())
 
     def _fwd(self, x, out):
         if self._plan is not None:
@@ -233,7 +244,9 @@ class _PDState:
         out.copy_(as_f32(self.op.adjoint(y)))
         return out
 
-    def step(self, k: int, want_loss: bool) -> None:
+    # one iteration in five phases, so that a batch of solves can group its convolutions
+    def _pre(self, k: int) -> None:
+        """y1 and y2 dual steps (pdhg_mlx.py:717-734)."""
         st = N.stream_ptr()
         nz, ny, nx = self.D3
         sc = self.sc.data_ptr()
@@ -241,10 +254,18 @@ class _PDState:
                self.data.numel(), sc, self.bg, self.lt, st)
         N.call("dh_pd_reg_dual", self.kind, self.y2.data_ptr(), self.x.data_ptr(), self.xp.data_ptr(), nz, ny, nx,
                sc, self.wp, self.alpha, self.l12, st)
-        self._adj(self.y1, self.g)
+
+    def _mid(self, k: int) -> None:
+        """Primal step with the regularizer's adjoint (pdhg_mlx.py:741-750)."""
+        nz, ny, nx = self.D3
         N.call("dh_pd_reg_primal", self.kind, self.xp.data_ptr(), self.x.data_ptr(), self.g.data_ptr(),
-               self.y2.data_ptr(), nz, ny, nx, sc, self.wp, self.part.data_ptr(), st)
-        self._fwd(self.xp, self.Axp)
+               self.y2.data_ptr(), nz, ny, nx, self.sc.data_ptr(), self.wp, self.part.data_ptr(), N.stream_ptr())
+
+    def _post(self, k: int, want_loss: bool) -> None:
+        """Step-size sums, the Malitsky-Pock update and the swap (pdhg_mlx.py:752-860)."""
+        st = N.stream_ptr()
+        nz, ny, nx = self.D3
+        sc = self.sc.data_ptr()
         dl = self.loss[2 * k:].data_ptr() if want_loss else None
         rl = self.loss[2 * k + 1:].data_ptr() if want_loss else None
         N.call("dh_pd_data_sums", self.Axp.data_ptr(), self.Ax.data_ptr(), self.data.data_ptr(),
@@ -255,12 +276,55 @@ class _PDState:
         self.x, self.xp = self.xp, self.x
         self.Ax, self.Axp = self.Axp, self.Ax
 
+    def _conv_item(self, adjoint: bool) -> N.BatchItem:
+        p = self._plan
+        src, dst = (self.y1, self.g) if adjoint else (self.xp, self.Axp)
+        return N.BatchItem(plan=p._h, otf=p.otf_buf.data_ptr(), work=p.work.data_ptr(),
+                           work_bytes=int(p.info.work_bytes), input=src.data_ptr(), out=dst.data_ptr(),
+                           fuse=N.FuseArgs())
+
+    def step(self, k: int, want_loss: bool) -> None:
+        self._pre(k)
+        self._adj(self.y1, self.g)
+        self._mid(k)
+        self._fwd(self.xp, self.Axp)
+        self._post(k, want_loss)
+
     def done(self) -> bool:
         return bool(self.sc[_DONE].item() != 0.0)
 
     def loss_value(self, k: int) -> float:
         v = self.loss[2 * k: 2 * k + 2].cpu().numpy()
         return float(v[0] + self.alpha * v[1])
+
+
+def pd_batch_step(states: List[_PDState], ks: List[int], want_loss=None, max_items: int = 256) -> None:
+    """One iteration of several PDHG solves (any shapes): the same kernels as
+    ``_PDState.step`` on each, with the adjoint and forward convolutions of the
+    plan-backed solves grouped per transform length (``dh_batch_conv_apply``), so
+    each solve's result equals its solo run bit for bit."""
+    want_loss = want_loss or [False] * len(states)
+    planned = [s for s in states if s._plan is not None]
+    for s, k in zip(states, ks):
+        s._pre(k)
+    for adjoint in (True, False):
+        for j in range(0, len(planned), max_items):
+            chunk = planned[j:j + max_items]
+            arr = (N.BatchItem * len(chunk))(*[s._conv_item(adjoint) for s in chunk])
+            N.call("dh_batch_conv_apply", len(chunk), arr, int(adjoint), N.stream_ptr())
+            for s in chunk:
+                s._plan.gen += 1
+        for s in states:
+            if s._plan is None:
+                if adjoint:
+                    s._adj(s.y1, s.g)
+                else:
+                    s._fwd(s.xp, s.Axp)
+        if adjoint:
+            for s, k in zip(states, ks):
+                s._mid(k)
+    for s, k, w in zip(states, ks, want_loss):
+        s._post(k, w)
 
 
 def _run(state: _PDState, num_iter, min_iter, patience, tol, eval_interval, callback, verbose,

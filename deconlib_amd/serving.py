@@ -6,31 +6,38 @@ HBM budget, builds its operator and state on the device (OTF, workspace,
 iterate, duals, sensitivity: all resident until the job finishes), and then
 advances every live job by one iteration per tick:
 
-* all live RL jobs together through ``BatchRunner`` (two batched convolution
-  calls per tick, volumes grouped by transform length);
-* each live PDHG job through its own device-resident iteration (its scalars
-  never leave the GPU).
+* all live RL jobs together through ``BatchRunner`` (the fused iteration of
+  every job, volumes grouped per transform length);
+* all live PDHG jobs together through ``pd_batch_step`` (their convolutions
+  grouped per transform length, their scalars never leaving the GPU).
 
-A job's latency is its completion time minus its arrival time (queueing,
-set-up and iterations included).  The result of a job does not depend on what
-else is in flight: the batched launches compute each volume exactly as a solo
-launch does.
+Nothing in the loop waits for the GPU except a bound on how far the host may
+run ahead (``max_ahead`` ticks enqueued): admission builds operators with
+asynchronous uploads and the OTF norm kept on the device, and a finished job
+is noticed by polling the event recorded after its last kernel, so host-side
+admission of new arrivals overlaps the device work of the jobs in flight.
+A job's latency is its completion time on the device (that event) minus its
+arrival time: queueing, set-up and iterations included.  The result of a job
+does not depend on what else is in flight: the grouped launches compute each
+volume exactly as a solo launch does.
 """
 
 from __future__ import annotations
 
 import math
 import time
-from dataclasses import dataclass, field
+from collections import deque
+from dataclasses import dataclass
 from typing import Callable, List, Optional
 
 import numpy as np
 import torch
 
 from . import _native as N
+from ._tensor import as_f32
 from .batch import BatchRunner, RLJob
 from .linops import LinearFFTConvolver
-from .pdhg import _PDState, _nonneg
+from .pdhg import _PDState, _nonneg, pd_batch_step
 
 
 @dataclass
@@ -38,8 +45,8 @@ class Request:
     job_id: int
     arrival: float                 # seconds after the server starts
     kind: str                      # "rl" or "pdhg"
-    observed: np.ndarray           # host float32 (z, y, x)
-    psf: np.ndarray                # host float32, same shape, corner origin
+    observed: object               # float32 (z, y, x): host array or device tensor
+    psf: object                    # float32, same shape, corner origin: host array or device tensor
     num_iter: int = 100
     alpha: float = 1e-3
     background: float = 0.0
@@ -61,11 +68,11 @@ class Completed:
         return self.finished - self.arrival
 
 
-def estimate_bytes(req: Request) -> int:
-    """Device bytes a request holds while live (plan buffers plus solver state)."""
+def plan_info(shape) -> N.PlanInfo:
+    """Sizes of the plan a job of this shape (PSF of the same shape) gets; host only."""
     import ctypes as C
 
-    shape = tuple(int(s) for s in req.observed.shape)
+    shape = tuple(int(s) for s in shape)
     nd = len(shape)
     h = C.c_void_p()
     arr = (C.c_int64 * nd)(*shape)
@@ -73,6 +80,13 @@ def estimate_bytes(req: Request) -> int:
     info = N.PlanInfo()
     N.call("dh_conv_plan_info", h, C.byref(info))
     N.fns["dh_conv_plan_destroy"](h)
+    return info
+
+
+def estimate_bytes(req: Request) -> int:
+    """Device bytes a request holds while live (plan buffers plus solver state)."""
+    shape = tuple(int(s) for s in req.observed.shape)
+    info = plan_info(shape)
     vox = int(np.prod(shape))
     # RL: obs, x, sens, ratio, pred; PDHG: data, x, xp, y1, y2 (6), Ax, Axp, g
     state = (5 if req.kind == "rl" else 14) * 4 * vox
@@ -83,15 +97,15 @@ class _Live:
     def __init__(self, req: Request, admitted: float):
         self.req = req
         self.admitted = admitted
-        op = LinearFFTConvolver(req.psf, req.observed.shape)
-        obs = torch.from_numpy(np.ascontiguousarray(req.observed, dtype=np.float32)).cuda()
+        obs = as_f32(req.observed)
+        op = LinearFFTConvolver(req.psf, tuple(obs.shape))
         if req.kind == "rl":
             self.job = RLJob(obs, op, num_iter=req.num_iter, background=req.background)
             self.pd = None
         else:
             self.job = None
             self.pd = _PDState(obs, op, _nonneg(obs), req.alpha, req.num_iter, req.background, tol=0.0)
-            self.k = 0
+        self.k = 0
         self.op = op
 
     @property
@@ -108,11 +122,13 @@ class DeconvServer:
     """Admits requests against an HBM budget and runs them with continuous batching."""
 
     def __init__(self, hbm_budget_bytes: Optional[int] = None, reserve_bytes: int = 8 << 30,
-                 keep_results: bool = False):
+                 keep_results: bool = False, max_ahead: int = 2):
         total = torch.cuda.get_device_properties(torch.cuda.current_device()).total_memory
         self.budget = int(hbm_budget_bytes if hbm_budget_bytes is not None else total - reserve_bytes)
         self.keep_results = keep_results
+        self.max_ahead = max(1, int(max_ahead))
         self.runner = BatchRunner(256)
+        self.ticks = 0
 
     def run(self, requests: List[Request], clock: Callable[[], float] = time.perf_counter) -> List[Completed]:
         queue = sorted(requests, key=lambda r: r.arrival)
@@ -121,11 +137,33 @@ class DeconvServer:
             if need[r.job_id] > self.budget:
                 raise ValueError(f"request {r.job_id} needs {need[r.job_id]} B, more than the HBM budget")
         live: List[_Live] = []
+        draining = []  # (live job, completion event, result)
+        ticks = deque()
         used = 0
         done: List[Completed] = []
+        # the start event is complete (its timestamp taken) before the host clock starts, so
+        # device completion times and host times share one origin
+        start = torch.cuda.Event(enable_timing=True)
+        start.record()
+        start.synchronize()
         t0 = clock()
         qi = 0
-        while qi < len(queue) or live:
+
+        def poll(block: bool = False) -> None:
+            nonlocal used, draining
+            keep = []
+            for l, ev, out in draining:
+                if (block and ev is draining[0][1]) or ev.query():
+                    ev.synchronize()
+                    t = start.elapsed_time(ev) / 1e3
+                    done.append(Completed(l.req.job_id, l.req.kind, tuple(l.req.observed.shape), l.req.num_iter,
+                                          l.req.arrival, l.admitted, t, out if self.keep_results else None))
+                    used -= need[l.req.job_id]
+                else:
+                    keep.append((l, ev, out))
+            draining = keep
+
+        while qi < len(queue) or live or draining:
             now = clock() - t0
             # admission: in arrival order while memory allows
             while qi < len(queue) and queue[qi].arrival <= now and used + need[queue[qi].job_id] <= self.budget:
@@ -133,30 +171,40 @@ class DeconvServer:
                 live.append(_Live(req, clock() - t0))
                 used += need[req.job_id]
                 qi += 1
-            if not live:
-                if qi < len(queue):
-                    time.sleep(max(0.0, min(queue[qi].arrival - (clock() - t0), 0.01)))
-                continue
-            # one iteration of every live job
-            rl = [l.job for l in live if l.job is not None and not l.job.done]
-            if rl:
-                self.runner.step(rl)
-            for l in live:
-                if l.pd is not None and l.k < l.req.num_iter:
-                    l.pd.step(l.k, False)
+            if live:
+                rl = [l.job for l in live if l.job is not None]
+                if rl:
+                    self.runner.step(rl)
+                pd = [l for l in live if l.pd is not None]
+                if pd:
+                    pd_batch_step([l.pd for l in pd], [l.k for l in pd])
+                for l in live:
                     l.k += 1
-            finished = [l for l in live if l.done]
-            if finished:
-                outs = [l.restored() for l in finished]
-                torch.cuda.current_stream().synchronize()
-                t = clock() - t0
-                for l, x in zip(finished, outs):
-                    done.append(Completed(l.req.job_id, l.req.kind, tuple(l.req.observed.shape), l.req.num_iter,
-                                          l.req.arrival, l.admitted, t, x if self.keep_results else None))
-                    used -= need[l.req.job_id]
+                for l in live:
+                    if l.done:
+                        out = l.restored()
+                        ev = torch.cuda.Event(enable_timing=True)
+                        ev.record()
+                        draining.append((l, ev, out))
                 live = [l for l in live if not l.done]
+                self.ticks += 1
+                tick = torch.cuda.Event()
+                tick.record()
+                ticks.append(tick)
+                while len(ticks) > self.max_ahead:
+                    ticks.popleft().synchronize()
+                poll()
+            elif draining:
+                # nothing to enqueue: wait for the oldest job in flight (or an arrival)
+                if qi < len(queue) and queue[qi].arrival <= clock() - t0 and used + need[queue[qi].job_id] > self.budget:
+                    poll(block=True)
+                else:
+                    poll()
+                    if draining:
+                        time.sleep(2e-4)
             else:
-                torch.cuda.current_stream().synchronize()
+                time.sleep(max(0.0, min(queue[qi].arrival - (clock() - t0), 0.01)))
+        torch.cuda.current_stream().synchronize()
         return done
 
 

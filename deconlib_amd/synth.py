@@ -110,6 +110,89 @@ def serving_requests(n_jobs: int, rate_per_s: float, seed: int, z_range=(16, 48)
     return out
 
 
+def blob_volume_device(shape, n_blobs: int, rng: np.random.Generator, background: float = 10.0,
+                       sigma=(1.0, 3.0), peak=(100.0, 1000.0)):
+    """``blob_volume`` rendered into a float32 device tensor (same parameter law and
+    draw order; separable Gaussians on +-4 sigma boxes, summed in float32)."""
+    import torch
+
+    shape = tuple(int(s) for s in shape)
+    vol = torch.full(shape, float(background), dtype=torch.float32, device="cuda")
+    centres = rng.uniform(0, 1, size=(n_blobs, 3)) * np.array(shape)
+    sig = rng.uniform(sigma[0], sigma[1], size=n_blobs)
+    amp = rng.uniform(peak[0], peak[1], size=n_blobs)
+    for c, s, a in zip(centres, sig, amp):
+        r = int(np.ceil(4 * s))
+        lo = [max(0, int(np.floor(c[d])) - r) for d in range(3)]
+        hi = [min(shape[d], int(np.floor(c[d])) + r + 1) for d in range(3)]
+        if any(h <= l for l, h in zip(lo, hi)):
+            continue
+        g = [np.exp(-0.5 * ((np.arange(lo[d], hi[d]) - c[d]) / s) ** 2) for d in range(3)]
+        box = (a * g[0][:, None, None] * g[1][None, :, None] * g[2][None, None, :]).astype(np.float32)
+        vol[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] += torch.from_numpy(box).to("cuda", non_blocking=False)
+    return vol
+
+
+def _synth_side(n: int) -> int:
+    """Largest length of the form 2^a * {1, 3, 5} (<= 4096) not above n: the sides the
+    GPU PSF synthesis computes."""
+    best = 1
+    for base in (1, 3, 5):
+        v = base
+        while v * 2 <= min(n, 4096):
+            v *= 2
+        if v <= n and v > best:
+            best = v
+    return best
+
+
+def serving_requests_device(n_jobs: int, rate_per_s: float, seed: int, z_range=(32, 128), xy_range=(128, 1024),
+                            rl_fraction=0.7, rl_iters=100, pdhg_iters=200, alpha_range=(1e-4, 1e-2),
+                            density=200 / (64 * 256 * 256), shapes=None, kinds=None):
+    """The serving configuration's request stream with inputs resident on the GPU.
+
+    Exponential inter-arrival times (rate ``rate_per_s``); z and y = x drawn
+    independently log-uniform over ``z_range`` / ``xy_range`` (any integer size);
+    70 % RL (``rl_iters``) / 30 % PDHG-Hessian (``pdhg_iters``, alpha log-uniform over
+    ``alpha_range``); each job with its own aberrated PSF (synthesised on the GPU, on the
+    largest 2^a * {1, 3, 5} grid not above the volume per axis: a linear convolution
+    takes any kernel extent up to the volume's),
+    a blob object at the config-0 density rendered on the GPU and a Poisson
+    observation of its blurred image drawn on the GPU.  ``shapes`` / ``kinds``
+    (lists of n_jobs) replace the drawn sizes / job kinds.
+    """
+    import torch
+
+    from .linops import LinearFFTConvolver
+    from .serving import Request
+
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for j in range(n_jobs):
+        t += float(rng.exponential(1.0 / rate_per_s))
+        z = int(round(np.exp(rng.uniform(np.log(z_range[0]), np.log(z_range[1])))))
+        xy = int(round(np.exp(rng.uniform(np.log(xy_range[0]), np.log(xy_range[1])))))
+        shape = tuple(shapes[j]) if shapes is not None else (z, xy, xy)
+        psf = aberrated_psf(tuple(_synth_side(n) for n in shape), rng, device=True)
+        obj = blob_volume_device(shape, max(1, int(round(density * np.prod(shape)))), rng)
+        clean = LinearFFTConvolver(psf, shape).forward(obj).clamp_min_(0.0)
+        del obj
+        gen = torch.Generator(device="cuda")
+        gen.manual_seed(int(rng.integers(1 << 31)))
+        obs = torch.poisson(clean, generator=gen)
+        del clean
+        is_rl = rng.uniform() < rl_fraction
+        if kinds is not None:
+            is_rl = kinds[j] == "rl"
+        if is_rl:
+            out.append(Request(j, t, "rl", obs, psf, num_iter=rl_iters))
+        else:
+            alpha = float(np.exp(rng.uniform(np.log(alpha_range[0]), np.log(alpha_range[1]))))
+            out.append(Request(j, t, "pdhg", obs, psf, num_iter=pdhg_iters, alpha=alpha))
+    return out
+
+
 def host_linear_blur(psf: np.ndarray, obj: np.ndarray) -> np.ndarray:
     """Wrap-free convolution of obj with a corner-origin psf (float64, numpy FFT)."""
     s = obj.shape

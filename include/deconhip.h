@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define DH_ABI_VERSION 7
+#define DH_ABI_VERSION 8
 
 #define DH_OK 0
 #define DH_EINVAL (-1)
@@ -89,7 +89,8 @@ const char* dh_last_error(void);
 int dh_conv_plan_create(int ndim, const int64_t* signal_shape, const int64_t* kernel_shape,
                         const int64_t* fft_shape, int circular, dh_conv_plan** out);
 int dh_conv_plan_info(const dh_conv_plan* plan, dh_plan_info* info);
-/* Upload the twiddle tables into twiddles_dev (twiddle_bytes). */
+/* Upload the twiddle tables into twiddles_dev (twiddle_bytes), asynchronously on the stream
+ * (dh_conv_plan_destroy waits for the upload). */
 int dh_conv_plan_bind(dh_conv_plan* plan, void* twiddles_dev, void* stream);
 int dh_conv_plan_destroy(dh_conv_plan* plan);
 /* Launch shape of one pass (DH_PASS_*): columns per workgroup for the Y/Z
@@ -109,6 +110,13 @@ int dh_conv_plan_get_tile(const dh_conv_plan* plan, int pass);
 int dh_otf_build(const dh_conv_plan* plan, const float* kernel_dev, int normalize,
                  void* otf_dev, void* work_dev, int64_t work_bytes,
                  double* norm_sq_host, void* stream);
+
+/* dh_otf_build without a host read: max|OTF|^2 goes to the device double *norm_sq_dev
+ * (may be NULL) and nothing synchronises, so operators can be built while other work runs
+ * (serving admission); the caller reads the value when it needs it. */
+int dh_otf_build_dev(const dh_conv_plan* plan, const float* kernel_dev, int normalize,
+                     void* otf_dev, void* work_dev, int64_t work_bytes,
+                     double* norm_sq_dev, void* stream);
 
 /* One forward (adjoint == 0: convolution, linops_mlx.py:520-524) or adjoint
  * (adjoint == 1: correlation, linops_mlx.py:526-530) application, S -> S.
@@ -290,6 +298,9 @@ int dh_pd_hess_sums(const float* a, const float* b, int nz, int ny, int nx, cons
 /* DA2 = sum (a - b)^2; if data_loss_out, the data loss of a + background against data. */
 int dh_pd_data_sums(const float* a, const float* b, const float* data, int64_t n, float background,
                     int loss_type, double* scalars, double* data_loss_out, double* partials, void* stream);
+/* Initial scalar block (pdhg_mlx.py:693-701): tau = sigma = step = 0.99 / sqrt(*blur_norm_sq_dev
+ * + reg_norm_sq), theta = 1, every other slot 0; the blur norm stays on the device. */
+int dh_pd_init_scalars(double* scalars, const double* blur_norm_sq_dev, double reg_norm_sq, void* stream);
 /* Malitsky-Pock step update and convergence test for iteration k in float64
  * (pdhg_mlx.py:761-835); hist[3k..3k+2] = tau_new, sigma_new, rel_change. */
 int dh_pd_scalars(double* scalars, double* hist, int k, const dh_pd_params* params, void* stream);
@@ -552,6 +563,13 @@ int dh_timing_read(double* ms, int64_t* counts, int reset);
  * (zero-pad) and Pad.adjoint / Crop.forward (slice) are both this call. */
 int dh_box_copy(const float* src, const int64_t* src_shape, float* dst, const int64_t* dst_shape,
                 const int64_t* src_start, const int64_t* dst_start, const int64_t* extent, void* stream);
+/* FractionalAreaDownsample with integer factors f that divide the fine extents n (the
+ * band tables are then unit block sums): dh_block_sum is its forward, coarse (n/f) =
+ * sum over each f0 x f1 x f2 block, accumulated along z, then y, then x in the order the
+ * three banded passes below use (bit-identical to them); dh_block_spread its adjoint,
+ * fine[z, y, x] = coarse[z / f0, y / f1, x / f2].  One pass each instead of three. */
+int dh_block_sum(const float* fine, float* coarse, int n0, int n1, int n2, int f0, int f1, int f2, void* stream);
+int dh_block_spread(const float* coarse, float* fine, int n0, int n1, int n2, int f0, int f1, int f2, void* stream);
 /* Banded resampling along one axis (core_operators.py:205-219): the volume is
  * viewed as (outer, n_in, inner) -> (outer, n_out, inner) with
  * out[o, i, q] = sum_k wt[i, k] * in[o, idx[i, k], q], k < width.  idx (int32)

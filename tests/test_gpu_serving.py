@@ -49,3 +49,29 @@ def test_serving_budget_limits_concurrency(gpu):
     assert order == [r.job_id for r in sorted(reqs, key=lambda r: r.arrival)]
     with pytest.raises(ValueError):
         DeconvServer(hbm_budget_bytes=budget // 100).run(reqs)
+
+
+def test_serving_config_sizes_equal_solo(gpu):
+    """Jobs at the serving configuration's sizes (32x128x128 .. 128x1024x1024, arbitrary
+    integer extents, per-job aberrated PSFs, inputs made on the GPU), RL and PDHG mixed,
+    admitted while others run: every result equals its solo run bit for bit."""
+    from deconlib_amd import LinearFFTConvolver, richardson_lucy_with_operator, solve_pdhg_mlx
+    from deconlib_amd.serving import DeconvServer
+    from deconlib_amd.synth import serving_requests_device
+
+    shapes = [(128, 1024, 1024), (37, 141, 141), (90, 700, 700), (32, 128, 128), (61, 333, 333)]
+    kinds = ["rl", "pdhg", "rl", "pdhg", "rl"]
+    print(f"serving at configuration sizes: {shapes}", flush=True)
+    reqs = serving_requests_device(5, 50.0, 93, shapes=shapes, kinds=kinds, rl_iters=4, pdhg_iters=5)
+    done = DeconvServer(keep_results=True).run(reqs)
+    by_id = {c.job_id: c for c in done}
+    assert sorted(by_id) == [r.job_id for r in reqs]
+    for r in reqs:
+        c = by_id[r.job_id]
+        assert c.shape == tuple(r.observed.shape) and c.finished >= c.arrival >= 0
+        if r.kind == "rl":
+            solo = richardson_lucy_with_operator(r.observed, LinearFFTConvolver(r.psf, tuple(r.observed.shape)),
+                                                 num_iter=r.num_iter).restored
+        else:
+            solo = solve_pdhg_mlx(r.observed, r.psf, alpha=r.alpha, num_iter=r.num_iter, tol=0.0).restored
+        assert torch.equal(c.restored, solo), (r.job_id, r.kind, c.shape)
