@@ -339,5 +339,7 @@ def tiles_richardson_lucy(tiles, model, num_iter: int = 50, background: float = 
     for s in range(0, len(tiles), group):
         jobs = [TileRLJob(t, model, num_iter, background, init_value, eval_interval) for t in tiles[s:s + group]]
         batched_richardson_lucy(jobs)
-        out.extend(j.state.x[model.valid_slices] for j in jobs)
+        # compact copies: the group's padded states are freed before the next group runs
+        out.extend(j.state.x[model.valid_slices].clone() for j in jobs)
+        del jobs
     return out

@@ -331,16 +331,24 @@ __global__ void k_block_sum(const float* __restrict__ in, float* __restrict__ ou
 }
 
 // Its adjoint (the banded adjoint passes with one unit tap each): in[z, y, x] of the fine
-// grid = coarse[z / f0, y / f1, x / f2].
+// grid = coarse[z / f0, y / f1, x / f2].  A thread writes four consecutive samples (one
+// 16-byte store) of a fine row; rows are dealt out grid-stride.
 __global__ void k_block_spread(const float* __restrict__ coarse, float* __restrict__ fine, int64_t rows, int n1,
                                int n2, int m1, int m2, int f0, int f1, int f2) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int x = 4 * (blockIdx.x * blockDim.x + threadIdx.x);
   if (x >= n2) return;
   for (int64_t r = blockIdx.y; r < rows; r += gridDim.y) {  // fine rows (z, y)
     const int64_t z = r / n1;
     const int y = (int)(r - z * n1);
-    const float v = coarse[((z / f0) * m1 + y / f1) * (int64_t)m2 + x / f2];
-    __builtin_nontemporal_store(v, fine + r * n2 + x);
+    const float* c = coarse + ((z / f0) * m1 + y / f1) * (int64_t)m2;
+    float* d = fine + r * n2 + x;
+    if (x + 3 < n2 && (n2 & 3) == 0) {
+      typedef float f4v __attribute__((ext_vector_type(4)));
+      const f4v v = {c[x / f2], c[(x + 1) / f2], c[(x + 2) / f2], c[(x + 3) / f2]};
+      __builtin_nontemporal_store(v, reinterpret_cast<f4v*>(d));
+    } else {
+      for (int j = 0; j < 4 && x + j < n2; ++j) __builtin_nontemporal_store(c[(x + j) / f2], d + j);
+    }
   }
 }
 
@@ -358,9 +366,12 @@ hipError_t launch_block_spread(const float* coarse, float* fine, int n0, int n1,
                                hipStream_t st) {
   const int m1 = n1 / f1, m2 = n2 / f2;
   const int64_t rows = (int64_t)n0 * n1;
-  const unsigned gy = (unsigned)std::min<int64_t>(rows, 65535);
-  hipLaunchKernelGGL(k_block_spread, dim3((unsigned)((n2 + 255) / 256), gy), dim3(256), 0, st, coarse, fine, rows,
-                     n1, n2, m1, m2, f0, f1, f2);
+  const int quads = (n2 + 3) / 4;
+  const unsigned gx = (unsigned)((quads + 127) / 128);
+  // about 8 rows per workgroup row slot: enough workgroups to fill the GPU, few enough to amortise them
+  const unsigned gy = (unsigned)std::max<int64_t>(1, std::min<int64_t>((rows + 7) / 8, 65535));
+  hipLaunchKernelGGL(k_block_spread, dim3(gx, gy), dim3(128), 0, st, coarse, fine, rows, n1, n2, m1, m2, f0, f1,
+                     f2);
   return hipGetLastError();
 }
 

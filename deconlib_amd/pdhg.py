@@ -233,16 +233,24 @@ class _PDState:
         return float(self.sc[_STEP].item())
 
     def _fwd(self, x, out):
+        """A x into ``out`` for a plan-backed operator; any other operator's own output
+        is returned as is (callers keep the returned tensor, no copy)."""
         if self._plan is not None:
             return self._plan.apply(x, False, out)
-        out.copy_(as_f32(self.op.forward(x)))
-        return out
+        return self._own(as_f32(self.op.forward(x)), x)
 
     def _adj(self, y, out):
         if self._plan is not None:
             return self._plan.apply(y, True, out)
-        out.copy_(as_f32(self.op.adjoint(y)))
-        return out
+        return self._own(as_f32(self.op.adjoint(y)), y)
+
+    def _own(self, r, arg):
+        """``r`` unless it shares memory with the argument or with a buffer the state
+        keeps (an operator that returns a persistent buffer): then a copy."""
+        held = [arg] + [getattr(self, n) for n in ("x", "xp", "Ax", "Axp", "g", "y1") if hasattr(self, n)]
+        if any(r.data_ptr() == h.data_ptr() for h in held):
+            return r.clone()
+        return r
 
     # one iteration in five phases, so that a batch of solves can group its convolutions
     def _pre(self, k: int) -> None:
@@ -285,9 +293,9 @@ class _PDState:
 
     def step(self, k: int, want_loss: bool) -> None:
         self._pre(k)
-        self._adj(self.y1, self.g)
+        self.g = self._adj(self.y1, self.g)
         self._mid(k)
-        self._fwd(self.xp, self.Axp)
+        self.Axp = self._fwd(self.xp, self.Axp)
         self._post(k, want_loss)
 
     def done(self) -> bool:
@@ -317,9 +325,9 @@ def pd_batch_step(states: List[_PDState], ks: List[int], want_loss=None, max_ite
         for s in states:
             if s._plan is None:
                 if adjoint:
-                    s._adj(s.y1, s.g)
+                    s.g = s._adj(s.y1, s.g)
                 else:
-                    s._fwd(s.xp, s.Axp)
+                    s.Axp = s._fwd(s.xp, s.Axp)
         if adjoint:
             for s, k in zip(states, ks):
                 s._mid(k)

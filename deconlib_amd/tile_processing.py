@@ -152,6 +152,8 @@ def process_tiles(data, psf, zoom: Union[float, Tuple[float, ...]], solve: Solve
             return _process_resident(data, plan, model, lambda tiles, m: [solve(tl, m) for tl in tiles], 1,
                                      verbose)
     output = np.zeros(plan.visible_shape, dtype=np.float32)
+    if batched is not None and group > 1:
+        group = _host_group(n, group, _tile_bytes(model), _free_device_bytes(), on_tile_done is not None)
 
     i = 0
     while i < n:
@@ -172,6 +174,37 @@ def process_tiles(data, psf, zoom: Union[float, Tuple[float, ...]], solve: Solve
             if on_tile_done is not None and on_tile_done(spec, output):
                 return output
     return output
+
+
+def _host_group(n_tiles: int, requested: int, tile_bytes: int, free_bytes: int, has_callback: bool) -> int:
+    """Tiles solved together on the host path (the volume does not fit the device):
+    one at a time when a per-tile callback may stop early (so no later tile is
+    solved in vain), else as many as the request allows within half of the free
+    device memory (each tile's solver state is resident while its group runs)."""
+    if has_callback:
+        return 1
+    fit = max(1, (free_bytes // 2) // max(int(tile_bytes), 1))
+    return max(1, min(int(requested), int(n_tiles), int(fit)))
+
+
+def _tile_bytes(model) -> int:
+    """Device bytes one tile's solve holds (padded-domain state and plan workspace)."""
+    from .batch import _tile_job_bytes, fused_tiles_ok
+
+    if fused_tiles_ok(model):
+        return _tile_job_bytes(model)
+    pad = 1
+    for v in model.padded_shape:
+        pad *= int(v)
+    return 8 * 4 * pad
+
+
+def _free_device_bytes() -> int:
+    import torch
+
+    if not torch.cuda.is_available():
+        return 0
+    return int(torch.cuda.mem_get_info()[0])
 
 
 def _fits_device(data, plan) -> bool:

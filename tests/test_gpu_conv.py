@@ -37,6 +37,8 @@ SHAPES = [
     ((2, 3, 2048), (2, 3, 2049)),      # x canvas 4096: 2048-point rows, twiddles read from global memory
     ((1, 2, 4096), (1, 1, 4097)),      # x canvas 8192: 4096-point rows
     ((2, 2048, 4), (1, 2049, 3)),      # y canvas 4096: blocks split over 2-column tiles (cached, XCD-major)
+    ((2, 2048, 40), (1, 2049, 9)),     # y canvas 4096, x canvas 48 (H = 25): 2-column tiles over three full
+                                       # 8-column blocks and a partial one
 ]
 
 

@@ -166,3 +166,16 @@ def test_oracle_tiled_rl_matches_full_interior():
     tiled = OT.process_tiles(data, psf, 1.0, solve, guard=6, tile_size=24)
     scale = float(np.max(full))
     np.testing.assert_allclose(tiled / scale, full / scale, atol=0.05)
+
+
+def test_host_path_group_sizes():
+    """Tiles solved together when the volume does not fit the device: bounded by half of
+    the free device memory per tile's footprint, by the request and the tile count, and
+    one at a time whenever a per-tile callback may stop the loop early."""
+    gib = 1 << 30
+    assert TP._host_group(20, 20, 2 * gib, 16 * gib, False) == 4
+    assert TP._host_group(20, 3, 2 * gib, 16 * gib, False) == 3
+    assert TP._host_group(2, 20, 1 * gib, 64 * gib, False) == 2
+    assert TP._host_group(20, 20, 64 * gib, 16 * gib, False) == 1
+    assert TP._host_group(20, 20, 1, 0, False) == 1
+    assert TP._host_group(20, 20, 1 * gib, 64 * gib, True) == 1
