@@ -141,7 +141,7 @@ __global__ void __launch_bounds__(kTileThreads) k_er_hess_apply_w_tiled(const fl
     if (t.in) {
       auto f = [&](int zz, int yy, int xx) { return lds_at(pl, slot_of(zz, t.z0), t, yy, xx); };
       float h[6];
-      hess6(f, d, z, t.y, t.x, w6.w, h);
+      hess6_halo(f, z, t.y, t.x, w6.w, h);
       const int64_t i = ((int64_t)z * d.ny + t.y) * d.nx + t.x;
       const float wv = __builtin_nontemporal_load(w + i);
 #pragma unroll

@@ -174,15 +174,22 @@ __device__ __forceinline__ int col_lds(int e, int ln, int L) { return e * L + ln
 __device__ __forceinline__ void opaque(int& a, int& b) { asm volatile("" : "+v"(a), "+v"(b)); }
 __device__ __forceinline__ void opaque(int& a) { asm volatile("" : "+v"(a)); }
 
-// Addressing of one lane's column in one tile (logical block lb of a volume).
+// Addressing of one lane's column in one tile (logical block lb of a volume):
+// the tile's plane bases sb/db are workgroup-uniform (scalar registers), the
+// lane's column offset and row stride are in bytes of type Off.  With Off =
+// uint32_t (every offset inside one plane below 2^32 bytes, col_narrow) an
+// access is a scalar base plus one 32-bit vector offset; uint64_t serves
+// planes larger than that.
+template <class Off>
 struct ColLane {
-  const float2* sb;
-  float2* db;
-  int64_t srs, drs;
+  const char* sb;
+  char* db;
+  Off so, dof, sr, dr;
   bool ok;
 };
 
-__device__ __forceinline__ ColLane col_lane(const ColArgs& a, int64_t lb, int L, int ln) {
+template <class Off>
+__device__ __forceinline__ ColLane<Off> col_lane(const ColArgs& a, int64_t lb, int L, int ln) {
   const int64_t tile = lb % a.ntiles, batch = lb / a.ntiles;
   const int64_t col = tile * L + ln;
   int64_t soff = col, doff = col, srs = a.src_rs, drs = a.dst_rs;
@@ -193,18 +200,41 @@ __device__ __forceinline__ ColLane col_lane(const ColArgs& a, int64_t lb, int L,
     doff = b * kBlk * a.dst_rs + j;
     srs = drs = w;
   }
-  return ColLane{a.src + batch * a.src_b + soff, a.dst + batch * a.dst_b + doff, srs, drs, col < a.ncols};
+  return ColLane<Off>{reinterpret_cast<const char*>(a.src + batch * a.src_b),
+                      reinterpret_cast<char*>(a.dst + batch * a.dst_b),
+                      (Off)(soff * 8),
+                      (Off)(doff * 8),
+                      (Off)(srs * 8),
+                      (Off)(drs * 8),
+                      col < a.ncols};
+}
+
+// True when every element offset of a volume's plane fits 32 bits in bytes:
+// blocked planes span ncols x rows elements, plain planes rows x row stride.
+__host__ __device__ inline bool col_narrow(const ColArgs& a, int N) {
+  constexpr int64_t kLim = int64_t(1) << 31;
+  auto span = [&](int64_t rs) {
+    return a.blk > 0 ? (a.ncols + kBlk) * (rs + N) : a.ncols + (int64_t)(N + 1) * rs;
+  };
+  return 8 * span(a.src_rs) < kLim && 8 * span(a.dst_rs) < kLim;
 }
 
 // Line element e of the zero-padded canvas: rows [0, lo) and [N-hi, N) hold
 // the source rows 0..lo+hi-1, the rest is zero.
-template <int N, bool NT = true>
-__device__ __forceinline__ float2 col_ld(const ColArgs& a, const ColLane& c, int e) {
+template <int N, bool NT = true, class Off>
+__device__ __forceinline__ float2 col_ld(const ColArgs& a, const ColLane<Off>& c, int e) {
   int r = -1;
   if (e < a.lo) r = e;
   else if (e >= N - a.hi) r = e - (N - a.lo - a.hi);
   if (!c.ok || r < 0) return make_float2(0.f, 0.f);
-  return ld_s<NT>(c.sb + (int64_t)r * c.srs);
+  return ld_s<NT>(reinterpret_cast<const float2*>(c.sb + (c.so + (Off)r * c.sr)));
+}
+
+// col_ld for hi == 0 (no wrapped rows: the Y and Z passes of an application).
+template <bool NT = true, class Off>
+__device__ __forceinline__ float2 col_ld_lo(int lo, const ColLane<Off>& c, int e) {
+  if (!c.ok || e >= lo) return make_float2(0.f, 0.f);
+  return ld_s<NT>(reinterpret_cast<const float2*>(c.sb + (c.so + (Off)e * c.sr)));
 }
 
 // Persistent column pass: each workgroup walks tiles b = blockIdx.x,
@@ -220,9 +250,8 @@ __device__ __forceinline__ int64_t xcd_major(int64_t b, int64_t G) {
   return x * q + (x < r ? x : r) + b / kXcd;
 }
 
-template <int N, int DIR, bool NT = true>
-__global__ void __launch_bounds__(kMaxThreads) k_col(Pack<ColArgs> pk, int L) {
-  extern __shared__ float2 lds[];
+template <int N, int DIR, bool NT, class Off>
+__device__ __forceinline__ void col_run(const Pack<ColArgs>& pk, int L, int64_t b, float2* lds) {
   constexpr int T = Geo<N>::T;
   constexpr Sched S = Geo<N>::F;
   constexpr int R0 = S.np > 0 ? S.r[0] : 1;
@@ -233,15 +262,14 @@ __global__ void __launch_bounds__(kMaxThreads) k_col(Pack<ColArgs> pk, int L) {
   const int ln0 = thr % G + G * (thr / (G * T));
   const int t0 = (thr / G) % T;
   const int64_t total = pk.prefix[pk.nvol];
-  int64_t b = NT ? (int64_t)blockIdx.x : xcd_major(blockIdx.x, gridDim.x);
-  if (b >= total) return;
   if constexpr (N == 1) {  // length-1 transform: scaled copy
     for (; b < total; b += gridDim.x) {
       const int v = pick(pk, b);
       const ColArgs& a = pk.a[v];
-      const ColLane c = col_lane(a, b - pk.prefix[v], L, ln0);
+      const ColLane<Off> c = col_lane<Off>(a, b - pk.prefix[v], L, ln0);
       const float2 x = col_ld<N, NT>(a, c, 0);
-      if (c.ok && a.n_out > 0) c.db[0] = make_float2(x.x * a.scale, x.y * a.scale);
+      if (c.ok && a.n_out > 0)
+        *reinterpret_cast<float2*>(c.db + c.dof) = make_float2(x.x * a.scale, x.y * a.scale);
     }
     return;
   }
@@ -251,8 +279,13 @@ __global__ void __launch_bounds__(kMaxThreads) k_col(Pack<ColArgs> pk, int L) {
     opaque(ln, t);
     const int v = pick(pk, bb);
     const ColArgs& a = pk.a[v];
-    const ColLane c = col_lane(a, bb - pk.prefix[v], L, ln);
-    pass_load<N, T, R0>(u, t, [&](int e) { return col_ld<N, NT>(a, c, e); });
+    const ColLane<Off> c = col_lane<Off>(a, bb - pk.prefix[v], L, ln);
+    if (a.hi == 0) {
+      const int lo = a.lo;
+      pass_load<N, T, R0>(u, t, [&](int e) { return col_ld_lo<NT>(lo, c, e); });
+    } else {
+      pass_load<N, T, R0>(u, t, [&](int e) { return col_ld<N, NT>(a, c, e); });
+    }
   };
   fetch(b);
   for (;;) {
@@ -262,9 +295,12 @@ __global__ void __launch_bounds__(kMaxThreads) k_col(Pack<ColArgs> pk, int L) {
     auto lst = [&](int e, float2 v) { lds[col_lds(e, ln, L)] = v; };
     const int v = pick(pk, b);
     const ColArgs& a = pk.a[v];
-    const ColLane c = col_lane(a, b - pk.prefix[v], L, ln);
+    const ColLane<Off> c = col_lane<Off>(a, b - pk.prefix[v], L, ln);
+    const int n_out = a.n_out;
+    const float scale = a.scale;
     auto gst = [&](int e, float2 val) {
-      if (c.ok && e < a.n_out) st_s<NT>(c.db + (int64_t)e * c.drs, make_float2(val.x * a.scale, val.y * a.scale));
+      if (c.ok && e < n_out)
+        st_s<NT>(reinterpret_cast<float2*>(c.db + (c.dof + (Off)e * c.dr)), make_float2(val.x * scale, val.y * scale));
     };
     const int64_t bn = b + gridDim.x;
     if constexpr (S.np == 1) {
@@ -280,6 +316,18 @@ __global__ void __launch_bounds__(kMaxThreads) k_col(Pack<ColArgs> pk, int L) {
     b = bn;
     __syncthreads();  // the next tile's first pass overwrites the LDS lines
   }
+}
+
+// WIDE: 64-bit lane offsets, for volumes that are not col_narrow (the launcher
+// picks the instantiation; a separate kernel, so that the 32-bit one keeps its
+// register count).
+template <int N, int DIR, bool NT = true, bool WIDE = false>
+__global__ void __launch_bounds__(kMaxThreads) k_col(Pack<ColArgs> pk, int L) {
+  extern __shared__ float2 lds[];
+  const int64_t total = pk.prefix[pk.nvol];
+  const int64_t b = NT ? (int64_t)blockIdx.x : xcd_major(blockIdx.x, gridDim.x);
+  if (b >= total) return;
+  col_run<N, DIR, NT, std::conditional_t<WIDE, uint64_t, uint32_t>>(pk, L, b, lds);
 }
 
 // ---- fused Z pass: forward FFT, x OTF, inverse FFT, in place ----
@@ -1077,16 +1125,27 @@ struct Launch {
   static unsigned grid(const void* k, int threads, size_t shm, int64_t nblocks, bool persist) {
     return persist ? persistent_grid(k, threads, shm, nblocks) : (unsigned)nblocks;
   }
-  template <int DIR, bool NT>
+  template <int DIR, bool NT, bool WIDE = false>
   static void col_as(const Pack<ColArgs>& pk, int L, int64_t nblocks, bool persist, size_t shm, hipStream_t st) {
-    const void* k = (const void*)k_col<N, DIR, NT>;
+    const void* k = (const void*)k_col<N, DIR, NT, WIDE>;
     (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    hipLaunchKernelGGL((k_col<N, DIR, NT>), dim3(grid(k, L * T, shm, nblocks, persist)), dim3(L * T), shm, st, pk, L);
+    hipLaunchKernelGGL((k_col<N, DIR, NT, WIDE>), dim3(grid(k, L * T, shm, nblocks, persist)), dim3(L * T), shm, st,
+                       pk, L);
   }
-  // blocked columns in tiles narrower than a block: ordinary accesses (ld_s/st_s)
+  // blocked columns in tiles narrower than a block: ordinary accesses (ld_s/st_s).
+  // Planes too large for 32-bit offsets (the OTF build's Z transform of a very large
+  // canvas: a forward transform of plain columns) take the 64-bit kernel; other
+  // shapes that large are not supported.
   static hipError_t col(int dir, const Pack<ColArgs>& pk, int L, int64_t nblocks, bool persist, hipStream_t st) {
     const size_t shm = (size_t)col_lds_elems(N, L) * sizeof(float2);
     const bool split = pk.a[0].blk > 0 && L < kBlk;
+    bool narrow = true;
+    for (int i = 0; i < pk.nvol; ++i) narrow = narrow && col_narrow(pk.a[i], N);
+    if (!narrow) {
+      if (dir >= 0 || split) return hipErrorInvalidValue;
+      col_as<-1, true, true>(pk, L, nblocks, persist, shm, st);
+      return hipGetLastError();
+    }
     if (dir < 0) {
       if (split) col_as<-1, false>(pk, L, nblocks, persist, shm, st);
       else col_as<-1, true>(pk, L, nblocks, persist, shm, st);

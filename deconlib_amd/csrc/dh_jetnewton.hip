@@ -160,7 +160,7 @@ __global__ void __launch_bounds__(kTileThreads) k_jn_hvp_operand_tiled(const flo
     if (t.in) {
       auto f = [&](int zz, int yy, int xx) { return lds_at(pl, slot_of(zz, t.z0), t, yy, xx); };
       float q[6], rr[6];
-      hess6(f, d, z, t.y, t.x, w6.w, q);
+      hess6_halo(f, z, t.y, t.x, w6.w, q);
       const int64_t i = ((int64_t)z * d.ny + t.y) * d.nx + t.x;
 #pragma unroll
       for (int k = 0; k < 6; ++k) rr[k] = __builtin_nontemporal_load(r + k * n + i);

@@ -166,7 +166,7 @@ __global__ void __launch_bounds__(kTileThreads) k_pd_dual(float* __restrict__ y2
     if (t.in) {
       auto f = [&](int zz, int yy, int xx) { return lds_at(pl, slot_of(zz, t.z0), t, yy, xx); };
       float h[6];
-      hess6(f, d, z, t.y, t.x, w.w, h);
+      hess6_halo(f, z, t.y, t.x, w.w, h);
       float av[6];
 #pragma unroll
       for (int q = 0; q < 6; ++q) av[q] = yc[q] + sf * h[q];
@@ -208,13 +208,23 @@ __global__ void __launch_bounds__(kTileThreads, 6) k_pd_primal(float* __restrict
   const float* wsrc[3] = {y2, y2 + 3 * n, y2 + 4 * n};
   const float* csrc[3] = {y2 + 1 * n, y2 + 2 * n, y2 + 5 * n};
   const float ww[3] = {w.w[0], w.w[3], w.w[4]}, cw[3] = {w.w[1], w.w[2], w.w[5]};
+  // staged weights of the thread's two samples with the reflections hess_adj_halo
+  // expects (exact sign flips): components 3 and 4 carry their y / x reflection, 5
+  // both; the z reflection of 3 and 4 is applied per plane
+  const HaloSign hs(d, t);
+  const float wws[3][2] = {{ww[0], ww[0]}, {ww[1] * hs.y0, ww[1] * hs.y1}, {ww[2] * hs.x0, ww[2] * hs.x1}};
+  const float cws[3][2] = {{cw[0], cw[0]}, {cw[1], cw[1]}, {cw[2] * hs.x0 * hs.y0, cw[2] * hs.x1 * hs.y1}};
   float wv[3][2], cv[3][2];
+  auto wstore = [&](int j, int zz) {
+    const float sz = j == 0 ? 1.0f : zsign(d, zz);
+    pp.store(&win[j][slot_of(zz, t.z0) * kPlane], (wv[j][0] * wws[j][0]) * sz, (wv[j][1] * wws[j][1]) * sz);
+  };
   for (int zz = t.z0 - 1; zz <= t.z0; ++zz) {
     const int64_t bz = pp.base(d, zz);
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       pp.load(wsrc[j], bz, wv[j][0], wv[j][1]);
-      pp.store(&win[j][slot_of(zz, t.z0) * kPlane], wv[j][0] * ww[j], wv[j][1] * ww[j]);
+      wstore(j, zz);
     }
   }
   {
@@ -235,8 +245,8 @@ __global__ void __launch_bounds__(kTileThreads, 6) k_pd_primal(float* __restrict
   for (int z = t.z0; z < t.z1; ++z) {
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      pp.store(&win[j][slot_of(z + 1, t.z0) * kPlane], wv[j][0] * ww[j], wv[j][1] * ww[j]);
-      pp.store(cur[j], cv[j][0] * cw[j], cv[j][1] * cw[j]);
+      wstore(j, z + 1);
+      pp.store(cur[j], cv[j][0] * cws[j][0], cv[j][1] * cws[j][1]);
     }
     __syncthreads();
     if (z + 1 < t.z1) {
@@ -265,7 +275,7 @@ __global__ void __launch_bounds__(kTileThreads, 6) k_pd_primal(float* __restrict
           default: return cur[2][o];
         }
       };
-      const float lt = hess_adj(hw, d, z, t.y, t.x);
+      const float lt = hess_adj_halo(hw, z, t.y, t.x);
       const int64_t i = ((int64_t)z * d.ny + t.y) * d.nx + t.x;
       const float grad = gc + lt;
       const float xv = xc;
@@ -309,13 +319,23 @@ __global__ void __launch_bounds__(kTileThreads, 6) k_hess_adj_tiled(const float*
   const float* wsrc[3] = {y2, y2 + 3 * n, y2 + 4 * n};
   const float* csrc[3] = {y2 + 1 * n, y2 + 2 * n, y2 + 5 * n};
   const float ww[3] = {w.w[0], w.w[3], w.w[4]}, cw[3] = {w.w[1], w.w[2], w.w[5]};
+  // staged weights of the thread's two samples with the reflections hess_adj_halo
+  // expects (exact sign flips): components 3 and 4 carry their y / x reflection, 5
+  // both; the z reflection of 3 and 4 is applied per plane
+  const HaloSign hs(d, t);
+  const float wws[3][2] = {{ww[0], ww[0]}, {ww[1] * hs.y0, ww[1] * hs.y1}, {ww[2] * hs.x0, ww[2] * hs.x1}};
+  const float cws[3][2] = {{cw[0], cw[0]}, {cw[1], cw[1]}, {cw[2] * hs.x0 * hs.y0, cw[2] * hs.x1 * hs.y1}};
   float wv[3][2], cv[3][2];
+  auto wstore = [&](int j, int zz) {
+    const float sz = j == 0 ? 1.0f : zsign(d, zz);
+    pp.store(&win[j][slot_of(zz, t.z0) * kPlane], (wv[j][0] * wws[j][0]) * sz, (wv[j][1] * wws[j][1]) * sz);
+  };
   for (int zz = t.z0 - 1; zz <= t.z0; ++zz) {
     const int64_t bz = pp.base(d, zz);
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       pp.load(wsrc[j], bz, wv[j][0], wv[j][1]);
-      pp.store(&win[j][slot_of(zz, t.z0) * kPlane], wv[j][0] * ww[j], wv[j][1] * ww[j]);
+      wstore(j, zz);
     }
   }
   {
@@ -329,8 +349,8 @@ __global__ void __launch_bounds__(kTileThreads, 6) k_hess_adj_tiled(const float*
   for (int z = t.z0; z < t.z1; ++z) {
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      pp.store(&win[j][slot_of(z + 1, t.z0) * kPlane], wv[j][0] * ww[j], wv[j][1] * ww[j]);
-      pp.store(cur[j], cv[j][0] * cw[j], cv[j][1] * cw[j]);
+      wstore(j, z + 1);
+      pp.store(cur[j], cv[j][0] * cws[j][0], cv[j][1] * cws[j][1]);
     }
     __syncthreads();
     if (z + 1 < t.z1) {
@@ -353,7 +373,7 @@ __global__ void __launch_bounds__(kTileThreads, 6) k_hess_adj_tiled(const float*
           default: return cur[2][o];
         }
       };
-      stnt(out + ((int64_t)z * d.ny + t.y) * d.nx + t.x, hess_adj(hw, d, z, t.y, t.x));
+      stnt(out + ((int64_t)z * d.ny + t.y) * d.nx + t.x, hess_adj_halo(hw, z, t.y, t.x));
     }
     __syncthreads();
   }
@@ -392,12 +412,12 @@ __global__ void __launch_bounds__(kTileThreads) k_pd_hess_sums(const float* __re
     if (t.in) {
       auto fd = [&](int zz, int yy, int xx) { return lds_at(pdl, slot_of(zz, t.z0), t, yy, xx); };
       float h[6];
-      hess6(fd, d, z, t.y, t.x, w.w, h);
+      hess6_halo(fd, z, t.y, t.x, w.w, h);
 #pragma unroll
       for (int q = 0; q < 6; ++q) s_sq += (double)(h[q] * h[q]);
       if (want_abs) {
         auto fa = [&](int zz, int yy, int xx) { return lds_at(pa, slot_of(zz, t.z0), t, yy, xx); };
-        hess6(fa, d, z, t.y, t.x, w.w, h);
+        hess6_halo(fa, z, t.y, t.x, w.w, h);
         float ss = 0.0f;
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
@@ -440,13 +460,12 @@ __global__ void k_pd_data_sums(const float* __restrict__ a, const float* __restr
                                double* __restrict__ part) {
   __shared__ double sh[16];
   double s_sq = 0.0, s_loss = 0.0;
-  PD_LOOP(i, n) {
-    const float av = ldnt(a + i);
-    const float dd = av - ldnt(b + i);
+  auto one = [&](float av, float bv, const float* dp) {
+    const float dd = av - bv;
     s_sq += (double)(dd * dd);
     if (want_loss) {
       const float fm = av + bg;
-      const float dv = ldnt(data + i);
+      const float dv = *dp;
       if (loss_type == 0) {
         const float fms = fmaxf(fm, 1e-6f);
         s_loss += (double)(dv * logf((dv + 1e-6f) / fms) - (dv - fm));
@@ -455,7 +474,25 @@ __global__ void k_pd_data_sums(const float* __restrict__ a, const float* __restr
         s_loss += 0.5 * (double)(r * r);
       }
     }
+  };
+  // 16-byte loads of a and b where both are 16-byte aligned (four samples per
+  // thread and iteration in flight), then the remainder one sample at a time
+  int64_t done = 0;
+  if ((((uintptr_t)a | (uintptr_t)b) & 15) == 0) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const int64_t n4 = n / 4;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n4; q += (int64_t)gridDim.x * blockDim.x) {
+      const f4 av = __builtin_nontemporal_load(reinterpret_cast<const f4*>(a) + q);
+      const f4 bv = __builtin_nontemporal_load(reinterpret_cast<const f4*>(b) + q);
+      one(av.x, bv.x, data + 4 * q);
+      one(av.y, bv.y, data + 4 * q + 1);
+      one(av.z, bv.z, data + 4 * q + 2);
+      one(av.w, bv.w, data + 4 * q + 3);
+    }
+    done = 4 * n4;
   }
+  for (int64_t i = done + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    one(ldnt(a + i), ldnt(b + i), data + i);
   double r0 = pd_block_sum(s_sq, sh);
   double r1 = pd_block_sum(s_loss, sh);
   if (threadIdx.x == 0) {

@@ -38,6 +38,30 @@ __device__ __forceinline__ void hess6(const F& f, const Dims& d, int z, int y, i
   for (int q = 0; q < 6; ++q) h[q] *= w[q];
 }
 
+// hess6 on an accessor whose neighbours one step outside the volume already hold the
+// replicated boundary values (an LDS window filled from clamped coordinates, as
+// the tiled kernels below stage it): the same reads in the same order, so the
+// same values, without the index clamps.
+template <class F>
+__device__ __forceinline__ void hess6_halo(const F& f, int z, int y, int x, const float* w, float* h) {
+  const int zm = z - 1, zp = z + 1, ym = y - 1, yp = y + 1, xm = x - 1, xp = x + 1;
+  const float c = f(z, y, x);
+  h[0] = (f(zm, y, x) - 2.0f * c) + f(zp, y, x);
+  h[1] = (f(z, ym, x) - 2.0f * c) + f(z, yp, x);
+  h[2] = (f(z, y, xm) - 2.0f * c) + f(z, y, xp);
+  const float dz_yp = (f(zp, yp, x) - f(zm, yp, x)) / 2.0f;
+  const float dz_ym = (f(zp, ym, x) - f(zm, ym, x)) / 2.0f;
+  h[3] = (dz_yp - dz_ym) / 2.0f;
+  const float dz_xp = (f(zp, y, xp) - f(zm, y, xp)) / 2.0f;
+  const float dz_xm = (f(zp, y, xm) - f(zm, y, xm)) / 2.0f;
+  h[4] = (dz_xp - dz_xm) / 2.0f;
+  const float dy_xp = (f(z, yp, xp) - f(z, ym, xp)) / 2.0f;
+  const float dy_xm = (f(z, yp, xm) - f(z, ym, xm)) / 2.0f;
+  h[5] = (dy_xp - dy_xm) / 2.0f;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) h[q] *= w[q];
+}
+
 // d1_cen_adj of g along one axis at index i (n samples):
 //   fpad = [-g0, g..., -g_{n-1}];  out[i] = (fpad[i] - fpad[i + 2]) / 2
 template <class G>
@@ -65,6 +89,26 @@ __device__ __forceinline__ float hess_adj(const HW& hw, const Dims& d, int z, in
   const float a4 = d1adj([&](int iz) { return d1adj([&](int ix) { return hw(4, iz, y, ix); }, x, d.nx); }, z, d.nz);
   // d1_cen_adj(d1_cen_adj(Hw5, x), y)
   const float a5 = d1adj([&](int iy) { return d1adj([&](int ix) { return hw(5, z, iy, ix); }, x, d.nx); }, y, d.ny);
+  return ((((a0 + a1) + a2) + a3) + a4) + a5;
+}
+
+// Hessian3D.adjoint at (z, y, x) on an accessor whose neighbours one step outside
+// the volume are staged so that plain differences reproduce hess_adj's boundary
+// rules: the d2 components (0, 1, 2) hold the replicated boundary value there, the
+// mixed components (3: y and z, 4: x and z, 5: x and y) the boundary value negated
+// once for each of their two axes crossed (d1_cen_adj's fpad = [-g0, g, -g_{n-1}]).
+// Same values, same order as hess_adj, without the boundary branches.
+template <class HW>
+__device__ __forceinline__ float hess_adj_halo(const HW& hw, int z, int y, int x) {
+  const float a0 = (hw(0, z - 1, y, x) - 2.0f * hw(0, z, y, x)) + hw(0, z + 1, y, x);
+  const float a1 = (hw(1, z, y - 1, x) - 2.0f * hw(1, z, y, x)) + hw(1, z, y + 1, x);
+  const float a2 = (hw(2, z, y, x - 1) - 2.0f * hw(2, z, y, x)) + hw(2, z, y, x + 1);
+  auto dy3 = [&](int iz) { return (hw(3, iz, y - 1, x) - hw(3, iz, y + 1, x)) / 2.0f; };
+  const float a3 = (dy3(z - 1) - dy3(z + 1)) / 2.0f;
+  auto dx4 = [&](int iz) { return (hw(4, iz, y, x - 1) - hw(4, iz, y, x + 1)) / 2.0f; };
+  const float a4 = (dx4(z - 1) - dx4(z + 1)) / 2.0f;
+  auto dx5 = [&](int iy) { return (hw(5, z, iy, x - 1) - hw(5, z, iy, x + 1)) / 2.0f; };
+  const float a5 = (dx5(y - 1) - dx5(y + 1)) / 2.0f;
   return ((((a0 + a1) + a2) + a3) + a4) + a5;
 }
 
@@ -157,6 +201,24 @@ struct PlanePos {
     if (has1) dst[threadIdx.x + kTileThreads] = v1;
   }
 };
+
+// Reflection signs of a thread's two staged samples (PlanePos order) for
+// hess_adj_halo: -1 where the sample's y (x) lies outside the volume, else +1;
+// zsign gives the same for a staged plane z.
+struct HaloSign {
+  float y0, x0, y1, x1;
+  __device__ HaloSign(const Dims& d, const Tile& t) {
+    auto sg = [](int v, int n) { return (v < 0 || v >= n) ? -1.0f : 1.0f; };
+    const int i0 = threadIdx.x, i1 = threadIdx.x + kTileThreads;
+    const int ly0 = i0 / kPW, lx0 = i0 - ly0 * kPW;
+    const int ly1 = i1 / kPW, lx1 = i1 - ly1 * kPW;
+    y0 = sg(t.y0 + ly0 - 1, d.ny);
+    x0 = sg(t.x0 + lx0 - 1, d.nx);
+    y1 = sg(t.y0 + ly1 - 1, d.ny);
+    x1 = sg(t.x0 + lx1 - 1, d.nx);
+  }
+};
+__device__ __forceinline__ float zsign(const Dims& d, int z) { return (z < 0 || z >= d.nz) ? -1.0f : 1.0f; }
 
 __device__ __forceinline__ float lds_at(const float* planes, int slot, const Tile& t, int yy, int xx) {
   return planes[slot * kPlane + (yy - t.y0 + 1) * kPW + (xx - t.x0 + 1)];
