@@ -725,14 +725,55 @@ struct XiEpt {
 // Loads (q) or places into LDS (!load) this thread's spectrum samples of the
 // tile whose rows start at row0: visit i of the blocked walk, or sample
 // k = t + i*T of the thread's own row for the plain layout.
-template <int M>
+// STEP: the stepped blocked walk (fewer instructions per load; the kernels whose
+// registers allow it use it, see k_xi).
+template <int M, bool STEP>
 __device__ __forceinline__ void xi_tile(const XiArgs& a, int64_t row0, int RPB,
                                         float2 (&q)[(M + kBlk + Geo<M>::T - 1) / Geo<M>::T], float2* lds, bool load,
                                         int thr) {
   constexpr int T = Geo<M>::T;
   constexpr int RS = xrow_stride(M);
   const int H = a.H;
-  if (a.blk > 0) {
+  if (STEP && T % kBlk == 0 && a.blk > 0) {
+    // A workgroup of RPB x T threads: visit i of thread thr is column j = thr % kBlk
+    // of row r = (thr / kBlk) % RPB in block bk0 + i * T / kBlk (the same visits as
+    // the general walk below), so the row's plane and offsets are formed once.
+    const int lg = __builtin_ctz(RPB);
+    const int nbk = (H + kBlk - 1) / kBlk;
+    const int j = thr & (kBlk - 1);
+    const int r = (thr >> 3) & (RPB - 1);
+    const int bk0 = thr >> (lg + 3);
+    constexpr int BS = T / kBlk;
+    if (!load) {
+      float2* lr = lds + r * RS;
+#pragma unroll
+      for (int i = 0; i < XiEpt<M>::value; ++i) {
+        const int bk = bk0 + i * BS, k = bk * kBlk + j;
+        if (bk < nbk && k < H) lr[xrow_lds(k)] = q[i];
+      }
+      return;
+    }
+    const bool rv = row0 + r < a.nrows;
+    int64_t z = row0 / a.prow;
+    int y = (int)(row0 - z * a.prow) + r;
+    if (y >= a.prow) {
+      z += y / a.prow;
+      y %= a.prow;
+    }
+    // the row's plane; offsets inside one plane (H x prow samples) fit 32 bits
+    const float2* zb = a.src + z * a.prow * (int64_t)H;
+    const int rowk = kBlk * a.prow;
+#pragma unroll
+    for (int i = 0; i < XiEpt<M>::value; ++i) {
+      const int bk = bk0 + i * BS, k = bk * kBlk + j;
+      float2 val = make_float2(0.f, 0.f);
+      if (rv && bk < nbk && k < H) {
+        const int w = (bk == nbk - 1) ? H - bk * kBlk : kBlk;
+        val = ld_nt(zb + (uint32_t)(bk * rowk + y * w + j));
+      }
+      q[i] = val;
+    }
+  } else if (a.blk > 0) {
     const int lg = __builtin_ctz(RPB);
     const int nbk = (H + kBlk - 1) / kBlk;
     const int total = nbk * RPB * kBlk;
@@ -948,6 +989,9 @@ __global__ void __launch_bounds__(kMaxThreads) k_xi(Pack<XiArgs> pk, int RPB) {
     twp = ltw + M;
   }
   float2 q[XiEpt<M>::value];
+  // the stepped load walk costs registers where a prefetched tile stays live
+  // across the body of a long row: there the general walk keeps the occupancy
+  constexpr bool kStep = RT || M <= 768;
   if constexpr (RT) {
     // the round trip holds two transforms' registers: load each tile at its start
     // rather than keeping a prefetched tile live across the body
@@ -957,8 +1001,8 @@ __global__ void __launch_bounds__(kMaxThreads) k_xi(Pack<XiArgs> pk, int RPB) {
       const int v = pick(pk, b);
       const XiArgs& a = pk.a[v];
       const int64_t lb = b - pk.prefix[v];
-      xi_tile<M>(a, lb * RPB, RPB, q, lds, true, thr);
-      xi_tile<M>(a, lb * RPB, RPB, q, lds, false, thr);
+      xi_tile<M, kStep>(a, lb * RPB, RPB, q, lds, true, thr);
+      xi_tile<M, kStep>(a, lb * RPB, RPB, q, lds, false, thr);
       __syncthreads();
       xi_body<M, RT, BOX>(a, lb, RPB, lds, twm, twp, thr);
       __syncthreads();
@@ -969,7 +1013,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_xi(Pack<XiArgs> pk, int RPB) {
     int thr = threadIdx.x;
     opaque(thr);
     const int v = pick(pk, b);
-    xi_tile<M>(pk.a[v], (b - pk.prefix[v]) * RPB, RPB, q, lds, true, thr);
+    xi_tile<M, kStep>(pk.a[v], (b - pk.prefix[v]) * RPB, RPB, q, lds, true, thr);
   }
   for (;;) {
     int thr = threadIdx.x;
@@ -977,13 +1021,13 @@ __global__ void __launch_bounds__(kMaxThreads) k_xi(Pack<XiArgs> pk, int RPB) {
     const int v = pick(pk, b);
     const XiArgs& a = pk.a[v];
     const int64_t lb = b - pk.prefix[v];
-    xi_tile<M>(a, lb * RPB, RPB, q, lds, false, thr);
+    xi_tile<M, kStep>(a, lb * RPB, RPB, q, lds, false, thr);
     const int64_t bn = b + gridDim.x;
     if (bn < total) {
       int thr2 = threadIdx.x;
       opaque(thr2);
       const int vn = pick(pk, bn);
-      xi_tile<M>(pk.a[vn], (bn - pk.prefix[vn]) * RPB, RPB, q, lds, true, thr2);
+      xi_tile<M, kStep>(pk.a[vn], (bn - pk.prefix[vn]) * RPB, RPB, q, lds, true, thr2);
     }
     __syncthreads();
     xi_body<M, RT>(a, lb, RPB, lds, twm, twp, thr);
