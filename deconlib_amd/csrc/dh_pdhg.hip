@@ -219,11 +219,23 @@ __global__ void __launch_bounds__(kTileThreads, 6) k_pd_primal(float* __restrict
     const float sz = j == 0 ? 1.0f : zsign(d, zz);
     pp.store(&win[j][slot_of(zz, t.z0) * kPlane], (wv[j][0] * wws[j][0]) * sz, (wv[j][1] * wws[j][1]) * sz);
   };
+  // each component fetches only the halo hess_adj_halo reads of it: zz none, yz the
+  // y halo, xz the x halo (windowed); yy y, xx x, xy both (current plane)
+  auto wload = [&](int j, int64_t b) {
+    if (j == 0) pp.load_need<false, false>(wsrc[0], b, wv[0][0], wv[0][1]);
+    else if (j == 1) pp.load_need<false, true>(wsrc[1], b, wv[1][0], wv[1][1]);
+    else pp.load_need<true, false>(wsrc[2], b, wv[2][0], wv[2][1]);
+  };
+  auto cload = [&](int j, int64_t b) {
+    if (j == 0) pp.load_need<false, true>(csrc[0], b, cv[0][0], cv[0][1]);
+    else if (j == 1) pp.load_need<true, false>(csrc[1], b, cv[1][0], cv[1][1]);
+    else pp.load_need<true, true>(csrc[2], b, cv[2][0], cv[2][1]);
+  };
   for (int zz = t.z0 - 1; zz <= t.z0; ++zz) {
     const int64_t bz = pp.base(d, zz);
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      pp.load(wsrc[j], bz, wv[j][0], wv[j][1]);
+      wload(j, bz);
       wstore(j, zz);
     }
   }
@@ -231,8 +243,8 @@ __global__ void __launch_bounds__(kTileThreads, 6) k_pd_primal(float* __restrict
     const int64_t bn = pp.base(d, t.z0 + 1), bc = pp.base(d, t.z0);
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      pp.load(wsrc[j], bn, wv[j][0], wv[j][1]);
-      pp.load(csrc[j], bc, cv[j][0], cv[j][1]);
+      wload(j, bn);
+      cload(j, bc);
     }
   }
   // this voxel's gradient and iterate, loaded one plane ahead
@@ -253,8 +265,8 @@ __global__ void __launch_bounds__(kTileThreads, 6) k_pd_primal(float* __restrict
       const int64_t bn = pp.base(d, z + 2), bc = pp.base(d, z + 1);
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
-        pp.load(wsrc[j], bn, wv[j][0], wv[j][1]);
-        pp.load(csrc[j], bc, cv[j][0], cv[j][1]);
+        wload(j, bn);
+        cload(j, bc);
       }
     }
     float gn = 0.f, xnext = 0.f;
@@ -330,11 +342,23 @@ __global__ void __launch_bounds__(kTileThreads, 6) k_hess_adj_tiled(const float*
     const float sz = j == 0 ? 1.0f : zsign(d, zz);
     pp.store(&win[j][slot_of(zz, t.z0) * kPlane], (wv[j][0] * wws[j][0]) * sz, (wv[j][1] * wws[j][1]) * sz);
   };
+  // each component fetches only the halo hess_adj_halo reads of it: zz none, yz the
+  // y halo, xz the x halo (windowed); yy y, xx x, xy both (current plane)
+  auto wload = [&](int j, int64_t b) {
+    if (j == 0) pp.load_need<false, false>(wsrc[0], b, wv[0][0], wv[0][1]);
+    else if (j == 1) pp.load_need<false, true>(wsrc[1], b, wv[1][0], wv[1][1]);
+    else pp.load_need<true, false>(wsrc[2], b, wv[2][0], wv[2][1]);
+  };
+  auto cload = [&](int j, int64_t b) {
+    if (j == 0) pp.load_need<false, true>(csrc[0], b, cv[0][0], cv[0][1]);
+    else if (j == 1) pp.load_need<true, false>(csrc[1], b, cv[1][0], cv[1][1]);
+    else pp.load_need<true, true>(csrc[2], b, cv[2][0], cv[2][1]);
+  };
   for (int zz = t.z0 - 1; zz <= t.z0; ++zz) {
     const int64_t bz = pp.base(d, zz);
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      pp.load(wsrc[j], bz, wv[j][0], wv[j][1]);
+      wload(j, bz);
       wstore(j, zz);
     }
   }
@@ -342,8 +366,8 @@ __global__ void __launch_bounds__(kTileThreads, 6) k_hess_adj_tiled(const float*
     const int64_t bn = pp.base(d, t.z0 + 1), bc = pp.base(d, t.z0);
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      pp.load(wsrc[j], bn, wv[j][0], wv[j][1]);
-      pp.load(csrc[j], bc, cv[j][0], cv[j][1]);
+      wload(j, bn);
+      cload(j, bc);
     }
   }
   for (int z = t.z0; z < t.z1; ++z) {
@@ -357,8 +381,8 @@ __global__ void __launch_bounds__(kTileThreads, 6) k_hess_adj_tiled(const float*
       const int64_t bn = pp.base(d, z + 2), bc = pp.base(d, z + 1);
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
-        pp.load(wsrc[j], bn, wv[j][0], wv[j][1]);
-        pp.load(csrc[j], bc, cv[j][0], cv[j][1]);
+        wload(j, bn);
+        cload(j, bc);
       }
     }
     if (t.in) {
@@ -413,8 +437,11 @@ __global__ void __launch_bounds__(kTileThreads) k_pd_hess_sums(const float* __re
       auto fd = [&](int zz, int yy, int xx) { return lds_at(pdl, slot_of(zz, t.z0), t, yy, xx); };
       float h[6];
       hess6_halo(fd, z, t.y, t.x, w.w, h);
+      // the voxel's six squares summed in float, then accumulated in double
+      float hh = 0.0f;
 #pragma unroll
-      for (int q = 0; q < 6; ++q) s_sq += (double)(h[q] * h[q]);
+      for (int q = 0; q < 6; ++q) hh = fmaf(h[q], h[q], hh);
+      s_sq += (double)hh;
       if (want_abs) {
         auto fa = [&](int zz, int yy, int xx) { return lds_at(pa, slot_of(zz, t.z0), t, yy, xx); };
         hess6_halo(fa, z, t.y, t.x, w.w, h);

@@ -181,6 +181,7 @@ __device__ __forceinline__ int slot_of(int z, int z0) { return (z - z0 + 1) % 3;
 struct PlanePos {
   int64_t off0, off1;
   bool has1;
+  bool hx0, hy0, hx1, hy1;  // sample in the x (y) halo column (row)
   __device__ PlanePos(const Dims& d, const Tile& t) {
     const int i0 = threadIdx.x, i1 = threadIdx.x + kTileThreads;
     const int ly0 = i0 / kPW, lx0 = i0 - ly0 * kPW;
@@ -188,6 +189,19 @@ struct PlanePos {
     has1 = i1 < kPlane;
     const int ly1 = has1 ? i1 / kPW : 0, lx1 = has1 ? i1 - ly1 * kPW : 0;
     off1 = (int64_t)clampi(t.y0 + ly1 - 1, d.ny) * d.nx + clampi(t.x0 + lx1 - 1, d.nx);
+    hx0 = lx0 == 0 || lx0 == kPW - 1;
+    hy0 = ly0 == 0 || ly0 == kPH - 1;
+    hx1 = lx1 == 0 || lx1 == kPW - 1;
+    hy1 = ly1 == 0 || ly1 == kPH - 1;
+  }
+  // load for a stencil that reads the x halo only if NX and the y halo only if NY:
+  // samples in a halo it never reads are not fetched (their LDS slots stay unread)
+  template <bool NX, bool NY>
+  __device__ __forceinline__ void load_need(const float* __restrict__ src, int64_t b, float& v0, float& v1) const {
+    const bool l0 = (NX || !hx0) && (NY || !hy0);
+    const bool l1 = has1 && (NX || !hx1) && (NY || !hy1);
+    v0 = l0 ? src[b + off0] : 0.0f;
+    v1 = l1 ? src[b + off1] : 0.0f;
   }
   __device__ __forceinline__ int64_t base(const Dims& d, int z) const {
     return (int64_t)clampi(z, d.nz) * d.ny * d.nx;
