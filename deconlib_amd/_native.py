@@ -21,7 +21,7 @@ if not _LIB_PATH.exists():
 
 lib = C.CDLL(str(_LIB_PATH))
 
-ABI_VERSION = 8
+ABI_VERSION = 9
 
 DH_OK, DH_EINVAL, DH_EHIP, DH_EWORKSPACE, DH_EUNSUPPORTED = 0, -1, -2, -3, -4
 
@@ -215,6 +215,7 @@ SYMBOLS = {
     "dh_jn_direction": (_int, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp),
     "dh_jn_trial": (_int, _vp, _vp, _f, _vp, _vp, _i64, _vp, _vp, _vp),
     "dh_timing_enable": (_int, _int),
+    "dh_timing_select": (_int, _int),
     "dh_timing_read": (_int, C.POINTER(_d), C.POINTER(_i64), _int),
 }
 

@@ -118,22 +118,41 @@ struct TimedPass {
   hipEvent_t a, b;
 };
 bool g_timing = false;
+int g_timing_mask = (1 << DH_PASS_COUNT) - 1;
 std::vector<TimedPass> g_pending;
+std::vector<hipEvent_t> g_free;  // read events, reused
 double g_ms[DH_PASS_COUNT] = {0};
 int64_t g_cnt[DH_PASS_COUNT] = {0};
 
+// Timing events record without the system-scope fence: a default event's record
+// writes back and invalidates the caches, which stalls the stream between the
+// passes it brackets and slows the next pass (the elapsed time between two events
+// needs no fence once the host has synchronised on the later one).
+int timing_event(hipEvent_t* e) {
+  if (!g_free.empty()) {
+    *e = g_free.back();
+    g_free.pop_back();
+    return DH_OK;
+  }
+  return hip_check(hipEventCreateWithFlags(e, hipEventDisableSystemFence), "event create");
+}
+
+bool g_open = false;  // timing_begin recorded the opening event of the current pass
+
 int timing_begin(int pass, hipStream_t st) {
-  if (!g_timing) return DH_OK;
+  g_open = g_timing && (g_timing_mask >> pass & 1);
+  if (!g_open) return DH_OK;
   TimedPass t{pass, nullptr, nullptr};
-  DH_TRY(hip_check(hipEventCreate(&t.a), "event create"));
-  DH_TRY(hip_check(hipEventCreate(&t.b), "event create"));
+  DH_TRY(timing_event(&t.a));
+  DH_TRY(timing_event(&t.b));
   DH_TRY(hip_check(hipEventRecord(t.a, st), "event record"));
   g_pending.push_back(t);
   return DH_OK;
 }
 
 int timing_end(hipStream_t st) {
-  if (!g_timing) return DH_OK;
+  if (!g_open) return DH_OK;
+  g_open = false;
   return hip_check(hipEventRecord(g_pending.back().b, st), "event record");
 }
 
@@ -1226,6 +1245,11 @@ int dh_timing_enable(int on) {
   return DH_OK;
 }
 
+int dh_timing_select(int mask) {
+  g_timing_mask = mask & ((1 << DH_PASS_COUNT) - 1);
+  return DH_OK;
+}
+
 int dh_timing_read(double* ms, int64_t* counts, int reset) {
   for (auto& t : g_pending) {
     DH_TRY(hip_check(hipEventSynchronize(t.b), "event sync"));
@@ -1233,8 +1257,8 @@ int dh_timing_read(double* ms, int64_t* counts, int reset) {
     DH_TRY(hip_check(hipEventElapsedTime(&e, t.a, t.b), "event elapsed"));
     g_ms[t.pass] += e;
     g_cnt[t.pass] += 1;
-    (void)hipEventDestroy(t.a);
-    (void)hipEventDestroy(t.b);
+    g_free.push_back(t.a);
+    g_free.push_back(t.b);
   }
   g_pending.clear();
   for (int i = 0; i < DH_PASS_COUNT; ++i) {

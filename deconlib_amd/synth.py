@@ -12,7 +12,7 @@ import numpy as np
 
 from .psf import Optics, widefield_psf
 
-CONFIG0_OPTICS = Optics(wavelength=0.525, na=1.4, ni=1.515, ns=1.334)
+WIDEFIELD_OPTICS = Optics(wavelength=0.525, na=1.4, ni=1.515, ns=1.334)
 
 
 def blob_volume(shape, n_blobs: int, rng: np.random.Generator, background: float = 10.0,
@@ -41,7 +41,7 @@ def rl_workload(shape=(64, 256, 256), n_blobs=200, seed=1000, lateral=0.085, axi
     to a float64 host FFT linear convolution.
     """
     rng = np.random.default_rng(seed)
-    psf = widefield_psf(shape, lateral, axial, CONFIG0_OPTICS)
+    psf = widefield_psf(shape, lateral, axial, WIDEFIELD_OPTICS)
     obj = blob_volume(shape, n_blobs, rng)
     if blur is None:
         blur = host_linear_blur
@@ -52,13 +52,13 @@ def rl_workload(shape=(64, 256, 256), n_blobs=200, seed=1000, lateral=0.085, axi
 
 
 def aberrated_psf(shape, rng: np.random.Generator, lateral=0.085, axial=0.1, device: bool = False):
-    """Config-0 optics with IndexMismatch depth ~ U[0, 20] um and Zernike spherical ~ U[-0.5, 0.5] rad
+    """The widefield optics (WIDEFIELD_OPTICS) with IndexMismatch depth ~ U[0, 20] um and Zernike spherical ~ U[-0.5, 0.5] rad
     (``device=True``: synthesised on the GPU, a float32 device tensor)."""
     from .psf import IndexMismatch, ZernikeAberration
 
     ab = [IndexMismatch(depth=float(rng.uniform(0.0, 20.0))),
           ZernikeAberration({ZernikeAberration.SPHERICAL: float(rng.uniform(-0.5, 0.5))})]
-    return widefield_psf(shape, lateral, axial, CONFIG0_OPTICS, aberrations=ab, device=device)
+    return widefield_psf(shape, lateral, axial, WIDEFIELD_OPTICS, aberrations=ab, device=device)
 
 
 def filaments(shape, n: int, rng: np.random.Generator, intensity: float = 300.0, vertices: int = 6) -> np.ndarray:
@@ -83,7 +83,7 @@ def serving_requests(n_jobs: int, rate_per_s: float, seed: int, z_range=(16, 48)
     """Poisson arrivals of mixed RL / PDHG-Hessian jobs with log-uniform volume sizes.
 
     Sizes are rounded to multiples of 8; each job has its own aberrated PSF,
-    a blob object at the config-0 blob density and a Poisson observation.
+    a blob object at the RL workload's blob density (200 per 64x256x256) and a Poisson observation.
     ``blur(psf, obj)`` computes the noiseless image (default: host float64
     FFT).  Returns a list of ``serving.Request``.
     """
@@ -157,7 +157,7 @@ def serving_requests_device(n_jobs: int, rate_per_s: float, seed: int, z_range=(
     ``alpha_range``); each job with its own aberrated PSF (synthesised on the GPU, on the
     largest 2^a * {1, 3, 5} grid not above the volume per axis: a linear convolution
     takes any kernel extent up to the volume's),
-    a blob object at the config-0 density rendered on the GPU and a Poisson
+    a blob object at that blob density rendered on the GPU and a Poisson
     observation of its blurred image drawn on the GPU.  ``shapes`` / ``kinds``
     (lists of n_jobs) replace the drawn sizes / job kinds.
     """

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define DH_ABI_VERSION 8
+#define DH_ABI_VERSION 9
 
 #define DH_OK 0
 #define DH_EINVAL (-1)
@@ -553,8 +553,12 @@ int dh_jn_trial(const float* x, const float* d, float t, const float* g, float* 
 /* When enabled, dh_conv_apply records a HIP event pair around each pass on
  * the caller's stream.  dh_timing_read synchronises on those events and
  * returns per-pass accumulated milliseconds and launch counts
- * (DH_PASS_COUNT entries each); reset != 0 clears the accumulators. */
+ * (DH_PASS_COUNT entries each); reset != 0 clears the accumulators.
+ * dh_timing_select limits the bracketed passes to those whose bit (1 << pass id)
+ * is set in mask (default: all), so that a timed run can bracket only the pass it
+ * reports.  The events record without a system-scope fence (timing only). */
 int dh_timing_enable(int on);
+int dh_timing_select(int mask);
 int dh_timing_read(double* ms, int64_t* counts, int reset);
 
 /* ---- layout operators (core_operators.py:29-352) ---- */

@@ -9,10 +9,10 @@ pytestmark = pytest.mark.gpu
 
 def _job_inputs(shape, seed):
     from deconlib_amd.psf import widefield_psf
-    from deconlib_amd.synth import CONFIG0_OPTICS
+    from deconlib_amd.synth import WIDEFIELD_OPTICS
 
     rng = np.random.default_rng(seed)
-    psf = widefield_psf(shape, 0.085, 0.1, CONFIG0_OPTICS)
+    psf = widefield_psf(shape, 0.085, 0.1, WIDEFIELD_OPTICS)
     obs = rng.poisson(40 * rng.random(shape) + 3).astype(np.float32)
     return psf, obs
 

@@ -55,11 +55,11 @@ def _gpu_observation(psf, obj, seed, scale_to=None):
 def test_rl_config_full_size(gpu):
     from deconlib_amd import LinearFFTConvolver, richardson_lucy_with_operator
     from deconlib_amd.psf import widefield_psf
-    from deconlib_amd.synth import CONFIG0_OPTICS, blob_volume
+    from deconlib_amd.synth import WIDEFIELD_OPTICS, blob_volume
 
     shape = (64, 256, 256)
     print(f"RL at {shape}", flush=True)
-    psf = widefield_psf(shape, 0.085, 0.1, CONFIG0_OPTICS).astype(np.float32)
+    psf = widefield_psf(shape, 0.085, 0.1, WIDEFIELD_OPTICS).astype(np.float32)
     obj = blob_volume(shape, 200, np.random.default_rng(1000))
     obs = _gpu_observation(psf, obj, 1000)
     op = LinearFFTConvolver(psf, shape)
@@ -81,11 +81,11 @@ def test_binned_pdhg_config_full_size(gpu):
     from deconlib_amd import (Crop, FractionalAreaDownsample, LinearFFTConvolver, compose,
                               solve_pdhg_mlx)
     from deconlib_amd.psf import widefield_psf
-    from deconlib_amd.synth import CONFIG0_OPTICS, blob_volume
+    from deconlib_amd.synth import WIDEFIELD_OPTICS, blob_volume
 
     hi, lo = (128, 512, 512), (64, 256, 256)
     print(f"binned PDHG {hi} -> {lo}", flush=True)
-    psf = widefield_psf(hi, 0.0425, 0.05, CONFIG0_OPTICS).astype(np.float32)
+    psf = widefield_psf(hi, 0.0425, 0.05, WIDEFIELD_OPTICS).astype(np.float32)
     obj = blob_volume(hi, 2000, np.random.default_rng(1001)).astype(np.float32)
     conv = LinearFFTConvolver(psf, hi)
     assert conv.canvas_shape == (256, 1024, 1024)
@@ -137,7 +137,7 @@ def test_large_volume_full_size(gpu):
     from deconlib_amd import LinearFFTConvolver
     from deconlib_amd.pdhg import _PDState, _nonneg
     from deconlib_amd.psf import IndexMismatch, widefield_psf
-    from deconlib_amd.synth import CONFIG0_OPTICS
+    from deconlib_amd.synth import WIDEFIELD_OPTICS
 
     free, _ = torch.cuda.mem_get_info()
     if free < 200e9:
@@ -145,7 +145,7 @@ def test_large_volume_full_size(gpu):
     shape = (256, 2048, 2048)
     print(f"large volume {shape}", flush=True)
     # the configuration's PSF: vectorial (isotropic emitter), NA 1.4, IndexMismatch 10 um, on the GPU
-    psf = widefield_psf(shape, 0.085, 0.1, CONFIG0_OPTICS, aberrations=[IndexMismatch(depth=10.0)], device=True,
+    psf = widefield_psf(shape, 0.085, 0.1, WIDEFIELD_OPTICS, aberrations=[IndexMismatch(depth=10.0)], device=True,
                         dipole="isotropic")
     assert abs(float(psf.double().sum()) - 1.0) < 1e-5
     op = LinearFFTConvolver(psf, shape)
@@ -170,12 +170,12 @@ def test_other_solvers_config_full_size(gpu):
     from deconlib_amd import (AnisotropicHessian3D, LinearFFTConvolver, erdecon_with_operator,
                               jetnewton_with_operator, nlcg_with_operator)
     from deconlib_amd.psf import widefield_psf
-    from deconlib_amd.synth import CONFIG0_OPTICS, blob_volume
+    from deconlib_amd.synth import WIDEFIELD_OPTICS, blob_volume
     from oracle import nlcg as NL
 
     shape = (64, 256, 256)
     print(f"NLCG / ER-Decon / JetNewton at {shape}", flush=True)
-    psf = widefield_psf(shape, 0.085, 0.1, CONFIG0_OPTICS).astype(np.float32)
+    psf = widefield_psf(shape, 0.085, 0.1, WIDEFIELD_OPTICS).astype(np.float32)
     obj = blob_volume(shape, 200, np.random.default_rng(1001))
     obs = _gpu_observation(psf, obj, 1001)
     op = LinearFFTConvolver(psf, shape)

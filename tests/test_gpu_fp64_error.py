@@ -56,11 +56,11 @@ def _poisson(clean, seed, peak=None):
 def test_rl_config_fp64_error(gpu):
     from deconlib_amd import LinearFFTConvolver, richardson_lucy_with_operator
     from deconlib_amd.psf import widefield_psf
-    from deconlib_amd.synth import CONFIG0_OPTICS, blob_volume
+    from deconlib_amd.synth import WIDEFIELD_OPTICS, blob_volume
 
     shape, it = (64, 256, 256), 50
     print(f"RL {shape}, {it} iterations", flush=True)
-    psf = widefield_psf(shape, 0.085, 0.1, CONFIG0_OPTICS).astype(np.float32)
+    psf = widefield_psf(shape, 0.085, 0.1, WIDEFIELD_OPTICS).astype(np.float32)
     obj = blob_volume(shape, 200, np.random.default_rng(1000))
     obs = _poisson(O.LinearFFTConvolver(psf, shape, dtype=np.float64).forward(obj), 1000)
     gpu = richardson_lucy_with_operator(obs, LinearFFTConvolver(psf, shape), num_iter=it).restored.cpu().numpy()
@@ -73,11 +73,11 @@ def test_rl_config_fp64_error(gpu):
 def test_binned_pdhg_config_fp64_error(gpu):
     from deconlib_amd import solve_pdhg_mlx
     from deconlib_amd.psf import widefield_psf
-    from deconlib_amd.synth import CONFIG0_OPTICS, blob_volume
+    from deconlib_amd.synth import WIDEFIELD_OPTICS, blob_volume
 
     hi, it = (64, 256, 256), 20
     print(f"binned PDHG {hi} -> half, {it} iterations", flush=True)
-    psf = widefield_psf(hi, 0.0425, 0.05, CONFIG0_OPTICS).astype(np.float32)
+    psf = widefield_psf(hi, 0.0425, 0.05, WIDEFIELD_OPTICS).astype(np.float32)
     obj = blob_volume(hi, 250, np.random.default_rng(1001))
     A = O.compose(O.FractionalAreaDownsample(2.0), O.LinearFFTConvolver(psf, hi, dtype=np.float64))
     obs = _poisson(A.forward(obj), 1001, peak=500.0)
@@ -125,11 +125,11 @@ LONG = os.environ.get("DECONLIB_AMD_LONG_TESTS") == "1"
 def test_large_volume_crop_fp64_error(gpu, shape, it):
     from deconlib_amd import solve_pdhg_mlx
     from deconlib_amd.psf import IndexMismatch, widefield_psf
-    from deconlib_amd.synth import CONFIG0_OPTICS, blob_volume, filaments
+    from deconlib_amd.synth import WIDEFIELD_OPTICS, blob_volume, filaments
 
     print(f"large-volume solver on a {shape} crop, {it} iterations", flush=True)
     rng = np.random.default_rng(1004)
-    psf = widefield_psf(shape, 0.085, 0.1, CONFIG0_OPTICS, aberrations=[IndexMismatch(depth=10.0)],
+    psf = widefield_psf(shape, 0.085, 0.1, WIDEFIELD_OPTICS, aberrations=[IndexMismatch(depth=10.0)],
                         dipole="isotropic")
     # 1e5 blobs over 256x2048x2048 voxels: the same density on the crop
     obj = blob_volume(shape, round(1e5 * np.prod(shape) / (256 * 2048 * 2048)), rng)

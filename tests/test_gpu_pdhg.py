@@ -34,10 +34,10 @@ def test_hessian_matches_oracle(gpu, shape, r):
 
 def _workload(shape=(12, 40, 40), seed=32, n_blobs=20):
     from deconlib_amd.psf import widefield_psf
-    from deconlib_amd.synth import CONFIG0_OPTICS, blob_volume
+    from deconlib_amd.synth import WIDEFIELD_OPTICS, blob_volume
 
     rng = np.random.default_rng(seed)
-    psf = widefield_psf(shape, 0.085, 0.1, CONFIG0_OPTICS)
+    psf = widefield_psf(shape, 0.085, 0.1, WIDEFIELD_OPTICS)
     obj = blob_volume(shape, n_blobs, rng)
     A = O.LinearFFTConvolver(psf, shape, dtype=np.float64)
     obs = rng.poisson(np.maximum(A.forward(obj), 0)).astype(np.float32)

@@ -60,10 +60,10 @@ def test_device_psf_parseval(gpu):
 def test_device_psf_config_shape_matches_host(gpu):
     """The 64x256x256 widefield PSF of the RL workload, GPU against the host synthesis."""
     from deconlib_amd.psf import widefield_psf
-    from deconlib_amd.synth import CONFIG0_OPTICS
+    from deconlib_amd.synth import WIDEFIELD_OPTICS
 
-    host = widefield_psf((64, 256, 256), 0.085, 0.1, CONFIG0_OPTICS, dtype=np.float64)
-    dev = widefield_psf((64, 256, 256), 0.085, 0.1, CONFIG0_OPTICS, device=True).cpu().numpy()
+    host = widefield_psf((64, 256, 256), 0.085, 0.1, WIDEFIELD_OPTICS, dtype=np.float64)
+    dev = widefield_psf((64, 256, 256), 0.085, 0.1, WIDEFIELD_OPTICS, device=True).cpu().numpy()
     assert rel_l2(dev, host) <= 1e-5
     assert abs(float(dev.sum(dtype=np.float64)) - 1.0) <= 1e-5
 
