@@ -21,12 +21,12 @@ if not _LIB_PATH.exists():
 
 lib = C.CDLL(str(_LIB_PATH))
 
-ABI_VERSION = 9
+ABI_VERSION = 10
 
 DH_OK, DH_EINVAL, DH_EHIP, DH_EWORKSPACE, DH_EUNSUPPORTED = 0, -1, -2, -3, -4
 
 PRO_NONE, PRO_ONES, PRO_SUB_FLOOR = 0, 1, 2
-EPI_NONE, EPI_FLOOR, EPI_RL_RATIO, EPI_RL_UPDATE, EPI_ADD = 0, 1, 2, 3, 4
+EPI_NONE, EPI_FLOOR, EPI_RL_RATIO, EPI_RL_UPDATE, EPI_ADD, EPI_SQDIFF = 0, 1, 2, 3, 4, 5
 RED_SUM, RED_MAX, RED_SUMSQ = 0, 1, 2
 REG_HESSIAN, REG_GRADIENT, REG_IDENTITY = 0, 1, 2
 STATIC_SCRATCH_BYTES = 128 << 20  # DH_STATIC_SCRATCH_BYTES
@@ -159,6 +159,8 @@ SYMBOLS = {
     "dh_pd_y1": (_int, _vp, _vp, _vp, _vp, _i64, _vp, _f, _int, _vp),
     "dh_pd_dual": (_int, _vp, _vp, _vp, _int, _int, _int, _vp, C.POINTER(_f), _f, _int, _vp),
     "dh_pd_primal": (_int, _vp, _vp, _vp, _vp, _int, _int, _int, _vp, C.POINTER(_f), _vp, _vp),
+    "dh_pd_dual_primal": (_int, _vp, _vp, _vp, _vp, _vp, _vp, _int, _int, _int, _vp, C.POINTER(_f), _f, _int, _vp,
+                          _vp),
     "dh_pd_hess_sums": (_int, _vp, _vp, _int, _int, _int, C.POINTER(_f), _vp, _vp, _int, _vp, _vp),
     "dh_pd_data_sums": (_int, _vp, _vp, _vp, _i64, _f, _int, _vp, _vp, _vp, _vp),
     "dh_pd_scalars": (_int, _vp, _vp, _int, C.POINTER(PdParams), _vp),

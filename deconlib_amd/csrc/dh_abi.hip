@@ -810,12 +810,12 @@ int dh_conv_apply(const dh_conv_plan* p, int adjoint, const float* in, float* ou
     epi.aux0 = fuse->aux0;
     epi.aux1 = fuse->aux1;
     if (pro.kind < DH_PRO_NONE || pro.kind > DH_PRO_SUB_FLOOR) return fail(DH_EINVAL, "unknown prologue");
-    if (epi.kind < DH_EPI_NONE || epi.kind > DH_EPI_ADD) return fail(DH_EINVAL, "unknown epilogue");
-    if ((epi.kind == DH_EPI_RL_RATIO || epi.kind == DH_EPI_RL_UPDATE) && !epi.aux0)
+    if (epi.kind < DH_EPI_NONE || epi.kind > DH_EPI_SQDIFF) return fail(DH_EINVAL, "unknown epilogue");
+    if ((epi.kind == DH_EPI_RL_RATIO || epi.kind == DH_EPI_RL_UPDATE || epi.kind == DH_EPI_SQDIFF) && !epi.aux0)
       return fail(DH_EINVAL, "epilogue needs aux0");
     if (epi.kind == DH_EPI_RL_UPDATE && !epi.aux1) return fail(DH_EINVAL, "epilogue needs aux1");
     if (fuse->reduce_out) {
-      if (!(epi.kind == DH_EPI_RL_RATIO || (epi.kind == DH_EPI_ADD && epi.aux0)))
+      if (!(epi.kind == DH_EPI_RL_RATIO || epi.kind == DH_EPI_SQDIFF || (epi.kind == DH_EPI_ADD && epi.aux0)))
         return fail(DH_EINVAL, "this epilogue has no reduction");
       reduce_out = fuse->reduce_out;
       epi.partials = (double*)(ws + p->part_off);
@@ -902,6 +902,28 @@ int dh_pd_primal(float* x_new, const float* x, const float* g, const float* y2, 
     return fail(DH_EINVAL, "bad pd_primal arguments");
   return hip_check(dh::pd_primal(x_new, x, g, y2, nz, ny, nx, scalars, w, partials, (hipStream_t)stream),
                    "pd_primal");
+}
+
+int dh_pd_dual_primal(float* y2_new, float* x_new, const float* x, const float* xp, const float* y2,
+                      const float* g, int nz, int ny, int nx, double* scalars, const float* w, float alpha,
+                      int l12, double* partials, void* stream) {
+  if (!y2_new || !x_new || !x || !xp || !y2 || !g || !scalars || !w || !partials || !dims_ok(nz, ny, nx))
+    return fail(DH_EINVAL, "bad pd_dual_primal arguments");
+  if ((int64_t)ny * nx > INT32_MAX) return fail(DH_EINVAL, "pd_dual_primal: a plane holds more than 2^31 voxels");
+  const int64_t n = (int64_t)nz * ny * nx;
+  auto overlaps = [](const void* a, int64_t an, const void* b, int64_t bn) {
+    const char* pa = (const char*)a;
+    const char* pb = (const char*)b;
+    return pa < pb + bn && pb < pa + an;
+  };
+  const int64_t vb = 4 * n, yb = 24 * n;
+  if (overlaps(y2_new, yb, y2, yb) || overlaps(y2_new, yb, x, vb) || overlaps(y2_new, yb, xp, vb) ||
+      overlaps(y2_new, yb, g, vb) || overlaps(x_new, vb, x, vb) || overlaps(x_new, vb, xp, vb) ||
+      overlaps(x_new, vb, y2, yb) || overlaps(x_new, vb, g, vb) || overlaps(x_new, vb, y2_new, yb))
+    return fail(DH_EINVAL, "pd_dual_primal outputs must not overlap its inputs or each other");
+  return hip_check(dh::pd_dual_primal(y2_new, x_new, x, xp, y2, g, nz, ny, nx, scalars, w, alpha, l12, partials,
+                                      (hipStream_t)stream),
+                   "pd_dual_primal");
 }
 
 int dh_pd_hess_sums(const float* a, const float* b, int nz, int ny, int nx, const float* w, double* scalars,
@@ -1290,12 +1312,12 @@ int fill_fuse(const dh_fuse_args* fuse, dh::Pro& pro, dh::Epi& epi, double** red
   epi.aux0 = fuse->aux0;
   epi.aux1 = fuse->aux1;
   if (pro.kind < DH_PRO_NONE || pro.kind > DH_PRO_SUB_FLOOR) return fail(DH_EINVAL, "unknown prologue");
-  if (epi.kind < DH_EPI_NONE || epi.kind > DH_EPI_ADD) return fail(DH_EINVAL, "unknown epilogue");
-  if ((epi.kind == DH_EPI_RL_RATIO || epi.kind == DH_EPI_RL_UPDATE) && !epi.aux0)
+  if (epi.kind < DH_EPI_NONE || epi.kind > DH_EPI_SQDIFF) return fail(DH_EINVAL, "unknown epilogue");
+  if ((epi.kind == DH_EPI_RL_RATIO || epi.kind == DH_EPI_RL_UPDATE || epi.kind == DH_EPI_SQDIFF) && !epi.aux0)
     return fail(DH_EINVAL, "epilogue needs aux0");
   if (epi.kind == DH_EPI_RL_UPDATE && !epi.aux1) return fail(DH_EINVAL, "epilogue needs aux1");
   if (fuse->reduce_out) {
-    if (!(epi.kind == DH_EPI_RL_RATIO || (epi.kind == DH_EPI_ADD && epi.aux0)))
+    if (!(epi.kind == DH_EPI_RL_RATIO || epi.kind == DH_EPI_SQDIFF || (epi.kind == DH_EPI_ADD && epi.aux0)))
       return fail(DH_EINVAL, "this epilogue has no reduction");
     *reduce_out = fuse->reduce_out;
   }

@@ -355,6 +355,11 @@ __device__ __forceinline__ float apply_epi(const Epi& e, float v, int64_t idx, d
       if (e.partials && e.aux0) acc += (double)idiv_term(e.aux0[idx], mdl);
       return mdl;
     }
+    case EPI_SQDIFF: {
+      const float dv = v - e.aux0[idx];
+      if (e.partials) acc += (double)(dv * dv);
+      return v;
+    }
     default: return v;
   }
 }

@@ -22,7 +22,7 @@ struct AxisSched {
 // Fused prologue (input sample of the forward X pass) and epilogue (cropped
 // output sample of the inverse X pass).
 enum ProKind { PRO_NONE = 0, PRO_ONES = 1, PRO_SUB_FLOOR = 2, PRO_DIV_DEV = 100 };
-enum EpiKind { EPI_NONE = 0, EPI_FLOOR = 1, EPI_RL_RATIO = 2, EPI_RL_UPDATE = 3, EPI_ADD = 4 };
+enum EpiKind { EPI_NONE = 0, EPI_FLOOR = 1, EPI_RL_RATIO = 2, EPI_RL_UPDATE = 3, EPI_ADD = 4, EPI_SQDIFF = 5 };
 
 struct Pro {
   int kind;

@@ -76,6 +76,11 @@ __device__ __forceinline__ float epi_apply(const Epi& e, float v, int64_t idx, d
       if (e.partials && e.aux0 && cnt(0)) acc += (double)idiv(e.aux0[idx], mdl);
       return mdl;
     }
+    case EPI_SQDIFF: {
+      const float dv = v - e.aux0[idx];
+      if (e.partials && cnt(0)) acc += (double)(dv * dv);
+      return v;
+    }
     default: return v;
   }
 }
@@ -109,6 +114,15 @@ __device__ __forceinline__ float2 epi_apply2(const Epi& e, float2 v, int64_t idx
         if (cnt(1)) acc += (double)idiv(o.y, m1);
       }
       return make_float2(m0, m1);
+    }
+    case EPI_SQDIFF: {
+      if (e.partials) {
+        const float2 o = *reinterpret_cast<const float2*>(e.aux0 + idx);
+        const float d0 = v.x - o.x, d1 = v.y - o.y;
+        if (cnt(0)) acc += (double)(d0 * d0);
+        if (cnt(1)) acc += (double)(d1 * d1);
+      }
+      return v;
     }
     default: return v;
   }
@@ -919,6 +933,7 @@ __device__ __forceinline__ void xi_body(const XiArgs& a, int64_t lb, int RPB, fl
       case EPI_RL_RATIO: sweep2(std::integral_constant<int, EPI_RL_RATIO>()); break;
       case EPI_RL_UPDATE: sweep2(std::integral_constant<int, EPI_RL_UPDATE>()); break;
       case EPI_ADD: sweep2(std::integral_constant<int, EPI_ADD>()); break;
+      case EPI_SQDIFF: sweep2(std::integral_constant<int, EPI_SQDIFF>()); break;
       default: sweep2(std::integral_constant<int, EPI_NONE>()); break;
     }
   } else {

@@ -119,6 +119,9 @@ __global__ void k_pd_y1(float* __restrict__ y1, const float* __restrict__ ax, co
 }
 
 
+// The dual and primal updates write their multiply-adds as fmaf (xbar, y2 + sigma H xbar,
+// the L1_2 norm's squares, x - tau grad), so that the fused sweep below and these
+// kernels round identically wherever the compiler inlines them.
 // y2 <- prox(y2 + sigma * H(xbar)), xbar = x + theta (x - xp)  (pdhg_mlx.py:729-734)
 __global__ void __launch_bounds__(kTileThreads) k_pd_dual(float* __restrict__ y2, const float* __restrict__ x,
                                                           const float* __restrict__ xp, Dims d,
@@ -128,7 +131,7 @@ __global__ void __launch_bounds__(kTileThreads) k_pd_dual(float* __restrict__ y2
   const PlanePos pp(d, t);
   const int64_t n = (int64_t)d.nz * d.ny * d.nx;
   const float sf = (float)sc[PD_SIGMA], th = (float)sc[PD_THETA];
-  auto xbar = [&](float v, float vp) { return v + th * (v - vp); };
+  auto xbar = [&](float v, float vp) { return fmaf(th, v - vp, v); };
   float a0, a1, b0, b1;
   for (int zz = t.z0 - 1; zz <= t.z0; ++zz) {
     const int64_t bz = pp.base(d, zz);
@@ -169,14 +172,14 @@ __global__ void __launch_bounds__(kTileThreads) k_pd_dual(float* __restrict__ y2
       hess6_halo(f, z, t.y, t.x, w.w, h);
       float av[6];
 #pragma unroll
-      for (int q = 0; q < 6; ++q) av[q] = yc[q] + sf * h[q];
+      for (int q = 0; q < 6; ++q) av[q] = fmaf(sf, h[q], yc[q]);
       if (!l12) {
 #pragma unroll
         for (int q = 0; q < 6; ++q) stnt(y2 + q * n + i, fminf(fmaxf(av[q], -alpha), alpha));
       } else {
         float ss = 0.0f;
 #pragma unroll
-        for (int q = 0; q < 6; ++q) ss += av[q] * av[q];
+        for (int q = 0; q < 6; ++q) ss = fmaf(av[q], av[q], ss);
         const float nrm = sqrtf(ss + 1e-12f);
         const float s = fmaxf(nrm / alpha, 1.0f);
 #pragma unroll
@@ -291,7 +294,7 @@ __global__ void __launch_bounds__(kTileThreads, 6) k_pd_primal(float* __restrict
       const int64_t i = ((int64_t)z * d.ny + t.y) * d.nx + t.x;
       const float grad = gc + lt;
       const float xv = xc;
-      const float v = fmaxf(xv - tf * grad, 0.0f);
+      const float v = fmaxf(fmaf(-tf, grad, xv), 0.0f);
       stnt(xn + i, v);
       const float dv = v - xv;
       s_dx += (double)(dv * dv);
@@ -306,6 +309,351 @@ __global__ void __launch_bounds__(kTileThreads, 6) k_pd_primal(float* __restrict
   double r0 = pd_block_sum(s_dx, sh);
   double r1 = pd_block_sum(s_x, sh);
   double r2 = pd_block_sum(s_xn, sh);
+  if (threadIdx.x == 0) {
+    part[b * 3 + 0] = r0;
+    part[b * 3 + 1] = r1;
+    part[b * 3 + 2] = r2;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused dual and primal steps (pdhg_mlx.py:729-750).  g = A^T y1_new does not
+// depend on y2, so the adjoint convolution runs before this kernel; one z sweep
+// per tile then computes y2_new = prox(y2 + sigma H xbar) on the tile and on the
+// one-voxel ring around it that H^T reads (each ring recomputed by the
+// neighbouring tiles from y2 and xbar, which no tile overwrites: y2_new and x_new
+// go to buffers of their own), and from the staged y2_new the primal step
+// x_new = max(0, x - tau (g + H^T y2_new)) with the partial sums of
+// (x_new - x)^2, x^2, x_new^2.  y2_new is written once and never read back: per
+// voxel x, xp, y2, g are read and y2_new, x_new written once (64 B, against 92 B
+// for k_pd_dual followed by k_pd_primal), with the same arithmetic in the same
+// order, so the same values.
+//
+// Thread t owns tile voxel (t % kTX, t / kTX) in every role (xbar, y2_new, x_new;
+// coalesced, and its x stays in registers from staging to the primal step); the
+// rings are spread over the first threads: the xbar ring (two voxels wide, 304
+// points) and the y2_new ring, where H^T reads only components 1, 3, 5 on the y
+// sides, 2, 4, 5 on the x sides and 5 at the corners (all six under the L1_2
+// projection, whose norm needs them).
+// ---------------------------------------------------------------------------
+
+constexpr int kFW = kTX + 4, kFH = kTY + 4, kFPlane = kFW * kFH;  // xbar window: two-voxel ring
+constexpr int kXRing = kFPlane - kTX * kTY;                         // 304
+// planes per workgroup: the z sweep restarts its pipeline (four xbar planes, two
+// y2_new planes) at every chunk, so chunks are longer than the other stencils'
+constexpr int kZCF = 64;
+constexpr int kYRing = kPlane - kTX * kTY;                          // 148
+static_assert(kXRing <= kTileThreads && kYRing <= kTileThreads, "one ring point per thread");
+
+// Weighted component q of Hessian3D.forward at (z, y, x): hess6_halo's expression
+// for that component alone (the same operations, so the same value).
+template <class F>
+__device__ __forceinline__ float hess_comp(const F& f, int q, int z, int y, int x, const float* w) {
+  const int zm = z - 1, zp = z + 1, ym = y - 1, yp = y + 1, xm = x - 1, xp = x + 1;
+  float h;
+  switch (q) {
+    case 0: h = (f(zm, y, x) - 2.0f * f(z, y, x)) + f(zp, y, x); break;
+    case 1: h = (f(z, ym, x) - 2.0f * f(z, y, x)) + f(z, yp, x); break;
+    case 2: h = (f(z, y, xm) - 2.0f * f(z, y, x)) + f(z, y, xp); break;
+    case 3: {
+      const float dz_yp = (f(zp, yp, x) - f(zm, yp, x)) / 2.0f;
+      const float dz_ym = (f(zp, ym, x) - f(zm, ym, x)) / 2.0f;
+      h = (dz_yp - dz_ym) / 2.0f;
+      break;
+    }
+    case 4: {
+      const float dz_xp = (f(zp, y, xp) - f(zm, y, xp)) / 2.0f;
+      const float dz_xm = (f(zp, y, xm) - f(zm, y, xm)) / 2.0f;
+      h = (dz_xp - dz_xm) / 2.0f;
+      break;
+    }
+    default: {
+      const float dy_xp = (f(z, yp, xp) - f(z, ym, xp)) / 2.0f;
+      const float dy_xm = (f(z, yp, xm) - f(z, ym, xm)) / 2.0f;
+      h = (dy_xp - dy_xm) / 2.0f;
+      break;
+    }
+  }
+  return h * w[q];
+}
+
+template <int L12>
+__global__ void __launch_bounds__(kTileThreads, 4) k_pd_dual_primal(float* __restrict__ y2o, float* __restrict__ xn,
+                                                                    const float* __restrict__ x,
+                                                                    const float* __restrict__ xp,
+                                                                    const float* __restrict__ y2i,
+                                                                    const float* __restrict__ g, Dims d,
+                                                                    const double* __restrict__ sc, W6 w,
+                                                                    float alpha, double* __restrict__ part) {
+  // four plane slots each (plane z in slot z & 3): one barrier per step suffices
+  // when step z stages xbar(z + 4), computes y2_new(z + 2) and x_new(z)
+  __shared__ float xb[4][kFPlane];    // xbar (window rows of kFW)
+  __shared__ float yw[3][4][kPlane];  // weighted y2_new components 0, 3, 4 (rows of kPW)
+  __shared__ float yc[3][4][kPlane];  // components 1, 2, 5
+  __shared__ double sh[16];
+  const Tile t = tile_of(d, kZCF);
+  const int z0 = t.z0, z1 = t.z1;
+  const int64_t n = (int64_t)d.nz * d.ny * d.nx;
+  const int64_t pstride = (int64_t)d.ny * d.nx;
+  const float sf = (float)sc[PD_SIGMA], th = (float)sc[PD_THETA], tf = (float)sc[PD_TAU];
+  const int thr = threadIdx.x;
+  auto plane = [&](int z) { return (int64_t)clampi(z, d.nz) * pstride; };
+
+  // the thread's voxel: clamped in-plane offset (also its y2_new / x_new offset when t.in;
+  // in-plane offsets fit 32 bits, host-checked), its xbar- and y2_new-window slots; a
+  // voxel past the volume's x or y end (a partial tile) is ring for its in-volume
+  // neighbours: computed at the clamped position (xbar-window centre ocy, ocx) and
+  // staged with the reflection signs osy, osx
+  const int ocyg = clampi(t.y, d.ny), ocxg = clampi(t.x, d.nx);
+  const int voff = ocyg * d.nx + ocxg;
+  const int xwin = (t.ty + 2) * kFW + (t.tx + 2);
+  const int ywin = (t.ty + 1) * kPW + (t.tx + 1);
+  const int ocy = ocyg - t.y0 + 2, ocx = ocxg - t.x0 + 2;
+  const float osy = t.y < d.ny ? 1.0f : -1.0f, osx = t.x < d.nx ? 1.0f : -1.0f;
+  // its xbar ring point (thr < kXRing): rows 0, 1 and kFH-2, kFH-1, then the side columns
+  int xr_ly, xr_lx;
+  {
+    const int h = thr;
+    if (h < 4 * kFW) {
+      const int r = h / kFW;
+      xr_ly = r < 2 ? r : kFH - 4 + r;
+      xr_lx = h - r * kFW;
+    } else {
+      const int k = h - 4 * kFW;  // 4 columns x kTY rows
+      xr_ly = 2 + (k >> 2);
+      const int c = k & 3;
+      xr_lx = c < 2 ? c : kFW - 4 + c;
+    }
+  }
+  const bool xr_has = thr < kXRing;
+  const int xr_off = clampi(t.y0 + xr_ly - 2, d.ny) * d.nx + clampi(t.x0 + xr_lx - 2, d.nx);
+  const int xr_win = xr_ly * kFW + xr_lx;
+  // its y2_new ring point (thr < kYRing): y sides, x sides, corners; computed at the
+  // clamped position (xbar-window centre yr_cy, yr_cx); reflection signs outside the volume
+  int yr_ly, yr_lx, yr_kind;  // kind 0: y side (components 1, 3, 5), 1: x side (2, 4, 5), 2: corner (5)
+  {
+    const int h = thr;
+    if (h < 2 * kTX) {
+      yr_kind = 0;
+      yr_ly = h < kTX ? 0 : kPH - 1;
+      yr_lx = 1 + (h & (kTX - 1));
+    } else if (h < 2 * kTX + 2 * kTY) {
+      yr_kind = 1;
+      const int k = h - 2 * kTX;
+      yr_ly = 1 + (k % kTY);
+      yr_lx = k < kTY ? 0 : kPW - 1;
+    } else {
+      yr_kind = 2;
+      const int k = (h - 2 * kTX - 2 * kTY) & 3;
+      yr_ly = (k & 2) ? kPH - 1 : 0;
+      yr_lx = (k & 1) ? kPW - 1 : 0;
+    }
+  }
+  const bool yr_has = thr < kYRing;
+  const int yr_gy = t.y0 + yr_ly - 1, yr_gx = t.x0 + yr_lx - 1;
+  const int yr_cyg = clampi(yr_gy, d.ny), yr_cxg = clampi(yr_gx, d.nx);
+  const int yr_off = yr_cyg * d.nx + yr_cxg;
+  const int yr_cy = yr_cyg - t.y0 + 2, yr_cx = yr_cxg - t.x0 + 2;
+  const int yr_win = yr_ly * kPW + yr_lx;
+  const float yr_sy = (yr_gy < 0 || yr_gy >= d.ny) ? -1.0f : 1.0f;
+  const float yr_sx = (yr_gx < 0 || yr_gx >= d.nx) ? -1.0f : 1.0f;
+  // components of the ring point (L1: the three H^T reads there; L1_2: all six)
+  constexpr int NR = L12 ? 6 : 3;
+  int rq[NR];
+  if (L12) {
+#pragma unroll
+    for (int q = 0; q < NR; ++q) rq[q] = q;
+  } else {
+    rq[0] = yr_kind == 0 ? 1 : (yr_kind == 1 ? 2 : 5);
+    rq[1] = yr_kind == 0 ? 3 : (yr_kind == 1 ? 4 : 5);
+    rq[2] = 5;
+  }
+
+  float ax, axp, rx, rxp;          // x, xp of the next xbar plane: own voxel, ring point
+  float yv[6], yr[NR];             // y2 of the next y2_new plane: own voxel, ring point
+  float yv1[6], yr1[NR], yv2[6], yr2[NR];  // the same one and two planes further (requested ahead)
+  float xq0 = 0.f, xq1 = 0.f, xq2 = 0.f, xq3 = 0.f;  // own x on planes z .. z+3 (the primal's x)
+  auto xload = [&](int z) {
+    const int64_t b = plane(z);
+    ax = x[b + voff];
+    axp = xp[b + voff];
+    rx = xr_has ? x[b + xr_off] : 0.0f;
+    rxp = xr_has ? xp[b + xr_off] : 0.0f;
+  };
+  auto xbar = [&](float v, float vp) { return fmaf(th, v - vp, v); };
+  auto xstage = [&](int z) {
+    xb[z & 3][xwin] = xbar(ax, axp);
+    if (xr_has) xb[z & 3][xr_win] = xbar(rx, rxp);
+  };
+  auto yload_to = [&](int p, float* ov, float* orr) {
+    const int64_t b = plane(p);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) ov[q] = y2i[q * n + b + voff];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) orr[q] = yr_has ? y2i[rq[q] * n + b + yr_off] : 0.0f;
+  };
+  auto yload = [&](int p) { yload_to(p, yv, yr); };
+  auto prox = [&](float* o, int m) {
+    if (!L12) {
+      for (int q = 0; q < m; ++q) o[q] = fminf(fmaxf(o[q], -alpha), alpha);
+    } else {
+      float ss = 0.0f;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) ss = fmaf(o[q], o[q], ss);
+      const float nrm = sqrtf(ss + 1e-12f);
+      const float s = fmaxf(nrm / alpha, 1.0f);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) o[q] = o[q] / s;
+    }
+  };
+  // y2_new of plane p (from yv / yr): own voxel (written out if `write`) and ring point,
+  // staged with the weights and reflections k_pd_primal stages y2 with
+  auto ycompute = [&](int p, bool write) {
+    const int a = p & 3;
+    auto f = [&](int zz, int yy, int xx) { return xb[zz & 3][yy * kFW + xx]; };
+    {
+      float h[6];
+      hess6_halo(f, p, ocy, ocx, w.w, h);
+      float o[6];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) o[q] = fmaf(sf, h[q], yv[q]);
+      prox(o, 6);
+      if (write && t.in) {
+        const int64_t b = plane(p);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) stnt(y2o + q * n + b + voff, o[q]);
+      }
+      yw[0][a][ywin] = o[0] * w.w[0];
+      yw[1][a][ywin] = o[3] * (w.w[3] * osy);
+      yw[2][a][ywin] = o[4] * (w.w[4] * osx);
+      yc[0][a][ywin] = o[1] * w.w[1];
+      yc[1][a][ywin] = o[2] * w.w[2];
+      yc[2][a][ywin] = o[5] * ((w.w[5] * osx) * osy);
+    }
+    if (yr_has) {
+      float o[NR];
+#pragma unroll
+      for (int q = 0; q < NR; ++q) o[q] = fmaf(sf, hess_comp(f, rq[q], p, yr_cy, yr_cx, w.w), yr[q]);
+      prox(o, NR);
+      // (ring points: the y sides carry the y reflection, the x sides the x one)
+#pragma unroll
+      for (int q = 0; q < NR; ++q) {
+        switch (rq[q]) {
+          case 0: yw[0][a][yr_win] = o[q] * w.w[0]; break;
+          case 1: yc[0][a][yr_win] = o[q] * w.w[1]; break;
+          case 2: yc[1][a][yr_win] = o[q] * w.w[2]; break;
+          case 3: yw[1][a][yr_win] = o[q] * (w.w[3] * yr_sy); break;
+          case 4: yw[2][a][yr_win] = o[q] * (w.w[4] * yr_sx); break;
+          default: yc[2][a][yr_win] = o[q] * ((w.w[5] * yr_sx) * yr_sy); break;
+        }
+      }
+    }
+  };
+  double s_dx = 0.0, s_x = 0.0, s_xn = 0.0;
+  // x_new of plane z at the own voxel; y2_new of planes z +- 1 beyond the volume's z
+  // ends is that of the end plane, components 3 and 4 reflected (k_pd_primal's staging)
+  auto xcompute = [&](int z, float gz, float xv) {
+    if (!t.in) return;
+    const int cm = (z > 0 ? z - 1 : z) & 3, c0 = z & 3, cp = (z + 1 < d.nz ? z + 1 : z) & 3;
+    const float szm = z > 0 ? 1.0f : -1.0f, szp = z + 1 < d.nz ? 1.0f : -1.0f;
+    auto hw = [&](int q, int zz, int yy, int xx) {
+      const int o = (yy - t.y0 + 1) * kPW + (xx - t.x0 + 1);
+      if (q == 1) return yc[0][c0][o];
+      if (q == 2) return yc[1][c0][o];
+      if (q == 5) return yc[2][c0][o];
+      const int j = q == 0 ? 0 : (q == 3 ? 1 : 2);
+      const float v = yw[j][zz < z ? cm : (zz > z ? cp : c0)][o];
+      if (q == 0) return v;
+      return v * (zz < z ? szm : (zz > z ? szp : 1.0f));
+    };
+    const float lt = hess_adj_halo(hw, z, t.y, t.x);
+    const int64_t i = (int64_t)z * pstride + voff;
+    const float grad = gz + lt;
+    const float v = fmaxf(fmaf(-tf, grad, xv), 0.0f);
+    stnt(xn + i, v);
+    const float dv = v - xv;
+    s_dx += (double)(dv * dv);
+    s_x += (double)(xv * xv);
+    s_xn += (double)(v * v);
+  };
+  auto gload = [&](int z) { return t.in ? ldnt(g + (int64_t)z * pstride + voff) : 0.0f; };
+  auto keep = [&](int q) {
+    if (q == z0) xq0 = ax;
+    else if (q == z0 + 1) xq1 = ax;
+    else if (q == z0 + 2) xq2 = ax;
+    else if (q == z0 + 3) xq3 = ax;
+  };
+
+  // Prologue: y2_new is needed on planes [pa, pb] (z0 - 1 .. z1 inside the volume); before
+  // step z0, planes pa .. z0 + 1 of it and xbar planes up to z0 + 3.  Plane q of xbar may
+  // replace plane q - 4 once every y2_new plane that reads q - 4 is done.
+  const int pa = z0 > 0 ? z0 - 1 : 0;
+  const int pb = z1 < d.nz ? z1 : d.nz - 1;
+  const int ylast = min(z0 + 1, pb);
+  int nx_ = pa - 1, ny_ = pa;  // next xbar plane to stage, next y2_new plane to compute
+  for (int k = 0; k < 4; ++k, ++nx_) {
+    xload(nx_);
+    xstage(nx_);
+    keep(nx_);
+  }
+  __syncthreads();
+  for (; ny_ <= ylast && ny_ + 1 < nx_; ++ny_) {
+    yload(ny_);
+    ycompute(ny_, ny_ >= z0 && ny_ < z1);
+  }
+  while (nx_ <= z0 + 3 || ny_ <= ylast) {
+    __syncthreads();
+    xload(nx_);
+    xstage(nx_);
+    keep(nx_);
+    ++nx_;
+    __syncthreads();
+    for (; ny_ <= ylast && ny_ + 1 < nx_; ++ny_) {
+      yload(ny_);
+      ycompute(ny_, ny_ >= z0 && ny_ < z1);
+    }
+  }
+  // operands of step z0 (xbar one step ahead, y2 two steps ahead from here on)
+  if (z0 + 3 <= pb) xload(z0 + 4);
+  if (z0 + 2 <= pb) yload(z0 + 2);
+  if (z0 + 3 <= pb) yload_to(z0 + 3, yv1, yr1);
+  float gz = gload(z0);
+  __syncthreads();
+  for (int z = z0; z < z1; ++z) {
+    // xbar(z + 4) replaces xbar(z), whose last reader (y2_new(z + 1)) ran before the barrier
+    const bool st4 = z + 3 <= pb;
+    if (st4) xstage(z + 4);
+    const float xq4 = ax;
+    // later steps' operands, requested before this step's work
+    if (z + 4 <= pb) {
+      xload(z + 5);
+      yload_to(z + 4, yv2, yr2);
+    }
+    const float gnext = z + 1 < z1 ? gload(z + 1) : 0.0f;
+    // y2_new(z + 2) replaces y2_new(z - 2), last read by x_new(z - 1)
+    if (z + 2 <= pb) ycompute(z + 2, z + 2 < z1);
+    xcompute(z, gz, xq0);
+    gz = gnext;
+    xq0 = xq1;
+    xq1 = xq2;
+    xq2 = xq3;
+    xq3 = xq4;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      yv[q] = yv1[q];
+      yv1[q] = yv2[q];
+    }
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      yr[q] = yr1[q];
+      yr1[q] = yr2[q];
+    }
+    __syncthreads();
+  }
+  const int64_t b = ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+  const double r0 = pd_block_sum(s_dx, sh);
+  const double r1 = pd_block_sum(s_x, sh);
+  const double r2 = pd_block_sum(s_xn, sh);
   if (threadIdx.x == 0) {
     part[b * 3 + 0] = r0;
     part[b * 3 + 1] = r1;
@@ -765,6 +1113,24 @@ hipError_t pd_primal(float* xn, const float* x, const float* g, const float* y2,
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   return finish(part, tile_blocks(nz, ny, nx), 3, sc + PD_DX2, st);  // DX2, XX, XNXN are consecutive
+}
+
+hipError_t pd_dual_primal(float* y2_new, float* xn, const float* x, const float* xp, const float* y2, const float* g,
+                          int nz, int ny, int nx, double* sc, const float* w, float alpha, int l12, double* part,
+                          hipStream_t st) {
+  Dims d{nz, ny, nx};
+  W6 ww;
+  for (int q = 0; q < 6; ++q) ww.w[q] = w[q];
+  const dim3 grid = tile_grid(nz, ny, nx, kZCF);
+  if (l12)
+    hipLaunchKernelGGL(k_pd_dual_primal<1>, grid, dim3(kTileThreads), 0, st, y2_new, xn, x, xp, y2, g, d, sc, ww,
+                       alpha, part);
+  else
+    hipLaunchKernelGGL(k_pd_dual_primal<0>, grid, dim3(kTileThreads), 0, st, y2_new, xn, x, xp, y2, g, d, sc, ww,
+                       alpha, part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return finish(part, (int)(grid.x * grid.y * grid.z), 3, sc + PD_DX2, st);  // DX2, XX, XNXN are consecutive
 }
 
 hipError_t pd_hess_sums(const float* a, const float* b, int nz, int ny, int nx, const float* w, int want_abs,

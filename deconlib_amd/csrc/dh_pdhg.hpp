@@ -33,6 +33,9 @@ hipError_t pd_dual(float* y2, const float* x, const float* xp, int nz, int ny, i
                    const float* w, float alpha, int l12, hipStream_t st);
 hipError_t pd_primal(float* xn, const float* x, const float* g, const float* y2, int nz, int ny, int nx,
                      double* sc, const float* w, double* part, hipStream_t st);
+hipError_t pd_dual_primal(float* y2_new, float* xn, const float* x, const float* xp, const float* y2, const float* g,
+                          int nz, int ny, int nx, double* sc, const float* w, float alpha, int l12, double* part,
+                          hipStream_t st);
 hipError_t pd_hess_sums(const float* a, const float* b, int nz, int ny, int nx, const float* w, int want_abs,
                         double* sc, double* loss_out, double* part, hipStream_t st);
 hipError_t pd_data_sums(const float* a, const float* b, const float* data, int64_t n, float bg, int want_loss,

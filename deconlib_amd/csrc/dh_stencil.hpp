@@ -145,14 +145,14 @@ __device__ __forceinline__ void tile_coords(int& bx, int& by, int& bz) {
   bz = (int)(l / ((int64_t)gridDim.x * gridDim.y));
 }
 
-__device__ __forceinline__ Tile tile_of(const Dims& d) {
+__device__ __forceinline__ Tile tile_of(const Dims& d, int zc = kZC) {
   Tile t;
   int bx, by, bz;
   tile_coords(bx, by, bz);
   t.x0 = bx * kTX;
   t.y0 = by * kTY;
-  t.z0 = bz * kZC;
-  t.z1 = min(t.z0 + kZC, d.nz);
+  t.z0 = bz * zc;
+  t.z1 = min(t.z0 + zc, d.nz);
   t.tx = threadIdx.x % kTX;
   t.ty = threadIdx.x / kTX;
   t.x = t.x0 + t.tx;
@@ -238,8 +238,8 @@ __device__ __forceinline__ float lds_at(const float* planes, int slot, const Til
   return planes[slot * kPlane + (yy - t.y0 + 1) * kPW + (xx - t.x0 + 1)];
 }
 
-static inline dim3 tile_grid(int nz, int ny, int nx) {
-  return dim3((unsigned)((nx + kTX - 1) / kTX), (unsigned)((ny + kTY - 1) / kTY), (unsigned)((nz + kZC - 1) / kZC));
+static inline dim3 tile_grid(int nz, int ny, int nx, int zc = kZC) {
+  return dim3((unsigned)((nx + kTX - 1) / kTX), (unsigned)((ny + kTY - 1) / kTY), (unsigned)((nz + zc - 1) / zc));
 }
 
 static inline int tile_blocks(int nz, int ny, int nx) {

@@ -41,6 +41,7 @@ __all__ = [
 ]
 
 _TAU, _SIGMA, _THETA, _STEP, _COUNT, _DONE, _FINAL = 0, 1, 2, 3, 4, 5, 6
+_DA2 = 10  # DH_PD_DA2
 
 
 class IdentityRegularizer:
@@ -210,6 +211,13 @@ class _PDState:
         nc = int(N.fns["dh_reg_components"](self.kind))
         self.y2 = torch.zeros((nc,) + self.D3, dtype=torch.float32, device=dev)
         self.g = empty_f32(self.S)
+        # Hessian: the dual and primal steps run as one sweep (dh_pd_dual_primal), which
+        # reads x, xp and y2 around each tile while it writes: the new iterate and dual
+        # go to buffers of their own (xn, y2n), rotated after the step
+        self.fused = self.kind == N.REG_HESSIAN and self.D3[1] * self.D3[2] < 2 ** 31
+        if self.fused:
+            self.xn = empty_f32(self.S)
+            self.y2n = torch.empty_like(self.y2)
         if self._plan is not None:
             # the step sizes start from the OTF norm on the device: no host round trip
             self.sc = torch.empty(N.PD_NSLOTS, dtype=torch.float64, device=dev)
@@ -247,55 +255,90 @@ class _PDState:
     def _own(self, r, arg):
         """``r`` unless it shares memory with the argument or with a buffer the state
         keeps (an operator that returns a persistent buffer): then a copy."""
-        held = [arg] + [getattr(self, n) for n in ("x", "xp", "Ax", "Axp", "g", "y1") if hasattr(self, n)]
+        held = [arg] + [getattr(self, n) for n in ("x", "xp", "xn", "Ax", "Axp", "g", "y1") if hasattr(self, n)]
         if any(r.data_ptr() == h.data_ptr() for h in held):
             return r.clone()
         return r
 
+    @property
+    def x_next(self) -> torch.Tensor:
+        """The buffer the primal step writes x_{k+1} into."""
+        return self.xn if self.fused else self.xp
+
     # one iteration in five phases, so that a batch of solves can group its convolutions
     def _pre(self, k: int) -> None:
-        """y1 and y2 dual steps (pdhg_mlx.py:717-734)."""
+        """y1 dual step, and the y2 dual step unless it is fused with the primal one
+        (pdhg_mlx.py:717-734)."""
         st = N.stream_ptr()
         nz, ny, nx = self.D3
         sc = self.sc.data_ptr()
         N.call("dh_pd_y1", self.y1.data_ptr(), self.Ax.data_ptr(), self.Axp.data_ptr(), self.data.data_ptr(),
                self.data.numel(), sc, self.bg, self.lt, st)
-        N.call("dh_pd_reg_dual", self.kind, self.y2.data_ptr(), self.x.data_ptr(), self.xp.data_ptr(), nz, ny, nx,
-               sc, self.wp, self.alpha, self.l12, st)
+        if not self.fused:
+            N.call("dh_pd_reg_dual", self.kind, self.y2.data_ptr(), self.x.data_ptr(), self.xp.data_ptr(), nz, ny,
+                   nx, sc, self.wp, self.alpha, self.l12, st)
 
     def _mid(self, k: int) -> None:
-        """Primal step with the regularizer's adjoint (pdhg_mlx.py:741-750)."""
+        """Primal step with the regularizer's adjoint (pdhg_mlx.py:741-750); for the
+        Hessian the y2 dual step runs in the same sweep (y2 does not enter A^T y1)."""
         nz, ny, nx = self.D3
-        N.call("dh_pd_reg_primal", self.kind, self.xp.data_ptr(), self.x.data_ptr(), self.g.data_ptr(),
-               self.y2.data_ptr(), nz, ny, nx, self.sc.data_ptr(), self.wp, self.part.data_ptr(), N.stream_ptr())
+        if self.fused:
+            N.call("dh_pd_dual_primal", self.y2n.data_ptr(), self.xn.data_ptr(), self.x.data_ptr(),
+                   self.xp.data_ptr(), self.y2.data_ptr(), self.g.data_ptr(), nz, ny, nx, self.sc.data_ptr(),
+                   self.wp, self.alpha, self.l12, self.part.data_ptr(), N.stream_ptr())
+            self.y2, self.y2n = self.y2n, self.y2
+        else:
+            N.call("dh_pd_reg_primal", self.kind, self.xp.data_ptr(), self.x.data_ptr(), self.g.data_ptr(),
+                   self.y2.data_ptr(), nz, ny, nx, self.sc.data_ptr(), self.wp, self.part.data_ptr(),
+                   N.stream_ptr())
+
+    def _fwd_fuse(self, want_loss: bool) -> Optional[N.FuseArgs]:
+        """Epilogue of the forward convolution A x_new on a plan: |A x_new - A x|^2 summed
+        into the DA2 slot in the inverse X pass (the loss iterations keep the separate
+        data-sums kernel, which also evaluates the data term)."""
+        if self._plan is None or want_loss:
+            return None
+        return N.FuseArgs(epilogue=N.EPI_SQDIFF, aux0=self.Ax.data_ptr(),
+                          reduce_out=self.sc.data_ptr() + 8 * _DA2)
 
     def _post(self, k: int, want_loss: bool) -> None:
-        """Step-size sums, the Malitsky-Pock update and the swap (pdhg_mlx.py:752-860)."""
+        """Step-size sums, the Malitsky-Pock update and the rotation of the iterates
+        (pdhg_mlx.py:752-860)."""
         st = N.stream_ptr()
         nz, ny, nx = self.D3
         sc = self.sc.data_ptr()
         dl = self.loss[2 * k:].data_ptr() if want_loss else None
         rl = self.loss[2 * k + 1:].data_ptr() if want_loss else None
-        N.call("dh_pd_data_sums", self.Axp.data_ptr(), self.Ax.data_ptr(), self.data.data_ptr(),
-               self.data.numel(), self.bg, self.lt, sc, dl, self.part.data_ptr(), st)
-        N.call("dh_pd_reg_sums", self.kind, self.xp.data_ptr(), self.x.data_ptr(), nz, ny, nx, self.wp, sc, rl,
+        xnew = self.x_next
+        if self._fwd_fuse(want_loss) is None:
+            N.call("dh_pd_data_sums", self.Axp.data_ptr(), self.Ax.data_ptr(), self.data.data_ptr(),
+                   self.data.numel(), self.bg, self.lt, sc, dl, self.part.data_ptr(), st)
+        N.call("dh_pd_reg_sums", self.kind, xnew.data_ptr(), self.x.data_ptr(), nz, ny, nx, self.wp, sc, rl,
                self.reg_norm, self.part.data_ptr(), st)
         N.call("dh_pd_scalars", sc, self.hist.data_ptr(), k, N.C.byref(self.params), st)
-        self.x, self.xp = self.xp, self.x
+        if self.fused:
+            self.x, self.xp, self.xn = self.xn, self.x, self.xp
+        else:
+            self.x, self.xp = self.xp, self.x
         self.Ax, self.Axp = self.Axp, self.Ax
 
-    def _conv_item(self, adjoint: bool) -> N.BatchItem:
+    def _conv_item(self, adjoint: bool, want_loss: bool = False) -> N.BatchItem:
         p = self._plan
-        src, dst = (self.y1, self.g) if adjoint else (self.xp, self.Axp)
+        src, dst = (self.y1, self.g) if adjoint else (self.x_next, self.Axp)
+        fuse = None if adjoint else self._fwd_fuse(want_loss)
         return N.BatchItem(plan=p._h, otf=p.otf_buf.data_ptr(), work=p.work.data_ptr(),
                            work_bytes=int(p.info.work_bytes), input=src.data_ptr(), out=dst.data_ptr(),
-                           fuse=N.FuseArgs())
+                           fuse=fuse if fuse is not None else N.FuseArgs())
 
     def step(self, k: int, want_loss: bool) -> None:
         self._pre(k)
         self.g = self._adj(self.y1, self.g)
         self._mid(k)
-        self.Axp = self._fwd(self.xp, self.Axp)
+        fuse = self._fwd_fuse(want_loss)
+        if fuse is not None:
+            self.Axp = self._plan.apply(self.x_next, False, self.Axp, fuse)
+        else:
+            self.Axp = self._fwd(self.x_next, self.Axp)
         self._post(k, want_loss)
 
     def done(self) -> bool:
@@ -313,12 +356,14 @@ def pd_batch_step(states: List[_PDState], ks: List[int], want_loss=None, max_ite
     each solve's result equals its solo run bit for bit."""
     want_loss = want_loss or [False] * len(states)
     planned = [s for s in states if s._plan is not None]
+    wl_of = {id(s): w for s, w in zip(states, want_loss)}
     for s, k in zip(states, ks):
         s._pre(k)
     for adjoint in (True, False):
         for j in range(0, len(planned), max_items):
             chunk = planned[j:j + max_items]
-            arr = (N.BatchItem * len(chunk))(*[s._conv_item(adjoint) for s in chunk])
+            wl = [wl_of[id(s)] for s in chunk]
+            arr = (N.BatchItem * len(chunk))(*[s._conv_item(adjoint, w) for s, w in zip(chunk, wl)])
             N.call("dh_batch_conv_apply", len(chunk), arr, int(adjoint), N.stream_ptr())
             for s in chunk:
                 s._plan.gen += 1
@@ -327,7 +372,7 @@ def pd_batch_step(states: List[_PDState], ks: List[int], want_loss=None, max_ite
                 if adjoint:
                     s.g = s._adj(s.y1, s.g)
                 else:
-                    s.Axp = s._fwd(s.xp, s.Axp)
+                    s.Axp = s._fwd(s.x_next, s.Axp)
         if adjoint:
             for s, k in zip(states, ks):
                 s._mid(k)

@@ -88,8 +88,8 @@ def estimate_bytes(req: Request) -> int:
     shape = tuple(int(s) for s in req.observed.shape)
     info = plan_info(shape)
     vox = int(np.prod(shape))
-    # RL: obs, x, sens, ratio, pred; PDHG: data, x, xp, y1, y2 (6), Ax, Axp, g
-    state = (5 if req.kind == "rl" else 14) * 4 * vox
+    # RL: obs, x, sens, ratio, pred; PDHG: data, x, xp, x_new, y1, y2 and y2_new (6 each), Ax, Axp, g
+    state = (5 if req.kind == "rl" else 21) * 4 * vox
     return int(info.otf_bytes + info.work_bytes + info.twiddle_bytes) + state + vox * 4  # + psf
 
 

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define DH_ABI_VERSION 9
+#define DH_ABI_VERSION 10
 
 #define DH_OK 0
 #define DH_EINVAL (-1)
@@ -68,6 +68,7 @@ typedef struct {
                              /*   reduce: sum aux0*log(max(aux0,1e-6)/max(m,1e-6)) - (aux0 - max(m,1e-6)) */
 #define DH_EPI_RL_UPDATE 3   /* out = max(aux0 * (v / aux1), b)  (aux1 = +inf off-support) */
 #define DH_EPI_ADD 4         /* out = v + a      reduce: same I-divergence as RL_RATIO when aux0 != NULL */
+#define DH_EPI_SQDIFF 5      /* out = v          reduce: sum (v - aux0)^2 (PDHG's |A x_new - A x|^2)     */
 
 typedef struct {
   int32_t prologue;
@@ -291,6 +292,14 @@ int dh_pd_dual(float* y2, const float* x, const float* xp, int nz, int ny, int n
 /* x_new = max(0, x - tau (g + H^T y2)); writes DX2, XX, XNXN  (pdhg_mlx.py:741-750, 805-806). */
 int dh_pd_primal(float* x_new, const float* x, const float* g, const float* y2, int nz, int ny, int nx,
                  double* scalars, const float* w, double* partials, void* stream);
+/* The dual and primal steps in one sweep (pdhg_mlx.py:729-750): y2_new <- prox(y2 + sigma
+ * H(x + theta (x - xp))) as dh_pd_dual, then x_new = max(0, x - tau (g + H^T y2_new)) as
+ * dh_pd_primal (writes DX2, XX, XNXN), with g = A^T y1_new computed beforehand.  y2_new and
+ * x_new must not overlap x, xp, y2 or g (neighbouring tiles re-read those).  Same values as
+ * the two calls. */
+int dh_pd_dual_primal(float* y2_new, float* x_new, const float* x, const float* xp, const float* y2,
+                      const float* g, int nz, int ny, int nx, double* scalars, const float* w, float alpha,
+                      int l12, double* partials, void* stream);
 /* DL2 = sum |H a - H b|^2; if reg_loss_out, *reg_loss_out = sum |H a| (reg_norm 1, "L1")
  * or sum sqrt(sum_q (H a)_q^2 + 1e-12) (reg_norm 2, "L1_2")  (pdhg_mlx.py:874-882). */
 int dh_pd_hess_sums(const float* a, const float* b, int nz, int ny, int nx, const float* w, double* scalars,

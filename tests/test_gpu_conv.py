@@ -170,3 +170,45 @@ def test_launch_shapes_are_bit_identical(gpu, sshape, kshape):
                 assert np.array_equal(op.adjoint(x).cpu().numpy(), ref_a), (p, v)
         N.call("dh_conv_plan_set_tile", h, p, 0)
     assert tried >= 10
+
+
+@pytest.mark.parametrize("sshape,kshape", [((16, 32, 32), (16, 32, 32)), ((9, 15, 11), (4, 6, 8)),
+                                           ((5, 7, 1000), (3, 3, 21))])
+def test_sqdiff_epilogue(gpu, sshape, kshape):
+    """DH_EPI_SQDIFF leaves the output unchanged and sums (out - aux0)^2 in float64
+    (each square formed in float32, as PDHG's data sums form it); the batched entry
+    point gives the same output and sum.  Shapes: specialised even canvas, odd rows,
+    and a runtime-length canvas."""
+    import ctypes as C
+
+    from deconlib_amd import LinearFFTConvolver
+    from deconlib_amd import _native as N
+
+    print(f"sqdiff epilogue {sshape} * {kshape}", flush=True)
+    rng = np.random.default_rng(13)
+    k = (rng.random(kshape) + 0.05).astype(np.float32)
+    x = torch.from_numpy(rng.standard_normal(sshape).astype(np.float32)).cuda()
+    prev = torch.from_numpy(rng.standard_normal(sshape).astype(np.float32)).cuda()
+    op = LinearFFTConvolver(k, sshape)
+    ref = op.forward(x)
+    red = torch.zeros(2, dtype=torch.float64, device="cuda")
+    out = torch.empty_like(ref)
+    fuse = N.FuseArgs(epilogue=N.EPI_SQDIFF, aux0=prev.data_ptr(), reduce_out=red.data_ptr())
+    op._plan.apply(x, False, out, fuse)
+    p = op._plan
+    out_b = torch.empty_like(ref)
+    fuse_b = N.FuseArgs(epilogue=N.EPI_SQDIFF, aux0=prev.data_ptr(), reduce_out=red.data_ptr() + 8)
+    item = N.BatchItem(plan=p._h, otf=p.otf_buf.data_ptr(), work=p.work.data_ptr(),
+                       work_bytes=int(p.info.work_bytes), input=x.data_ptr(), out=out_b.data_ptr(), fuse=fuse_b)
+    N.call("dh_batch_conv_apply", 1, (N.BatchItem * 1)(item), 0, N.stream_ptr())
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref) and torch.equal(out_b, ref)
+    d = (ref - prev).double().cpu().numpy().astype(np.float32)
+    want = float(np.sum((d * d).astype(np.float64)))
+    got = red.cpu().numpy()
+    np.testing.assert_allclose(got, [want, want], rtol=1e-12)
+    assert got[0] == got[1]
+    # aux0 is required
+    bad = N.FuseArgs(epilogue=N.EPI_SQDIFF, reduce_out=red.data_ptr())
+    assert N.fns["dh_conv_apply"](p._h, 0, x.data_ptr(), out.data_ptr(), p.otf_buf.data_ptr(), p.work.data_ptr(),
+                                  int(p.info.work_bytes), C.byref(bad), N.stream_ptr()) == N.DH_EINVAL

@@ -99,3 +99,53 @@ def test_pdhg_errors_like_reference(gpu):
         solve_pdhg_mlx(obs, psf, icf_sigmas=(1.0, 1.0, 1.0))
     with pytest.raises(ValueError):
         solve_pdhg_mlx(obs, psf, bin_factors=3)
+
+
+@pytest.mark.parametrize("shape", [(37, 21, 70), (1, 13, 130), (1, 1, 77), (16, 8, 64), (18, 9, 65)])
+@pytest.mark.parametrize("l12", [0, 1])
+def test_fused_dual_primal_equals_separate_kernels(gpu, shape, l12):
+    """dh_pd_dual_primal equals dh_pd_dual followed by dh_pd_primal: y2_new and x_new bit
+    for bit, the DX2 / XX / XNXN sums to float64 rounding (several z chunks, partial tiles
+    in x and y, 1-D and 2-D volumes, clipped and L1_2-projected duals)."""
+    import ctypes as C
+
+    from deconlib_amd import _native as N
+
+    print(f"fused dual/primal {shape} l12={l12}", flush=True)
+    rng = np.random.default_rng(40 + l12)
+    nz, ny, nx = shape
+    dev = "cuda"
+    x = torch.from_numpy(rng.random(shape).astype(np.float32)).to(dev)
+    xp = torch.from_numpy(rng.random(shape).astype(np.float32)).to(dev)
+    y2 = torch.from_numpy(0.02 * rng.standard_normal((6,) + shape).astype(np.float32)).to(dev)
+    g = torch.from_numpy(rng.standard_normal(shape).astype(np.float32)).to(dev)
+    w = (C.c_float * 6)(1.7, 1.0, 1.0, 1.3, 1.3, 2 ** 0.5)
+    sc0 = np.zeros(16, np.float64)
+    sc0[0], sc0[1], sc0[2] = 0.3, 0.7, 0.9  # tau, sigma, theta
+    part = torch.empty(int(N.fns["dh_pd_partials"](nz, ny, nx)), dtype=torch.float64, device=dev)
+    st = N.stream_ptr()
+    alpha = 0.01
+    # separate kernels
+    sc_a = torch.from_numpy(sc0.copy()).to(dev)
+    y2_a = y2.clone()
+    xn_a = torch.empty_like(x)
+    N.call("dh_pd_dual", y2_a.data_ptr(), x.data_ptr(), xp.data_ptr(), nz, ny, nx, sc_a.data_ptr(), w, alpha,
+           l12, st)
+    N.call("dh_pd_primal", xn_a.data_ptr(), x.data_ptr(), g.data_ptr(), y2_a.data_ptr(), nz, ny, nx,
+           sc_a.data_ptr(), w, part.data_ptr(), st)
+    # fused
+    sc_b = torch.from_numpy(sc0.copy()).to(dev)
+    y2_b = torch.full_like(y2, float("nan"))
+    xn_b = torch.full_like(x, float("nan"))
+    N.call("dh_pd_dual_primal", y2_b.data_ptr(), xn_b.data_ptr(), x.data_ptr(), xp.data_ptr(), y2.data_ptr(),
+           g.data_ptr(), nz, ny, nx, sc_b.data_ptr(), w, alpha, l12, part.data_ptr(), st)
+    torch.cuda.synchronize()
+    assert torch.equal(y2_a, y2_b)
+    assert torch.equal(xn_a, xn_b)
+    a, b = sc_a.cpu().numpy(), sc_b.cpu().numpy()
+    # the same float32 terms summed in float64 over longer z chunks: equal to rounding
+    np.testing.assert_allclose(b[7:10], a[7:10], rtol=1e-12)
+    # the clip (or the projection onto the ball) was active somewhere
+    mag = y2_b.norm(dim=0) if l12 else y2_b.abs()
+    assert float(mag.max()) <= alpha * (1 + 1e-5)
+    assert float((mag >= alpha * 0.999).float().mean()) > 0

@@ -98,3 +98,26 @@ def test_solver_entry_points_reject_bad_arguments(prefix):
         with pytest.raises(ValueError, match="arguments"):
             N.check(rc, name)
 
+
+
+def test_pd_dual_primal_rejects_overlapping_buffers():
+    """The fused PDHG sweep re-reads x, xp, y2 and g around every tile while it writes:
+    outputs that overlap an input (or each other) are refused before any device work.
+    The pointers are never dereferenced on this path."""
+    fn = N.fns["dh_pd_dual_primal"]
+    nz, ny, nx = 4, 5, 6
+    n = nz * ny * nx
+    w = (C.c_float * 6)(1, 1, 1, 1, 1, 1)
+    base = 1 << 20
+    x, xp, g = base, base + 4 * n, base + 8 * n
+    y2, y2n, xn = base + 12 * n, base + 40 * n, base + 70 * n
+    sc, part = base + 80 * n, base + 90 * n
+
+    def call(y2n_, xn_, x_=x, xp_=xp, y2_=y2, g_=g):
+        return fn(y2n_, xn_, x_, xp_, y2_, g_, nz, ny, nx, sc, w, 1.0, 0, part, None)
+
+    for bad in [call(y2, xn), call(y2n, x), call(y2n, xp), call(y2n, g), call(y2n, y2 + 4),
+                call(y2n, y2n + 8), call(x, xn), call(y2n, xn, x_=xn)]:
+        assert bad == N.DH_EINVAL
+    assert fn(None, xn, x, xp, y2, g, nz, ny, nx, sc, w, 1.0, 0, part, None) == N.DH_EINVAL
+    assert fn(y2n, xn, x, xp, y2, g, 0, ny, nx, sc, w, 1.0, 0, part, None) == N.DH_EINVAL
