@@ -167,7 +167,7 @@ def bench_large_volume(torch, steps=3, warmup=1, seed=1004):
     + 10 and 64 thin filaments (intensity 300); vectorial PSF of an isotropic emitter
     (pupil_to_vectorial_psf) of the volume's size with IndexMismatch depth 10 um,
     synthesised on the GPU; Poisson data.  Skipped below
-    200 GB of free device memory (about 160 GB stay resident)."""
+    200 GB of free device memory (about 150 GB stay resident)."""
     import ctypes as C
 
     from deconlib_amd import LinearFFTConvolver
