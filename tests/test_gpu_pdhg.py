@@ -101,12 +101,14 @@ def test_pdhg_errors_like_reference(gpu):
         solve_pdhg_mlx(obs, psf, bin_factors=3)
 
 
-@pytest.mark.parametrize("shape", [(37, 21, 70), (1, 13, 130), (1, 1, 77), (16, 8, 64), (18, 9, 65)])
+@pytest.mark.parametrize("shape", [(37, 21, 70), (1, 13, 130), (1, 1, 77), (16, 8, 64), (18, 9, 65), (65, 9, 66),
+                                   (130, 17, 40), (128, 8, 64)])
 @pytest.mark.parametrize("l12", [0, 1])
 def test_fused_dual_primal_equals_separate_kernels(gpu, shape, l12):
     """dh_pd_dual_primal equals dh_pd_dual followed by dh_pd_primal: y2_new and x_new bit
-    for bit, the DX2 / XX / XNXN sums to float64 rounding (several z chunks, partial tiles
-    in x and y, 1-D and 2-D volumes, clipped and L1_2-projected duals)."""
+    for bit, the DX2 / XX / XNXN sums to float64 rounding (one to three 64-plane z chunks,
+    a last chunk of one or two planes, partial tiles in x and y, 1-D and 2-D volumes,
+    clipped and L1_2-projected duals)."""
     import ctypes as C
 
     from deconlib_amd import _native as N
