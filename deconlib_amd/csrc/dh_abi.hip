@@ -200,7 +200,7 @@ bool ct_col_tile_ok(int n, int L, int kind) {
 }
 // rows per workgroup: a power of two (the blocked row walk relies on it)
 bool ct_row_tile_ok(int m, int R) {
-  return R >= 1 && (R & (R - 1)) == 0 && R * dh::ct_threads(m) <= kCtMaxThreads &&
+  return R >= 1 && (R & (R - 1)) == 0 && R * dh::ct_row_threads(m) <= kCtMaxThreads &&
          dh::xrow_lds_elems(m, R) * 8 <= kCtMaxLds;
 }
 // Default launch shapes (what the autotuner settles on for the 64x512x512
@@ -231,7 +231,7 @@ int ct_col_tile(int n, int64_t ncols, int over, int kind) {
 constexpr int64_t kMinRowTiles = 2048;  // 8 workgroups per CU
 int ct_row_tile(int m, int over, int64_t nrows) {
   if (over > 0 && ct_row_tile_ok(m, over)) return over;
-  const int T = dh::ct_threads(m);
+  const int T = dh::ct_row_threads(m);
   int R = 1;
   while (R < 64 && 2 * R * T <= kCtMaxThreads && dh::xrow_lds_elems(m, 2 * R) * 8 <= kCtRowDefaultLds) R *= 2;
   const int r_min = T >= 64 ? 1 : 64 / T;
@@ -507,7 +507,7 @@ int dh_conv_plan_create(int ndim, const int64_t* signal_shape, const int64_t* ke
   if (p->ct) push_pass_tables(p->tw_host, dh::ct::make_sched(p->P[1]));
   else push_table(p->tw_host, p->P[1], p->P[1]);
   p->tw_xm = (int64_t)p->tw_host.size();
-  if (p->ct) push_pass_tables(p->tw_host, dh::ct::make_sched(p->M));
+  if (p->ct) push_pass_tables(p->tw_host, dh::ct::make_row_sched(p->M));
   else push_table(p->tw_host, p->M, p->M);
   p->tw_xp = (int64_t)p->tw_host.size();
   push_table(p->tw_host, p->P[2], p->M + 1);

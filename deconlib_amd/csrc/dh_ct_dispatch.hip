@@ -25,6 +25,7 @@ bool ct_supported(int n) {
 }
 
 int ct_threads(int n) { return n / ct::rmax(ct::make_sched(n)); }
+int ct_row_threads(int n) { return n / ct::rmax(ct::make_row_sched(n)); }
 
 hipError_t ct_launch(int kind, int n, int dir, const void* pk, int tile, int64_t nblocks, hipStream_t st) {
   using namespace ct;

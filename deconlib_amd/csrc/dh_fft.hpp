@@ -198,6 +198,8 @@ enum CtKind { CT_COL = 0, CT_Z = 1, CT_XF = 2, CT_XI = 3, CT_XR = 4, CT_PSF_Y = 
 bool ct_supported(int n);
 // Threads per line of the specialised kernels for length n.
 int ct_threads(int n);
+// Threads per row of the specialised X kernels (their row schedule).
+int ct_row_threads(int n);
 // Launches kind for length n; pk points to the matching Pack<...>.
 // tile = columns (CT_COL, CT_Z) or rows (CT_XF, CT_XI) per workgroup, plus
 // kTilePersistent for a persistent grid (one workgroup per resident slot, each

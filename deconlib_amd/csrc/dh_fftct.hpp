@@ -19,7 +19,7 @@ struct Sched {
 };
 
 // 16s first, then 8/4/2, then 5s and 3s (the same rule as the runtime plans).
-constexpr Sched make_sched(int n) {
+constexpr Sched default_sched(int n) {
   Sched s{0, {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
   int m = n, c2 = 0, c3 = 0, c5 = 0;
   while (m % 2 == 0) m /= 2, ++c2;
@@ -60,11 +60,44 @@ constexpr int rmax(Sched s) {
   return v;
 }
 
-template <int N>
+// Measured exceptions to the rule (one MI355X; others measured no better).
+// Columns (Y and Z passes): 1024 points as 16 x 8 x 8 (Y passes at 64x512x512 82.3 ->
+// 79.5 us and 85.3 -> 82.9 us; 8 x 8 x 16 no gain, 8 x 8 x 8 x 2 slower); 128 stays
+// 16 x 8 (8 x 16 and 8 x 8 x 2 slower), 256 stays 16 x 16 (8 x 8 x 4 slower).
+constexpr Sched make_sched(int n) {
+  if (n == 1024) return Sched{3, {16, 8, 8, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
+  return default_sched(n);
+}
+
+// Rows (the X passes) up to 2048 points: 8s first, then 4 or 2, then 5s and 3s.  A
+// row of 512 points as 8 x 8 x 8 runs 64 threads of 8 points instead of 32 of 16, so
+// an X workgroup holds half the registers per thread and the CU more workgroups (X
+// round trip at 64x512x512 103 -> 90 us, X forward 63 -> 52 us; 16 x 16 x 2 was slower,
+// 147 us); measured the same way 128 points as 8 x 8 x 2 (X forward 14.2 -> 11.8 us at
+// 64x256x128), 256 as 8 x 8 x 4 (27.7 -> 16.6 us) and 1024 as 8 x 8 x 8 x 2 (103 ->
+// 67 us at 32x512x1024), 2048 unchanged.  A column or Z pass of 512 points keeps
+// 16 x 8 x 4: as 8 x 8 x 8 the 256x2048x2048 Z pass took 32 ms instead of 17.
+constexpr Sched make_row_sched(int n) {
+  if (n > 2048) return default_sched(n);
+  Sched s{0, {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
+  int m = n, c2 = 0, c3 = 0, c5 = 0;
+  while (m % 2 == 0) m /= 2, ++c2;
+  while (m % 3 == 0) m /= 3, ++c3;
+  while (m % 5 == 0) m /= 5, ++c5;
+  for (int i = 0; i < c2 / 3; ++i) s.r[s.np++] = 8;
+  if (c2 % 3 == 2) s.r[s.np++] = 4;
+  if (c2 % 3 == 1) s.r[s.np++] = 2;
+  for (int i = 0; i < c5; ++i) s.r[s.np++] = 5;
+  for (int i = 0; i < c3; ++i) s.r[s.np++] = 3;
+  return s;
+}
+
+// ROW: the row (X) schedule, else the column (Y, Z) one.
+template <int N, bool ROW = false>
 struct Geo {
-  static constexpr Sched F = make_sched(N);
-  static constexpr Sched I = reversed(make_sched(N));
-  static constexpr int T = N / rmax(make_sched(N));  // threads per line
+  static constexpr Sched F = ROW ? make_row_sched(N) : make_sched(N);
+  static constexpr Sched I = reversed(F);
+  static constexpr int T = N / rmax(F);  // threads per line
 };
 
 // ---- per-pass twiddle tables ----
@@ -185,14 +218,15 @@ __device__ __forceinline__ void pass(int t, const float2* __restrict__ tw, Ld ld
 // LDS.  LAST_LDS: gst writes the same LDS the last pass reads, so that pass
 // needs the barrier between its loads and stores.
 // tw = the schedule's tables (pass rows in schedule order).
-template <int N, bool INV, int DIR, int p, bool LAST_LDS = false, class GLd, class GSt, class LLd, class LSt>
+template <int N, bool INV, int DIR, int p, bool LAST_LDS = false, bool ROW = false, class GLd, class GSt, class LLd,
+          class LSt>
 __device__ __forceinline__ void passes(int t, const float2* __restrict__ tw, GLd gld, GSt gst, LLd lld, LSt lst,
                                        bool first_from_global) {
-  constexpr Sched S = INV ? Geo<N>::I : Geo<N>::F;
+  constexpr Sched S = INV ? Geo<N, ROW>::I : Geo<N, ROW>::F;
   if constexpr (p < S.np) {
     constexpr int R = S.r[p];
     constexpr int P = span_of(S, p);
-    constexpr int T = Geo<N>::T;
+    constexpr int T = Geo<N, ROW>::T;
     constexpr bool last = (p == S.np - 1);
     const bool first = (p == 0) && first_from_global;
     constexpr int off = tw_pass_offset(S, p);
@@ -202,7 +236,7 @@ __device__ __forceinline__ void passes(int t, const float2* __restrict__ tw, GLd
     else if (last) pass<N, T, R, P, DIR>(t, twp, lld, gst, LAST_LDS);
     else pass<N, T, R, P, DIR>(t, twp, lld, lst, true);
     if (!last) __syncthreads();
-    passes<N, INV, DIR, p + 1, LAST_LDS>(t, tw, gld, gst, lld, lst, first_from_global);
+    passes<N, INV, DIR, p + 1, LAST_LDS, ROW>(t, tw, gld, gst, lld, lst, first_from_global);
   }
 }
 

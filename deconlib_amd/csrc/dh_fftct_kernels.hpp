@@ -585,7 +585,7 @@ __device__ __forceinline__ void blocked_rows_t(int thr, int64_t row0, int RPB, i
 // [M, 2M].  One transform length per launch, so one copy serves every volume.
 template <int M>
 __device__ __forceinline__ void stage_x_tables(float2* ltw, const float2* twm, const float2* twp) {
-  constexpr int TWM = tw_table_size(Geo<M>::F);
+  constexpr int TWM = tw_table_size(Geo<M, true>::F);
   static_assert(TWM <= M, "the pass tables fit the LDS slot reserved for them");
   for (int i = threadIdx.x; i < 2 * M + 1; i += blockDim.x) {
     if (i < TWM) ltw[i] = twm[i];
@@ -658,8 +658,8 @@ __device__ __forceinline__ float2 xf_post(const float2* lr, const float2* twp, i
 template <int M>
 __global__ void __launch_bounds__(kMaxThreads) k_xf(Pack<XfArgs> pk, int RPB) {
   extern __shared__ float2 lds[];
-  constexpr int T = Geo<M>::T;
-  constexpr Sched S = Geo<M>::F;
+  constexpr int T = Geo<M, true>::T;
+  constexpr Sched S = Geo<M, true>::F;
   constexpr int R0 = S.np > 0 ? S.r[0] : 1;
   constexpr int RS = xrow_stride(M);
   const int rl0 = threadIdx.x / T;
@@ -709,7 +709,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_xf(Pack<XfArgs> pk, int RPB) {
     if (bn < total) fetch(bn);
     __syncthreads();
     // remaining passes, in LDS (the last one reads and writes the row: LAST_LDS)
-    passes<M, false, -1, 1, true>(t, twm, lld, lst, lld, lst, false);
+    passes<M, false, -1, 1, true, true>(t, twm, lld, lst, lld, lst, false);
     __syncthreads();
     if (a.blk > 0) {
       int thr = threadIdx.x;
@@ -733,7 +733,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_xf(Pack<XfArgs> pk, int RPB) {
 // visits) and plain rows (ceil(H/T)).
 template <int M>
 struct XiEpt {
-  static constexpr int value = (M + kBlk + Geo<M>::T - 1) / Geo<M>::T;
+  static constexpr int value = (M + kBlk + Geo<M, true>::T - 1) / Geo<M, true>::T;
 };
 
 // Loads (q) or places into LDS (!load) this thread's spectrum samples of the
@@ -743,9 +743,9 @@ struct XiEpt {
 // registers allow it use it, see k_xi).
 template <int M, bool STEP>
 __device__ __forceinline__ void xi_tile(const XiArgs& a, int64_t row0, int RPB,
-                                        float2 (&q)[(M + kBlk + Geo<M>::T - 1) / Geo<M>::T], float2* lds, bool load,
+                                        float2 (&q)[(M + kBlk + Geo<M, true>::T - 1) / Geo<M, true>::T], float2* lds, bool load,
                                         int thr) {
-  constexpr int T = Geo<M>::T;
+  constexpr int T = Geo<M, true>::T;
   constexpr int RS = xrow_stride(M);
   const int H = a.H;
   if (STEP && T % kBlk == 0 && a.blk > 0) {
@@ -840,7 +840,7 @@ __device__ __forceinline__ void xi_tile(const XiArgs& a, int64_t row0, int RPB,
 template <int M, bool RT, bool BOX = false>
 __device__ __forceinline__ void xi_body(const XiArgs& a, int64_t lb, int RPB, float2* lds, const float2* twm,
                                         const float2* twp, int thr) {
-  constexpr int T = Geo<M>::T;
+  constexpr int T = Geo<M, true>::T;
   const int rl = thr / T;
   const int t = thr % T;
   const int64_t row = lb * RPB + rl;
@@ -890,12 +890,12 @@ __device__ __forceinline__ void xi_body(const XiArgs& a, int64_t lb, int RPB, fl
   } else {
     // the first pass reads the LDS row and overwrites it: full barrier between;
     // so does the last one (LAST_LDS)
-    constexpr Sched S = Geo<M>::F;
+    constexpr Sched S = Geo<M, true>::F;
     constexpr int R0 = S.r[0];
     pass<M, T, R0, 1, 1>(t, twm, pre, lst, true);
     if constexpr (S.np > 1) {
       __syncthreads();
-      passes<M, false, 1, 1, true>(t, twm, pre, lst, lld, lst, false);
+      passes<M, false, 1, 1, true, true>(t, twm, pre, lst, lld, lst, false);
     }
   }
   __syncthreads();
@@ -966,7 +966,7 @@ __device__ __forceinline__ void xi_body(const XiArgs& a, int64_t lb, int RPB, fl
     auto lld2 = [&](int e) { return lrow2[xrow_lds(e)]; };
     auto lst2 = [&](int e, float2 v) { lrow2[xrow_lds(e)] = v; };
     __syncthreads();
-    passes<M, false, -1, 0, true>(t2, twm, lld2, lst2, lld2, lst2, false);
+    passes<M, false, -1, 0, true, true>(t2, twm, lld2, lst2, lld2, lst2, false);
     __syncthreads();
     if (a.blk > 0) {
       int thr3 = thr;
@@ -1179,7 +1179,8 @@ __global__ void __launch_bounds__(kMaxThreads) k_psf_x(Pack<PsfXArgs> pk, int R)
 
 template <int N>
 struct Launch {
-  static constexpr int T = Geo<N>::T;
+  static constexpr int T = Geo<N>::T;         // column / Z passes
+  static constexpr int TX = Geo<N, true>::T;  // X passes (row schedule)
   // one workgroup per tile, or a persistent grid
   static unsigned grid(const void* k, int threads, size_t shm, int64_t nblocks, bool persist) {
     return persist ? persistent_grid(k, threads, shm, nblocks) : (unsigned)nblocks;
@@ -1225,7 +1226,7 @@ struct Launch {
     const size_t shm = (size_t)xrow_lds_elems(N, R) * sizeof(float2);
     const void* k = (const void*)k_xf<N>;
     (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    hipLaunchKernelGGL((k_xf<N>), dim3(grid(k, R * T, shm, nblocks, persist)), dim3(R * T), shm, st, pk, R);
+    hipLaunchKernelGGL((k_xf<N>), dim3(grid(k, R * TX, shm, nblocks, persist)), dim3(R * TX), shm, st, pk, R);
     return hipGetLastError();
   }
   static hipError_t xi(const Pack<XiArgs>& pk, int R, int64_t nblocks, bool persist, hipStream_t st) {
@@ -1233,7 +1234,7 @@ struct Launch {
     if (shm < 16 * sizeof(double)) shm = 16 * sizeof(double);
     const void* k = (const void*)k_xi<N, false>;
     (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    hipLaunchKernelGGL((k_xi<N, false>), dim3(grid(k, R * T, shm, nblocks, persist)), dim3(R * T), shm, st, pk, R);
+    hipLaunchKernelGGL((k_xi<N, false>), dim3(grid(k, R * TX, shm, nblocks, persist)), dim3(R * TX), shm, st, pk, R);
     return hipGetLastError();
   }
   static hipError_t psfy(const Pack<PsfYArgs>& pk, int L, int64_t nblocks, hipStream_t st) {
@@ -1260,10 +1261,10 @@ struct Launch {
     const void* k = boxed ? (const void*)k_xi<N, true, true> : (const void*)k_xi<N, true, false>;
     (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (boxed)
-      hipLaunchKernelGGL((k_xi<N, true, true>), dim3(grid(k, R * T, shm, nblocks, persist)), dim3(R * T), shm, st,
+      hipLaunchKernelGGL((k_xi<N, true, true>), dim3(grid(k, R * TX, shm, nblocks, persist)), dim3(R * TX), shm, st,
                          pk, R);
     else
-      hipLaunchKernelGGL((k_xi<N, true, false>), dim3(grid(k, R * T, shm, nblocks, persist)), dim3(R * T), shm, st,
+      hipLaunchKernelGGL((k_xi<N, true, false>), dim3(grid(k, R * TX, shm, nblocks, persist)), dim3(R * TX), shm, st,
                          pk, R);
     return hipGetLastError();
   }
