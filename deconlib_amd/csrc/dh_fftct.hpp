@@ -62,10 +62,12 @@ constexpr int rmax(Sched s) {
 
 // Measured exceptions to the rule (one MI355X; others measured no better).
 // Columns (Y and Z passes): 1024 points as 16 x 8 x 8 (Y passes at 64x512x512 82.3 ->
-// 79.5 us and 85.3 -> 82.9 us; 8 x 8 x 16 no gain, 8 x 8 x 8 x 2 slower); 128 stays
-// 16 x 8 (8 x 16 and 8 x 8 x 2 slower), 256 stays 16 x 16 (8 x 8 x 4 slower).
+// 79.5 us and 85.3 -> 82.9 us; 8 x 8 x 16 no gain, 8 x 8 x 8 x 2 slower); 64 as 8 x 8
+// (Z pass of a 64x128x128 canvas 9.6 -> 6.8 us; 4 x 16 8.1 us); 128 stays 16 x 8
+// (8 x 16 and 8 x 8 x 2 slower), 256 stays 16 x 16 (8 x 8 x 4 slower).
 constexpr Sched make_sched(int n) {
   if (n == 1024) return Sched{3, {16, 8, 8, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
+  if (n == 64) return Sched{2, {8, 8, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
   return default_sched(n);
 }
 
