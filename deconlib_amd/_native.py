@@ -21,11 +21,11 @@ if not _LIB_PATH.exists():
 
 lib = C.CDLL(str(_LIB_PATH))
 
-ABI_VERSION = 10
+ABI_VERSION = 11
 
 DH_OK, DH_EINVAL, DH_EHIP, DH_EWORKSPACE, DH_EUNSUPPORTED = 0, -1, -2, -3, -4
 
-PRO_NONE, PRO_ONES, PRO_SUB_FLOOR = 0, 1, 2
+PRO_NONE, PRO_ONES, PRO_SUB_FLOOR, PRO_SPREAD, PRO_PD_Y1 = 0, 1, 2, 3, 4
 EPI_NONE, EPI_FLOOR, EPI_RL_RATIO, EPI_RL_UPDATE, EPI_ADD, EPI_SQDIFF = 0, 1, 2, 3, 4, 5
 RED_SUM, RED_MAX, RED_SUMSQ = 0, 1, 2
 REG_HESSIAN, REG_GRADIENT, REG_IDENTITY = 0, 1, 2
@@ -58,6 +58,11 @@ class FuseArgs(C.Structure):
         ("aux0", C.c_void_p),
         ("aux1", C.c_void_p),
         ("reduce_out", C.c_void_p),
+        ("spread", C.c_int32 * 3),
+        ("pro_loss", C.c_int32),
+        ("pro_aux", C.c_void_p * 3),
+        ("pro_out", C.c_void_p),
+        ("pro_scalars", C.c_void_p),
     ]
 
 

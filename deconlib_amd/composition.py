@@ -48,6 +48,14 @@ class Compose:
         return self.outer.forward(self.inner.forward(x))
 
     def adjoint(self, y: Any) -> Any:
+        # FractionalAreaDownsample after a convolution: the integer-factor spread runs
+        # inside the correlation's first pass (same values as the two steps)
+        spread = getattr(self.outer, "spread_factors", None)
+        fused = getattr(self.inner, "adjoint_spread", None)
+        if spread is not None and fused is not None:
+            f = spread(y)
+            if f is not None:
+                return fused(y, f)
         return self.inner.adjoint(self.outer.adjoint(y))
 
     def __call__(self, x: Any) -> Any:

@@ -9,7 +9,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
-from typing import Tuple, Union
+from typing import Optional, Tuple, Union
 
 import numpy as np
 import torch
@@ -245,6 +245,17 @@ class FractionalAreaDownsample:
             i_f, w_f, _, _ = self._get_band(nl, ns)
             r = _apply_banded(r, i_f, w_f, a)
         return r
+
+    def spread_factors(self, y) -> Optional[Tuple[int, int, int]]:
+        """Integer factors (z, y, x of the volume promoted to 3-D) when the adjoint of
+        ``y`` is a unit block spread with an x factor of 1 or 2 -- then a convolution can
+        read ``y`` through its spread prologue instead of a materialised fine volume --
+        else None."""
+        if not isinstance(y, torch.Tensor) or y.dim() > 3:
+            return None
+        pairs = self.axis_pairs(y.dim(), y.shape, False)
+        f = self._block_factors(pairs, y.dim()) if pairs else None
+        return f if f is not None and f[2] <= 2 else None
 
     def adjoint(self, y) -> torch.Tensor:
         r = as_f32(y)

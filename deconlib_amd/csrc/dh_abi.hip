@@ -291,8 +291,10 @@ CtPasses ct_passes(const dh_conv_plan* p, int adjoint, const float* in, float* o
   c.t_xf = ct_row_tile(p->M, tile_override(p, DH_PASS_X_FWD), nrows);
   c.n_xf = (nrows + c.t_xf - 1) / c.t_xf;
   c.t_xf |= persist_flag(p, DH_PASS_X_FWD);
-  c.xf = dh::XfArgs{in, B1, tw + p->tw_xm, tw + p->tw_xp, nrows, s2, s2, 0, (int)H, pro, xf_vec2(in, s2),
-                    dh::kBlk, s1};
+  int vec2 = xf_vec2(in, s2);
+  if (pro.kind == dh::PRO_PD_Y1)
+    vec2 = vec2 && aligned8(pro.ax) && aligned8(pro.axp) && aligned8(pro.obs) && aligned8(pro.y1w);
+  c.xf = dh::XfArgs{in, B1, tw + p->tw_xm, tw + p->tw_xp, nrows, s2, s2, 0, (int)H, pro, vec2, dh::kBlk, s1};
   c.has_y = P1 > 1;
   if (c.has_y) {
     c.t_yf = ct_col_tile(P1, H, tile_override(p, DH_PASS_Y_FWD), COL_BLOCKED);
@@ -791,6 +793,45 @@ int dh_otf_build_dev(const dh_conv_plan* p, const float* kernel_dev, int normali
   return otf_build(p, kernel_dev, normalize, otf_dev, work_dev, work_bytes, nullptr, norm_sq_dev, stream);
 }
 
+}  // extern "C"
+
+namespace {
+
+// DH_PRO_SPREAD set-up: the factors must divide the plan's signal extents (the
+// input is then the coarse volume), the x factor is 1 or 2, and the rows of one
+// signal fit 32 bits.
+// DH_PRO_PD_Y1 set-up: operands present, a specialised canvas.
+int prologue_setup(const dh_conv_plan* p, const dh_fuse_args* fuse, dh::Pro& pro) {
+  if (pro.kind == DH_PRO_PD_Y1) {
+    if (!p->ct) return fail(DH_EINVAL, "the y1 prologue needs a canvas with specialised kernels");
+    if (!fuse->pro_aux[0] || !fuse->pro_aux[1] || !fuse->pro_aux[2] || !fuse->pro_out || !fuse->pro_scalars)
+      return fail(DH_EINVAL, "the y1 prologue needs pro_aux[0..2], pro_out and pro_scalars");
+    if (fuse->pro_loss != 0 && fuse->pro_loss != 1) return fail(DH_EINVAL, "pro_loss must be 0 or 1");
+    pro.ax = fuse->pro_aux[0];
+    pro.axp = fuse->pro_aux[1];
+    pro.obs = fuse->pro_aux[2];
+    pro.y1w = fuse->pro_out;
+    pro.sc = fuse->pro_scalars;
+    pro.loss = fuse->pro_loss;
+    return DH_OK;
+  }
+  if (pro.kind != DH_PRO_SPREAD) return DH_OK;
+  const int f0 = fuse->spread[0], f1 = fuse->spread[1], f2 = fuse->spread[2];
+  if (f0 < 1 || f1 < 1 || f2 < 1 || p->s[0] % f0 || p->s[1] % f1 || p->s[2] % f2)
+    return fail(DH_EINVAL, "spread factors must divide the signal extents");
+  if (f2 > 2) return fail(DH_EINVAL, "the spread prologue takes an x factor of 1 or 2");
+  if ((int64_t)p->s[0] * p->s[1] > INT32_MAX) return fail(DH_EINVAL, "spread prologue: too many rows");
+  pro.f0 = f0;
+  pro.f1 = f1;
+  pro.f2 = f2;
+  pro.rows1 = p->s[1];
+  return DH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int dh_conv_apply(const dh_conv_plan* p, int adjoint, const float* in, float* out, const void* otf_dev,
                   void* work_dev, int64_t work_bytes, const dh_fuse_args* fuse, void* stream) {
   if (!p || !out || !otf_dev || !work_dev) return fail(DH_EINVAL, "null argument");
@@ -809,7 +850,8 @@ int dh_conv_apply(const dh_conv_plan* p, int adjoint, const float* in, float* ou
     epi.b = fuse->epi_b;
     epi.aux0 = fuse->aux0;
     epi.aux1 = fuse->aux1;
-    if (pro.kind < DH_PRO_NONE || pro.kind > DH_PRO_SUB_FLOOR) return fail(DH_EINVAL, "unknown prologue");
+    if (pro.kind < DH_PRO_NONE || pro.kind > DH_PRO_PD_Y1) return fail(DH_EINVAL, "unknown prologue");
+    DH_TRY(prologue_setup(p, fuse, pro));
     if (epi.kind < DH_EPI_NONE || epi.kind > DH_EPI_SQDIFF) return fail(DH_EINVAL, "unknown epilogue");
     if ((epi.kind == DH_EPI_RL_RATIO || epi.kind == DH_EPI_RL_UPDATE || epi.kind == DH_EPI_SQDIFF) && !epi.aux0)
       return fail(DH_EINVAL, "epilogue needs aux0");
@@ -1311,7 +1353,7 @@ int fill_fuse(const dh_fuse_args* fuse, dh::Pro& pro, dh::Epi& epi, double** red
   epi.b = fuse->epi_b;
   epi.aux0 = fuse->aux0;
   epi.aux1 = fuse->aux1;
-  if (pro.kind < DH_PRO_NONE || pro.kind > DH_PRO_SUB_FLOOR) return fail(DH_EINVAL, "unknown prologue");
+  if (pro.kind < DH_PRO_NONE || pro.kind > DH_PRO_PD_Y1) return fail(DH_EINVAL, "unknown prologue");
   if (epi.kind < DH_EPI_NONE || epi.kind > DH_EPI_SQDIFF) return fail(DH_EINVAL, "unknown epilogue");
   if ((epi.kind == DH_EPI_RL_RATIO || epi.kind == DH_EPI_RL_UPDATE || epi.kind == DH_EPI_SQDIFF) && !epi.aux0)
     return fail(DH_EINVAL, "epilogue needs aux0");
@@ -1392,6 +1434,7 @@ int ct_batch_apply(int nitems, const dh_batch_item* items, int adjoint, hipStrea
     dh::Epi epi;
     double* red = nullptr;
     DH_TRY(fill_fuse(&it.fuse, pro, epi, &red));
+    DH_TRY(prologue_setup(p, &it.fuse, pro));
     if (!it.in && pro.kind != DH_PRO_ONES) return fail(DH_EINVAL, "null input");
     char* ws = (char*)it.work;
     if (red) epi.partials = (double*)(ws + p->part_off);

@@ -297,14 +297,14 @@ __device__ __forceinline__ void x_r2c_body(const AxisSched& s, const float2* __r
   const bool ok = active && row < nrows;
   const int P2 = 2 * m;
   const int shift = P2 - lo - hi;
-  const float* sr = src + row * rowlen;
+  const float* sr = pro_row(pro, src, ok ? row : 0, rowlen);
   float2* lrow = lds + rl * (m + 1);
   auto sample = [&](int q) -> float {
     int c = -1;
     if (q < lo) c = q;
     else if (q >= P2 - hi) c = q - shift;
     if (!ok || c < 0) return 0.0f;
-    return apply_pro(pro, sr, c);
+    return apply_pro(pro, sr, pro_col(pro, c));
   };
   auto gld = [&](int e) -> float2 { return make_float2(sample(2 * e), sample(2 * e + 1)); };
   auto lld = [&](int e) { return lrow[e]; };

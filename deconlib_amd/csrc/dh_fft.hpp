@@ -21,14 +21,40 @@ struct AxisSched {
 
 // Fused prologue (input sample of the forward X pass) and epilogue (cropped
 // output sample of the inverse X pass).
-enum ProKind { PRO_NONE = 0, PRO_ONES = 1, PRO_SUB_FLOOR = 2, PRO_DIV_DEV = 100 };
+enum ProKind { PRO_NONE = 0, PRO_ONES = 1, PRO_SUB_FLOOR = 2, PRO_SPREAD = 3, PRO_PD_Y1 = 4, PRO_DIV_DEV = 100 };
 enum EpiKind { EPI_NONE = 0, EPI_FLOOR = 1, EPI_RL_RATIO = 2, EPI_RL_UPDATE = 3, EPI_ADD = 4, EPI_SQDIFF = 5 };
 
 struct Pro {
   int kind;
   float a, b;
   const double* dptr;  // PRO_DIV_DEV: divide by (float)*dptr
+  // PRO_SPREAD: sample (z, y, x) of the input is src[z / f0, y / f1, x / f2] of a
+  // coarse volume; rows1 = input rows per plane (the fine grid's y extent).
+  int f0 = 1, f1 = 1, f2 = 1, rows1 = 1;
+  // PRO_PD_Y1: the sample is PDHG's data-term dual step y1' = prox(y1 + sigma A xbar)
+  // of y1 = src and the same sample of A x_k (ax), A x_{k-1} (axp) and the data (obs),
+  // with sigma and theta from the float64 scalar block sc and the background in a;
+  // y1' is also stored to y1w (which may be src).  loss: 0 Poisson, 1 Gaussian.
+  const float* ax = nullptr;
+  const float* axp = nullptr;
+  const float* obs = nullptr;
+  float* y1w = nullptr;
+  const double* sc = nullptr;
+  int loss = 0;
 };
+
+// First sample of input row `row` (plane-major rows of length rowlen): the row
+// itself, or under PRO_SPREAD the coarse row it replicates (rows < 2^31).
+__host__ __device__ __forceinline__ const float* pro_row(const Pro& p, const float* src, int64_t row, int rowlen) {
+  if (p.kind != PRO_SPREAD) return src + row * rowlen;
+  const unsigned r = (unsigned)row, n1 = (unsigned)p.rows1;
+  const unsigned z = r / n1, y = r - z * n1;
+  return src + (int64_t)((z / p.f0) * (n1 / p.f1) + y / p.f1) * (rowlen / p.f2);
+}
+// Offset within that row of row sample c.
+__host__ __device__ __forceinline__ int pro_col(const Pro& p, int c) {
+  return p.kind == PRO_SPREAD ? c / p.f2 : c;
+}
 
 struct Epi {
   int kind;

@@ -9,6 +9,7 @@
 #include <type_traits>
 
 #include "dh_fftct.hpp"
+#include "dh_prox.hpp"
 
 namespace dh {
 namespace ct {
@@ -29,15 +30,6 @@ __device__ __forceinline__ int pick(const Pack<A>& pk, int64_t b) {
   for (int i = 1; i < pk.nvol; ++i)
     if (pk.prefix[i] <= b) v = i;
   return v;
-}
-
-__device__ __forceinline__ float pro_apply(const Pro& p, const float* src, int64_t idx) {
-  switch (p.kind) {
-    case PRO_ONES: return 1.0f;
-    case PRO_SUB_FLOOR: return fmaxf(src[idx] - p.a, p.b);
-    case PRO_DIV_DEV: return src[idx] / (float)(*p.dptr);
-    default: return src[idx];
-  }
 }
 
 __device__ __forceinline__ float idiv(float o, float mdl) {
@@ -597,16 +589,68 @@ __device__ __forceinline__ void stage_x_tables(float2* ltw, const float2* twm, c
 // Complex sample e (real samples 2e, 2e+1 of the zero-padded row of length 2M,
 // prologue applied) of row `row` of volume a.
 // PRO_DIV_DEV (the OTF build's kernel / sum) is applied afterwards by xf_div.
-template <int M>
-__device__ __forceinline__ float2 xf_ld(const XfArgs& a, int64_t row, int e) {
+// EXT: 0 the plain prologues; 1 also PRO_SPREAD; 2 also PRO_PD_Y1.  A launch takes
+// the smallest instantiation that covers its pack, so the others keep lean code.
+// Row `row` of volume a: ok = row < nrows, rb = row * rowlen (its offset in
+// signal-shaped arrays), sr = its first source sample (xf_row).
+template <int EXT>
+__device__ __forceinline__ const float* xf_row(const XfArgs& a, int64_t row, bool ok) {
+  if constexpr (EXT > 0) return pro_row(a.pro, a.src, ok ? row : 0, a.rowlen);
+  return a.src + row * a.rowlen;
+}
+
+template <int M, int EXT>
+__device__ __forceinline__ float2 xf_ld(const XfArgs& a, const float* sr, int64_t rb, bool ok, int e) {
   constexpr int P2 = 2 * M;
-  const bool ok = row < a.nrows;
   const int kind = a.pro.kind;
   // PRO_ONES never reads the source (which may then be null)
   const bool read = kind != PRO_ONES;
-  const float* sr = a.src + row * a.rowlen;
   float2 raw = make_float2(0.f, 0.f);
   bool ok0, ok1;
+  if constexpr (EXT > 0) {
+    if (kind == PRO_SPREAD) {  // coarse rows (hi == 0), each sample replicated f2 (1 or 2) times
+      ok0 = ok && 2 * e < a.lo;
+      ok1 = ok && 2 * e + 1 < a.lo;
+      if (a.pro.f2 == 2) {
+        if (ok0) raw.x = raw.y = __builtin_nontemporal_load(sr + e);
+      } else {
+        if (ok0) raw.x = __builtin_nontemporal_load(sr + 2 * e);
+        if (ok1) raw.y = __builtin_nontemporal_load(sr + 2 * e + 1);
+      }
+      return raw;
+    }
+    if (EXT > 1 && kind == PRO_PD_Y1) {  // convolution input rows (hi == 0)
+      const PdY1Consts kc = pd_y1_consts(a.pro.sc, a.pro.a);
+      const int64_t i0 = rb + 2 * e;
+      ok0 = ok && 2 * e < a.lo;
+      ok1 = ok && 2 * e + 1 < a.lo;
+      if (a.vec2) {  // every operand row 8-byte aligned, even row length: ok0 == ok1
+        if (ok0) {
+          const float2 y = ld_nt(reinterpret_cast<const float2*>(a.src + i0));
+          const float2 ax = ld_nt(reinterpret_cast<const float2*>(a.pro.ax + i0));
+          const float2 axp = ld_nt(reinterpret_cast<const float2*>(a.pro.axp + i0));
+          const float2 d = ld_nt(reinterpret_cast<const float2*>(a.pro.obs + i0));
+          raw = make_float2(pd_y1_prox(y.x, ax.x, axp.x, d.x, kc, a.pro.loss),
+                            pd_y1_prox(y.y, ax.y, axp.y, d.y, kc, a.pro.loss));
+          st_nt(reinterpret_cast<float2*>(a.pro.y1w + i0), raw);
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          if (j == 0 ? ok0 : ok1) {
+            const float v = pd_y1_prox(__builtin_nontemporal_load(a.src + i0 + j),
+                                       __builtin_nontemporal_load(a.pro.ax + i0 + j),
+                                       __builtin_nontemporal_load(a.pro.axp + i0 + j),
+                                       __builtin_nontemporal_load(a.pro.obs + i0 + j), kc, a.pro.loss);
+            __builtin_nontemporal_store(v, a.pro.y1w + i0 + j);
+            if (j == 0) raw.x = v;
+            else raw.y = v;
+          }
+        }
+      }
+      return raw;
+    }
+  }
   if (a.vec2) {  // hi == 0, even row length, 8-byte aligned rows
     ok0 = ok1 = ok && 2 * e + 1 < a.lo;
     if (ok0 && read) raw = ld_nt(reinterpret_cast<const float2*>(sr + 2 * e));
@@ -655,7 +699,7 @@ __device__ __forceinline__ float2 xf_post(const float2* lr, const float2* twp, i
 // Persistent: workgroups walk tiles of RPB rows, prefetching the next tile's
 // samples while the current tile's passes and stores run.  Both twiddle tables
 // sit in LDS after the rows (one length per launch, so one copy per workgroup).
-template <int M>
+template <int M, int EXT>
 __global__ void __launch_bounds__(kMaxThreads) k_xf(Pack<XfArgs> pk, int RPB) {
   extern __shared__ float2 lds[];
   constexpr int T = Geo<M, true>::T;
@@ -682,10 +726,13 @@ __global__ void __launch_bounds__(kMaxThreads) k_xf(Pack<XfArgs> pk, int RPB) {
     const int v = pick(pk, bb);
     const XfArgs& a = pk.a[v];
     const int64_t row = (bb - pk.prefix[v]) * RPB + rl;
+    const bool ok = row < a.nrows;
+    const int64_t rb = row * a.rowlen;
+    const float* sr = xf_row<EXT>(a, row, ok);
     if constexpr (M == 1) {
-      u[0][0] = xf_ld<M>(a, row, 0);
+      u[0][0] = xf_ld<M, EXT>(a, sr, rb, ok, 0);
     } else {
-      pass_load<M, T, R0>(u, t, [&](int e) { return xf_ld<M>(a, row, e); });
+      pass_load<M, T, R0>(u, t, [&](int e) { return xf_ld<M, EXT>(a, sr, rb, ok, e); });
     }
     xf_div<M, T, R0>(u, a.pro);
   };
@@ -1224,9 +1271,15 @@ struct Launch {
   }
   static hipError_t xf(const Pack<XfArgs>& pk, int R, int64_t nblocks, bool persist, hipStream_t st) {
     const size_t shm = (size_t)xrow_lds_elems(N, R) * sizeof(float2);
-    const void* k = (const void*)k_xf<N>;
+    int ext = 0;
+    for (int i = 0; i < pk.nvol; ++i)
+      ext = std::max(ext, pk.a[i].pro.kind == PRO_PD_Y1 ? 2 : (pk.a[i].pro.kind == PRO_SPREAD ? 1 : 0));
+    const void* k = ext == 2 ? (const void*)k_xf<N, 2> : ext == 1 ? (const void*)k_xf<N, 1> : (const void*)k_xf<N, 0>;
     (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    hipLaunchKernelGGL((k_xf<N>), dim3(grid(k, R * TX, shm, nblocks, persist)), dim3(R * TX), shm, st, pk, R);
+    const dim3 g(grid(k, R * TX, shm, nblocks, persist)), blk(R * TX);
+    if (ext == 2) hipLaunchKernelGGL((k_xf<N, 2>), g, blk, shm, st, pk, R);
+    else if (ext == 1) hipLaunchKernelGGL((k_xf<N, 1>), g, blk, shm, st, pk, R);
+    else hipLaunchKernelGGL((k_xf<N, 0>), g, blk, shm, st, pk, R);
     return hipGetLastError();
   }
   static hipError_t xi(const Pack<XiArgs>& pk, int R, int64_t nblocks, bool persist, hipStream_t st) {

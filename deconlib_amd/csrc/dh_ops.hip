@@ -330,11 +330,54 @@ __global__ void k_block_sum(const float* __restrict__ in, float* __restrict__ ou
   }
 }
 
+// k_block_sum for factors 2 x 2 x 2 and an output row length divisible by 2: a thread
+// forms two outputs from four 16-byte loads (rows (2 o0 + a, 2 o1 + b), a, b in {0, 1}),
+// adding in k_block_sum's order, so the values are the same bit for bit.
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+__global__ void k_block_sum2(const float* __restrict__ in, float* __restrict__ out, int64_t rows, int n1, int n2,
+                             int m1, int m2) {
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;  // output pair
+  if (2 * h >= m2) return;
+  for (int64_t r = blockIdx.y; r < rows; r += gridDim.y) {
+    const int64_t o0 = r / m1;
+    const int o1 = (int)(r - o0 * m1);
+    const float* p = in + ((2 * o0) * n1 + 2 * (int64_t)o1) * n2 + 4 * (int64_t)h;
+    const int64_t pl = (int64_t)n1 * n2;
+    const f4v v00 = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(p));            // a 0, b 0
+    const f4v v10 = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(p + pl));       // a 1, b 0
+    const f4v v01 = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(p + n2));       // a 0, b 1
+    const f4v v11 = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(p + pl + n2));  // a 1, b 1
+    f2v o;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      float r3 = 0.0f;
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const int i = 2 * q + c;
+        float r2 = 0.0f;
+        float r1 = 0.0f;
+        r1 += v00[i];
+        r1 += v10[i];
+        r2 += r1;
+        r1 = 0.0f;
+        r1 += v01[i];
+        r1 += v11[i];
+        r2 += r1;
+        r3 += r2;
+      }
+      o[q] = r3;
+    }
+    __builtin_nontemporal_store(o, reinterpret_cast<f2v*>(out + r * m2 + 2 * h));
+  }
+}
+
 // Its adjoint (the banded adjoint passes with one unit tap each): in[z, y, x] of the fine
 // grid = coarse[z / f0, y / f1, x / f2].  A thread writes four consecutive samples (one
 // 16-byte store) of a fine row; rows are dealt out grid-stride.
+// vec: fine 16-byte and coarse 8-byte aligned.
 __global__ void k_block_spread(const float* __restrict__ coarse, float* __restrict__ fine, int64_t rows, int n1,
-                               int n2, int m1, int m2, int f0, int f1, int f2) {
+                               int n2, int m1, int m2, int f0, int f1, int f2, int vec) {
   const int x = 4 * (blockIdx.x * blockDim.x + threadIdx.x);
   if (x >= n2) return;
   for (int64_t r = blockIdx.y; r < rows; r += gridDim.y) {  // fine rows (z, y)
@@ -342,8 +385,11 @@ __global__ void k_block_spread(const float* __restrict__ coarse, float* __restri
     const int y = (int)(r - z * n1);
     const float* c = coarse + ((z / f0) * m1 + y / f1) * (int64_t)m2;
     float* d = fine + r * n2 + x;
-    if (x + 3 < n2 && (n2 & 3) == 0) {
-      typedef float f4v __attribute__((ext_vector_type(4)));
+    if (vec && f2 == 2 && (n2 & 3) == 0) {  // x + 3 < n2; one 8-byte load
+      const f2v w = *reinterpret_cast<const f2v*>(c + (x >> 1));
+      const f4v v = {w[0], w[0], w[1], w[1]};
+      __builtin_nontemporal_store(v, reinterpret_cast<f4v*>(d));
+    } else if (vec && x + 3 < n2 && (n2 & 3) == 0) {
       const f4v v = {c[x / f2], c[(x + 1) / f2], c[(x + 2) / f2], c[(x + 3) / f2]};
       __builtin_nontemporal_store(v, reinterpret_cast<f4v*>(d));
     } else {
@@ -357,6 +403,12 @@ hipError_t launch_block_sum(const float* in, float* out, int n0, int n1, int n2,
   const int m1 = n1 / f1, m2 = n2 / f2;
   const int64_t rows = (int64_t)(n0 / f0) * m1;
   const unsigned gy = (unsigned)std::min<int64_t>(rows, 65535);
+  if (f0 == 2 && f1 == 2 && f2 == 2 && (m2 & 1) == 0 && (reinterpret_cast<uintptr_t>(in) & 15) == 0 &&
+      (reinterpret_cast<uintptr_t>(out) & 7) == 0) {
+    hipLaunchKernelGGL(k_block_sum2, dim3((unsigned)((m2 / 2 + 127) / 128), gy), dim3(128), 0, st, in, out, rows,
+                       n1, n2, m1, m2);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(k_block_sum, dim3((unsigned)((m2 + 127) / 128), gy), dim3(128), 0, st, in, out, rows, n1, n2,
                      m1, m2, f0, f1, f2);
   return hipGetLastError();
@@ -370,8 +422,9 @@ hipError_t launch_block_spread(const float* coarse, float* fine, int n0, int n1,
   const unsigned gx = (unsigned)((quads + 127) / 128);
   // about 8 rows per workgroup row slot: enough workgroups to fill the GPU, few enough to amortise them
   const unsigned gy = (unsigned)std::max<int64_t>(1, std::min<int64_t>((rows + 7) / 8, 65535));
+  const int vec = (reinterpret_cast<uintptr_t>(fine) & 15) == 0 && (reinterpret_cast<uintptr_t>(coarse) & 7) == 0;
   hipLaunchKernelGGL(k_block_spread, dim3(gx, gy), dim3(128), 0, st, coarse, fine, rows, n1, n2, m1, m2, f0, f1,
-                     f2);
+                     f2, vec);
   return hipGetLastError();
 }
 

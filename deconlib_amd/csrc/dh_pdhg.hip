@@ -12,6 +12,7 @@
 #include <algorithm>
 
 #include "dh_pdhg.hpp"
+#include "dh_prox.hpp"
 #include "dh_stencil.hpp"
 
 namespace dh {
@@ -91,33 +92,9 @@ static int pd_blocks(int64_t n) {
 __global__ void k_pd_y1(float* __restrict__ y1, const float* __restrict__ ax, const float* __restrict__ axp,
                         const float* __restrict__ data, int64_t n, const double* __restrict__ sc, float bg,
                         int loss_type) {
-  const double sigma = sc[PD_SIGMA], theta = sc[PD_THETA];
-  const float sf = (float)sigma, th = (float)theta;
-  const float one_pth = (float)(1.0 + theta);
-  const float sb = (float)(sigma * (double)bg), s4 = (float)(4.0 * sigma), s2 = (float)(2.0 * sigma);
-  const float inv1ps = (float)(1.0 + sigma);
-  PD_LOOP(i, n) {
-    // A xbar by linearity: (1 + theta) A x_k - theta A x_{k-1}
-    const float axb = one_pth * ldnt(ax + i) - th * ldnt(axp + i);
-    const float arg = ldnt(y1 + i) + sf * axb;
-    const float d = ldnt(data + i);
-    float out;
-    if (loss_type == 0) {
-      const float c = 1.0f - arg - sb;
-      const float ds = fmaxf(d, 0.0f);
-      const float disc = c * c + s4 * ds;
-      const float den = sqrtf(disc) - c + 1e-12f;
-      float z = (s2 * ds) / den;
-      if (ds < 1e-6f) z = fmaxf(c, 1e-6f);
-      z = fmaxf(z, 1e-6f);
-      out = 1.0f - z;
-    } else {
-      out = (arg - sf * (d - bg)) / inv1ps;
-    }
-    stnt(y1 + i, out);
-  }
+  const PdY1Consts k = pd_y1_consts(sc, bg);
+  PD_LOOP(i, n) stnt(y1 + i, pd_y1_prox(ldnt(y1 + i), ldnt(ax + i), ldnt(axp + i), ldnt(data + i), k, loss_type));
 }
-
 
 // The dual and primal updates write their multiply-adds as fmaf (xbar, y2 + sigma H xbar,
 // the L1_2 norm's squares, x - tau grad), so that the fused sweep below and these

@@ -384,8 +384,27 @@ class FFTConvolver:
         y = as_f32(y)
         return self._plan.apply(y, True, empty_f32(self.shape))
 
+    def adjoint_spread(self, y, factors) -> torch.Tensor:
+        """adjoint(spread(y)) in one correlation (see LinearFFTConvolver.adjoint_spread)."""
+        return _adjoint_spread(self._plan, self.shape, y, factors)
+
     def __call__(self, x) -> torch.Tensor:
         return self.forward(x)
+
+
+def _adjoint_spread(plan: _ConvPlan, shape, y, factors) -> torch.Tensor:
+    """Correlation of the integer-factor block spread of ``y`` (each coarse sample
+    repeated f0 x f1 x f2 times, the adjoint of FractionalAreaDownsample's block sum,
+    core_operators.py:205-219) with the plan's kernel: DH_PRO_SPREAD."""
+    y = as_f32(y)
+    f = tuple(int(v) for v in factors)
+    full = (1,) * (3 - len(shape)) + tuple(int(v) for v in shape)
+    got = (1,) * (3 - y.dim()) + tuple(int(v) for v in y.shape)
+    if len(f) != 3 or y.dim() != len(shape) or any(n != g * q for n, g, q in zip(full, got, f)):
+        raise ValueError(f"spread of input {tuple(y.shape)} by {f} does not give the signal shape {tuple(shape)}")
+    fz = N.FuseArgs(prologue=N.PRO_SPREAD)
+    fz.spread[:] = f
+    return plan.apply(y, True, empty_f32(tuple(shape)), fz)
 
 
 class LinearFFTConvolver:
@@ -443,6 +462,12 @@ class LinearFFTConvolver:
         y = as_f32(y)
         self._check(y)
         return self._plan.apply(y, True, empty_f32(self.signal_shape))
+
+    def adjoint_spread(self, y, factors) -> torch.Tensor:
+        """adjoint(spread(y)) for a coarse ``y`` whose integer-factor block spread
+        (FractionalAreaDownsample's adjoint) is the signal: one correlation whose first
+        pass reads ``y`` through the spread prologue (the fine volume never exists)."""
+        return _adjoint_spread(self._plan, self.signal_shape, y, factors)
 
     def _check(self, x: torch.Tensor) -> None:
         if tuple(x.shape) != self.signal_shape:

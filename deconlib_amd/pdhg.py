@@ -218,6 +218,9 @@ class _PDState:
         if self.fused:
             self.xn = empty_f32(self.S)
             self.y2n = torch.empty_like(self.y2)
+        # the y1 dual step runs in the adjoint's first pass (DH_PRO_PD_Y1) on canvases with
+        # specialised kernels
+        self.y1_pro = self._plan is not None and int(self._plan.info.spectrum_block) > 0
         if self._plan is not None:
             # the step sizes start from the OTF norm on the device: no host round trip
             self.sc = torch.empty(N.PD_NSLOTS, dtype=torch.float64, device=dev)
@@ -267,13 +270,14 @@ class _PDState:
 
     # one iteration in five phases, so that a batch of solves can group its convolutions
     def _pre(self, k: int) -> None:
-        """y1 dual step, and the y2 dual step unless it is fused with the primal one
-        (pdhg_mlx.py:717-734)."""
+        """y1 dual step (unless the adjoint's first pass runs it), and the y2 dual step
+        unless it is fused with the primal one (pdhg_mlx.py:717-734)."""
         st = N.stream_ptr()
         nz, ny, nx = self.D3
         sc = self.sc.data_ptr()
-        N.call("dh_pd_y1", self.y1.data_ptr(), self.Ax.data_ptr(), self.Axp.data_ptr(), self.data.data_ptr(),
-               self.data.numel(), sc, self.bg, self.lt, st)
+        if not self.y1_pro:
+            N.call("dh_pd_y1", self.y1.data_ptr(), self.Ax.data_ptr(), self.Axp.data_ptr(), self.data.data_ptr(),
+                   self.data.numel(), sc, self.bg, self.lt, st)
         if not self.fused:
             N.call("dh_pd_reg_dual", self.kind, self.y2.data_ptr(), self.x.data_ptr(), self.xp.data_ptr(), nz, ny,
                    nx, sc, self.wp, self.alpha, self.l12, st)
@@ -291,6 +295,17 @@ class _PDState:
             N.call("dh_pd_reg_primal", self.kind, self.xp.data_ptr(), self.x.data_ptr(), self.g.data_ptr(),
                    self.y2.data_ptr(), nz, ny, nx, self.sc.data_ptr(), self.wp, self.part.data_ptr(),
                    N.stream_ptr())
+
+    def _adj_fuse(self) -> Optional[N.FuseArgs]:
+        """Prologue of the adjoint A^T y1 on a plan with specialised kernels: the y1
+        dual step (dh_pd_y1's values) on each sample as the first pass reads it, y1
+        updated in place."""
+        if not self.y1_pro:
+            return None
+        f = N.FuseArgs(prologue=N.PRO_PD_Y1, pro_a=self.bg, pro_loss=self.lt, pro_out=self.y1.data_ptr(),
+                       pro_scalars=self.sc.data_ptr())
+        f.pro_aux[:] = (self.Ax.data_ptr(), self.Axp.data_ptr(), self.data.data_ptr())
+        return f
 
     def _fwd_fuse(self, want_loss: bool) -> Optional[N.FuseArgs]:
         """Epilogue of the forward convolution A x_new on a plan: |A x_new - A x|^2 summed
@@ -325,14 +340,18 @@ class _PDState:
     def _conv_item(self, adjoint: bool, want_loss: bool = False) -> N.BatchItem:
         p = self._plan
         src, dst = (self.y1, self.g) if adjoint else (self.x_next, self.Axp)
-        fuse = None if adjoint else self._fwd_fuse(want_loss)
+        fuse = self._adj_fuse() if adjoint else self._fwd_fuse(want_loss)
         return N.BatchItem(plan=p._h, otf=p.otf_buf.data_ptr(), work=p.work.data_ptr(),
                            work_bytes=int(p.info.work_bytes), input=src.data_ptr(), out=dst.data_ptr(),
                            fuse=fuse if fuse is not None else N.FuseArgs())
 
     def step(self, k: int, want_loss: bool) -> None:
         self._pre(k)
-        self.g = self._adj(self.y1, self.g)
+        afz = self._adj_fuse()
+        if afz is not None:
+            self.g = self._plan.apply(self.y1, True, self.g, afz)
+        else:
+            self.g = self._adj(self.y1, self.g)
         self._mid(k)
         fuse = self._fwd_fuse(want_loss)
         if fuse is not None:

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define DH_ABI_VERSION 10
+#define DH_ABI_VERSION 11
 
 #define DH_OK 0
 #define DH_EINVAL (-1)
@@ -60,6 +60,17 @@ typedef struct {
 #define DH_PRO_NONE 0        /* v = in[i]                                 */
 #define DH_PRO_ONES 1        /* v = 1            (RL sensitivity A^T 1)   */
 #define DH_PRO_SUB_FLOOR 2   /* v = max(in[i] - a, b)  (RL max(obs-bg,1e-6)) */
+#define DH_PRO_SPREAD 3      /* v[z,y,x] = in[z/f0, y/f1, x/f2], (f0,f1,f2) = spread[], f2 1 or 2: the input is the
+                              * coarse volume (signal/f per axis) of FractionalAreaDownsample's
+                              * integer-factor adjoint (core_operators.py:205-219, its banded passes
+                              * with unit taps), so compose(Downsample, Convolver)'s adjoint is one
+                              * application; convolution plans only */
+#define DH_PRO_PD_Y1 4       /* v = y1' = prox(in + sigma ((1 + theta) pro_aux[0] - theta pro_aux[1]))
+                              * with data pro_aux[2] (PDHG's data-term dual step, pdhg_mlx.py:115-189,
+                              * 717-723: the same values as dh_pd_y1), y1' also stored to pro_out
+                              * (which may be in); sigma and theta from pro_scalars (dh_pd_* block),
+                              * pro_a = background, pro_loss 0 Poisson / 1 Gaussian.  Plans whose
+                              * canvas has specialised kernels only (dh_plan_info.spectrum_block > 0) */
 
 /* Fused epilogue applied to each cropped output sample v of the inverse pass. */
 #define DH_EPI_NONE 0        /* out = v                                   */
@@ -78,6 +89,11 @@ typedef struct {
   const float* aux0;         /* epilogue operand (observed / x), S-shaped */
   const float* aux1;         /* epilogue operand (sensitivity), S-shaped  */
   double* reduce_out;        /* device; if non-NULL the epilogue's sum is written here */
+  int32_t spread[3];         /* DH_PRO_SPREAD factors (z, y, x of the signal promoted to 3-D) */
+  int32_t pro_loss;          /* DH_PRO_PD_Y1 loss: 0 Poisson, 1 Gaussian */
+  const float* pro_aux[3];   /* DH_PRO_PD_Y1 operands: A x_k, A x_{k-1}, data (signal-shaped) */
+  float* pro_out;            /* DH_PRO_PD_Y1: y1' (signal-shaped; may be the input) */
+  const double* pro_scalars; /* DH_PRO_PD_Y1: device scalar block of the dh_pd_* calls */
 } dh_fuse_args;
 
 int dh_abi_version(void);

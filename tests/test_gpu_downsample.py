@@ -24,7 +24,7 @@ def _banded_chain(op, x, forward):
 
 @pytest.mark.parametrize("shape,scale", [
     ((16, 24, 32), 2.0), ((16, 24, 32), (1.0, 2.0, 2.0)), ((9, 12, 10), (3.0, 1.0, 2.0)), ((20, 36), (2.0, 3.0)),
-    ((128, 512, 512), 2.0),
+    ((128, 512, 512), 2.0), ((8, 10, 18), 2.0),
 ])
 def test_block_sum_equals_banded_passes(gpu, shape, scale):
     from deconlib_amd import FractionalAreaDownsample
@@ -47,6 +47,24 @@ def test_block_sum_equals_banded_passes(gpu, shape, scale):
         np.testing.assert_allclose(back.cpu().numpy(), ref.adjoint(y.cpu().numpy().astype(np.float64)), rtol=1e-6)
 
 
+def test_block_sum_and_spread_on_unaligned_views(gpu):
+    """Inputs and outputs 4 bytes off a 16-byte boundary take the scalar paths."""
+    from deconlib_amd import FractionalAreaDownsample
+    from deconlib_amd import _native as N
+
+    rng = np.random.default_rng(72)
+    op = FractionalAreaDownsample(2.0)
+    base = torch.from_numpy(rng.standard_normal(16 * 24 * 32 + 1).astype(np.float32)).cuda()
+    x = base[1:].view(16, 24, 32)
+    assert x.data_ptr() % 16 == 4
+    assert torch.equal(op.forward(x), _banded_chain(op, x, True))
+    cb = torch.from_numpy(rng.standard_normal(8 * 12 * 16 + 1).astype(np.float32)).cuda()
+    y = cb[1:].view(8, 12, 16)
+    fine = torch.empty(16 * 24 * 32 + 1, device="cuda")[1:].view(16, 24, 32)
+    N.call("dh_block_spread", y.data_ptr(), fine.data_ptr(), 16, 24, 32, 2, 2, 2, N.stream_ptr())
+    assert torch.equal(fine, _banded_chain(op, y, False))
+
+
 def test_non_integer_scale_keeps_banded_passes(gpu):
     from deconlib_amd import FractionalAreaDownsample
 
@@ -56,3 +74,71 @@ def test_non_integer_scale_keeps_banded_passes(gpu):
     ref = O.FractionalAreaDownsample(1.325)
     np.testing.assert_allclose(op.forward(x).cpu().numpy(), ref.forward(x.cpu().numpy().astype(np.float64)),
                                rtol=1e-5)
+
+
+@pytest.mark.parametrize("shape,kshape,scale", [
+    ((16, 24, 32), (5, 7, 9), 2.0),
+    ((16, 24, 32), (3, 3, 3), (1.0, 2.0, 2.0)),
+    ((9, 12, 30), (4, 5, 6), (3.0, 1.0, 1.0)),   # x factor 1
+    ((21, 36), (5, 7), (3.0, 2.0)),
+    ((7000,), (3001,), 2.0),                      # canvas 10000: runtime-length kernels
+    ((128, 512, 512), (33, 65, 65), 2.0),         # the binned configuration's object grid
+])
+def test_composed_adjoint_spreads_in_the_first_pass(gpu, shape, kshape, scale):
+    """compose(FractionalAreaDownsample, convolver).adjoint reads the coarse input
+    through DH_PRO_SPREAD: bit for bit the block spread followed by the correlation,
+    through dh_conv_apply and dh_batch_conv_apply; the float64 oracle on small cases."""
+    from deconlib_amd import FFTConvolver, FractionalAreaDownsample, LinearFFTConvolver, compose
+    from deconlib_amd import _native as N
+
+    print(f"spread prologue {shape} * {kshape} scale {scale}", flush=True)
+    rng = np.random.default_rng(71)
+    k = (rng.random(kshape) + 0.05).astype(np.float32)
+    conv = LinearFFTConvolver(k, shape)
+    ds = FractionalAreaDownsample(scale)
+    A = compose(ds, conv)
+    coarse = tuple(int(v) for v in ds.forward(torch.zeros(shape, device="cuda")).shape)
+    y = torch.from_numpy(rng.standard_normal(coarse).astype(np.float32)).cuda()
+    f = ds.spread_factors(y)
+    assert f is not None
+    two = conv.adjoint(ds.adjoint(y))
+    got = A.adjoint(y)
+    assert torch.equal(got, two)
+    p = conv._plan
+    out_b = torch.empty_like(two)
+    fz = N.FuseArgs(prologue=N.PRO_SPREAD)
+    fz.spread[:] = f
+    item = N.BatchItem(plan=p._h, otf=p.otf_buf.data_ptr(), work=p.work.data_ptr(),
+                       work_bytes=int(p.info.work_bytes), input=y.data_ptr(), out=out_b.data_ptr(), fuse=fz)
+    N.call("dh_batch_conv_apply", 1, (N.BatchItem * 1)(item), 1, N.stream_ptr())
+    assert torch.equal(out_b, two)
+    if np.prod(shape) < 1e5:
+        conv64 = O.LinearFFTConvolver(k.astype(np.float64), shape, dtype=np.float64)
+        ref = O.compose(O.FractionalAreaDownsample(scale), conv64)
+        want = ref.adjoint(y.cpu().numpy().astype(np.float64))
+        g = got.cpu().numpy().astype(np.float64)
+        assert np.linalg.norm(g - want) <= 1e-5 * np.linalg.norm(want)
+    if len(shape) == 3 and np.prod(shape) < 1e5:
+        circ = FFTConvolver(np.pad(k, [(0, s - q) for s, q in zip(shape, kshape)]))
+        assert torch.equal(compose(ds, circ).adjoint(y), circ.adjoint(ds.adjoint(y)))
+
+
+def test_spread_prologue_rejects_bad_factors(gpu):
+    import ctypes as C
+
+    from deconlib_amd import FractionalAreaDownsample, LinearFFTConvolver
+    from deconlib_amd import _native as N
+
+    conv = LinearFFTConvolver(np.ones((3, 3, 3), np.float32), (8, 12, 16))
+    p = conv._plan
+    y = torch.zeros((4, 6, 8), device="cuda")
+    out = torch.empty((8, 12, 16), device="cuda")
+    for f in [(2, 2, 3), (0, 2, 2), (3, 1, 1), (1, 1, 4)]:
+        fz = N.FuseArgs(prologue=N.PRO_SPREAD)
+        fz.spread[:] = f
+        assert N.fns["dh_conv_apply"](p._h, 1, y.data_ptr(), out.data_ptr(), p.otf_buf.data_ptr(), p.work.data_ptr(),
+                                      int(p.info.work_bytes), C.byref(fz), N.stream_ptr()) == N.DH_EINVAL
+    with pytest.raises(ValueError):
+        conv.adjoint_spread(torch.zeros((4, 6, 9), device="cuda"), (2, 2, 2))
+    # an x factor above 2 keeps the two steps
+    assert FractionalAreaDownsample((1.0, 1.0, 4.0)).spread_factors(torch.zeros((8, 12, 4), device="cuda")) is None

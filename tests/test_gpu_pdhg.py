@@ -151,3 +151,87 @@ def test_fused_dual_primal_equals_separate_kernels(gpu, shape, l12):
     mag = y2_b.norm(dim=0) if l12 else y2_b.abs()
     assert float(mag.max()) <= alpha * (1 + 1e-5)
     assert float((mag >= alpha * 0.999).float().mean()) > 0
+
+
+@pytest.mark.parametrize("shape,off", [((16, 24, 32), 0), ((5, 7, 33), 0), ((6, 10, 40), 1), ((64, 128, 128), 0)])
+@pytest.mark.parametrize("loss", [0, 1])
+def test_y1_prologue_equals_separate_kernel(gpu, shape, off, loss):
+    """DH_PRO_PD_Y1: the adjoint's first pass runs the y1 dual step.  y1' and A^T y1'
+    equal dh_pd_y1 followed by the plain adjoint bit for bit, through dh_conv_apply and
+    dh_batch_conv_apply (row pairs as 8-byte loads; odd rows and operands 4 bytes off an
+    8-byte boundary take the scalar loads; data with zeros takes the D ~ 0 branch)."""
+    from deconlib_amd import LinearFFTConvolver
+    from deconlib_amd import _native as N
+
+    print(f"y1 prologue {shape} off={off} loss={loss}", flush=True)
+    rng = np.random.default_rng(90 + loss)
+    n = int(np.prod(shape))
+    dev = "cuda"
+
+    def arr(v):
+        t = torch.empty(n + off, dtype=torch.float32, device=dev)[off:].view(shape)
+        t.copy_(torch.from_numpy(v.astype(np.float32)))
+        return t
+
+    k = (rng.random((5, 5, 7)) + 0.05).astype(np.float32)
+    op = LinearFFTConvolver(k, shape)
+    p = op._plan
+    assert int(p.info.spectrum_block) > 0
+    y1 = arr(rng.standard_normal(shape) * 0.3)
+    ax = arr(rng.random(shape) * 5)
+    axp = arr(rng.random(shape) * 5)
+    d = rng.poisson(3.0, shape).astype(np.float32)
+    d[rng.random(shape) < 0.2] = 0.0
+    data = arr(d)
+    sc = torch.zeros(16, dtype=torch.float64, device=dev)
+    sc[0], sc[1], sc[2] = 0.4, 0.6, 0.85
+    bg = 0.5
+    st = N.stream_ptr()
+    # separate
+    y1_a = y1.clone()
+    N.call("dh_pd_y1", y1_a.data_ptr(), ax.data_ptr(), axp.data_ptr(), data.data_ptr(), n, sc.data_ptr(), bg, loss, st)
+    g_a = op.adjoint(y1_a)
+    # fused, in place and through the batch entry point into another buffer
+    y1_b = arr(y1.cpu().numpy())
+    g_b = torch.full(shape, float("nan"), device=dev)
+
+    def fz(out):
+        f = N.FuseArgs(prologue=N.PRO_PD_Y1, pro_a=bg, pro_loss=loss, pro_out=out.data_ptr(),
+                       pro_scalars=sc.data_ptr())
+        f.pro_aux[:] = (ax.data_ptr(), axp.data_ptr(), data.data_ptr())
+        return f
+
+    p.apply(y1_b, True, g_b, fz(y1_b))
+    y1_c = arr(np.full(shape, np.nan))
+    g_c = torch.full(shape, float("nan"), device=dev)
+    item = N.BatchItem(plan=p._h, otf=p.otf_buf.data_ptr(), work=p.work.data_ptr(),
+                       work_bytes=int(p.info.work_bytes), input=y1.data_ptr(), out=g_c.data_ptr(), fuse=fz(y1_c))
+    N.call("dh_batch_conv_apply", 1, (N.BatchItem * 1)(item), 1, st)
+    torch.cuda.synchronize()
+    assert torch.equal(y1_a, y1_b) and torch.equal(y1_a, y1_c)
+    assert torch.equal(g_a, g_b) and torch.equal(g_a, g_c)
+    # the prox moved y1 (and the Poisson branch's floor held)
+    assert not torch.equal(y1_a, y1)
+    if loss == 0:
+        assert float(y1_a.max()) <= 1.0 - 1e-6
+
+
+def test_y1_prologue_rejects_missing_operands(gpu):
+    import ctypes as C
+
+    from deconlib_amd import LinearFFTConvolver
+    from deconlib_amd import _native as N
+
+    op = LinearFFTConvolver(np.ones((3, 3, 3), np.float32), (8, 12, 16))
+    p = op._plan
+    y = torch.zeros((8, 12, 16), device="cuda")
+    out = torch.empty_like(y)
+    sc = torch.zeros(16, dtype=torch.float64, device="cuda")
+    good = [y.data_ptr()] * 3
+    for aux, po, ps, lt in [(good, 0, sc.data_ptr(), 0), ([y.data_ptr(), 0, y.data_ptr()], y.data_ptr(),
+                                                         sc.data_ptr(), 0),
+                            (good, y.data_ptr(), 0, 0), (good, y.data_ptr(), sc.data_ptr(), 2)]:
+        f = N.FuseArgs(prologue=N.PRO_PD_Y1, pro_loss=lt, pro_out=po or None, pro_scalars=ps or None)
+        f.pro_aux[:] = [a or None for a in aux]
+        assert N.fns["dh_conv_apply"](p._h, 1, y.data_ptr(), out.data_ptr(), p.otf_buf.data_ptr(), p.work.data_ptr(),
+                                      int(p.info.work_bytes), C.byref(f), N.stream_ptr()) == N.DH_EINVAL
