@@ -119,13 +119,14 @@ def bench_pdhg(torch, N, obs, blur, steps, warmup):
     }
 
 
-def bench_pdhg_binned(torch, steps=10, warmup=2, seed=1001):
+def bench_pdhg_binned(torch, steps=10, warmup=2, seed=1001, tune=False, tune_cache=None):
     """The binned PDHG configuration at full size: object 128x512x512 (2000 blobs + 10),
     widefield PSF of the same optics at 0.0425/0.05 um (synthesised on the GPU), data
     64x256x256 = FractionalAreaDownsample(2) o LinearFFTConvolver (canvas 256x1024x1024),
     Poisson at peak ~500 counts; PDHG-Hessian (alpha 1e-3, L1) iterations from the start
     solve_pdhg_mlx(bin_factors=2) uses.  Voxel-iterations count the 128x512x512 object."""
     from deconlib_amd import FractionalAreaDownsample, LinearFFTConvolver, compose
+    from deconlib_amd import _native as N
     from deconlib_amd.pdhg import _PDState, _nonneg
     from deconlib_amd.psf import widefield_psf
     from deconlib_amd.synth import WIDEFIELD_OPTICS, blob_volume
@@ -134,6 +135,11 @@ def bench_pdhg_binned(torch, steps=10, warmup=2, seed=1001):
     psf = widefield_psf(hi, 0.0425, 0.05, WIDEFIELD_OPTICS, device=True)
     obj = blob_volume(hi, 2000, np.random.default_rng(seed)).astype(np.float32)
     conv = LinearFFTConvolver(psf, hi)
+    tiles = {}
+    if tune:  # launch shapes of this canvas's passes, as for the metric shape
+        from deconlib_amd.autotune import autotune
+
+        tiles = autotune(conv, verbose=True, cache_file=tune_cache)
     A = compose(FractionalAreaDownsample(scale=(2.0, 2.0, 2.0)), conv)
     clean = A.forward(torch.from_numpy(obj).cuda()).clamp_min(0.0)
     gen = torch.Generator(device="cuda")
@@ -156,6 +162,7 @@ def bench_pdhg_binned(torch, steps=10, warmup=2, seed=1001):
         "steps": steps,
         "warmup": warmup,
         "canvas": list(conv.canvas_shape),
+        "tiles": {N.PASS_NAMES[p]: t for p, t in tiles.items()},
         "data_shape": list(obs.shape),
         "finite": bool(torch.isfinite(st.x).all().item()),
     }
@@ -964,7 +971,7 @@ def main():
     ok = bool(torch.isfinite(st.x).all().item())
     pdhg = bench_pdhg(torch, N, obs, blur, args.steps, args.warmup)
     log(f"pdhg: {json.dumps(pdhg)}")
-    pdhg["binned_configuration"] = bench_pdhg_binned(torch)
+    pdhg["binned_configuration"] = bench_pdhg_binned(torch, tune=not args.no_tune, tune_cache=args.tune_cache)
     log(f"pdhg binned configuration: {json.dumps(pdhg['binned_configuration'])}")
     pdhg["large_volume"] = bench_large_volume(torch)
     log(f"pdhg large volume: {json.dumps(pdhg['large_volume'])}")

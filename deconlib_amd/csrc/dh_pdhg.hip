@@ -728,38 +728,54 @@ __global__ void __launch_bounds__(kTileThreads, 6) k_hess_adj_tiled(const float*
   }
 }
 
+constexpr int kZCS = 32;  // planes per workgroup of the sums
+// Planes rotate through four LDS slots, so one barrier per plane suffices: the slot
+// a plane's store reuses was last read two planes earlier, before the previous
+// barrier.  Plane loads run two planes ahead of the computed one.
 __global__ void __launch_bounds__(kTileThreads) k_pd_hess_sums(const float* __restrict__ a,
                                                                const float* __restrict__ b, Dims d, W6 w,
                                                                int want_abs, double* __restrict__ part) {
-  __shared__ float pdl[3 * kPlane], pa[3 * kPlane];
+  __shared__ float pdl[4 * kPlane], pa[4 * kPlane];
   __shared__ double sh[16];
-  const Tile t = tile_of(d);
+  const Tile t = tile_of(d, kZCS);
   const PlanePos pp(d, t);
+  auto slot = [&](int z) { return (z - t.z0 + 1) & 3; };
   double s_sq = 0.0, s_abs = 0.0;
   float a0, a1, b0, b1;
   for (int zz = t.z0 - 1; zz <= t.z0; ++zz) {
     const int64_t bz = pp.base(d, zz);
     pp.load(a, bz, a0, a1);
     pp.load(b, bz, b0, b1);
-    pp.store(pdl + slot_of(zz, t.z0) * kPlane, a0 - b0, a1 - b1);
-    if (want_abs) pp.store(pa + slot_of(zz, t.z0) * kPlane, a0, a1);
+    pp.store(pdl + slot(zz) * kPlane, a0 - b0, a1 - b1);
+    if (want_abs) pp.store(pa + slot(zz) * kPlane, a0, a1);
   }
+  // planes z + 1 (r1) and z + 2 (r2) in registers
+  float r1a0, r1a1, r1b0, r1b1, r2a0 = 0.f, r2a1 = 0.f, r2b0 = 0.f, r2b1 = 0.f;
   {
     const int64_t bz = pp.base(d, t.z0 + 1);
-    pp.load(a, bz, a0, a1);
-    pp.load(b, bz, b0, b1);
+    pp.load(a, bz, r1a0, r1a1);
+    pp.load(b, bz, r1b0, r1b1);
+  }
+  if (t.z0 + 2 <= t.z1) {
+    const int64_t bz = pp.base(d, t.z0 + 2);
+    pp.load(a, bz, r2a0, r2a1);
+    pp.load(b, bz, r2b0, r2b1);
   }
   for (int z = t.z0; z < t.z1; ++z) {
-    pp.store(pdl + slot_of(z + 1, t.z0) * kPlane, a0 - b0, a1 - b1);
-    if (want_abs) pp.store(pa + slot_of(z + 1, t.z0) * kPlane, a0, a1);
-    __syncthreads();
-    if (z + 1 < t.z1) {
-      const int64_t bz = pp.base(d, z + 2);
-      pp.load(a, bz, a0, a1);
-      pp.load(b, bz, b0, b1);
+    pp.store(pdl + slot(z + 1) * kPlane, r1a0 - r1b0, r1a1 - r1b1);
+    if (want_abs) pp.store(pa + slot(z + 1) * kPlane, r1a0, r1a1);
+    r1a0 = r2a0;
+    r1a1 = r2a1;
+    r1b0 = r2b0;
+    r1b1 = r2b1;
+    if (z + 3 <= t.z1) {
+      const int64_t bz = pp.base(d, z + 3);
+      pp.load(a, bz, r2a0, r2a1);
+      pp.load(b, bz, r2b0, r2b1);
     }
+    __syncthreads();
     if (t.in) {
-      auto fd = [&](int zz, int yy, int xx) { return lds_at(pdl, slot_of(zz, t.z0), t, yy, xx); };
+      auto fd = [&](int zz, int yy, int xx) { return lds_at(pdl, slot(zz), t, yy, xx); };
       float h[6];
       hess6_halo(fd, z, t.y, t.x, w.w, h);
       // the voxel's six squares summed in float, then accumulated in double
@@ -768,7 +784,7 @@ __global__ void __launch_bounds__(kTileThreads) k_pd_hess_sums(const float* __re
       for (int q = 0; q < 6; ++q) hh = fmaf(h[q], h[q], hh);
       s_sq += (double)hh;
       if (want_abs) {
-        auto fa = [&](int zz, int yy, int xx) { return lds_at(pa, slot_of(zz, t.z0), t, yy, xx); };
+        auto fa = [&](int zz, int yy, int xx) { return lds_at(pa, slot(zz), t, yy, xx); };
         hess6_halo(fa, z, t.y, t.x, w.w, h);
         float ss = 0.0f;
 #pragma unroll
@@ -779,7 +795,6 @@ __global__ void __launch_bounds__(kTileThreads) k_pd_hess_sums(const float* __re
         if (want_abs == 2) s_abs += (double)sqrtf(ss + 1e-12f);
       }
     }
-    __syncthreads();
   }
   const int64_t blk = ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
   double r0 = pd_block_sum(s_sq, sh);
@@ -1115,10 +1130,11 @@ hipError_t pd_hess_sums(const float* a, const float* b, int nz, int ny, int nx, 
   Dims d{nz, ny, nx};
   W6 ww;
   for (int q = 0; q < 6; ++q) ww.w[q] = w[q];
-  hipLaunchKernelGGL(k_pd_hess_sums, tile_grid(nz, ny, nx), dim3(kTileThreads), 0, st, a, b, d, ww, want_abs, part);
+  const dim3 grid = tile_grid(nz, ny, nx, kZCS);
+  hipLaunchKernelGGL(k_pd_hess_sums, grid, dim3(kTileThreads), 0, st, a, b, d, ww, want_abs, part);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  return finish2(part, tile_blocks(nz, ny, nx), sc + PD_DL2, loss_out, st);
+  return finish2(part, (int)(grid.x * grid.y * grid.z), sc + PD_DL2, loss_out, st);
 }
 
 hipError_t pd_data_sums(const float* a, const float* b, const float* data, int64_t n, float bg, int want_loss,
