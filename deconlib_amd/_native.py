@@ -26,7 +26,7 @@ ABI_VERSION = 11
 DH_OK, DH_EINVAL, DH_EHIP, DH_EWORKSPACE, DH_EUNSUPPORTED = 0, -1, -2, -3, -4
 
 PRO_NONE, PRO_ONES, PRO_SUB_FLOOR, PRO_SPREAD, PRO_PD_Y1 = 0, 1, 2, 3, 4
-EPI_NONE, EPI_FLOOR, EPI_RL_RATIO, EPI_RL_UPDATE, EPI_ADD, EPI_SQDIFF = 0, 1, 2, 3, 4, 5
+EPI_NONE, EPI_FLOOR, EPI_RL_RATIO, EPI_RL_UPDATE, EPI_ADD, EPI_SQDIFF, EPI_BLOCKSUM2 = 0, 1, 2, 3, 4, 5, 6
 RED_SUM, RED_MAX, RED_SUMSQ = 0, 1, 2
 REG_HESSIAN, REG_GRADIENT, REG_IDENTITY = 0, 1, 2
 STATIC_SCRATCH_BYTES = 128 << 20  # DH_STATIC_SCRATCH_BYTES

@@ -45,6 +45,15 @@ class Compose:
         self.operator_norm_sq = float(outer.operator_norm_sq * inner.operator_norm_sq)
 
     def forward(self, x: Any) -> Any:
+        # FractionalAreaDownsample(2) after a convolution: the 2x2x2 block sum runs in the
+        # convolution's last pass (same values as the two steps)
+        fac = getattr(self.outer, "sum_factors", None)
+        fused = getattr(self.inner, "forward_block_sum", None)
+        shape = getattr(self.inner, "out_shape", None)
+        if fac is not None and fused is not None and shape is not None and fac(shape) == (2, 2, 2):
+            r = fused(x)
+            if r is not None:
+                return r
         return self.outer.forward(self.inner.forward(x))
 
     def adjoint(self, y: Any) -> Any:

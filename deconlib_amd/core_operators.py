@@ -246,6 +246,15 @@ class FractionalAreaDownsample:
             r = _apply_banded(r, i_f, w_f, a)
         return r
 
+    def sum_factors(self, shape) -> Optional[Tuple[int, int, int]]:
+        """Integer factors (z, y, x promoted to 3-D) when the forward of a volume of
+        ``shape`` is a unit block sum, else None."""
+        shape = tuple(int(v) for v in shape)
+        if len(shape) > 3:
+            return None
+        pairs = self.axis_pairs(len(shape), shape, True)
+        return self._block_factors(pairs, len(shape)) if pairs else None
+
     def spread_factors(self, y) -> Optional[Tuple[int, int, int]]:
         """Integer factors (z, y, x of the volume promoted to 3-D) when the adjoint of
         ``y`` is a unit block spread with an x factor of 1 or 2 -- then a convolution can

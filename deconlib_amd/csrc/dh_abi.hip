@@ -315,9 +315,15 @@ CtPasses ct_passes(const dh_conv_plan* p, int adjoint, const float* in, float* o
   c.n_z = (ncols + c.t_z - 1) / c.t_z;
   c.t_z |= persist_flag(p, DH_PASS_Z);
   c.z = dh::ZArgs{c.has_y ? B2 : B1, (const float2*)otf, tw + p->tw_z, ncols, s0, s0, adjoint ? 1 : 0};
-  c.t_xi = ct_row_tile(p->M, tile_override(p, DH_PASS_X_INV), nrows);
-  c.n_xi = (nrows + c.t_xi - 1) / c.t_xi;
-  c.t_xi |= persist_flag(p, DH_PASS_X_INV);
+  if (epi.kind == dh::EPI_BLOCKSUM2) {  // k_xi_bs: four fine rows per coarse row, persistent
+    c.t_xi = 4;
+    c.n_xi = nrows / 4;
+    c.t_xi |= dh::kTilePersistent;
+  } else {
+    c.t_xi = ct_row_tile(p->M, tile_override(p, DH_PASS_X_INV), nrows);
+    c.n_xi = (nrows + c.t_xi - 1) / c.t_xi;
+    c.t_xi |= persist_flag(p, DH_PASS_X_INV);
+  }
   c.xi = dh::XiArgs{B1, out, nullptr, tw + p->tw_xm, tw + p->tw_xp, nrows, s2, (int)H, epi, xi_vec2(out, s2, epi),
                     dh::kBlk, s1};
   return c;
@@ -852,7 +858,13 @@ int dh_conv_apply(const dh_conv_plan* p, int adjoint, const float* in, float* ou
     epi.aux1 = fuse->aux1;
     if (pro.kind < DH_PRO_NONE || pro.kind > DH_PRO_PD_Y1) return fail(DH_EINVAL, "unknown prologue");
     DH_TRY(prologue_setup(p, fuse, pro));
-    if (epi.kind < DH_EPI_NONE || epi.kind > DH_EPI_SQDIFF) return fail(DH_EINVAL, "unknown epilogue");
+    if (epi.kind < DH_EPI_NONE || epi.kind > DH_EPI_BLOCKSUM2) return fail(DH_EINVAL, "unknown epilogue");
+    if (epi.kind == DH_EPI_BLOCKSUM2) {
+      if (!p->ct) return fail(DH_EINVAL, "the block-sum epilogue needs a canvas with specialised kernels");
+      if (p->s[0] % 2 || p->s[1] % 2 || p->s[2] % 2)
+        return fail(DH_EINVAL, "the block-sum epilogue needs even signal extents");
+      if (!ct_row_tile_ok(p->M, 4)) return fail(DH_EINVAL, "the block-sum epilogue's four-row tile does not fit");
+    }
     if ((epi.kind == DH_EPI_RL_RATIO || epi.kind == DH_EPI_RL_UPDATE || epi.kind == DH_EPI_SQDIFF) && !epi.aux0)
       return fail(DH_EINVAL, "epilogue needs aux0");
     if (epi.kind == DH_EPI_RL_UPDATE && !epi.aux1) return fail(DH_EINVAL, "epilogue needs aux1");

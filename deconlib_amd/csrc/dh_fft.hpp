@@ -22,7 +22,15 @@ struct AxisSched {
 // Fused prologue (input sample of the forward X pass) and epilogue (cropped
 // output sample of the inverse X pass).
 enum ProKind { PRO_NONE = 0, PRO_ONES = 1, PRO_SUB_FLOOR = 2, PRO_SPREAD = 3, PRO_PD_Y1 = 4, PRO_DIV_DEV = 100 };
-enum EpiKind { EPI_NONE = 0, EPI_FLOOR = 1, EPI_RL_RATIO = 2, EPI_RL_UPDATE = 3, EPI_ADD = 4, EPI_SQDIFF = 5 };
+enum EpiKind {
+  EPI_NONE = 0,
+  EPI_FLOOR = 1,
+  EPI_RL_RATIO = 2,
+  EPI_RL_UPDATE = 3,
+  EPI_ADD = 4,
+  EPI_SQDIFF = 5,
+  EPI_BLOCKSUM2 = 6  // out = 2x2x2 block sums of the cropped output (k_xi_bs)
+};
 
 struct Pro {
   int kind;

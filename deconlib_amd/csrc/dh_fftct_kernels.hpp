@@ -789,9 +789,10 @@ struct XiEpt {
 // STEP: the stepped blocked walk (fewer instructions per load; the kernels whose
 // registers allow it use it, see k_xi).
 template <int M, bool STEP>
+// nthr: threads that walk these rows (the workgroup, or half of it in k_xi_bs).
 __device__ __forceinline__ void xi_tile(const XiArgs& a, int64_t row0, int RPB,
                                         float2 (&q)[(M + kBlk + Geo<M, true>::T - 1) / Geo<M, true>::T], float2* lds, bool load,
-                                        int thr) {
+                                        int thr, int nthr) {
   constexpr int T = Geo<M, true>::T;
   constexpr int RS = xrow_stride(M);
   const int H = a.H;
@@ -842,7 +843,7 @@ __device__ __forceinline__ void xi_tile(const XiArgs& a, int64_t row0, int RPB,
     const int y0 = (int)(row0 - z0 * a.prow);
 #pragma unroll
     for (int i = 0; i < XiEpt<M>::value; ++i) {
-      const int f = thr + i * blockDim.x;
+      const int f = thr + i * nthr;
       const int j = f & (kBlk - 1);
       const int r = (f / kBlk) & (RPB - 1);
       const int bk = f >> (lg + 3);
@@ -884,7 +885,7 @@ __device__ __forceinline__ void xi_tile(const XiArgs& a, int64_t row0, int RPB,
 // writing the rows followed by an X forward pass reading them.
 // BOX: the epilogue's reduction honours a.epi.box (a separate instantiation, so
 // that launches without a box carry none of its registers).
-template <int M, bool RT, bool BOX = false>
+template <int M, bool RT, bool BOX = false, bool BS = false>
 __device__ __forceinline__ void xi_body(const XiArgs& a, int64_t lb, int RPB, float2* lds, const float2* twm,
                                         const float2* twp, int thr) {
   constexpr int T = Geo<M, true>::T;
@@ -946,6 +947,33 @@ __device__ __forceinline__ void xi_body(const XiArgs& a, int64_t lb, int RPB, fl
     }
   }
   __syncthreads();
+  if constexpr (BS) {
+    // k_xi_bs: LDS rows 0..3 hold fine rows (2z', 2y'), (2z', 2y'+1), (2z'+1, 2y'),
+    // (2z'+1, 2y'+1) of coarse row lb; output sample x' sums fine x = 2x', 2x'+1 (the
+    // .x and .y of LDS pair x') in k_block_sum's order: z innermost, then y, then x.
+    const int m2 = s2 / 2;
+    float* orow = a.dst + lb * m2;
+    for (int x = thr; x < m2; x += RPB * T) {
+      const int sl = xrow_lds(x);
+      const float2 v00 = lds[sl], v01 = lds[RS + sl], v10 = lds[2 * RS + sl], v11 = lds[3 * RS + sl];
+      float r3 = 0.0f;
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        float r2 = 0.0f;
+        float r1 = 0.0f;
+        r1 += c ? v00.y : v00.x;
+        r1 += c ? v10.y : v10.x;
+        r2 += r1;
+        r1 = 0.0f;
+        r1 += c ? v01.y : v01.x;
+        r1 += c ? v11.y : v11.x;
+        r2 += r1;
+        r3 += r2;
+      }
+      __builtin_nontemporal_store(r3, orow + x);
+    }
+    return;
+  }
   // The epilogue over this thread's sample pairs e = t, t + T, ... of its row:
   // one loop per epilogue kind, so that each kind's code appears once (not once
   // per pair of the unrolled transform) and four pairs' operand loads are in
@@ -1063,8 +1091,8 @@ __global__ void __launch_bounds__(kMaxThreads) k_xi(Pack<XiArgs> pk, int RPB) {
       const int v = pick(pk, b);
       const XiArgs& a = pk.a[v];
       const int64_t lb = b - pk.prefix[v];
-      xi_tile<M, kStep>(a, lb * RPB, RPB, q, lds, true, thr);
-      xi_tile<M, kStep>(a, lb * RPB, RPB, q, lds, false, thr);
+      xi_tile<M, kStep>(a, lb * RPB, RPB, q, lds, true, thr, blockDim.x);
+      xi_tile<M, kStep>(a, lb * RPB, RPB, q, lds, false, thr, blockDim.x);
       __syncthreads();
       xi_body<M, RT, BOX>(a, lb, RPB, lds, twm, twp, thr);
       __syncthreads();
@@ -1075,7 +1103,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_xi(Pack<XiArgs> pk, int RPB) {
     int thr = threadIdx.x;
     opaque(thr);
     const int v = pick(pk, b);
-    xi_tile<M, kStep>(pk.a[v], (b - pk.prefix[v]) * RPB, RPB, q, lds, true, thr);
+    xi_tile<M, kStep>(pk.a[v], (b - pk.prefix[v]) * RPB, RPB, q, lds, true, thr, blockDim.x);
   }
   for (;;) {
     int thr = threadIdx.x;
@@ -1083,16 +1111,67 @@ __global__ void __launch_bounds__(kMaxThreads) k_xi(Pack<XiArgs> pk, int RPB) {
     const int v = pick(pk, b);
     const XiArgs& a = pk.a[v];
     const int64_t lb = b - pk.prefix[v];
-    xi_tile<M, kStep>(a, lb * RPB, RPB, q, lds, false, thr);
+    xi_tile<M, kStep>(a, lb * RPB, RPB, q, lds, false, thr, blockDim.x);
     const int64_t bn = b + gridDim.x;
     if (bn < total) {
       int thr2 = threadIdx.x;
       opaque(thr2);
       const int vn = pick(pk, bn);
-      xi_tile<M, kStep>(pk.a[vn], (bn - pk.prefix[vn]) * RPB, RPB, q, lds, true, thr2);
+      xi_tile<M, kStep>(pk.a[vn], (bn - pk.prefix[vn]) * RPB, RPB, q, lds, true, thr2, blockDim.x);
     }
     __syncthreads();
     xi_body<M, RT>(a, lb, RPB, lds, twm, twp, thr);
+    if (bn >= total) break;
+    b = bn;
+    __syncthreads();
+  }
+}
+
+// ---- X inverse with the 2x2x2 block sum (FractionalAreaDownsample(2) after the
+// convolution, its block sum core_operators.py:205-219 fused): tile lb is coarse row
+// (z', y') = (lb / (prow/2), lb % (prow/2)); the first half of the workgroup loads fine
+// rows (2z', 2y'), (2z', 2y'+1) into LDS rows 0-1, the second half (2z'+1, 2y'),
+// (2z'+1, 2y'+1) into rows 2-3 (prow and the plane count even).  Persistent, with the
+// next tile's spectrum prefetched into registers as in k_xi.
+template <int M>
+__global__ void __launch_bounds__(kMaxThreads) k_xi_bs(Pack<XiArgs> pk, int RPB) {
+  extern __shared__ float2 lds[];
+  constexpr int T = Geo<M, true>::T;
+  constexpr int RS = xrow_stride(M);
+  const int64_t total = pk.prefix[pk.nvol];
+  int64_t b = blockIdx.x;
+  if (b >= total) return;
+  const float2* twm = pk.a[0].twm;
+  const float2* twp = pk.a[0].twp;
+  if constexpr (!xrow_tw_global(M)) {
+    float2* ltw = lds + RPB * RS;
+    stage_x_tables<M>(ltw, pk.a[0].twm, pk.a[0].twp);
+    twm = ltw;
+    twp = ltw + M;
+  }
+  float2 q[XiEpt<M>::value];
+  constexpr bool kStep = M <= 768;
+  const int half = threadIdx.x >= 2 * T ? 1 : 0;
+  auto tile = [&](int64_t bb, bool load) {
+    int thr = threadIdx.x - half * 2 * T;
+    opaque(thr);
+    const int v = pick(pk, bb);
+    const XiArgs& a = pk.a[v];
+    const int64_t lb = bb - pk.prefix[v];
+    const int64_t cy = a.prow / 2;
+    const int64_t zc = lb / cy, yc = lb - zc * cy;
+    xi_tile<M, kStep>(a, (2 * zc + half) * a.prow + 2 * yc, 2, q, lds + half * 2 * RS, load, thr, 2 * T);
+  };
+  tile(b, true);
+  for (;;) {
+    tile(b, false);
+    const int64_t bn = b + gridDim.x;
+    if (bn < total) tile(bn, true);
+    __syncthreads();
+    int thr = threadIdx.x;
+    opaque(thr);
+    const int v = pick(pk, b);
+    xi_body<M, false, false, true>(pk.a[v], b - pk.prefix[v], 4, lds, twm, twp, thr);
     if (bn >= total) break;
     b = bn;
     __syncthreads();
@@ -1285,6 +1364,13 @@ struct Launch {
   static hipError_t xi(const Pack<XiArgs>& pk, int R, int64_t nblocks, bool persist, hipStream_t st) {
     size_t shm = (size_t)xrow_lds_elems(N, R) * sizeof(float2);
     if (shm < 16 * sizeof(double)) shm = 16 * sizeof(double);
+    if (pk.a[0].epi.kind == EPI_BLOCKSUM2) {  // one volume per launch, four rows per tile
+      if (R != 4) return hipErrorInvalidValue;
+      const void* kb = (const void*)k_xi_bs<N>;
+      (void)hipFuncSetAttribute(kb, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+      hipLaunchKernelGGL((k_xi_bs<N>), dim3(grid(kb, R * TX, shm, nblocks, persist)), dim3(R * TX), shm, st, pk, R);
+      return hipGetLastError();
+    }
     const void* k = (const void*)k_xi<N, false>;
     (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     hipLaunchKernelGGL((k_xi<N, false>), dim3(grid(k, R * TX, shm, nblocks, persist)), dim3(R * TX), shm, st, pk, R);

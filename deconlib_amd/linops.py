@@ -463,6 +463,20 @@ class LinearFFTConvolver:
         self._check(y)
         return self._plan.apply(y, True, empty_f32(self.signal_shape))
 
+    def forward_block_sum(self, x) -> Optional[torch.Tensor]:
+        """FractionalAreaDownsample(2)(forward(x)) of a 3-D signal with even extents in one
+        application: the inverse X pass sums each 2x2x2 block (DH_EPI_BLOCKSUM2) and the
+        fine result never exists.  None where the plan cannot (odd extents, a canvas
+        without specialised kernels or with rows beyond 4096 samples)."""
+        s = self.signal_shape
+        if (len(s) != 3 or any(v % 2 for v in s) or int(self._plan.info.spectrum_block) <= 0
+                or self.canvas_shape[2] > 4096):
+            return None
+        x = as_f32(x)
+        self._check(x)
+        return self._plan.apply(x, False, empty_f32(tuple(v // 2 for v in s)),
+                                N.FuseArgs(epilogue=N.EPI_BLOCKSUM2))
+
     def adjoint_spread(self, y, factors) -> torch.Tensor:
         """adjoint(spread(y)) for a coarse ``y`` whose integer-factor block spread
         (FractionalAreaDownsample's adjoint) is the signal: one correlation whose first

@@ -80,6 +80,12 @@ typedef struct {
 #define DH_EPI_RL_UPDATE 3   /* out = max(aux0 * (v / aux1), b)  (aux1 = +inf off-support) */
 #define DH_EPI_ADD 4         /* out = v + a      reduce: same I-divergence as RL_RATIO when aux0 != NULL */
 #define DH_EPI_SQDIFF 5      /* out = v          reduce: sum (v - aux0)^2 (PDHG's |A x_new - A x|^2)     */
+#define DH_EPI_BLOCKSUM2 6   /* out[z,y,x] = sum of v over the 2x2x2 block at (2z, 2y, 2x): out is the
+                              * coarse (S0/2, S1/2, S2/2) volume, FractionalAreaDownsample(2)'s block
+                              * sum (core_operators.py:205-219) in its passes' order, so
+                              * compose(Downsample(2), Convolver) is one application.  Even signal
+                              * extents, dh_conv_apply on plans with specialised kernels only;
+                              * no reduction */
 
 typedef struct {
   int32_t prologue;

@@ -142,3 +142,65 @@ def test_spread_prologue_rejects_bad_factors(gpu):
         conv.adjoint_spread(torch.zeros((4, 6, 9), device="cuda"), (2, 2, 2))
     # an x factor above 2 keeps the two steps
     assert FractionalAreaDownsample((1.0, 1.0, 4.0)).spread_factors(torch.zeros((8, 12, 4), device="cuda")) is None
+
+
+@pytest.mark.parametrize("shape,kshape", [
+    ((16, 24, 32), (5, 7, 9)),
+    ((8, 12, 18), (3, 4, 5)),        # coarse rows of 9 samples
+    ((6, 10, 64), (5, 5, 5)),
+    ((4, 6, 1600), (3, 3, 3)),       # 1024-point rows: the general load walk
+    ((128, 512, 512), (33, 65, 65)),  # the binned configuration's object grid
+])
+def test_composed_forward_block_sums_in_the_last_pass(gpu, shape, kshape):
+    """compose(FractionalAreaDownsample(2), convolver).forward sums the 2x2x2 blocks in
+    the inverse X pass (DH_EPI_BLOCKSUM2): bit for bit the convolution followed by the
+    block sum; the float64 oracle on small cases."""
+    from deconlib_amd import FractionalAreaDownsample, LinearFFTConvolver, compose
+
+    print(f"block-sum epilogue {shape} * {kshape}", flush=True)
+    rng = np.random.default_rng(73)
+    k = (rng.random(kshape) + 0.05).astype(np.float32)
+    conv = LinearFFTConvolver(k, shape)
+    ds = FractionalAreaDownsample(2.0)
+    assert ds.sum_factors(shape) == (2, 2, 2)
+    x = torch.from_numpy(rng.standard_normal(shape).astype(np.float32)).cuda()
+    got = conv.forward_block_sum(x)
+    assert got is not None and tuple(got.shape) == tuple(v // 2 for v in shape)
+    two = ds.forward(conv.forward(x))
+    assert torch.equal(got, two)
+    assert torch.equal(compose(ds, conv).forward(x), two)
+    if np.prod(shape) < 1e5:
+        conv64 = O.LinearFFTConvolver(k.astype(np.float64), shape, dtype=np.float64)
+        want = O.compose(O.FractionalAreaDownsample(2.0), conv64).forward(x.cpu().numpy().astype(np.float64))
+        g = got.cpu().numpy().astype(np.float64)
+        assert np.linalg.norm(g - want) <= 1e-5 * np.linalg.norm(want)
+
+
+def test_block_sum_epilogue_limits(gpu):
+    """Odd extents and 2-D signals keep the two steps; the C-ABI rejects the epilogue there
+    and in the batched entry point."""
+    import ctypes as C
+
+    from deconlib_amd import FractionalAreaDownsample, LinearFFTConvolver, compose
+    from deconlib_amd import _native as N
+
+    k = np.ones((3, 3, 3), np.float32)
+    odd = LinearFFTConvolver(k, (6, 9, 8))
+    assert odd.forward_block_sum(torch.zeros((6, 9, 8), device="cuda")) is None
+    p = odd._plan
+    x = torch.zeros((6, 9, 8), device="cuda")
+    out = torch.empty((3, 4, 4), device="cuda")
+    f = N.FuseArgs(epilogue=N.EPI_BLOCKSUM2)
+    assert N.fns["dh_conv_apply"](p._h, 0, x.data_ptr(), out.data_ptr(), p.otf_buf.data_ptr(), p.work.data_ptr(),
+                                  int(p.info.work_bytes), C.byref(f), N.stream_ptr()) == N.DH_EINVAL
+    flat = LinearFFTConvolver(np.ones((3, 3), np.float32), (8, 8))
+    y = torch.rand((8, 8), device="cuda")
+    ds = FractionalAreaDownsample(2.0)
+    assert torch.equal(compose(ds, flat).forward(y), ds.forward(flat.forward(y)))
+    ev = LinearFFTConvolver(k, (6, 8, 8))
+    pe = ev._plan
+    xe = torch.rand((6, 8, 8), device="cuda")
+    oe = torch.empty((3, 4, 4), device="cuda")
+    item = N.BatchItem(plan=pe._h, otf=pe.otf_buf.data_ptr(), work=pe.work.data_ptr(),
+                       work_bytes=int(pe.info.work_bytes), input=xe.data_ptr(), out=oe.data_ptr(), fuse=f)
+    assert N.fns["dh_batch_conv_apply"](1, (N.BatchItem * 1)(item), 0, N.stream_ptr()) == N.DH_EINVAL
