@@ -6,7 +6,8 @@ and composition, ``LinearFFTConvolver`` / ``FFTConvolver``, ``Pad`` / ``Crop`` /
 PDHG with identity, gradient and Hessian regularization, NLCG (``nlcg_with_operator``,
 ``nlcg_solver``), ER-Decon (``erdecon_with_operator``, ``erdecon_solver``), JetNewton
 (``jetnewton_with_operator``), and the tiled path (``make_forward_model``,
-``plan_tiles`` / ``process_tiles``, ``richardson_lucy_solver``).  All arithmetic runs in hand-written HIP kernels for
+``plan_tiles`` / ``process_tiles``, ``richardson_lucy_solver``), and ``.psf.h5`` files (``Psf``,
+``save_psf``, ``load_psf``, through the system's HDF5 C library).  All arithmetic runs in hand-written HIP kernels for
 gfx950 (``deconlib_amd/_lib/libdeconhip.so``, C ABI in ``include/deconhip.h``);
 PyTorch only owns device buffers and streams.
 """
@@ -24,6 +25,7 @@ from .pdhg import (GradientRegularizer, HessianRegularizer, IdentityRegularizer,
                    solve_pdhg_with_operator)
 from .rl import RLResult, poisson_i_divergence, richardson_lucy_solver, richardson_lucy_with_operator
 from .forward_model import ForwardModel, make_forward_model
+from .io import Psf, load_psf, save_psf
 from .shapes import compute_padded_shape, compute_visible_shape, get_valid_slices
 from .tile_processing import TilePlan, TileSpec, optimal_tile_size, plan_tiles, process_tiles
 
@@ -56,6 +58,9 @@ __all__ = [
     "richardson_lucy_solver",
     "ForwardModel",
     "make_forward_model",
+    "Psf",
+    "save_psf",
+    "load_psf",
     "TileSpec",
     "TilePlan",
     "plan_tiles",
