@@ -2,6 +2,7 @@
 // application, and the RL elementwise / reduction entry points.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>

@@ -468,14 +468,19 @@ class LinearFFTConvolver:
         application: the inverse X pass sums each 2x2x2 block (DH_EPI_BLOCKSUM2) and the
         fine result never exists.  None where the plan cannot (odd extents, a canvas
         without specialised kernels or with rows beyond 4096 samples)."""
-        s = self.signal_shape
-        if (len(s) != 3 or any(v % 2 for v in s) or int(self._plan.info.spectrum_block) <= 0
-                or self.canvas_shape[2] > 4096):
+        if not self.block_sum_fusable():
             return None
         x = as_f32(x)
         self._check(x)
-        return self._plan.apply(x, False, empty_f32(tuple(v // 2 for v in s)),
+        return self._plan.apply(x, False, empty_f32(tuple(v // 2 for v in self.signal_shape)),
                                 N.FuseArgs(epilogue=N.EPI_BLOCKSUM2))
+
+    def block_sum_fusable(self) -> bool:
+        """Whether the 2x2x2 block-sum epilogue applies: a 3-D signal with even extents on a
+        canvas with specialised kernels and rows of at most 4096 samples."""
+        s = self.signal_shape
+        return (len(s) == 3 and not any(v % 2 for v in s) and int(self._plan.info.spectrum_block) > 0
+                and self.canvas_shape[2] <= 4096)
 
     def adjoint_spread(self, y, factors) -> torch.Tensor:
         """adjoint(spread(y)) for a coarse ``y`` whose integer-factor block spread

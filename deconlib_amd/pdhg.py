@@ -296,6 +296,22 @@ class _PDState:
                    self.y2.data_ptr(), nz, ny, nx, self.sc.data_ptr(), self.wp, self.part.data_ptr(),
                    N.stream_ptr())
 
+    def _adjoint(self) -> None:
+        """g = A^T y1_new, the y1 step in the first pass where the operator allows."""
+        afz = self._adj_fuse()
+        if afz is not None:
+            self.g = self._plan.apply(self.y1, True, self.g, afz)
+        else:
+            self.g = self._adj(self.y1, self.g)
+
+    def _forward(self, want_loss: bool) -> None:
+        """A x_new into Axp, |A x_new - A x|^2 in its last pass where the operator allows."""
+        fuse = self._fwd_fuse(want_loss)
+        if fuse is not None:
+            self.Axp = self._plan.apply(self.x_next, False, self.Axp, fuse)
+        else:
+            self.Axp = self._fwd(self.x_next, self.Axp)
+
     def _adj_fuse(self) -> Optional[N.FuseArgs]:
         """Prologue of the adjoint A^T y1 on a plan with specialised kernels: the y1
         dual step (dh_pd_y1's values) on each sample as the first pass reads it, y1
@@ -347,17 +363,9 @@ class _PDState:
 
     def step(self, k: int, want_loss: bool) -> None:
         self._pre(k)
-        afz = self._adj_fuse()
-        if afz is not None:
-            self.g = self._plan.apply(self.y1, True, self.g, afz)
-        else:
-            self.g = self._adj(self.y1, self.g)
+        self._adjoint()
         self._mid(k)
-        fuse = self._fwd_fuse(want_loss)
-        if fuse is not None:
-            self.Axp = self._plan.apply(self.x_next, False, self.Axp, fuse)
-        else:
-            self.Axp = self._fwd(self.x_next, self.Axp)
+        self._forward(want_loss)
         self._post(k, want_loss)
 
     def done(self) -> bool:
@@ -389,9 +397,9 @@ def pd_batch_step(states: List[_PDState], ks: List[int], want_loss=None, max_ite
         for s in states:
             if s._plan is None:
                 if adjoint:
-                    s.g = s._adj(s.y1, s.g)
+                    s._adjoint()
                 else:
-                    s.Axp = s._fwd(s.x_next, s.Axp)
+                    s._forward(wl_of[id(s)])
         if adjoint:
             for s, k in zip(states, ks):
                 s._mid(k)
