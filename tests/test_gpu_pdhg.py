@@ -235,3 +235,13 @@ def test_y1_prologue_rejects_missing_operands(gpu):
         f.pro_aux[:] = [a or None for a in aux]
         assert N.fns["dh_conv_apply"](p._h, 1, y.data_ptr(), out.data_ptr(), p.otf_buf.data_ptr(), p.work.data_ptr(),
                                       int(p.info.work_bytes), C.byref(f), N.stream_ptr()) == N.DH_EINVAL
+    # a canvas without specialised kernels (10000 samples) keeps dh_pd_y1: the prologue is refused
+    rl = LinearFFTConvolver(np.ones(3001, np.float32), (7000,))
+    assert int(rl._plan.info.spectrum_block) == 0
+    q = rl._plan
+    y7 = torch.zeros(7000, device="cuda")
+    o7 = torch.empty_like(y7)
+    f = N.FuseArgs(prologue=N.PRO_PD_Y1, pro_out=y7.data_ptr(), pro_scalars=sc.data_ptr())
+    f.pro_aux[:] = [y7.data_ptr()] * 3
+    assert N.fns["dh_conv_apply"](q._h, 1, y7.data_ptr(), o7.data_ptr(), q.otf_buf.data_ptr(), q.work.data_ptr(),
+                                  int(q.info.work_bytes), C.byref(f), N.stream_ptr()) == N.DH_EINVAL
