@@ -515,12 +515,34 @@ def bench_batched_small(torch, n_vol=32, shape=(16, 64, 64), iters=50, seed=1005
     eager = timed(lambda: batched_richardson_lucy([RLJob(v, o, num_iter=iters) for v, o in zip(vols, ops)]))
     graph = timed(lambda: batched_richardson_lucy([RLJob(v, o, num_iter=iters) for v, o in zip(vols, ops)],
                                                   graphs=True))
+
+    # steady state: per-iteration time of a running batch once the graph exists (its
+    # capture is a one-time cost that the end-to-end figures above include), the same
+    # iterations eager; every 10th iteration evaluates the loss and runs eagerly in both
+    from deconlib_amd.batch import BatchRunner
+
+    def steady(graphs: bool, n: int = 40) -> float:
+        jobs = [RLJob(v, o, num_iter=n + 10) for v, o in zip(vols, ops)]
+        runner = BatchRunner(len(jobs), graphs=graphs)
+        for _ in range(5):
+            runner.step(jobs)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            runner.step(jobs)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / n
+
+    steady_eager, steady_graph = steady(False), steady(True)
     return {
         "workload": f"{n_vol} volumes {shape[0]}x{shape[1]}x{shape[2]}, RL {iters} it (canvas 32x128x128)",
         "plain_loop_voxel_iters_per_s": plain,
         "batched_voxel_iters_per_s": eager,
         "batched_graph_voxel_iters_per_s": graph,
         "graph_speedup_vs_plain_loop": graph / plain,
+        "steady_iteration_ms": {"eager": steady_eager, "graph": steady_graph,
+                                "note": "one batched iteration of all volumes after 5 warm-up iterations "
+                                        "(graph captured), mean of 40"},
     }
 
 
