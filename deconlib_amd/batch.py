@@ -99,6 +99,18 @@ class RLJob:
         return self.result
 
 
+_CAPTURE_STREAMS = {}
+
+
+def _capture_stream():
+    """One side stream per device for graph captures, created on first use (creating a
+    stream costs milliseconds, more than a capture)."""
+    dev = torch.cuda.current_device()
+    if dev not in _CAPTURE_STREAMS:
+        _CAPTURE_STREAMS[dev] = torch.cuda.Stream()
+    return _CAPTURE_STREAMS[dev]
+
+
 class BatchRunner:
     """Advances a changing set of RL jobs one iteration at a time (continuous batching).
 
@@ -136,7 +148,7 @@ class BatchRunner:
             # empties the caching allocator (the capture allocates nothing)
             cur = torch.cuda.current_stream()
             if self._stream is None:
-                self._stream = torch.cuda.Stream()
+                self._stream = _capture_stream()
             self._stream.wait_stream(cur)
             try:
                 with torch.cuda.stream(self._stream):
