@@ -197,10 +197,12 @@ __device__ __forceinline__ void pass_store(float2 (&u)[(N / R + T - 1) / T][R], 
   for (int m = 0; m < M; ++m) {
     const int i = t + m * T;
     if ((M * T == NB) || i < NB) {
-      const int k = i % P;
+      // the last pass of a line (P == NB) has i < P: j = i, written so that a store's
+      // index stays the thread's index plus a constant
+      const int k = (P == NB) ? i : i % P;
       if constexpr (P > 1) twiddle<R, DIR, P>(u[m], tw, k);
       Dft<R, DIR>::run(u[m]);
-      const int j = (i / P) * P * R + k;
+      const int j = (P == NB) ? i : (i / P) * P * R + k;
 #pragma unroll
       for (int r = 0; r < R; ++r) st(j + r * P, u[m][r]);
     }

@@ -366,6 +366,7 @@ __device__ __forceinline__ void z_mid(int t, const float2* __restrict__ tw, floa
   if (from_lds) __syncthreads();
   constexpr int off = tw_pass_offset(Geo<N>::F, Geo<N>::F.np - 1);
   const float2* twp = tw + off;
+  const float2* ot = otf + col + (int64_t)t * rs;  // OTF element t + c at c rows' uniform offset
 #pragma unroll
   for (int m = 0; m < M; ++m) {
     const int i = t + m * T;
@@ -375,7 +376,7 @@ __device__ __forceinline__ void z_mid(int t, const float2* __restrict__ tw, floa
       Dft<R, -1>::run(u[m]);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const float2 o = ok ? ld_nt(otf + (int64_t)(i + r * NB) * rs + col) : make_float2(0.f, 0.f);
+        const float2 o = ok ? ld_nt(ot + (int64_t)(m * T + r * NB) * rs) : make_float2(0.f, 0.f);
         u[m][r] = conj ? cmulc(u[m][r], o) : cmul(u[m][r], o);
       }
       Dft<R, 1>::run(u[m]);
@@ -467,9 +468,11 @@ __global__ void __launch_bounds__(kMaxThreads) k_z(Pack<ZArgs> pk, int L) {
       const ZArgs& a = pk.a[v];
       const int64_t col = (bb - pk.prefix[v]) * L + ln;
       const bool ok = col < a.ncols;
-      const float2* cb = a.buf + col;
+      // element e = t + c sits c rows after the thread's first one: c * ncols is
+      // uniform, so no per-element 64-bit multiply
+      const float2* cbt = a.buf + col + (int64_t)t * a.ncols;
       pass_load<N, T, R0>(u, t, [&](int e) {
-        return (ok && e < a.lo) ? ld_nt(cb + (int64_t)e * a.ncols) : make_float2(0.f, 0.f);
+        return (ok && e < a.lo) ? ld_nt(cbt + (int64_t)(e - t) * a.ncols) : make_float2(0.f, 0.f);
       });
     };
     fetch(b);
@@ -484,8 +487,9 @@ __global__ void __launch_bounds__(kMaxThreads) k_z(Pack<ZArgs> pk, int L) {
       const bool ok = col < a.ncols;
       float2* cb = a.buf + col;
       const int64_t rs = a.ncols;
+      float2* cbt = cb + (int64_t)t * rs;  // as in fetch: the last pass stores at t + constant
       auto gst = [&](int e, float2 val) {
-        if (ok && e < a.n_out) st_nt(cb + (int64_t)e * rs, val);
+        if (ok && e < a.n_out) st_nt(cbt + (int64_t)(e - t) * rs, val);
       };
       pass_store<N, T, R0, 1, -1>(u, t, a.tw, lst);
       const int64_t bn = b + gridDim.x;
