@@ -1367,6 +1367,7 @@ int fill_fuse(const dh_fuse_args* fuse, dh::Pro& pro, dh::Epi& epi, double** red
   epi.aux0 = fuse->aux0;
   epi.aux1 = fuse->aux1;
   if (pro.kind < DH_PRO_NONE || pro.kind > DH_PRO_PD_Y1) return fail(DH_EINVAL, "unknown prologue");
+  if (epi.kind == DH_EPI_BLOCKSUM2) return fail(DH_EINVAL, "the block-sum epilogue is not available in batches");
   if (epi.kind < DH_EPI_NONE || epi.kind > DH_EPI_SQDIFF) return fail(DH_EINVAL, "unknown epilogue");
   if ((epi.kind == DH_EPI_RL_RATIO || epi.kind == DH_EPI_RL_UPDATE || epi.kind == DH_EPI_SQDIFF) && !epi.aux0)
     return fail(DH_EINVAL, "epilogue needs aux0");
