@@ -546,7 +546,7 @@ def bench_batched_small(torch, n_vol=32, shape=(16, 64, 64), iters=50, seed=1005
     }
 
 
-def bench_batched_config(torch, n_vol=64, iters=100, seed=1002):
+def bench_batched_config(torch, n_vol=64, iters=100, seed=1002, tune=False):
     """The batched configuration at full size: 64 volumes with z in {16,32,48,64} and
     y = x in {128,192,256,384,512}, each with its own aberrated PSF (IndexMismatch depth
     U[0,20] um, spherical U[-0.5,0.5] rad, synthesised on the GPU) and blob object
@@ -555,7 +555,8 @@ def bench_batched_config(torch, n_vol=64, iters=100, seed=1002):
     another; the batched run advances all of them together, each pass one grouped
     launch per transform length and 16 volumes.  Both timings include every job's
     set-up (sensitivity) and result; the batched results must equal the plain ones
-    bit for bit."""
+    bit for bit.  With ``tune`` the launch shapes come from the autotuner, one tuning
+    per shape class (reported as ``autotune_s``, outside both timings)."""
     from deconlib_amd import LinearFFTConvolver, richardson_lucy_with_operator
     from deconlib_amd.batch import RLJob, batched_richardson_lucy
     from deconlib_amd.synth import aberrated_psf, blob_volume
@@ -578,6 +579,14 @@ def bench_batched_config(torch, n_vol=64, iters=100, seed=1002):
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
     vox = sum(int(o.numel()) for o in obs)
+    tune_s = None
+    if tune:  # launch shapes per shape class (plans of one shape share the tuned result)
+        from deconlib_amd.autotune import autotune
+
+        t0 = time.perf_counter()
+        for op in ops:
+            autotune(op, reps=5)
+        tune_s = time.perf_counter() - t0
 
     def plain(n):
         return [richardson_lucy_with_operator(o, op, num_iter=n) for o, op in zip(obs, ops)]
@@ -604,6 +613,7 @@ def bench_batched_config(torch, n_vol=64, iters=100, seed=1002):
         "voxels": vox,
         "shapes": sorted({tuple(int(s) for s in o.shape) for o in obs}),
         "setup_s": setup_s,
+        "autotune_s": tune_s,
         "plain_loop_s": tp,
         "plain_loop_voxel_iters_per_s": vox * iters / tp,
         "batched_s": tb,
@@ -1010,7 +1020,7 @@ def main():
         log(f"host-to-host solve: {json.dumps(host)}")
         small = bench_batched_small(torch)
         log(f"batched small volumes: {json.dumps(small)}")
-        batched_cfg = bench_batched_config(torch)
+        batched_cfg = bench_batched_config(torch, tune=not args.no_tune)
         log(f"batched configuration: {json.dumps(batched_cfg)}")
         tiled = bench_tiled(torch)
         log(f"tiled: {json.dumps(tiled)}")
