@@ -168,13 +168,15 @@ def bench_pdhg_binned(torch, steps=10, warmup=2, seed=1001, tune=False, tune_cac
     }
 
 
-def bench_large_volume(torch, steps=3, warmup=1, seed=1004):
+def bench_large_volume(torch, steps=3, warmup=1, seed=1004, tune=False):
     """The large single-volume configuration at full size: 256x2048x2048 float32 on a
     512x4096x4096 canvas, PDHG-Hessian (alpha 5e-4, L1) iterations.  Object: 1e5 blobs
     + 10 and 64 thin filaments (intensity 300); vectorial PSF of an isotropic emitter
     (pupil_to_vectorial_psf) of the volume's size with IndexMismatch depth 10 um,
     synthesised on the GPU; Poisson data.  Skipped below
-    200 GB of free device memory (about 150 GB stay resident)."""
+    200 GB of free device memory (about 150 GB stay resident).  With ``tune`` the two Y
+    passes' launch shapes are autotuned (the 4096-point Y passes' best shape differs
+    from box to box; a whole-canvas tuning of every pass would take minutes)."""
     import ctypes as C
 
     from deconlib_amd import LinearFFTConvolver
@@ -193,6 +195,12 @@ def bench_large_volume(torch, steps=3, warmup=1, seed=1004):
                         device=True, dipole="isotropic")
     op = LinearFFTConvolver(psf, shape)
     del psf
+    tiles = {}
+    if tune:
+        from deconlib_amd.autotune import autotune
+
+        tiles = autotune(op, reps=2, candidates=(1, 2), passes=(1, 3))
+        torch.cuda.empty_cache()
     obj = blob_volume(shape, 100000, rng)
     obj = torch.from_numpy(np.maximum(obj, filaments(shape, 64, rng)).astype(np.float32)).cuda()
     clean = op.forward(obj).clamp_min_(0.0)
@@ -231,6 +239,7 @@ def bench_large_volume(torch, steps=3, warmup=1, seed=1004):
         "warmup": warmup,
         "setup_s": setup_s,
         "canvas": list(op.canvas_shape),
+        "tiles": {N.PASS_NAMES[p]: t for p, t in tiles.items()},
         "device_gb_resident": torch.cuda.memory_allocated() / 1e9,
         "passes": passes,
         "finite": bool(torch.isfinite(st.x).all().item()),
@@ -1005,7 +1014,7 @@ def main():
     log(f"pdhg: {json.dumps(pdhg)}")
     pdhg["binned_configuration"] = bench_pdhg_binned(torch, tune=not args.no_tune, tune_cache=args.tune_cache)
     log(f"pdhg binned configuration: {json.dumps(pdhg['binned_configuration'])}")
-    pdhg["large_volume"] = bench_large_volume(torch)
+    pdhg["large_volume"] = bench_large_volume(torch, tune=not args.no_tune)
     log(f"pdhg large volume: {json.dumps(pdhg['large_volume'])}")
     nlcg = bench_nlcg(torch, obs, blur, args.steps, args.warmup)
     nlcg["vs_rl_to_equal_fit"] = nlcg_vs_rl(torch, obs, blur)

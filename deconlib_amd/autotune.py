@@ -57,10 +57,12 @@ def _time_passes(op, x, reps: int):
 
 
 def autotune(op, reps: int = 10, candidates=(1, 2, 4, 8, 16, 32, 64), cache_file: Optional[str] = None,
-             verbose: bool = False) -> Dict[int, int]:
-    """Picks the fastest tile for every pass of ``op`` (a LinearFFTConvolver/FFTConvolver)."""
+             verbose: bool = False, passes=range(5)) -> Dict[int, int]:
+    """Picks the fastest tile for every pass in ``passes`` (0..4: x_fwd, y_fwd, z_conv,
+    y_inv, x_inv) of ``op`` (a LinearFFTConvolver/FFTConvolver); the others keep theirs."""
     plan = op._plan
-    key = _key(plan)
+    passes = tuple(sorted(set(int(p) for p in passes)))
+    key = _key(plan) + ("" if passes == tuple(range(5)) else "-p" + "".join(map(str, passes)))
     if cache_file and key not in _CACHE and os.path.exists(cache_file):
         with open(cache_file) as f:
             for k, v in json.load(f).items():
@@ -71,7 +73,7 @@ def autotune(op, reps: int = 10, candidates=(1, 2, 4, 8, 16, 32, 64), cache_file
         return dict(_CACHE[key])
     x = torch.rand(tuple(op.in_shape), dtype=torch.float32, device="cuda")
     best: Dict[int, int] = {}
-    for p in range(5):
+    for p in passes:
         timings = {}
         for t in candidates:
             for v in (t, t | N.TILE_PERSISTENT):
