@@ -212,3 +212,26 @@ def test_sqdiff_epilogue(gpu, sshape, kshape):
     bad = N.FuseArgs(epilogue=N.EPI_SQDIFF, reduce_out=red.data_ptr())
     assert N.fns["dh_conv_apply"](p._h, 0, x.data_ptr(), out.data_ptr(), p.otf_buf.data_ptr(), p.work.data_ptr(),
                                   int(p.info.work_bytes), C.byref(bad), N.stream_ptr()) == N.DH_EINVAL
+
+
+def test_autotune_restricted_passes(gpu):
+    """autotune(passes=...) times and sets only the named passes; the others keep their
+    launch shapes, and the tuned operator computes the same values."""
+    from deconlib_amd import LinearFFTConvolver
+    from deconlib_amd import _native as N
+    from deconlib_amd.autotune import autotune
+
+    rng = np.random.default_rng(21)
+    shape = (16, 64, 96)
+    op = LinearFFTConvolver((rng.random((5, 7, 9)) + 0.05).astype(np.float32), shape)
+    x = torch.from_numpy(rng.standard_normal(shape).astype(np.float32)).cuda()
+    ref = op.forward(x)
+    h = op._plan._h
+    before = [N.fns["dh_conv_plan_get_tile"](h, p) for p in range(5)]
+    best = autotune(op, reps=1, candidates=(1, 2), passes=(1, 3))
+    assert set(best) <= {1, 3}
+    after = [N.fns["dh_conv_plan_get_tile"](h, p) for p in range(5)]
+    assert [after[p] for p in (0, 2, 4)] == [before[p] for p in (0, 2, 4)]
+    for p, t in best.items():
+        assert after[p] == t
+    assert torch.equal(op.forward(x), ref)
