@@ -14,7 +14,7 @@ requires the GPU's to stay within the parity bar (relative L2 <= 1e-4) and withi
   48x192x192) with their own aberrated PSFs, run together in one batch, 50 iterations;
 * the large-volume configuration's solver (PDHG-Hessian alpha 5e-4), 20 iterations, on
   a 64x256x256 crop and (with DECONLIB_AMD_LONG_TESTS=1: the float64 oracle takes
-  about 10 minutes of host time there) on the configuration's 128x512x512 crop:
+  several minutes of host time there, about 3.5 on 16 cores) on the configuration's 128x512x512 crop:
   blobs at the configuration's density plus thin filaments, the configuration's
   vectorial PSF (isotropic emitter, NA 1.4) with IndexMismatch depth 10 um.
 
@@ -119,7 +119,7 @@ LONG = os.environ.get("DECONLIB_AMD_LONG_TESTS") == "1"
 @pytest.mark.parametrize("shape,it", [
     ((64, 256, 256), 20),
     pytest.param((128, 512, 512), 20, marks=pytest.mark.skipif(
-        not LONG, reason="the float64 oracle needs ~10 minutes of host time at this size; "
+        not LONG, reason="the float64 oracle needs several minutes of host time at this size; "
                          "run with DECONLIB_AMD_LONG_TESTS=1")),
 ])
 def test_large_volume_crop_fp64_error(gpu, shape, it):
